@@ -1,0 +1,160 @@
+/*
+ * topodda.h -- C ABI of the MI355X DDA solve engine (libtopodda_hip.so).
+ *
+ * Drop-in boundary for the Aproduct + BiCGSTAB path of Topo-DDA-forWin64.
+ * Reference paths are relative to its Topo-DDA-forWin64/ directory.
+ *
+ *   reference                                          this ABI
+ *   AProductCore::AProductCore  AProductCore.cpp:12-249   tdda_op_create
+ *   AProductCore::~AProductCore AProductCore.cpp:538-558  tdda_op_destroy
+ *   AProductCore::Aproduct      AProductCore.cpp:697-758  tdda_aproduct
+ *   AProductCore::LDR_inter     AProductCore.cpp:588-623  tdda_green_tensor
+ *   DDAModel state (P, E, al)   DDAModel.h:14-30          tdda_model_create/_set/_get
+ *   DDAModel::Aproductwithalb   DDAModel.cpp:577-586      tdda_model_apply
+ *   DDAModel::bicgstab          DDAModel.cpp:131-260      tdda_model_solve
+ *   (kernel.cu A2As/B2Bs/Conv/Conv2B and the cuFFT calls are internal.)
+ *
+ * Conventions
+ *   - Every function returns TDDA_OK (0) or a negative status; the message of
+ *     the last failure on the calling thread is tdda_last_error().
+ *   - Complex vectors are interleaved (re, im) doubles; a site vector holds
+ *     3N complex values, entry 3*i+c is component c of dipole i
+ *     (AProductCore.cpp:705-714).  Batched calls take nrhs such vectors back
+ *     to back.
+ *   - Handles own their device memory; callers own host buffers.  A handle is
+ *     not thread-safe: one host thread at a time per context.
+ *   - No exceptions cross this boundary.
+ */
+#ifndef TOPODDA_H_
+#define TOPODDA_H_
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define TDDA_OK 0
+#define TDDA_ERR_ARG (-1)
+#define TDDA_ERR_HIP (-2)
+#define TDDA_ERR_OOM (-3)
+#define TDDA_ERR_STATE (-4)
+#define TDDA_ERR_UNSUPPORTED (-5)
+
+/* precision bits */
+#define TDDA_C128 1
+#define TDDA_C64 2
+
+/* model vectors */
+#define TDDA_VEC_AL 0 /* 1/alpha per component (DDAModel::al)            */
+#define TDDA_VEC_E 1  /* right-hand side (DDAModel::E)                     */
+#define TDDA_VEC_P 2  /* solution / initial guess (DDAModel::P)            */
+#define TDDA_VEC_Y 3  /* output of tdda_model_apply                        */
+
+/* solve status */
+#define TDDA_SOLVE_CONVERGED 1
+#define TDDA_SOLVE_BREAKDOWN 2 /* r0.rl == 0, DDAModel.cpp:193-216         */
+#define TDDA_SOLVE_MAXIT 3     /* "does not converge", DDAModel.cpp:256-259 */
+
+/* solve modes */
+#define TDDA_MODE_FP64 0  /* reference algorithm in complex128 on the GPU   */
+#define TDDA_MODE_MIXED 1 /* complex64 inner BiCGSTAB, FP64 refinement      */
+
+typedef struct tdda_ctx_s* tdda_ctx;
+typedef struct tdda_op_s* tdda_op;
+typedef struct tdda_model_s* tdda_model;
+
+/* Last error message of the calling thread ("" if none). */
+const char* tdda_last_error(void);
+int tdda_version(void);
+
+/* Device context: one HIP device and one stream. */
+int tdda_ctx_create(int device, tdda_ctx* out);
+int tdda_ctx_destroy(tdda_ctx ctx);
+int tdda_ctx_sync(tdda_ctx ctx);
+
+/* Operator description.  AProductCore.cpp:12-44 takes these from
+ * CoreStructure (N, Nx, Ny, Nz, R, d) and its own arguments (lam, nback).
+ * R holds 3N integer lattice coordinates with 0 <= R[3i+a] < N_a; sites must
+ * be distinct.  grid = (Mx, My, Mz) chooses the zero-padded FFT grid; 0 lets
+ * the planner pick the smallest {2,3,5}-smooth sizes >= 2N_a - 1.  Any grid
+ * >= 2N_a - 1 gives the same linear convolution as the reference's 2N_a - 1.
+ * K = 2 pi nback / lam (AProductCore.cpp:17).                               */
+typedef struct {
+    int N;
+    int Nx, Ny, Nz;
+    const int* R;
+    double d;
+    double lam;
+    double nback;
+    int Mx, My, Mz;
+    int precisions; /* TDDA_C128 | TDDA_C64 */
+} tdda_op_desc;
+
+/* AProductCore::AProductCore (AProductCore.cpp:12-249): builds the Green
+ * tensor on the device, transforms it and keeps one octant of its spectrum. */
+int tdda_op_create(tdda_ctx ctx, const tdda_op_desc* desc, tdda_op* out);
+/* AProductCore::~AProductCore (AProductCore.cpp:538-558). */
+int tdda_op_destroy(tdda_op op);
+/* grid3 <- (Mx, My, Mz); K <- wavenumber.  Either may be NULL. */
+int tdda_op_info(tdda_op op, int* grid3, double* K);
+
+/* AProductCore::Aproduct (AProductCore.cpp:697-758): y = A_offdiag b for
+ * nrhs site vectors (no diagonal term, AProductCore.h:56).  If al is not
+ * NULL, adds al o b per right-hand side (DDAModel::Aproductwithalb,
+ * DDAModel.cpp:577-586); al_per_rhs = 0 shares one al across the batch.
+ * precision selects the arithmetic (TDDA_C128 or TDDA_C64; inputs and
+ * outputs are complex128 either way).  Host buffers, synchronous.           */
+int tdda_aproduct(tdda_op op, const double* b, double* y, int nrhs, const double* al, int al_per_rhs,
+                  int precision);
+
+/* AProductCore::LDR_inter (AProductCore.cpp:588-623) on the host: the six
+ * unique components xx xy xz yy yz zz of the interaction tensor at offset
+ * (x, y, z) for wavenumber K, as 12 doubles.                                */
+int tdda_green_tensor(double K, double x, double y, double z, double* out12);
+
+/* DDAModel device state for nrhs right-hand sides sharing one operator:
+ * al, E, P per right-hand side, plus solver work space.                    */
+int tdda_model_create(tdda_op op, int nrhs, tdda_model* out);
+int tdda_model_destroy(tdda_model m);
+/* Copy one vector kind (TDDA_VEC_*) to or from the device.  rhs = -1 moves
+ * all nrhs vectors (nrhs * 3N complex); otherwise one right-hand side.     */
+int tdda_model_set(tdda_model m, int which, int rhs, const double* host);
+int tdda_model_get(tdda_model m, int which, int rhs, double* host);
+/* Y = A P + al o P on the device (DDAModel::Aproductwithalb). */
+int tdda_model_apply(tdda_model m, int precision);
+
+typedef struct {
+    int max_it;        /* MAX_ITERATION of DDAModel::bicgstab             */
+    double tol;        /* MAX_ERROR: stop when ||r|| / ||E|| <= tol       */
+    int mode;          /* TDDA_MODE_FP64 or TDDA_MODE_MIXED               */
+    int x0_zero;       /* 1: start from P = 0 (skips the A P product)     */
+    double inner_tol;  /* mixed: relative tolerance of each inner solve   */
+    int inner_max_it;  /* mixed: iteration cap of each inner solve        */
+    int max_outer;     /* mixed: refinement cycles                        */
+    int check_every;   /* iterations between host status polls (0: auto)  */
+    int true_residual; /* 1: also compute ||E - A P|| / ||E|| in FP64     */
+} tdda_solve_opts;
+
+typedef struct {
+    int status;      /* TDDA_SOLVE_*                                        */
+    int iterations;  /* ITERATION (FP64 mode) or total inner iterations     */
+    double error;    /* recurrence residual ||r|| / ||E|| (DDAModel::Error) */
+    double true_residual; /* ||E - A P|| / ||E|| in FP64, if requested      */
+    int outer;       /* mixed: refinement cycles used                       */
+} tdda_solve_info;
+
+void tdda_solve_opts_default(tdda_solve_opts* o);
+
+/* DDAModel::bicgstab (DDAModel.cpp:131-260) for every right-hand side of
+ * the model, on the device.  FP64 mode follows the reference algorithm
+ * statement by statement; MIXED runs complex64 BiCGSTAB inner solves inside
+ * an FP64 residual-refinement loop whose stop test uses the true residual.
+ * info must hold nrhs entries.  P is updated in device memory.            */
+int tdda_model_solve(tdda_model m, const tdda_solve_opts* opts, tdda_solve_info* info);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* TOPODDA_H_ */

@@ -1,0 +1,332 @@
+"""CPU oracle for the DDA operator and BiCGSTAB solve (Topo-DDA-forWin64).
+
+TEST INFRASTRUCTURE ONLY.  Imported by tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py, always as the checker.  The product path never
+imports it.
+
+Two layers:
+
+* ``liboracle.so`` (oracle/dda_oracle.c): FP64 C restatement of the reference
+  path -- interaction tensor (AProductCore.cpp:588-623), Get_Alpha
+  (tools.cpp:617-630), incident field (DDAModel.cpp:47-51), permittivity
+  interpolation and 1/alpha (DDAModel.cpp:54-63), FFT operator
+  (AProductCore.cpp:12-249, 697-758; kernel.cu:74-95) and BiCGSTAB
+  (DDAModel.cpp:131-260).  Wrapped by the ``Orc*`` helpers below.
+* numpy references used to pin the C restatement: an O(N^2) direct sum in
+  long double, dense assembly of the 3N x 3N matrix, and an ``numpy.fft``
+  convolution.  They restate the same formulas independently (different
+  algebraic form for the tensor: -G with the textbook dyadic Green function).
+
+``oracle/_ref/libref_tools.so`` is the reference's own tools.cpp compiled from
+its sources (oracle/Makefile target ``ref``); ``ref_lib()`` loads it when present.
+"""
+from __future__ import annotations
+
+import ctypes
+import math
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = None
+_REF = None
+
+_dp = ctypes.POINTER(ctypes.c_double)
+_ip = ctypes.POINTER(ctypes.c_int)
+
+
+def _ptr(a, t=_dp):
+    return a.ctypes.data_as(t)
+
+
+def build(force: bool = False) -> None:
+    """Compile liboracle.so (and the reference tools library when mounted)."""
+    args = ["make", "-s", "-C", HERE, "all"]
+    if force:
+        args.insert(1, "-B")
+    subprocess.check_call(args)
+    if os.path.isdir("/root/reference/Topo-DDA-forWin64"):
+        subprocess.check_call(["make", "-s", "-C", HERE, "ref"])
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        path = os.path.join(HERE, "liboracle.so")
+        if not os.path.exists(path):
+            build()
+        L = ctypes.CDLL(path)
+        L.orc_ldr_inter.argtypes = [ctypes.c_double] * 4 + [_dp]
+        L.orc_get_alpha.argtypes = [ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_double, _dp, _dp, _dp]
+        L.orc_wavenumber.argtypes = [ctypes.c_double, ctypes.c_double]
+        L.orc_wavenumber.restype = ctypes.c_double
+        L.orc_material_rule.argtypes = [ctypes.c_int, _dp, ctypes.c_double]
+        L.orc_einc.argtypes = [ctypes.c_int, _ip, ctypes.c_double, ctypes.c_double, ctypes.c_double, _dp, _dp, _dp]
+        L.orc_diel_al.argtypes = [ctypes.c_int, _dp, ctypes.c_int, _dp, ctypes.c_double, ctypes.c_double,
+                                  ctypes.c_double, _dp, _dp, _dp, _dp]
+        L.orc_local_field.argtypes = [ctypes.c_int, _dp, _dp, _dp, _dp]
+        L.orc_aproduct_direct.argtypes = [ctypes.c_int, _ip, ctypes.c_double, ctypes.c_double, _dp, _dp]
+        L.orc_fft1d.argtypes = [ctypes.c_int, _dp, ctypes.c_int]
+        L.orc_op_create.argtypes = [ctypes.c_int, _ip] + [ctypes.c_int] * 6 + [ctypes.c_double, ctypes.c_double]
+        L.orc_op_create.restype = ctypes.c_void_p
+        L.orc_op_destroy.argtypes = [ctypes.c_void_p]
+        L.orc_op_apply.argtypes = [ctypes.c_void_p, _dp, _dp, _dp]
+        L.orc_bicgstab.argtypes = [ctypes.c_void_p, _dp, _dp, _dp, ctypes.c_int, ctypes.c_double, ctypes.c_int,
+                                   _ip, _dp]
+        L.orc_bicgstab.restype = ctypes.c_int
+        L.orc_num_threads.restype = ctypes.c_int
+        _LIB = L
+    return _LIB
+
+
+def ref_lib():
+    """The reference's own tools.cpp (compiled into oracle/_ref), or None."""
+    global _REF
+    if _REF is None:
+        path = os.path.join(HERE, "_ref", "libref_tools.so")
+        if not os.path.exists(path):
+            return None
+        L = ctypes.CDLL(path)
+        L.ref_get_alpha.argtypes = [ctypes.c_double] * 5 + [_dp, _dp, _dp]
+        L.ref_nE_perp_in_xz.argtypes = [ctypes.c_double, ctypes.c_double, _dp]
+        L.ref_build_a_bulk.argtypes = [ctypes.c_int] * 3 + [_ip]
+        _REF = L
+    return _REF
+
+
+# ---------------------------------------------------------------------------
+# Scalars and per-site quantities (C restatement)
+# ---------------------------------------------------------------------------
+def wavenumber(lam: float, nback: float = 1.0) -> float:
+    """AProductCore.cpp:17."""
+    return lib().orc_wavenumber(lam, nback)
+
+
+def material_rule(mat, nback: float = 1.0) -> np.ndarray:
+    """AProductCore.cpp:19-24 (divide by nback^2, +0.01 when |eps| <= 1.01)."""
+    m = np.ascontiguousarray(np.asarray(mat, dtype=np.complex128).copy())
+    lib().orc_material_rule(len(m), _ptr(m.view(np.float64)), nback)
+    return m
+
+
+def ldr_inter(K: float, x: float, y: float, z: float) -> np.ndarray:
+    """AProductCore.cpp:588-623; returns [xx, xy, xz, yy, yz, zz]."""
+    out = np.zeros(6, np.complex128)
+    lib().orc_ldr_inter(K, x, y, z, _ptr(out.view(np.float64)))
+    return out
+
+
+def get_alpha(K: float, d: float, diel: complex, nE, nK) -> complex:
+    """tools.cpp:617-630."""
+    out = np.zeros(2)
+    e = np.ascontiguousarray(nE, dtype=np.float64)
+    k = np.ascontiguousarray(nK, dtype=np.float64)
+    lib().orc_get_alpha(K, d, diel.real, diel.imag, _ptr(e), _ptr(k), _ptr(out))
+    return complex(out[0], out[1])
+
+
+def einc(R: np.ndarray, K: float, d: float, E0: float, nK, nE) -> np.ndarray:
+    """DDAModel.cpp:47-51."""
+    R = np.ascontiguousarray(R, dtype=np.int32)
+    N = len(R) // 3
+    E = np.zeros(3 * N, np.complex128)
+    lib().orc_einc(N, _ptr(R, _ip), K, d, E0, _ptr(np.ascontiguousarray(nK, np.float64)),
+                   _ptr(np.ascontiguousarray(nE, np.float64)), _ptr(E.view(np.float64)))
+    return E
+
+
+def diel_al(diel_old: np.ndarray, mat: np.ndarray, lam: float, K: float, d: float, nE, nK):
+    """DDAModel.cpp:54-63: (eps per component, 1/alpha per component)."""
+    t = np.ascontiguousarray(diel_old, dtype=np.float64)
+    m = np.ascontiguousarray(mat, dtype=np.complex128)
+    diel = np.zeros(len(t), np.complex128)
+    al = np.zeros(len(t), np.complex128)
+    lib().orc_diel_al(len(t), _ptr(t), len(m), _ptr(m.view(np.float64)), lam, K, d,
+                      _ptr(np.ascontiguousarray(nE, np.float64)), _ptr(np.ascontiguousarray(nK, np.float64)),
+                      _ptr(diel.view(np.float64)), _ptr(al.view(np.float64)))
+    return diel, al
+
+
+def local_field(al: np.ndarray, P: np.ndarray, diel: np.ndarray) -> np.ndarray:
+    """DDAModel.cpp:567-575."""
+    out = np.zeros_like(P)
+    lib().orc_local_field(len(P), _ptr(np.ascontiguousarray(al).view(np.float64)),
+                          _ptr(np.ascontiguousarray(P).view(np.float64)),
+                          _ptr(np.ascontiguousarray(diel).view(np.float64)), _ptr(out.view(np.float64)))
+    return out
+
+
+def aproduct_direct(R: np.ndarray, d: float, K: float, b: np.ndarray) -> np.ndarray:
+    """O(N^2) direct sum of AProductCore::Aproduct's off-diagonal product (C)."""
+    R = np.ascontiguousarray(R, dtype=np.int32)
+    b = np.ascontiguousarray(b, dtype=np.complex128)
+    y = np.zeros_like(b)
+    lib().orc_aproduct_direct(len(R) // 3, _ptr(R, _ip), d, K, _ptr(b.view(np.float64)), _ptr(y.view(np.float64)))
+    return y
+
+
+def fft1d(x: np.ndarray, sign: int = -1) -> np.ndarray:
+    y = np.ascontiguousarray(x, dtype=np.complex128).copy()
+    lib().orc_fft1d(len(y), _ptr(y.view(np.float64)), sign)
+    return y
+
+
+class OrcOperator:
+    """FFT-based operator of the C oracle (AProductCore.cpp:12-249, 697-758)."""
+
+    def __init__(self, R, Nx, Ny, Nz, d, K, grid=None):
+        self.R = np.ascontiguousarray(R, dtype=np.int32)
+        self.N = len(self.R) // 3
+        Mx, My, Mz = grid if grid is not None else (2 * Nx - 1, 2 * Ny - 1, 2 * Nz - 1)
+        self.grid = (Mx, My, Mz)
+        self.h = lib().orc_op_create(self.N, _ptr(self.R, _ip), Nx, Ny, Nz, Mx, My, Mz, d, K)
+        if not self.h:
+            raise ValueError("grid smaller than 2N-1")
+
+    def apply(self, b, al=None):
+        b = np.ascontiguousarray(b, dtype=np.complex128)
+        y = np.zeros_like(b)
+        alp = _ptr(np.ascontiguousarray(al, np.complex128).view(np.float64)) if al is not None else None
+        lib().orc_op_apply(self.h, _ptr(b.view(np.float64)), alp, _ptr(y.view(np.float64)))
+        return y
+
+    def bicgstab(self, al, E, P0=None, max_it=10000, tol=1e-8, max_aproducts=0):
+        """DDAModel.cpp:131-260.  Returns (P, status, iterations, error)."""
+        al = np.ascontiguousarray(al, np.complex128)
+        E = np.ascontiguousarray(E, np.complex128)
+        P = np.zeros_like(E) if P0 is None else np.ascontiguousarray(P0, np.complex128).copy()
+        it = ctypes.c_int(0)
+        err = ctypes.c_double(0)
+        st = lib().orc_bicgstab(self.h, _ptr(al.view(np.float64)), _ptr(E.view(np.float64)),
+                                _ptr(P.view(np.float64)), max_it, tol, max_aproducts,
+                                ctypes.byref(it), ctypes.byref(err))
+        return P, st, it.value, err.value
+
+    def close(self):
+        if self.h:
+            lib().orc_op_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+# ---------------------------------------------------------------------------
+# Independent numpy restatements used to pin the C layer
+# ---------------------------------------------------------------------------
+def green_textbook(K, x, y, z, dtype=np.complex128):
+    """-G(r) with G the free-space dyadic Green function written as
+    e^{ikr}/r [k^2 (I - r^r^) + (ikr - 1)/r^2 (I - 3 r^r^)].  The reference's
+    LDR_inter (AProductCore.cpp:588-623) is algebraically -G; this form shares
+    no intermediate with it.  Returns [..., 6] as xx xy xz yy yz zz."""
+    ftype = np.longdouble if dtype == np.clongdouble else np.float64
+    x = np.asarray(x, ftype)
+    y = np.asarray(y, ftype)
+    z = np.asarray(z, ftype)
+    K = ftype(K)
+    r = np.sqrt(x * x + y * y + z * z)
+    rs = np.where(r == 0, ftype(1), r)
+    u = np.stack([x / rs, y / rs, z / rs], axis=-1)
+    ph = np.exp(1j * (K * rs).astype(dtype)) if dtype != np.clongdouble else (
+        np.cos(K * rs) + 1j * np.sin(K * rs)).astype(np.clongdouble)
+    pre = ph / rs
+    a = K * K
+    bco = (1j * K * rs - 1) / (rs * rs)
+    pairs = [(0, 0), (0, 1), (0, 2), (1, 1), (1, 2), (2, 2)]
+    out = []
+    for (i, j) in pairs:
+        dij = 1.0 if i == j else 0.0
+        uu = u[..., i] * u[..., j]
+        g = pre * (a * (dij - uu) + bco * (dij - 3 * uu))
+        out.append(np.where(r == 0, 0, -g))
+    return np.stack(out, axis=-1).astype(dtype)
+
+
+def assemble_dense(R, d, K, al=None):
+    """Explicit 3N x 3N matrix of Aproductwithalb (DDAModel.cpp:577-586)."""
+    R = np.asarray(R, np.int64).reshape(-1, 3)
+    N = len(R)
+    D = (R[:, None, :] - R[None, :, :]) * d
+    G = green_textbook(K, D[..., 0], D[..., 1], D[..., 2])
+    T = np.zeros((N, N, 3, 3), np.complex128)
+    idx = [[0, 1, 2], [1, 3, 4], [2, 4, 5]]
+    for i in range(3):
+        for j in range(3):
+            T[:, :, i, j] = G[..., idx[i][j]]
+    A = T.transpose(0, 2, 1, 3).reshape(3 * N, 3 * N)
+    if al is not None:
+        A = A + np.diag(np.asarray(al, np.complex128))
+    return A
+
+
+def aproduct_direct_ld(R, d, K, b):
+    """Long-double O(N^2) direct sum; tiny lattices only."""
+    R = np.asarray(R, np.int64).reshape(-1, 3)
+    D = (R[:, None, :] - R[None, :, :]).astype(np.longdouble) * np.longdouble(d)
+    G = green_textbook(K, D[..., 0], D[..., 1], D[..., 2], dtype=np.clongdouble)
+    b3 = np.asarray(b, np.clongdouble).reshape(-1, 3)
+    idx = [[0, 1, 2], [1, 3, 4], [2, 4, 5]]
+    y = np.zeros_like(b3)
+    for i in range(3):
+        for j in range(3):
+            y[:, i] += np.einsum("ab,b->a", G[..., idx[i][j]], b3[:, j])
+    return y.reshape(-1)
+
+
+def aproduct_numpy_fft(R, Nx, Ny, Nz, d, K, b, grid=None):
+    """numpy.fft convolution on grid (Mx, My, Mz) >= 2N-1 (pocketfft, FP64)."""
+    Mx, My, Mz = grid if grid is not None else (2 * Nx, 2 * Ny, 2 * Nz)
+    R = np.asarray(R, np.int64).reshape(-1, 3)
+
+    def offs(M, Nn):
+        i = np.arange(M)
+        o = np.where(i <= Nn - 1, i, i - M).astype(np.float64)
+        ok = (i <= Nn - 1) | (i >= M - (Nn - 1))
+        return o, ok
+
+    ox, kx = offs(Mx, Nx)
+    oy, ky = offs(My, Ny)
+    oz, kz = offs(Mz, Nz)
+    X, Y, Z = np.meshgrid(ox * d, oy * d, oz * d, indexing="ij")
+    mask = kx[:, None, None] & ky[None, :, None] & kz[None, None, :]
+    G = green_textbook(K, X, Y, Z) * mask[..., None]
+    Gh = np.fft.fftn(G, axes=(0, 1, 2))
+    B = np.zeros((3, Mx, My, Mz), np.complex128)
+    b3 = np.asarray(b, np.complex128).reshape(-1, 3)
+    for c in range(3):
+        B[c][R[:, 0], R[:, 1], R[:, 2]] = b3[:, c]
+    Bh = np.fft.fftn(B, axes=(1, 2, 3))
+    idx = [[0, 1, 2], [1, 3, 4], [2, 4, 5]]
+    C = np.zeros_like(Bh)
+    for i in range(3):
+        for j in range(3):
+            C[i] += Gh[..., idx[i][j]] * Bh[j]
+    y = np.fft.ifftn(C, axes=(1, 2, 3))
+    out = np.stack([y[c][R[:, 0], R[:, 1], R[:, 2]] for c in range(3)], axis=1)
+    return out.reshape(-1)
+
+
+# ---------------------------------------------------------------------------
+# Lattices and synthetic problems
+# ---------------------------------------------------------------------------
+def build_a_bulk(Nx, Ny, Nz) -> np.ndarray:
+    """tools.cpp:497-510: x-major, then y, then z."""
+    x, y, z = np.meshgrid(np.arange(Nx), np.arange(Ny), np.arange(Nz), indexing="ij")
+    return np.stack([x.ravel(), y.ravel(), z.ravel()], axis=1).astype(np.int32).reshape(-1)
+
+
+def ne_perp_in_xz(theta: float) -> np.ndarray:
+    """p-polarisation for k = (sin t, 0, cos t): tools.cpp:686-709 at phi = 0."""
+    return np.array([math.cos(theta), 0.0, -math.sin(theta)])
+
+
+def rel_l2(a, b) -> float:
+    a = np.asarray(a)
+    b = np.asarray(b)
+    nb = np.linalg.norm(b)
+    return float(np.linalg.norm(a - b) / (nb if nb > 0 else 1.0))

@@ -1,0 +1,918 @@
+// tdda_capi.hip -- C ABI of the DDA solve engine: contexts, operators (Green
+// spectrum on the device), models (device-resident solver state) and the
+// solver drivers.  Declarations and reference mapping: include/topodda.h.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "topodda.h"
+#include "tdda_kernels.hpp"
+
+using namespace tdda;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string& msg)
+{
+    g_err = msg;
+    return code;
+}
+
+#define HIPCHK(x)                                                                                  \
+    do {                                                                                           \
+        hipError_t e_ = (x);                                                                       \
+        if (e_ != hipSuccess) return fail(TDDA_ERR_HIP, std::string(#x) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+#define CHK(x)                 \
+    do {                       \
+        int rc_ = (x);         \
+        if (rc_ != TDDA_OK) return rc_; \
+    } while (0)
+
+bool smooth235(int n)
+{
+    if (n <= 0) return false;
+    for (int p : {2, 3, 5})
+        while (n % p == 0) n /= p;
+    return n == 1;
+}
+
+int next_smooth(int n)
+{
+    if (n < 1) n = 1;
+    while (!smooth235(n)) n++;
+    return n;
+}
+
+FftPlan make_plan(int L)
+{
+    FftPlan p{};
+    p.L = L;
+    int n = L, e2 = 0;
+    while (n % 2 == 0) {
+        n /= 2;
+        e2++;
+    }
+    while (e2 >= 3) {
+        p.rad[p.nst++] = 8;
+        e2 -= 3;
+    }
+    if (e2 == 2) p.rad[p.nst++] = 4;
+    if (e2 == 1) p.rad[p.nst++] = 2;
+    while (n % 3 == 0) {
+        p.rad[p.nst++] = 3;
+        n /= 3;
+    }
+    while (n % 5 == 0) {
+        p.rad[p.nst++] = 5;
+        n /= 5;
+    }
+    return p;
+}
+
+// roots of unity exp(-2 pi i q / L) in long double
+std::vector<Cx<double>> twiddles(int L)
+{
+    std::vector<Cx<double>> t(L);
+    const long double pi = 3.141592653589793238462643383279502884L;
+    for (int q = 0; q < L; q++) {
+        const long double a = -2.0L * pi * (long double)q / (long double)L;
+        t[q] = Cx<double>{(double)cosl(a), (double)sinl(a)};
+    }
+    return t;
+}
+
+int env_int(const char* name, int dflt)
+{
+    const char* v = getenv(name);
+    return v ? atoi(v) : dflt;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// handles
+// ---------------------------------------------------------------------------
+struct tdda_ctx_s {
+    int device;
+    hipStream_t stream;
+    size_t lds_budget;
+    int ncu;
+};
+
+struct TilePlan {
+    int pz, tfy, tfx, ps_z, ps_y, ps_x;
+    size_t lds_z, lds_y, lds_x;
+};
+
+struct tdda_op_s {
+    tdda_ctx ctx;
+    int N, Nx, Ny, Nz, Mx, My, Mz, Hx, Hy, Hz;
+    double d, lam, nback, K;
+    int prec;
+    std::vector<int> R;
+    int* cellmap = nullptr;
+    Cx<double>* green64 = nullptr;
+    Cx<float>* green32 = nullptr;
+    Cx<double>*tw64[3] = {nullptr, nullptr, nullptr};
+    Cx<float>*tw32[3] = {nullptr, nullptr, nullptr};
+    FftPlan pl[3];
+    TilePlan tp64, tp32;
+    tdda_model scratch = nullptr;
+};
+
+template <typename T>
+struct VecSet {
+    Cx<T>*al = nullptr, *e = nullptr, *x = nullptr, *r = nullptr, *r0 = nullptr, *p = nullptr, *ap = nullptr,
+    *t = nullptr, *at = nullptr, *w1 = nullptr, *w2 = nullptr;
+};
+
+template <typename T>
+struct JobTable {
+    std::vector<JobDesc<T>> h;
+    JobDesc<T>* d = nullptr;
+    int* tz = nullptr;  // tile starts, njobs+1 each
+    int* ty = nullptr;
+    int* tx = nullptr;
+    int* tv = nullptr;
+    int nz = 0, ny = 0, nx = 0, nv = 0;
+    SolverState* st = nullptr;
+    double* part = nullptr;
+    unsigned* cnt = nullptr;
+    TilePlan tp;
+};
+
+struct tdda_model_s {
+    tdda_op op;
+    int nrhs;
+    size_t n3;
+    VecSet<double> v64;
+    VecSet<float> v32;
+    JobTable<double> j64;  // FP64 solver and operator applications
+    JobTable<double> jo;   // FP64 outer residual of the mixed solver
+    JobTable<float> j32;   // complex64 inner solver
+    SolverState* h_st = nullptr;  // pinned host mirror, 3 * nrhs
+    std::vector<void*> allocs;
+};
+
+namespace {
+
+template <typename T>
+TilePlan plan_tiles(const tdda_op_s* op, size_t budget)
+{
+    TilePlan t{};
+    const size_t s = sizeof(Cx<T>);
+    t.ps_z = op->Mz + 1;
+    t.ps_y = op->My + 1;
+    t.ps_x = op->Mx + 1;
+    const int maxz = (int)(budget / (2 * 3 * t.ps_z * s));
+    t.pz = std::max(1, std::min({maxz, 64, op->Nx * op->Ny}));
+    const int maxy = (int)(budget / (2 * t.ps_y * s));
+    t.tfy = std::max(1, std::min({maxy, 48, 3 * op->Mz}));
+    int maxx = (int)(budget / (2 * t.ps_x * s));
+    maxx = std::min({maxx, 48, 3 * op->Mz});
+    t.tfx = std::max(3, (maxx / 3) * 3);
+    t.lds_z = 2 * 3 * (size_t)t.pz * t.ps_z * s;
+    t.lds_y = 2 * (size_t)t.tfy * t.ps_y * s;
+    t.lds_x = 2 * (size_t)t.tfx * t.ps_x * s;
+    return t;
+}
+
+template <typename T>
+int dev_alloc(tdda_model_s* m, T** p, size_t count)
+{
+    void* q = nullptr;
+    if (count == 0) count = 1;
+    hipError_t e = hipMalloc(&q, count * sizeof(T));
+    if (e != hipSuccess) return fail(TDDA_ERR_OOM, std::string("hipMalloc failed: ") + hipGetErrorString(e));
+    m->allocs.push_back(q);
+    *p = reinterpret_cast<T*>(q);
+    return TDDA_OK;
+}
+
+template <typename T>
+int alloc_vecs(tdda_model_s* m, VecSet<T>& v)
+{
+    const size_t n = m->n3 * m->nrhs;
+    const tdda_op_s* op = m->op;
+    CHK(dev_alloc(m, &v.al, n));
+    CHK(dev_alloc(m, &v.e, n));
+    CHK(dev_alloc(m, &v.x, n));
+    CHK(dev_alloc(m, &v.r, n));
+    CHK(dev_alloc(m, &v.r0, n));
+    CHK(dev_alloc(m, &v.p, n));
+    CHK(dev_alloc(m, &v.ap, n));
+    CHK(dev_alloc(m, &v.t, n));
+    CHK(dev_alloc(m, &v.at, n));
+    CHK(dev_alloc(m, &v.w1, (size_t)op->Nx * op->Ny * op->Mz * 3 * m->nrhs));
+    CHK(dev_alloc(m, &v.w2, (size_t)op->Nx * op->My * op->Mz * 3 * m->nrhs));
+    HIPCHK(hipMemsetAsync(v.x, 0, n * sizeof(Cx<T>), op->ctx->stream));
+    HIPCHK(hipMemsetAsync(v.al, 0, n * sizeof(Cx<T>), op->ctx->stream));
+    HIPCHK(hipMemsetAsync(v.e, 0, n * sizeof(Cx<T>), op->ctx->stream));
+    return TDDA_OK;
+}
+
+// Builds the job table of one model for precision T.  r0_override lets the
+// outer residual table write its r0 copy into a scratch vector.
+template <typename T>
+int build_table(tdda_model_s* m, JobTable<T>& jt, const VecSet<T>& v, const Cx<T>* green, Cx<T>* const* tw,
+                const TilePlan& tp, Cx<T>* r_vec, Cx<T>* r0_vec)
+{
+    const tdda_op_s* op = m->op;
+    const int nj = m->nrhs;
+    const size_t n3 = m->n3;
+    jt.tp = tp;
+    const int tiles_z = (op->Nx * op->Ny + tp.pz - 1) / tp.pz;
+    const int nfy = (3 * op->Mz + tp.tfy - 1) / tp.tfy;
+    const int nfx = (3 * op->Mz + tp.tfx - 1) / tp.tfx;
+    const int tiles_y = op->Nx * nfy;
+    const int tiles_x = op->My * nfx;
+    const int tiles_v = (int)((n3 + kThreads * kVecPer - 1) / (kThreads * kVecPer));
+    const int maxparts = std::max(tiles_z, tiles_v);
+    CHK(dev_alloc(m, &jt.st, nj));
+    CHK(dev_alloc(m, &jt.part, (size_t)maxparts * 4 * nj));
+    CHK(dev_alloc(m, &jt.cnt, nj));
+    HIPCHK(hipMemsetAsync(jt.cnt, 0, sizeof(unsigned) * nj, op->ctx->stream));
+    HIPCHK(hipMemsetAsync(jt.st, 0, sizeof(SolverState) * nj, op->ctx->stream));
+    jt.h.resize(nj);
+    const size_t w1n = (size_t)op->Nx * op->Ny * op->Mz * 3;
+    const size_t w2n = (size_t)op->Nx * op->My * op->Mz * 3;
+    for (int j = 0; j < nj; j++) {
+        JobDesc<T>& J = jt.h[j];
+        memset(&J, 0, sizeof(J));
+        J.Nx = op->Nx; J.Ny = op->Ny; J.Nz = op->Nz;
+        J.Mx = op->Mx; J.My = op->My; J.Mz = op->Mz;
+        J.Hx = op->Hx; J.Hy = op->Hy; J.Hz = op->Hz;
+        J.N = op->N;
+        J.pz = tp.pz; J.tfy = tp.tfy; J.tfx = tp.tfx;
+        J.nfy = nfy; J.nfx = nfx;
+        J.ps_z = tp.ps_z; J.ps_y = tp.ps_y; J.ps_x = tp.ps_x;
+        J.cellmap = op->cellmap;
+        J.green = green;
+        J.twx = tw[0]; J.twy = tw[1]; J.twz = tw[2];
+        J.plx = op->pl[0]; J.ply = op->pl[1]; J.plz = op->pl[2];
+        J.w1 = v.w1 + j * w1n;
+        J.w2 = v.w2 + j * w2n;
+        J.in = v.x + j * n3;
+        J.out = v.ap + j * n3;
+        J.al = v.al + j * n3;
+        J.x = v.x + j * n3;
+        J.r = r_vec + j * n3;
+        J.r0 = r0_vec + j * n3;
+        J.p = v.p + j * n3;
+        J.ap = v.ap + j * n3;
+        J.t = v.t + j * n3;
+        J.at = v.at + j * n3;
+        J.e = v.e + j * n3;
+        J.st = jt.st + j;
+        J.part = jt.part + (size_t)j * maxparts * 4;
+        J.cnt = jt.cnt + j;
+    }
+    CHK(dev_alloc(m, &jt.d, nj));
+    HIPCHK(hipMemcpy(jt.d, jt.h.data(), sizeof(JobDesc<T>) * nj, hipMemcpyHostToDevice));
+    std::vector<int> hz(nj + 1), hy(nj + 1), hx(nj + 1), hv(nj + 1);
+    for (int j = 0; j <= nj; j++) {
+        hz[j] = j * tiles_z;
+        hy[j] = j * tiles_y;
+        hx[j] = j * tiles_x;
+        hv[j] = j * tiles_v;
+    }
+    jt.nz = hz[nj]; jt.ny = hy[nj]; jt.nx = hx[nj]; jt.nv = hv[nj];
+    CHK(dev_alloc(m, &jt.tz, nj + 1));
+    CHK(dev_alloc(m, &jt.ty, nj + 1));
+    CHK(dev_alloc(m, &jt.tx, nj + 1));
+    CHK(dev_alloc(m, &jt.tv, nj + 1));
+    HIPCHK(hipMemcpy(jt.tz, hz.data(), sizeof(int) * (nj + 1), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(jt.ty, hy.data(), sizeof(int) * (nj + 1), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(jt.tx, hx.data(), sizeof(int) * (nj + 1), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(jt.tv, hv.data(), sizeof(int) * (nj + 1), hipMemcpyHostToDevice));
+    return TDDA_OK;
+}
+
+template <typename T>
+void launch_p1(const JobTable<T>& jt, int nj, int mode, hipStream_t s)
+{
+    const dim3 g(jt.nz), b(kThreads);
+    switch (mode) {
+    case IN_PLAIN: k_p1<T, IN_PLAIN><<<g, b, jt.tp.lds_z, s>>>(jt.d, jt.tz, nj); break;
+    case IN_P: k_p1<T, IN_P><<<g, b, jt.tp.lds_z, s>>>(jt.d, jt.tz, nj); break;
+    case IN_T: k_p1<T, IN_T><<<g, b, jt.tp.lds_z, s>>>(jt.d, jt.tz, nj); break;
+    default: k_p1<T, IN_X><<<g, b, jt.tp.lds_z, s>>>(jt.d, jt.tz, nj); break;
+    }
+}
+
+template <typename T>
+void launch_p5(const JobTable<T>& jt, int nj, int mode, int diag, hipStream_t s)
+{
+    const dim3 g(jt.nz), b(kThreads);
+    switch (mode) {
+    case OUT_PLAIN: k_p5<T, OUT_PLAIN><<<g, b, jt.tp.lds_z, s>>>(jt.d, jt.tz, nj, diag); break;
+    case OUT_RES: k_p5<T, OUT_RES><<<g, b, jt.tp.lds_z, s>>>(jt.d, jt.tz, nj, 1); break;
+    case OUT_AP: k_p5<T, OUT_AP><<<g, b, jt.tp.lds_z, s>>>(jt.d, jt.tz, nj, 1); break;
+    default: k_p5<T, OUT_AT><<<g, b, jt.tp.lds_z, s>>>(jt.d, jt.tz, nj, 1); break;
+    }
+}
+
+// one operator application: P1 .. P5
+template <typename T>
+void launch_apply(const JobTable<T>& jt, int nj, int inmode, int outmode, int diag, int solver, hipStream_t s)
+{
+    launch_p1<T>(jt, nj, inmode, s);
+    k_py<T, false><<<dim3(jt.ny), dim3(kThreads), jt.tp.lds_y, s>>>(jt.d, jt.ty, nj, solver);
+    k_px<T><<<dim3(jt.nx), dim3(kThreads), jt.tp.lds_x, s>>>(jt.d, jt.tx, nj, solver);
+    k_py<T, true><<<dim3(jt.ny), dim3(kThreads), jt.tp.lds_y, s>>>(jt.d, jt.ty, nj, solver);
+    launch_p5<T>(jt, nj, outmode, diag, s);
+}
+
+// one BiCGSTAB iteration (or the unrolled first step when state.first)
+template <typename T>
+void launch_iteration(const JobTable<T>& jt, int nj, hipStream_t s)
+{
+    launch_apply<T>(jt, nj, IN_P, OUT_AP, 1, 1, s);
+    launch_apply<T>(jt, nj, IN_T, OUT_AT, 1, 1, s);
+    k_update<T><<<dim3(jt.nv), dim3(kThreads), 0, s>>>(jt.d, jt.tv, nj);
+}
+
+int check_launch()
+{
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return fail(TDDA_ERR_HIP, std::string("kernel launch: ") + hipGetErrorString(e));
+    return TDDA_OK;
+}
+
+// Runs BiCGSTAB iterations for every running job until each has stopped.
+// States must already be initialised (first = 1) and r, r0, x, rho set.
+template <typename T>
+int run_iterations(tdda_model_s* m, JobTable<T>& jt, SolverState* hst, int max_total, int check_every)
+{
+    hipStream_t s = m->op->ctx->stream;
+    const int nj = m->nrhs;
+    int done = 0;
+    int chunk = check_every > 0 ? check_every : 4;
+    while (done <= max_total + 1) {
+        for (int k = 0; k < chunk; k++) launch_iteration<T>(jt, nj, s);
+        CHK(check_launch());
+        done += chunk;
+        HIPCHK(hipMemcpyAsync(hst, jt.st, sizeof(SolverState) * nj, hipMemcpyDeviceToHost, s));
+        HIPCHK(hipStreamSynchronize(s));
+        bool running = false;
+        for (int j = 0; j < nj; j++) running |= (hst[j].status == ST_RUNNING);
+        if (!running) break;
+        if (check_every <= 0) chunk = std::min(32, std::max(4, done / 3));
+    }
+    return TDDA_OK;
+}
+
+template <typename T>
+int init_states(tdda_model_s* m, JobTable<T>& jt, SolverState* hst, const std::vector<int>& active, double tol,
+                int maxit)
+{
+    const int nj = m->nrhs;
+    for (int j = 0; j < nj; j++) {
+        SolverState& S = hst[j];
+        memset(&S, 0, sizeof(S));
+        S.tol = tol;
+        S.maxit = maxit;
+        S.first = 1;
+        S.status = active[j] ? ST_RUNNING : ST_CONVERGED;
+    }
+    HIPCHK(hipMemcpyAsync(jt.st, hst, sizeof(SolverState) * nj, hipMemcpyHostToDevice, m->op->ctx->stream));
+    return TDDA_OK;
+}
+
+template <typename T>
+void launch_init_zero(const JobTable<T>& jt, int nj, hipStream_t s)
+{
+    k_init_zero<T><<<dim3(jt.nv), dim3(kThreads), 0, s>>>(jt.d, jt.tv, nj);
+}
+
+int solve_fp64(tdda_model_s* m, const tdda_solve_opts* o, tdda_solve_info* info)
+{
+    hipStream_t s = m->op->ctx->stream;
+    const int nj = m->nrhs;
+    SolverState* hst = m->h_st;
+    std::vector<int> active(nj, 1);
+    CHK(init_states<double>(m, m->j64, hst, active, o->tol, o->max_it));
+    if (o->x0_zero) {
+        launch_init_zero<double>(m->j64, nj, s);
+    } else {
+        launch_apply<double>(m->j64, nj, IN_X, OUT_RES, 1, 1, s);
+    }
+    CHK(check_launch());
+    CHK(run_iterations<double>(m, m->j64, hst, o->max_it, o->check_every));
+    for (int j = 0; j < nj; j++) {
+        info[j].status = hst[j].status;
+        info[j].iterations = hst[j].iter;
+        info[j].error = hst[j].err;
+        info[j].outer = 0;
+        info[j].true_residual = -1.0;
+    }
+    return TDDA_OK;
+}
+
+// FP64 true residual r = E - A x into jo's r vector; returns ||r||/||E||.
+int true_residual(tdda_model_s* m, std::vector<double>& rel, const std::vector<int>& active)
+{
+    hipStream_t s = m->op->ctx->stream;
+    const int nj = m->nrhs;
+    SolverState* hst = m->h_st + nj;
+    std::vector<int> act = active;
+    CHK(init_states<double>(m, m->jo, hst, act, 0.0, 0));
+    launch_apply<double>(m->jo, nj, IN_X, OUT_RES, 1, 1, s);
+    CHK(check_launch());
+    HIPCHK(hipMemcpyAsync(hst, m->jo.st, sizeof(SolverState) * nj, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    rel.resize(nj);
+    for (int j = 0; j < nj; j++) rel[j] = active[j] ? hst[j].rnorm / hst[j].enorm : rel[j];
+    return TDDA_OK;
+}
+
+int solve_mixed(tdda_model_s* m, const tdda_solve_opts* o, tdda_solve_info* info)
+{
+    hipStream_t s = m->op->ctx->stream;
+    const int nj = m->nrhs;
+    const size_t n3 = m->n3;
+    if (!m->op->green32) return fail(TDDA_ERR_STATE, "mixed solve needs an operator built with TDDA_C64");
+    if (o->x0_zero) HIPCHK(hipMemsetAsync(m->v64.x, 0, sizeof(Cx<double>) * n3 * nj, s));
+    std::vector<int> active(nj, 1);
+    std::vector<double> rel(nj, 0.0);
+    std::vector<int> inner_it(nj, 0), outer(nj, 0);
+    CHK(true_residual(m, rel, active));
+    SolverState* hin = m->h_st + 2 * nj;
+    const int blocks = (int)((n3 + kThreads - 1) / kThreads);
+    for (int c = 0; c < o->max_outer; c++) {
+        bool any = false;
+        for (int j = 0; j < nj; j++) {
+            active[j] = active[j] && !(rel[j] <= o->tol) && std::isfinite(rel[j]);
+            any |= active[j] != 0;
+        }
+        if (!any) break;
+        // inner right-hand side = r (complex64)
+        for (int j = 0; j < nj; j++)
+            if (active[j])
+                k_demote<<<blocks, kThreads, 0, s>>>(m->jo.h[j].r, m->v32.e + j * n3, n3);
+        CHK(init_states<float>(m, m->j32, hin, active, o->inner_tol, o->inner_max_it));
+        launch_init_zero<float>(m->j32, nj, s);
+        CHK(check_launch());
+        CHK(run_iterations<float>(m, m->j32, hin, o->inner_max_it, o->check_every));
+        for (int j = 0; j < nj; j++) {
+            if (!active[j]) continue;
+            inner_it[j] += hin[j].iter;
+            outer[j] += 1;
+            k_accum<<<blocks, kThreads, 0, s>>>(m->v64.x + j * n3, m->v32.x + j * n3, n3);
+        }
+        CHK(check_launch());
+        CHK(true_residual(m, rel, active));
+    }
+    for (int j = 0; j < nj; j++) {
+        info[j].status = (rel[j] <= o->tol) ? TDDA_SOLVE_CONVERGED : TDDA_SOLVE_MAXIT;
+        info[j].iterations = inner_it[j];
+        info[j].error = rel[j];
+        info[j].true_residual = rel[j];
+        info[j].outer = outer[j];
+    }
+    return TDDA_OK;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// exported functions
+// ---------------------------------------------------------------------------
+extern "C" {
+
+const char* tdda_last_error(void) { return g_err.c_str(); }
+
+int tdda_version(void) { return 100; }
+
+int tdda_ctx_create(int device, tdda_ctx* out)
+{
+    if (!out) return fail(TDDA_ERR_ARG, "null output handle");
+    int n = 0;
+    HIPCHK(hipGetDeviceCount(&n));
+    if (device < 0 || device >= n) return fail(TDDA_ERR_ARG, "device index out of range");
+    HIPCHK(hipSetDevice(device));
+    hipDeviceProp_t prop;
+    HIPCHK(hipGetDeviceProperties(&prop, device));
+    tdda_ctx c = new tdda_ctx_s();
+    c->device = device;
+    c->ncu = prop.multiProcessorCount;
+    const size_t lim = std::min<size_t>(prop.sharedMemPerBlock, 65536);
+    c->lds_budget = std::min<size_t>(lim, (size_t)env_int("TDDA_LDS_BUDGET", 49152));
+    hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+    if (e != hipSuccess) {
+        delete c;
+        return fail(TDDA_ERR_HIP, std::string("hipStreamCreate: ") + hipGetErrorString(e));
+    }
+    *out = c;
+    return TDDA_OK;
+}
+
+int tdda_ctx_destroy(tdda_ctx ctx)
+{
+    if (!ctx) return TDDA_OK;
+    hipSetDevice(ctx->device);
+    hipStreamSynchronize(ctx->stream);
+    hipStreamDestroy(ctx->stream);
+    delete ctx;
+    return TDDA_OK;
+}
+
+int tdda_ctx_sync(tdda_ctx ctx)
+{
+    if (!ctx) return fail(TDDA_ERR_ARG, "null context");
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    return TDDA_OK;
+}
+
+int tdda_green_tensor(double K, double x, double y, double z, double* out12)
+{
+    if (!out12) return fail(TDDA_ERR_ARG, "null output");
+    const double xs = x * x, ys = y * y, zs = z * z;
+    const double rn = std::sqrt(xs + ys + zs);
+    if (rn == 0.0) {
+        for (int l = 0; l < 12; l++) out12[l] = 0.0;
+        return TDDA_OK;
+    }
+    const double r2 = rn * rn, r3 = rn * rn * rn;
+    const double c1r = std::cos(K * rn) / r3, c1i = std::sin(K * rn) / r3;
+    const double c2r = 1.0 / r2, c2i = -K * rn / r2;
+    const double c3r = K * K - 3.0 * c2r, c3i = -3.0 * c2i;
+    const double kk = K * K;
+    double a[6][2] = {{c2r * (r2 - 3.0 * xs) - kk * (ys + zs), c2i * (r2 - 3.0 * xs)},
+                      {c3r * x * y, c3i * x * y},
+                      {c3r * z * x, c3i * z * x},
+                      {c2r * (r2 - 3.0 * ys) - kk * (zs + xs), c2i * (r2 - 3.0 * ys)},
+                      {c3r * y * z, c3i * y * z},
+                      {c2r * (r2 - 3.0 * zs) - kk * (xs + ys), c2i * (r2 - 3.0 * zs)}};
+    for (int l = 0; l < 6; l++) {
+        out12[2 * l] = c1r * a[l][0] - c1i * a[l][1];
+        out12[2 * l + 1] = c1r * a[l][1] + c1i * a[l][0];
+    }
+    return TDDA_OK;
+}
+
+int tdda_op_create(tdda_ctx ctx, const tdda_op_desc* desc, tdda_op* out)
+{
+    if (!ctx || !desc || !out) return fail(TDDA_ERR_ARG, "null argument");
+    const tdda_op_desc& D = *desc;
+    if (D.N <= 0 || D.Nx <= 0 || D.Ny <= 0 || D.Nz <= 0 || !D.R)
+        return fail(TDDA_ERR_ARG, "lattice sizes must be positive and R non-null");
+    if (!(D.d > 0) || !(D.lam > 0) || !(D.nback > 0)) return fail(TDDA_ERR_ARG, "d, lam and nback must be positive");
+    const long long box = (long long)D.Nx * D.Ny * D.Nz;
+    if (box > (1LL << 30) || D.N > box) return fail(TDDA_ERR_ARG, "lattice too large or N exceeds the box");
+    int prec = D.precisions ? D.precisions : (TDDA_C128 | TDDA_C64);
+    if (prec & ~(TDDA_C128 | TDDA_C64)) return fail(TDDA_ERR_ARG, "unknown precision bits");
+    int M[3] = {D.Mx, D.My, D.Mz};
+    const int Nn[3] = {D.Nx, D.Ny, D.Nz};
+    for (int a = 0; a < 3; a++) {
+        if (M[a] == 0) M[a] = next_smooth(2 * Nn[a] - 1);
+        if (M[a] < 2 * Nn[a] - 1) return fail(TDDA_ERR_ARG, "FFT grid smaller than 2N-1 along an axis");
+        if (!smooth235(M[a])) return fail(TDDA_ERR_UNSUPPORTED, "FFT grid sizes must have prime factors 2, 3, 5 only");
+        if (M[a] > 4096) return fail(TDDA_ERR_UNSUPPORTED, "FFT length above 4096");
+    }
+    HIPCHK(hipSetDevice(ctx->device));
+    tdda_op op = new tdda_op_s();
+    op->ctx = ctx;
+    op->N = D.N; op->Nx = D.Nx; op->Ny = D.Ny; op->Nz = D.Nz;
+    op->Mx = M[0]; op->My = M[1]; op->Mz = M[2];
+    op->Hx = M[0] / 2 + 1; op->Hy = M[1] / 2 + 1; op->Hz = M[2] / 2 + 1;
+    op->d = D.d; op->lam = D.lam; op->nback = D.nback;
+    op->K = 2.0 * M_PI / (D.lam / D.nback);  // AProductCore.cpp:17
+    op->prec = prec;
+    op->R.assign(D.R, D.R + 3 * (size_t)D.N);
+    // bulk order (tools.cpp:497-510) on the full box needs no cell map
+    bool bulk = (box == D.N);
+    std::vector<int> cmap((size_t)box, -1);
+    for (int i = 0; i < D.N; i++) {
+        const int x = D.R[3 * i], y = D.R[3 * i + 1], z = D.R[3 * i + 2];
+        if (x < 0 || x >= D.Nx || y < 0 || y >= D.Ny || z < 0 || z >= D.Nz) {
+            delete op;
+            return fail(TDDA_ERR_ARG, "site coordinate outside the Nx x Ny x Nz box");
+        }
+        const size_t cell = ((size_t)x * D.Ny + y) * D.Nz + z;
+        if (cmap[cell] != -1) {
+            delete op;
+            return fail(TDDA_ERR_ARG, "two sites share one lattice cell");
+        }
+        cmap[cell] = i;
+        if ((size_t)i != cell) bulk = false;
+    }
+    auto cleanup = [&](int rc) {
+        tdda_op_destroy(op);
+        return rc;
+    };
+    if (!bulk) {
+        if (hipMalloc(&op->cellmap, sizeof(int) * box) != hipSuccess) return cleanup(fail(TDDA_ERR_OOM, "cell map"));
+        if (hipMemcpy(op->cellmap, cmap.data(), sizeof(int) * box, hipMemcpyHostToDevice) != hipSuccess)
+            return cleanup(fail(TDDA_ERR_HIP, "cell map upload"));
+    }
+    for (int a = 0; a < 3; a++) {
+        op->pl[a] = make_plan(M[a]);
+        std::vector<Cx<double>> t = twiddles(M[a]);
+        std::vector<Cx<float>> tf(M[a]);
+        for (int q = 0; q < M[a]; q++) tf[q] = Cx<float>{(float)t[q].x, (float)t[q].y};
+        if (hipMalloc(&op->tw64[a], sizeof(Cx<double>) * M[a]) != hipSuccess ||
+            hipMalloc(&op->tw32[a], sizeof(Cx<float>) * M[a]) != hipSuccess)
+            return cleanup(fail(TDDA_ERR_OOM, "twiddles"));
+        hipMemcpy(op->tw64[a], t.data(), sizeof(Cx<double>) * M[a], hipMemcpyHostToDevice);
+        hipMemcpy(op->tw32[a], tf.data(), sizeof(Cx<float>) * M[a], hipMemcpyHostToDevice);
+    }
+    // Green tensor on the full grid, FP64, then its 3D forward transform
+    // (AProductCore.cpp:54-229).
+    const size_t Mt = (size_t)M[0] * M[1] * M[2];
+    Cx<double>* full = nullptr;
+    if (hipMalloc(&full, sizeof(Cx<double>) * Mt * 6) != hipSuccess)
+        return cleanup(fail(TDDA_ERR_OOM, "Green tensor grid"));
+    hipStream_t s = ctx->stream;
+    k_green_fill<<<(unsigned)((Mt + kThreads - 1) / kThreads), kThreads, 0, s>>>(full, D.Nx, D.Ny, D.Nz, M[0], M[1],
+                                                                                M[2], D.d, op->K);
+    {
+        // z: contiguous pencils [x][y][z][6]
+        const size_t budget = ctx->lds_budget;
+        const int ps = M[2] + 1;
+        int P = (int)(budget / (2 * 6 * ps * sizeof(Cx<double>)));
+        P = std::max(1, std::min(P, 32));
+        if (2 * 6 * (size_t)ps * sizeof(Cx<double>) > 65536) {
+            hipFree(full);
+            return cleanup(fail(TDDA_ERR_UNSUPPORTED, "z FFT length too large for the Green set-up pass"));
+        }
+        const int Q = M[0] * M[1];
+        k_fft_contig<double, false><<<(Q + P - 1) / P, kThreads, 2 * 6 * P * ps * sizeof(Cx<double>), s>>>(
+            full, Q, M[2], 6, P, ps, op->pl[2], op->tw64[2]);
+        // y: outer x, element (x, y, f) at x*My*Mz*6 + y*Mz*6 + f
+        const int F = M[2] * 6;
+        for (int ax = 1; ax >= 0; ax--) {
+            const int L = M[ax];
+            const int psa = L + 1;
+            int tf = (int)(budget / (2 * psa * sizeof(Cx<double>)));
+            tf = std::max(1, std::min(tf, 48));
+            const int ntf = (F + tf - 1) / tf;
+            const int outer = (ax == 1) ? M[0] : M[1];
+            const size_t so = (ax == 1) ? (size_t)M[1] * F : (size_t)F;
+            const size_t sa = (ax == 1) ? (size_t)F : (size_t)M[1] * F;
+            k_fft_strided<double, false><<<outer * ntf, kThreads, 2 * tf * psa * sizeof(Cx<double>), s>>>(
+                full, L, so, sa, F, tf, ntf, psa, op->pl[ax], op->tw64[ax]);
+        }
+    }
+    const size_t Ht = (size_t)op->Hx * op->Hy * op->Hz * 6;
+    const double scale = 1.0 / (double)Mt;
+    const unsigned gb = (unsigned)((Ht + kThreads - 1) / kThreads);
+    if (prec & TDDA_C128) {
+        if (hipMalloc(&op->green64, sizeof(Cx<double>) * Ht) != hipSuccess) {
+            hipFree(full);
+            return cleanup(fail(TDDA_ERR_OOM, "Green spectrum"));
+        }
+        k_green_octant<double><<<gb, kThreads, 0, s>>>(full, op->green64, M[0], M[1], M[2], op->Hx, op->Hy, op->Hz,
+                                                       scale);
+    }
+    if (prec & TDDA_C64) {
+        if (hipMalloc(&op->green32, sizeof(Cx<float>) * Ht) != hipSuccess) {
+            hipFree(full);
+            return cleanup(fail(TDDA_ERR_OOM, "Green spectrum"));
+        }
+        k_green_octant<float><<<gb, kThreads, 0, s>>>(full, op->green32, M[0], M[1], M[2], op->Hx, op->Hy, op->Hz,
+                                                      scale);
+    }
+    hipError_t e = hipGetLastError();
+    hipError_t e2 = hipStreamSynchronize(s);
+    hipFree(full);
+    if (e != hipSuccess || e2 != hipSuccess)
+        return cleanup(fail(TDDA_ERR_HIP, std::string("Green set-up: ") + hipGetErrorString(e != hipSuccess ? e : e2)));
+    op->tp64 = plan_tiles<double>(op, ctx->lds_budget);
+    op->tp32 = plan_tiles<float>(op, ctx->lds_budget);
+    *out = op;
+    return TDDA_OK;
+}
+
+int tdda_op_destroy(tdda_op op)
+{
+    if (!op) return TDDA_OK;
+    hipSetDevice(op->ctx->device);
+    hipStreamSynchronize(op->ctx->stream);
+    if (op->scratch) tdda_model_destroy(op->scratch);
+    hipFree(op->cellmap);
+    hipFree(op->green64);
+    hipFree(op->green32);
+    for (int a = 0; a < 3; a++) {
+        hipFree(op->tw64[a]);
+        hipFree(op->tw32[a]);
+    }
+    delete op;
+    return TDDA_OK;
+}
+
+int tdda_op_info(tdda_op op, int* grid3, double* K)
+{
+    if (!op) return fail(TDDA_ERR_ARG, "null operator");
+    if (grid3) {
+        grid3[0] = op->Mx;
+        grid3[1] = op->My;
+        grid3[2] = op->Mz;
+    }
+    if (K) *K = op->K;
+    return TDDA_OK;
+}
+
+int tdda_model_create(tdda_op op, int nrhs, tdda_model* out)
+{
+    if (!op || !out) return fail(TDDA_ERR_ARG, "null argument");
+    if (nrhs <= 0 || nrhs > 4096) return fail(TDDA_ERR_ARG, "nrhs out of range");
+    HIPCHK(hipSetDevice(op->ctx->device));
+    tdda_model m = new tdda_model_s();
+    m->op = op;
+    m->nrhs = nrhs;
+    m->n3 = 3 * (size_t)op->N;
+    auto bail = [&](int rc) {
+        tdda_model_destroy(m);
+        return rc;
+    };
+    int rc;
+    if (op->green64) {
+        if ((rc = alloc_vecs<double>(m, m->v64)) != TDDA_OK) return bail(rc);
+        Cx<double>* tw[3] = {op->tw64[0], op->tw64[1], op->tw64[2]};
+        if ((rc = build_table<double>(m, m->j64, m->v64, op->green64, tw, op->tp64, m->v64.r, m->v64.r0)) != TDDA_OK)
+            return bail(rc);
+        // outer residual of the mixed solver: r -> p, its r0 copy -> t (scratch)
+        if ((rc = build_table<double>(m, m->jo, m->v64, op->green64, tw, op->tp64, m->v64.p, m->v64.t)) != TDDA_OK)
+            return bail(rc);
+    }
+    if (op->green32) {
+        if ((rc = alloc_vecs<float>(m, m->v32)) != TDDA_OK) return bail(rc);
+        Cx<float>* tw[3] = {op->tw32[0], op->tw32[1], op->tw32[2]};
+        if ((rc = build_table<float>(m, m->j32, m->v32, op->green32, tw, op->tp32, m->v32.r, m->v32.r0)) != TDDA_OK)
+            return bail(rc);
+    }
+    if (hipHostMalloc((void**)&m->h_st, sizeof(SolverState) * 3 * nrhs, hipHostMallocDefault) != hipSuccess)
+        return bail(fail(TDDA_ERR_OOM, "pinned host state"));
+    if (hipStreamSynchronize(op->ctx->stream) != hipSuccess) return bail(fail(TDDA_ERR_HIP, "model set-up"));
+    *out = m;
+    return TDDA_OK;
+}
+
+int tdda_model_destroy(tdda_model m)
+{
+    if (!m) return TDDA_OK;
+    hipSetDevice(m->op->ctx->device);
+    hipStreamSynchronize(m->op->ctx->stream);
+    for (void* p : m->allocs) hipFree(p);
+    if (m->h_st) hipHostFree(m->h_st);
+    delete m;
+    return TDDA_OK;
+}
+
+static Cx<double>* vec64(tdda_model m, int which)
+{
+    switch (which) {
+    case TDDA_VEC_AL: return m->v64.al;
+    case TDDA_VEC_E: return m->v64.e;
+    case TDDA_VEC_P: return m->v64.x;
+    case TDDA_VEC_Y: return m->v64.ap;
+    default: return nullptr;
+    }
+}
+
+int tdda_model_set(tdda_model m, int which, int rhs, const double* host)
+{
+    if (!m || !host) return fail(TDDA_ERR_ARG, "null argument");
+    if (rhs < -1 || rhs >= m->nrhs) return fail(TDDA_ERR_ARG, "rhs index out of range");
+    if (which != TDDA_VEC_AL && which != TDDA_VEC_E && which != TDDA_VEC_P)
+        return fail(TDDA_ERR_ARG, "vector kind not settable");
+    if (!m->v64.x) return fail(TDDA_ERR_STATE, "model has no complex128 storage");
+    HIPCHK(hipSetDevice(m->op->ctx->device));
+    hipStream_t s = m->op->ctx->stream;
+    const size_t n3 = m->n3;
+    const size_t off = rhs < 0 ? 0 : (size_t)rhs * n3;
+    const size_t cnt = rhs < 0 ? n3 * m->nrhs : n3;
+    Cx<double>* dst = vec64(m, which) + off;
+    HIPCHK(hipMemcpyAsync(dst, host, cnt * sizeof(Cx<double>), hipMemcpyHostToDevice, s));
+    if (which == TDDA_VEC_AL && m->v32.al) {
+        const unsigned blocks = (unsigned)((cnt + kThreads - 1) / kThreads);
+        k_demote<<<blocks, kThreads, 0, s>>>(dst, m->v32.al + off, cnt);
+        CHK(check_launch());
+    }
+    HIPCHK(hipStreamSynchronize(s));
+    return TDDA_OK;
+}
+
+int tdda_model_get(tdda_model m, int which, int rhs, double* host)
+{
+    if (!m || !host) return fail(TDDA_ERR_ARG, "null argument");
+    if (rhs < -1 || rhs >= m->nrhs) return fail(TDDA_ERR_ARG, "rhs index out of range");
+    Cx<double>* src = vec64(m, which);
+    if (!src) return fail(TDDA_ERR_ARG, "unknown vector kind");
+    HIPCHK(hipSetDevice(m->op->ctx->device));
+    hipStream_t s = m->op->ctx->stream;
+    const size_t n3 = m->n3;
+    const size_t off = rhs < 0 ? 0 : (size_t)rhs * n3;
+    const size_t cnt = rhs < 0 ? n3 * m->nrhs : n3;
+    HIPCHK(hipMemcpyAsync(host, src + off, cnt * sizeof(Cx<double>), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    return TDDA_OK;
+}
+
+static int model_apply(tdda_model m, int precision, int diag)
+{
+    hipStream_t s = m->op->ctx->stream;
+    const int nj = m->nrhs;
+    const size_t n = m->n3 * nj;
+    if (precision == TDDA_C128) {
+        if (!m->v64.x) return fail(TDDA_ERR_STATE, "operator built without TDDA_C128");
+        launch_apply<double>(m->j64, nj, IN_PLAIN, OUT_PLAIN, diag, 0, s);
+    } else if (precision == TDDA_C64) {
+        if (!m->v32.x) return fail(TDDA_ERR_STATE, "operator built without TDDA_C64");
+        const unsigned blocks = (unsigned)((n + kThreads - 1) / kThreads);
+        k_demote<<<blocks, kThreads, 0, s>>>(m->v64.x, m->v32.x, n);
+        launch_apply<float>(m->j32, nj, IN_PLAIN, OUT_PLAIN, diag, 0, s);
+        // promote: y64 = c128(y32)
+        HIPCHK(hipMemsetAsync(m->v64.ap, 0, n * sizeof(Cx<double>), s));
+        k_accum<<<blocks, kThreads, 0, s>>>(m->v64.ap, m->v32.ap, n);
+    } else {
+        return fail(TDDA_ERR_ARG, "precision must be TDDA_C128 or TDDA_C64");
+    }
+    CHK(check_launch());
+    HIPCHK(hipStreamSynchronize(s));
+    return TDDA_OK;
+}
+
+int tdda_model_apply(tdda_model m, int precision)
+{
+    if (!m) return fail(TDDA_ERR_ARG, "null model");
+    HIPCHK(hipSetDevice(m->op->ctx->device));
+    return model_apply(m, precision, 1);
+}
+
+int tdda_aproduct(tdda_op op, const double* b, double* y, int nrhs, const double* al, int al_per_rhs, int precision)
+{
+    if (!op || !b || !y) return fail(TDDA_ERR_ARG, "null argument");
+    if (nrhs <= 0) return fail(TDDA_ERR_ARG, "nrhs must be positive");
+    if (!op->green64) return fail(TDDA_ERR_STATE, "host-buffer Aproduct needs an operator built with TDDA_C128");
+    HIPCHK(hipSetDevice(op->ctx->device));
+    if (op->scratch && op->scratch->nrhs != nrhs) {
+        tdda_model_destroy(op->scratch);
+        op->scratch = nullptr;
+    }
+    if (!op->scratch) CHK(tdda_model_create(op, nrhs, &op->scratch));
+    tdda_model m = op->scratch;
+    const size_t n3 = m->n3;
+    CHK(tdda_model_set(m, TDDA_VEC_P, -1, b));
+    if (al) {
+        if (al_per_rhs) {
+            CHK(tdda_model_set(m, TDDA_VEC_AL, -1, al));
+        } else {
+            for (int j = 0; j < nrhs; j++) CHK(tdda_model_set(m, TDDA_VEC_AL, j, al));
+        }
+    }
+    (void)n3;
+    CHK(model_apply(m, precision, al ? 1 : 0));
+    return tdda_model_get(m, TDDA_VEC_Y, -1, y);
+}
+
+void tdda_solve_opts_default(tdda_solve_opts* o)
+{
+    if (!o) return;
+    o->max_it = 100000;
+    o->tol = 1e-8;
+    o->mode = TDDA_MODE_FP64;
+    o->x0_zero = 0;
+    o->inner_tol = 1e-5;
+    o->inner_max_it = 2000;
+    o->max_outer = 8;
+    o->check_every = 0;
+    o->true_residual = 0;
+}
+
+int tdda_model_solve(tdda_model m, const tdda_solve_opts* opts, tdda_solve_info* info)
+{
+    if (!m || !opts || !info) return fail(TDDA_ERR_ARG, "null argument");
+    if (opts->max_it < 0 || !(opts->tol >= 0)) return fail(TDDA_ERR_ARG, "bad iteration limit or tolerance");
+    HIPCHK(hipSetDevice(m->op->ctx->device));
+    if (!m->v64.x) return fail(TDDA_ERR_STATE, "solves need an operator built with TDDA_C128");
+    int rc;
+    if (opts->mode == TDDA_MODE_FP64) {
+        rc = solve_fp64(m, opts, info);
+        if (rc == TDDA_OK && opts->true_residual) {
+            std::vector<double> rel(m->nrhs, 0.0);
+            std::vector<int> act(m->nrhs, 1);
+            rc = true_residual(m, rel, act);
+            for (int j = 0; j < m->nrhs; j++) info[j].true_residual = rel[j];
+        }
+    } else if (opts->mode == TDDA_MODE_MIXED) {
+        if (opts->inner_max_it <= 0 || opts->max_outer <= 0) return fail(TDDA_ERR_ARG, "bad inner/outer limits");
+        rc = solve_mixed(m, opts, info);
+    } else {
+        return fail(TDDA_ERR_ARG, "unknown solve mode");
+    }
+    return rc;
+}
+
+}  // extern "C"

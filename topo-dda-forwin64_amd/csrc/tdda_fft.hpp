@@ -1,0 +1,207 @@
+// tdda_fft.hpp -- complex arithmetic and the LDS-staged mixed-radix FFT used by
+// every pass of the DDA operator.
+//
+// The 3D transforms of the operator are split into one pass per axis.  A pass
+// stages a tile of pencils in LDS, runs self-sorting (Stockham) radix stages
+// over them and writes the tile back.  Radices 2, 3, 4, 5 and 8 are supported,
+// so any FFT length whose prime factors are 2, 3 and 5 can be planned.
+//
+// Twiddles come from tables of the L-th roots of unity computed on the host in
+// long double (forward sign, exp(-2 pi i q / L)); the inverse uses conjugates.
+#pragma once
+#include <hip/hip_runtime.h>
+
+namespace tdda {
+
+template <typename T>
+struct alignas(2 * sizeof(T)) Cx {
+    T x, y;
+};
+
+template <typename T>
+__device__ __forceinline__ Cx<T> cadd(Cx<T> a, Cx<T> b) { return {a.x + b.x, a.y + b.y}; }
+template <typename T>
+__device__ __forceinline__ Cx<T> csub(Cx<T> a, Cx<T> b) { return {a.x - b.x, a.y - b.y}; }
+template <typename T>
+__device__ __forceinline__ Cx<T> cmul(Cx<T> a, Cx<T> b)
+{
+    return {a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x};
+}
+// a * conj(b)
+template <typename T>
+__device__ __forceinline__ Cx<T> cmulc(Cx<T> a, Cx<T> b)
+{
+    return {a.x * b.x + a.y * b.y, a.y * b.x - a.x * b.y};
+}
+template <typename T>
+__device__ __forceinline__ Cx<T> cscale(Cx<T> a, T s) { return {a.x * s, a.y * s}; }
+// multiply by -i (forward) or +i (inverse)
+template <typename T, bool INV>
+__device__ __forceinline__ Cx<T> rot90(Cx<T> a)
+{
+    if (INV) return {-a.y, a.x};
+    return {a.y, -a.x};
+}
+
+// Plan of one 1D transform: length and its radix sequence.
+struct FftPlan {
+    int L;
+    int nst;
+    int rad[12];
+};
+
+// ---------------------------------------------------------------------------
+// Small DFTs in registers.  Forward: y_q = sum_m v_m exp(-2 pi i m q / R).
+// ---------------------------------------------------------------------------
+template <typename T, bool INV>
+__device__ __forceinline__ void dft2(Cx<T>* v)
+{
+    Cx<T> a = v[0], b = v[1];
+    v[0] = cadd(a, b);
+    v[1] = csub(a, b);
+}
+
+template <typename T, bool INV>
+__device__ __forceinline__ void dft4(Cx<T>* v)
+{
+    Cx<T> t0 = cadd(v[0], v[2]);
+    Cx<T> t1 = csub(v[0], v[2]);
+    Cx<T> t2 = cadd(v[1], v[3]);
+    Cx<T> t3 = rot90<T, INV>(csub(v[1], v[3]));
+    v[0] = cadd(t0, t2);
+    v[2] = csub(t0, t2);
+    v[1] = cadd(t1, t3);
+    v[3] = csub(t1, t3);
+}
+
+template <typename T, bool INV>
+__device__ __forceinline__ void dft8(Cx<T>* v)
+{
+    const T h = T(0.70710678118654752440084436210484903928);
+    Cx<T> a[4], b[4];
+#pragma unroll
+    for (int m = 0; m < 4; m++) {
+        a[m] = cadd(v[m], v[m + 4]);
+        b[m] = csub(v[m], v[m + 4]);
+    }
+    // b[m] *= W8^m, W8 = exp(-+ i pi / 4)
+    b[1] = INV ? Cx<T>{h * (b[1].x - b[1].y), h * (b[1].x + b[1].y)}
+               : Cx<T>{h * (b[1].x + b[1].y), h * (b[1].y - b[1].x)};
+    b[2] = rot90<T, INV>(b[2]);
+    b[3] = INV ? Cx<T>{-h * (b[3].x + b[3].y), h * (b[3].x - b[3].y)}
+               : Cx<T>{h * (b[3].y - b[3].x), -h * (b[3].x + b[3].y)};
+    dft4<T, INV>(a);
+    dft4<T, INV>(b);
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+        v[2 * q] = a[q];
+        v[2 * q + 1] = b[q];
+    }
+}
+
+template <typename T, bool INV>
+__device__ __forceinline__ void dft3(Cx<T>* v)
+{
+    const T c = T(-0.5);
+    const T s = T(0.86602540378443864676372317075293618347);
+    Cx<T> t1 = cadd(v[1], v[2]);
+    Cx<T> y0 = cadd(v[0], t1);
+    Cx<T> m1 = {v[0].x + c * t1.x, v[0].y + c * t1.y};
+    Cx<T> d = csub(v[1], v[2]);
+    Cx<T> m2 = rot90<T, INV>(Cx<T>{s * d.x, s * d.y});
+    v[0] = y0;
+    v[1] = cadd(m1, m2);
+    v[2] = csub(m1, m2);
+}
+
+template <typename T, bool INV>
+__device__ __forceinline__ void dft5(Cx<T>* v)
+{
+    const T c1 = T(0.30901699437494742410229341718281905886);
+    const T c2 = T(-0.80901699437494742410229341718281905886);
+    const T s1 = T(0.95105651629515357211643933337938214340);
+    const T s2 = T(0.58778525229247312916870595463907276860);
+    Cx<T> t1 = cadd(v[1], v[4]);
+    Cx<T> t2 = cadd(v[2], v[3]);
+    Cx<T> t3 = csub(v[1], v[4]);
+    Cx<T> t4 = csub(v[2], v[3]);
+    Cx<T> y0 = cadd(v[0], cadd(t1, t2));
+    Cx<T> a1 = {v[0].x + c1 * t1.x + c2 * t2.x, v[0].y + c1 * t1.y + c2 * t2.y};
+    Cx<T> a2 = {v[0].x + c2 * t1.x + c1 * t2.x, v[0].y + c2 * t1.y + c1 * t2.y};
+    Cx<T> b1 = rot90<T, INV>(Cx<T>{s1 * t3.x + s2 * t4.x, s1 * t3.y + s2 * t4.y});
+    Cx<T> b2 = rot90<T, INV>(Cx<T>{s2 * t3.x - s1 * t4.x, s2 * t3.y - s1 * t4.y});
+    v[0] = y0;
+    v[1] = cadd(a1, b1);
+    v[4] = csub(a1, b1);
+    v[2] = cadd(a2, b2);
+    v[3] = csub(a2, b2);
+}
+
+template <typename T, int R, bool INV>
+__device__ __forceinline__ void dft(Cx<T>* v)
+{
+    if constexpr (R == 2) dft2<T, INV>(v);
+    else if constexpr (R == 3) dft3<T, INV>(v);
+    else if constexpr (R == 4) dft4<T, INV>(v);
+    else if constexpr (R == 5) dft5<T, INV>(v);
+    else dft8<T, INV>(v);
+}
+
+// One Stockham stage over NP pencils of length L held in LDS (pencil stride
+// ps).  Ns is the length of the sub-transforms already completed.
+template <typename T, int R, bool INV>
+__device__ __forceinline__ void stockham_stage(const Cx<T>* __restrict__ src, Cx<T>* __restrict__ dst,
+                                               int NP, int ps, int L, int Ns, const Cx<T>* __restrict__ tw)
+{
+    const int LR = L / R;
+    const int total = NP * LR;
+    const int twstep = L / (Ns * R);
+    for (int b = threadIdx.x; b < total; b += blockDim.x) {
+        const int p = b / LR;
+        const int j = b - p * LR;
+        const int k = j % Ns;
+        const Cx<T>* s = src + p * ps + j;
+        Cx<T> v[R];
+#pragma unroll
+        for (int m = 0; m < R; m++) v[m] = s[m * LR];
+        if (Ns > 1) {
+#pragma unroll
+            for (int m = 1; m < R; m++) {
+                const Cx<T> w = tw[m * k * twstep];
+                v[m] = INV ? cmulc(v[m], w) : cmul(v[m], w);
+            }
+        }
+        dft<T, R, INV>(v);
+        Cx<T>* d = dst + p * ps + (j - k) * R + k;
+#pragma unroll
+        for (int m = 0; m < R; m++) d[m * Ns] = v[m];
+    }
+}
+
+// Full transform of NP pencils: data starts in a, b is scratch; returns the
+// buffer that holds the result.  Ends with a barrier.
+template <typename T, bool INV>
+__device__ Cx<T>* lds_fft(Cx<T>* a, Cx<T>* b, int NP, int ps, const FftPlan& pl, const Cx<T>* __restrict__ tw)
+{
+    int Ns = 1;
+    const int L = pl.L;
+    const int nst = pl.nst;
+    for (int s = 0; s < nst; s++) {
+        const int R = pl.rad[s];
+        switch (R) {
+        case 8: stockham_stage<T, 8, INV>(a, b, NP, ps, L, Ns, tw); break;
+        case 4: stockham_stage<T, 4, INV>(a, b, NP, ps, L, Ns, tw); break;
+        case 2: stockham_stage<T, 2, INV>(a, b, NP, ps, L, Ns, tw); break;
+        case 3: stockham_stage<T, 3, INV>(a, b, NP, ps, L, Ns, tw); break;
+        default: stockham_stage<T, 5, INV>(a, b, NP, ps, L, Ns, tw); break;
+        }
+        __syncthreads();
+        Cx<T>* t = a;
+        a = b;
+        b = t;
+        Ns *= R;
+    }
+    return a;
+}
+
+}  // namespace tdda

@@ -1,0 +1,768 @@
+// tdda_kernels.hpp -- device kernels of the DDA operator and solver.
+//
+// Operator (AProductCore::Aproduct, AProductCore.cpp:697-758, plus the
+// al o b diagonal of DDAModel::Aproductwithalb, DDAModel.cpp:577-586):
+//
+//   y = IFFT( G^ . FFT(b) ) restricted to the lattice box, + al o b
+//
+// on a zero-padded grid Mx x My x Mz >= (2Nx-1, 2Ny-1, 2Nz-1).  The 3D
+// transforms are pruned and split into five passes, each a kernel:
+//
+//   P1 z-forward  : box [Nx][Ny][Nz] (site vector)  -> w1 [Nx][Ny][Mz][3]
+//   P2 y-forward  : w1                               -> w2 [Nx][My][Mz][3]
+//   P3 x-fused    : w2 -> FFT_x, 6x3 symmetric Green product, IFFT_x -> w2
+//   P4 y-inverse  : w2                               -> w1
+//   P5 z-inverse  : w1 -> box, + diagonal, BiCGSTAB epilogues and reductions
+//
+// The Green spectrum is stored as one octant [Hx][Hy][Hz][6] (H = M/2 + 1):
+// every tensor component is even or odd along each axis, so the other
+// octants follow by sign flips.  The 1/M of the inverse transform is folded
+// into the stored spectrum.
+//
+// Work is described by a table of jobs (one per structure x right-hand side)
+// and each launch walks a flattened list of tiles over all jobs, so uneven
+// structures share one launch.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "tdda_fft.hpp"
+
+namespace tdda {
+
+constexpr int kThreads = 256;
+
+enum SolveStatus : int {
+    ST_RUNNING = 0,
+    ST_CONVERGED = 1,
+    ST_BREAKDOWN = 2,
+    ST_MAXIT = 3,
+};
+
+// Per right-hand-side BiCGSTAB scalars (always FP64, also for the complex64
+// inner solver) -- the scalars of DDAModel::bicgstab, DDAModel.cpp:145-160.
+struct SolverState {
+    double alpha[2], zeta[2], rho[2], rho_prev[2];
+    double enorm, rnorm, err, tol;
+    int iter, maxit, status, first;
+};
+
+template <typename T>
+struct JobDesc {
+    int Nx, Ny, Nz, Mx, My, Mz, Hx, Hy, Hz;
+    int N;       // sites
+    int pz;      // pencils per tile in P1/P5
+    int tfy;     // flattened (kz, c) columns per tile in P2/P4
+    int tfx;     // flattened (kz, c) columns per tile in P3 (multiple of 3)
+    int nfy;     // tiles per x-slab in P2/P4: ceil(3 Mz / tfy)
+    int nfx;     // tiles per ky row in P3: ceil(3 Mz / tfx)
+    int ps_z, ps_y, ps_x;  // LDS pencil strides
+    const int* cellmap;    // [Nx][Ny][Nz] -> site or -1; nullptr for the full box in bulk order
+    const Cx<T>* green;
+    const Cx<T>* twx;
+    const Cx<T>* twy;
+    const Cx<T>* twz;
+    FftPlan plx, ply, plz;
+    Cx<T>* w1;
+    Cx<T>* w2;
+    // plain operator application
+    const Cx<T>* in;
+    Cx<T>* out;
+    const Cx<T>* al;
+    // solver vectors (3N each)
+    Cx<T>* x;
+    Cx<T>* r;
+    Cx<T>* r0;
+    Cx<T>* p;
+    Cx<T>* ap;
+    Cx<T>* t;
+    Cx<T>* at;
+    const Cx<T>* e;
+    SolverState* st;
+    double* part;   // [tiles][6] partial sums
+    unsigned* cnt;  // last-tile counter
+};
+
+// P1 input modes
+enum InMode : int { IN_PLAIN = 0, IN_P = 1, IN_T = 2, IN_X = 3 };
+// P5 output modes
+enum OutMode : int { OUT_PLAIN = 0, OUT_RES = 1, OUT_AP = 2, OUT_AT = 3 };
+
+__device__ __forceinline__ int find_job(const int* __restrict__ tstart, int njobs, int b)
+{
+    int lo = 0, hi = njobs - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (tstart[mid] <= b) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+template <typename T>
+__device__ __forceinline__ Cx<T> ld(const Cx<T>* p) { return *p; }
+
+__device__ __forceinline__ double2 to_d(Cx<float> a) { return make_double2(a.x, a.y); }
+__device__ __forceinline__ double2 to_d(Cx<double> a) { return make_double2(a.x, a.y); }
+
+template <typename T>
+__device__ __forceinline__ Cx<T> from_d(double re, double im) { return Cx<T>{T(re), T(im)}; }
+
+__device__ __forceinline__ void cdiv(double ar, double ai, double br, double bi, double& cr, double& ci)
+{
+    // complex division with scaling (Smith's method)
+    if (fabs(br) >= fabs(bi)) {
+        if (br == 0.0 && bi == 0.0) {
+            cr = ar / br;
+            ci = ai / br;
+            return;
+        }
+        const double rr = bi / br, den = br + bi * rr;
+        cr = (ar + ai * rr) / den;
+        ci = (ai - ar * rr) / den;
+    } else {
+        const double rr = br / bi, den = br * rr + bi;
+        cr = (ar * rr + ai) / den;
+        ci = (ai * rr - ar) / den;
+    }
+}
+
+// Block-wide deterministic sum of NV doubles held per thread; result valid in
+// thread 0.  Uses wavefront shuffles then LDS.
+template <int NV>
+__device__ __forceinline__ void block_sum(double* v, double* red)
+{
+#pragma unroll
+    for (int k = 0; k < NV; k++) {
+        double s = v[k];
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
+        v[k] = s;
+    }
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    __syncthreads();
+    if (lane == 0) {
+#pragma unroll
+        for (int k = 0; k < NV; k++) red[wid * NV + k] = v[k];
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const int nw = blockDim.x >> 6;
+#pragma unroll
+        for (int k = 0; k < NV; k++) {
+            double s = 0.0;
+            for (int w = 0; w < nw; w++) s += red[w * NV + k];
+            v[k] = s;
+        }
+    }
+}
+
+// Publishes this tile's partial sums; returns true in thread 0 of the last
+// tile of the job to finish, after which the totals are in tot[].
+template <int NV>
+__device__ __forceinline__ bool publish_partials(double* part, unsigned* cnt, int tile, int ntiles,
+                                                 const double* v, double* tot, int* s_last)
+{
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int k = 0; k < NV; k++) part[tile * NV + k] = v[k];
+        __threadfence();
+        const unsigned prev = atomicAdd(cnt, 1u);
+        *s_last = (prev == (unsigned)(ntiles - 1));
+    }
+    __syncthreads();
+    if (!*s_last) return false;
+    __threadfence();
+    // deterministic final sum: fixed per-thread strided order, then block sum
+    double acc[NV];
+#pragma unroll
+    for (int k = 0; k < NV; k++) acc[k] = 0.0;
+    for (int i = threadIdx.x; i < ntiles; i += blockDim.x) {
+#pragma unroll
+        for (int k = 0; k < NV; k++)
+            acc[k] += __hip_atomic_load(&part[i * NV + k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    __shared__ double red2[16 * NV];
+    block_sum<NV>(acc, red2);
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int k = 0; k < NV; k++) tot[k] = acc[k];
+        *cnt = 0u;
+    }
+    return threadIdx.x == 0;
+}
+
+__device__ __forceinline__ bool job_inactive(const SolverState* st)
+{
+    return st != nullptr && st->status != ST_RUNNING;
+}
+
+// ---------------------------------------------------------------------------
+// P1: z-forward.  Tile = pz consecutive (x, y) pencils.
+// ---------------------------------------------------------------------------
+template <typename T, int MODE>
+__global__ void __launch_bounds__(kThreads) k_p1(const JobDesc<T>* __restrict__ jobs, const int* __restrict__ tstart,
+                                                 int njobs)
+{
+    extern __shared__ __align__(16) char smem[];
+    const int job = find_job(tstart, njobs, blockIdx.x);
+    const JobDesc<T>& J = jobs[job];
+    if (MODE != IN_PLAIN && job_inactive(J.st)) return;
+    const int tile = blockIdx.x - tstart[job];
+    const int Ny = J.Ny, Nz = J.Nz, Mz = J.Mz;
+    const int Q = J.Nx * Ny;
+    const int q0 = tile * J.pz;
+    const int np = min(J.pz, Q - q0);
+    const int ps = J.ps_z;
+    Cx<T>* A = reinterpret_cast<Cx<T>*>(smem);
+    Cx<T>* B = A + 3 * J.pz * ps;
+
+    // scalars for the fused vector updates
+    Cx<T> sa{T(0), T(0)}, sb{T(0), T(0)};
+    bool first = false;
+    if (MODE == IN_P) {
+        const SolverState* st = J.st;
+        first = st->first != 0;
+        if (!first) {
+            // beta = (alpha / zeta) * rho / rho_prev   (DDAModel.cpp:218)
+            double q1r, q1i, q2r, q2i;
+            cdiv(st->alpha[0], st->alpha[1], st->zeta[0], st->zeta[1], q1r, q1i);
+            cdiv(st->rho[0], st->rho[1], st->rho_prev[0], st->rho_prev[1], q2r, q2i);
+            const double br = q1r * q2r - q1i * q2i, bi = q1r * q2i + q1i * q2r;
+            sa = from_d<T>(br, bi);
+            sb = from_d<T>(st->zeta[0], st->zeta[1]);
+        }
+    } else if (MODE == IN_T) {
+        sa = from_d<T>(J.st->alpha[0], J.st->alpha[1]);
+    }
+
+    const int nin = np * Nz * 3;
+    for (int e = threadIdx.x; e < nin; e += blockDim.x) {
+        const int pl = e / (Nz * 3);
+        const int rem = e - pl * (Nz * 3);
+        const int z = rem / 3;
+        const int c = rem - z * 3;
+        const int cell = (q0 + pl) * Nz + z;
+        const int site = J.cellmap ? J.cellmap[cell] : cell;
+        Cx<T> v{T(0), T(0)};
+        if (site >= 0) {
+            const int i = site * 3 + c;
+            if (MODE == IN_PLAIN) {
+                v = J.in[i];
+            } else if (MODE == IN_X) {
+                v = J.x[i];
+            } else if (MODE == IN_P) {
+                // p = r + beta (p - u), u = zeta Ap   (DDAModel.cpp:219)
+                const Cx<T> r = J.r[i];
+                if (first) {
+                    v = r;
+                } else {
+                    const Cx<T> u = cmul(sb, J.ap[i]);
+                    v = cadd(r, cmul(sa, csub(J.p[i], u)));
+                }
+                J.p[i] = v;
+            } else {  // IN_T: t = r - alpha Ap   (DDAModel.cpp:222)
+                v = csub(J.r[i], cmul(sa, J.ap[i]));
+                J.t[i] = v;
+            }
+        }
+        A[(pl * 3 + c) * ps + z] = v;
+    }
+    const int npad = Mz - Nz;
+    for (int e = threadIdx.x; e < np * 3 * npad; e += blockDim.x) {
+        const int pc = e / npad;
+        const int z = Nz + (e - pc * npad);
+        A[pc * ps + z] = Cx<T>{T(0), T(0)};
+    }
+    __syncthreads();
+    const Cx<T>* res = lds_fft<T, false>(A, B, np * 3, ps, J.plz, J.twz);
+    const int nout = np * Mz * 3;
+    Cx<T>* w1 = J.w1 + (size_t)q0 * Mz * 3;
+    for (int e = threadIdx.x; e < nout; e += blockDim.x) {
+        const int pl = e / (Mz * 3);
+        const int rem = e - pl * (Mz * 3);
+        const int kz = rem / 3;
+        const int c = rem - kz * 3;
+        w1[e] = res[(pl * 3 + c) * ps + kz];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// P2 / P4: y pass.  Tile = (x, tfy consecutive flattened (kz, c) columns).
+// FWD: w1[x][y<Ny][.] -> w2[x][ky<My][.];  INV: w2[x][ky][.] -> w1[x][y<Ny][.]
+// ---------------------------------------------------------------------------
+template <typename T, bool INV>
+__global__ void __launch_bounds__(kThreads) k_py(const JobDesc<T>* __restrict__ jobs, const int* __restrict__ tstart,
+                                                 int njobs, int solver)
+{
+    extern __shared__ __align__(16) char smem[];
+    const int job = find_job(tstart, njobs, blockIdx.x);
+    const JobDesc<T>& J = jobs[job];
+    if (solver && job_inactive(J.st)) return;
+    const int tile = blockIdx.x - tstart[job];
+    const int x = tile / J.nfy;
+    const int f0 = (tile - x * J.nfy) * J.tfy;
+    const int F = J.Mz * 3;
+    const int nf = min(J.tfy, F - f0);
+    const int Ny = J.Ny, My = J.My;
+    const int ps = J.ps_y;
+    Cx<T>* A = reinterpret_cast<Cx<T>*>(smem);
+    Cx<T>* B = A + J.tfy * ps;
+    const int nin = INV ? My : Ny;
+    const int nout = INV ? Ny : My;
+    const Cx<T>* src = (INV ? J.w2 + (size_t)x * My * F : J.w1 + (size_t)x * Ny * F) + f0;
+    Cx<T>* dst = (INV ? J.w1 + (size_t)x * Ny * F : J.w2 + (size_t)x * My * F) + f0;
+    for (int e = threadIdx.x; e < nin * nf; e += blockDim.x) {
+        const int y = e / nf;
+        const int f = e - y * nf;
+        A[f * ps + y] = src[(size_t)y * F + f];
+    }
+    if (!INV) {
+        for (int e = threadIdx.x; e < (My - Ny) * nf; e += blockDim.x) {
+            const int y = Ny + e / nf;
+            const int f = e % nf;
+            A[f * ps + y] = Cx<T>{T(0), T(0)};
+        }
+    }
+    __syncthreads();
+    const Cx<T>* res = lds_fft<T, INV>(A, B, nf, ps, J.ply, J.twy);
+    for (int e = threadIdx.x; e < nout * nf; e += blockDim.x) {
+        const int y = e / nf;
+        const int f = e - y * nf;
+        dst[(size_t)y * F + f] = res[f * ps + y];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// P3: x pass fused with the Green product (kernel.cu:74-95).
+// Tile = (ky, tfx consecutive flattened (kz, c) columns, tfx % 3 == 0).
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ void __launch_bounds__(kThreads) k_px(const JobDesc<T>* __restrict__ jobs, const int* __restrict__ tstart,
+                                                 int njobs, int solver)
+{
+    extern __shared__ __align__(16) char smem[];
+    const int job = find_job(tstart, njobs, blockIdx.x);
+    const JobDesc<T>& J = jobs[job];
+    if (solver && job_inactive(J.st)) return;
+    const int tile = blockIdx.x - tstart[job];
+    const int ky = tile / J.nfx;
+    const int f0 = (tile - ky * J.nfx) * J.tfx;
+    const int F = J.Mz * 3;
+    const int nf = min(J.tfx, F - f0);
+    const int Nx = J.Nx, Mx = J.Mx, My = J.My, Mz = J.Mz;
+    const int ps = J.ps_x;
+    Cx<T>* A = reinterpret_cast<Cx<T>*>(smem);
+    Cx<T>* B = A + J.tfx * ps;
+    const size_t sx = (size_t)My * F;
+    Cx<T>* base = J.w2 + (size_t)ky * F + f0;
+    for (int e = threadIdx.x; e < Nx * nf; e += blockDim.x) {
+        const int x = e / nf;
+        const int f = e - x * nf;
+        A[f * ps + x] = base[x * sx + f];
+    }
+    for (int e = threadIdx.x; e < (Mx - Nx) * nf; e += blockDim.x) {
+        const int x = Nx + e / nf;
+        const int f = e % nf;
+        A[f * ps + x] = Cx<T>{T(0), T(0)};
+    }
+    __syncthreads();
+    Cx<T>* res = lds_fft<T, false>(A, B, nf, ps, J.plx, J.twx);
+    Cx<T>* other = (res == A) ? B : A;
+
+    // Green product.  Octant fold per axis: k' = min(k, M - k); odd components
+    // flip sign when k was folded.
+    const int kyf = (ky > My / 2) ? My - ky : ky;
+    const T sy = (ky > My / 2) ? T(-1) : T(1);
+    const int nkz = nf / 3;
+    const int Hy = J.Hy, Hz = J.Hz;
+    for (int e = threadIdx.x; e < Mx * nkz; e += blockDim.x) {
+        const int kx = e / nkz;
+        const int kl = e - kx * nkz;
+        const int kz = f0 / 3 + kl;
+        const int kxf = (kx > Mx / 2) ? Mx - kx : kx;
+        const T sxv = (kx > Mx / 2) ? T(-1) : T(1);
+        const int kzf = (kz > Mz / 2) ? Mz - kz : kz;
+        const T szv = (kz > Mz / 2) ? T(-1) : T(1);
+        const Cx<T>* g = J.green + ((size_t)(kxf * Hy + kyf) * Hz + kzf) * 6;
+        const Cx<T> gxx = g[0], gyy = g[3], gzz = g[5];
+        const Cx<T> gxy = cscale(g[1], sxv * sy);
+        const Cx<T> gxz = cscale(g[2], sxv * szv);
+        const Cx<T> gyz = cscale(g[4], sy * szv);
+        Cx<T>* v = res + (kl * 3) * ps + kx;
+        const Cx<T> bx = v[0], by = v[ps], bz = v[2 * ps];
+        v[0] = cadd(cadd(cmul(gxx, bx), cmul(gxy, by)), cmul(gxz, bz));
+        v[ps] = cadd(cadd(cmul(gxy, bx), cmul(gyy, by)), cmul(gyz, bz));
+        v[2 * ps] = cadd(cadd(cmul(gxz, bx), cmul(gyz, by)), cmul(gzz, bz));
+    }
+    __syncthreads();
+    const Cx<T>* res2 = lds_fft<T, true>(res, other, nf, ps, J.plx, J.twx);
+    for (int e = threadIdx.x; e < Nx * nf; e += blockDim.x) {
+        const int x = e / nf;
+        const int f = e - x * nf;
+        base[x * sx + f] = res2[f * ps + x];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// P5: z-inverse, crop to the box, diagonal term and solver epilogues.
+// ---------------------------------------------------------------------------
+template <typename T, int MODE>
+__global__ void __launch_bounds__(kThreads) k_p5(const JobDesc<T>* __restrict__ jobs, const int* __restrict__ tstart,
+                                                 int njobs, int diag)
+{
+    extern __shared__ __align__(16) char smem[];
+    __shared__ double red[16 * 4];
+    __shared__ int s_last;
+    const int job = find_job(tstart, njobs, blockIdx.x);
+    const JobDesc<T>& J = jobs[job];
+    if (MODE != OUT_PLAIN && job_inactive(J.st)) return;
+    const int tile = blockIdx.x - tstart[job];
+    const int ntiles = tstart[job + 1] - tstart[job];
+    const int Ny = J.Ny, Nz = J.Nz, Mz = J.Mz;
+    const int Q = J.Nx * Ny;
+    const int q0 = tile * J.pz;
+    const int np = min(J.pz, Q - q0);
+    const int ps = J.ps_z;
+    Cx<T>* A = reinterpret_cast<Cx<T>*>(smem);
+    Cx<T>* B = A + 3 * J.pz * ps;
+    const Cx<T>* w1 = J.w1 + (size_t)q0 * Mz * 3;
+    const int nin = np * Mz * 3;
+    for (int e = threadIdx.x; e < nin; e += blockDim.x) {
+        const int pl = e / (Mz * 3);
+        const int rem = e - pl * (Mz * 3);
+        const int kz = rem / 3;
+        const int c = rem - kz * 3;
+        A[(pl * 3 + c) * ps + kz] = w1[e];
+    }
+    __syncthreads();
+    const Cx<T>* res = lds_fft<T, true>(A, B, np * 3, ps, J.plz, J.twz);
+
+    double acc[4] = {0.0, 0.0, 0.0, 0.0};
+    const int nout = np * Nz * 3;
+    for (int e = threadIdx.x; e < nout; e += blockDim.x) {
+        const int pl = e / (Nz * 3);
+        const int rem = e - pl * (Nz * 3);
+        const int z = rem / 3;
+        const int c = rem - z * 3;
+        const int cell = (q0 + pl) * Nz + z;
+        const int site = J.cellmap ? J.cellmap[cell] : cell;
+        if (site < 0) continue;
+        const int i = site * 3 + c;
+        const Cx<T> conv = res[(pl * 3 + c) * ps + z];
+        if (MODE == OUT_PLAIN) {
+            Cx<T> y = conv;
+            if (diag) y = cadd(y, cmul(J.al[i], J.in[i]));
+            J.out[i] = y;
+        } else if (MODE == OUT_RES) {
+            // r = E - A x ; r0 = r   (DDAModel.cpp:163-166)
+            const Cx<T> ax = cadd(conv, cmul(J.al[i], J.x[i]));
+            const Cx<T> ev = J.e[i];
+            const Cx<T> r = csub(ev, ax);
+            J.r[i] = r;
+            J.r0[i] = r;
+            const double2 rd = to_d(r), ed = to_d(ev);
+            acc[0] += rd.x * rd.x + rd.y * rd.y;
+            acc[1] += ed.x * ed.x + ed.y * ed.y;
+        } else if (MODE == OUT_AP) {
+            const Cx<T> ap = cadd(conv, cmul(J.al[i], J.p[i]));
+            J.ap[i] = ap;
+            const double2 a = to_d(J.r0[i]), b = to_d(ap);
+            acc[0] += a.x * b.x + a.y * b.y;  // conj(r0) Ap
+            acc[1] += a.x * b.y - a.y * b.x;
+        } else {  // OUT_AT
+            const Cx<T> at = cadd(conv, cmul(J.al[i], J.t[i]));
+            J.at[i] = at;
+            const double2 a = to_d(at), b = to_d(J.t[i]);
+            acc[0] += a.x * b.x + a.y * b.y;  // conj(At) t
+            acc[1] += a.x * b.y - a.y * b.x;
+            acc[2] += a.x * a.x + a.y * a.y;  // conj(At) At
+        }
+    }
+    if (MODE == OUT_PLAIN) return;
+    block_sum<4>(acc, red);
+    double tot[4];
+    if (!publish_partials<4>(J.part, J.cnt, tile, ntiles, acc, tot, &s_last)) return;
+    SolverState* st = J.st;
+    if (MODE == OUT_RES) {
+        st->rho[0] = tot[0];  // r0.r = |r|^2
+        st->rho[1] = 0.0;
+        st->enorm = sqrt(tot[1]);
+        st->rnorm = sqrt(tot[0]);
+    } else if (MODE == OUT_AP) {
+        // alpha = r0.r / r0.Ap   (DDAModel.cpp:168, 221)
+        cdiv(st->rho[0], st->rho[1], tot[0], tot[1], st->alpha[0], st->alpha[1]);
+    } else {
+        // zeta = At.t / At.At   (DDAModel.cpp:172, 225)
+        cdiv(tot[0], tot[1], tot[2], 0.0, st->zeta[0], st->zeta[1]);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Vector kernels of the solver (one tile = kThreads * kVecPer elements).
+// ---------------------------------------------------------------------------
+constexpr int kVecPer = 4;
+
+// x0 = 0 start: r = r0 = E, x = 0 (A 0 = 0 exactly, so this equals
+// DDAModel.cpp:163-166 with P = 0).
+template <typename T>
+__global__ void __launch_bounds__(kThreads) k_init_zero(const JobDesc<T>* __restrict__ jobs,
+                                                        const int* __restrict__ tstart, int njobs)
+{
+    __shared__ double red[16 * 2];
+    __shared__ int s_last;
+    const int job = find_job(tstart, njobs, blockIdx.x);
+    const JobDesc<T>& J = jobs[job];
+    if (job_inactive(J.st)) return;
+    const int tile = blockIdx.x - tstart[job];
+    const int ntiles = tstart[job + 1] - tstart[job];
+    const int n = J.N * 3;
+    double acc[2] = {0.0, 0.0};
+    const int i0 = tile * kThreads * kVecPer;
+    for (int k = 0; k < kVecPer; k++) {
+        const int i = i0 + k * kThreads + threadIdx.x;
+        if (i < n) {
+            const Cx<T> ev = J.e[i];
+            J.r[i] = ev;
+            J.r0[i] = ev;
+            J.x[i] = Cx<T>{T(0), T(0)};
+            const double2 d = to_d(ev);
+            acc[0] += d.x * d.x + d.y * d.y;
+        }
+    }
+    block_sum<2>(acc, red);
+    double tot[2];
+    if (!publish_partials<2>(J.part, J.cnt, tile, ntiles, acc, tot, &s_last)) return;
+    SolverState* st = J.st;
+    st->rho[0] = tot[0];
+    st->rho[1] = 0.0;
+    st->enorm = sqrt(tot[0]);
+    st->rnorm = sqrt(tot[0]);
+}
+
+// u = zeta Ap; z = zeta r - alpha u; P += alpha p + z; rl = r; r = t - zeta At
+// (DDAModel.cpp:226-230) with the stop test of DDAModel.cpp:232 and the
+// breakdown test of DDAModel.cpp:193 for the next iteration.
+template <typename T>
+__global__ void __launch_bounds__(kThreads) k_update(const JobDesc<T>* __restrict__ jobs,
+                                                     const int* __restrict__ tstart, int njobs)
+{
+    __shared__ double red[16 * 4];
+    __shared__ int s_last;
+    const int job = find_job(tstart, njobs, blockIdx.x);
+    const JobDesc<T>& J = jobs[job];
+    if (job_inactive(J.st)) return;
+    const int tile = blockIdx.x - tstart[job];
+    const int ntiles = tstart[job + 1] - tstart[job];
+    const int n = J.N * 3;
+    const SolverState* stc = J.st;
+    const Cx<T> al = from_d<T>(stc->alpha[0], stc->alpha[1]);
+    const Cx<T> ze = from_d<T>(stc->zeta[0], stc->zeta[1]);
+    double acc[4] = {0.0, 0.0, 0.0, 0.0};
+    const int i0 = tile * kThreads * kVecPer;
+    for (int k = 0; k < kVecPer; k++) {
+        const int i = i0 + k * kThreads + threadIdx.x;
+        if (i < n) {
+            const Cx<T> r = J.r[i];
+            const Cx<T> u = cmul(ze, J.ap[i]);
+            const Cx<T> z = csub(cmul(ze, r), cmul(al, u));
+            J.x[i] = cadd(cadd(J.x[i], cmul(al, J.p[i])), z);
+            const Cx<T> rn = csub(J.t[i], cmul(ze, J.at[i]));
+            J.r[i] = rn;
+            const double2 a = to_d(J.r0[i]), b = to_d(rn);
+            acc[0] += b.x * b.x + b.y * b.y;
+            acc[1] += a.x * b.x + a.y * b.y;
+            acc[2] += a.x * b.y - a.y * b.x;
+        }
+    }
+    block_sum<4>(acc, red);
+    double tot[4];
+    if (!publish_partials<4>(J.part, J.cnt, tile, ntiles, acc, tot, &s_last)) return;
+    SolverState* st = J.st;
+    st->rho_prev[0] = st->rho[0];
+    st->rho_prev[1] = st->rho[1];
+    st->rho[0] = tot[1];
+    st->rho[1] = tot[2];
+    st->rnorm = sqrt(tot[0]);
+    st->err = st->rnorm / st->enorm;
+    if (st->first) {
+        st->first = 0;
+    } else {
+        st->iter += 1;
+        if (st->err <= st->tol) {
+            st->status = ST_CONVERGED;
+            return;
+        }
+    }
+    if (st->iter >= st->maxit) {
+        st->status = ST_MAXIT;
+        return;
+    }
+    if (st->rho_prev[0] == 0.0 && st->rho_prev[1] == 0.0) {
+        st->status = ST_BREAKDOWN;
+        st->iter += 1;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Mixed precision glue: inner right-hand side and solution accumulation.
+// ---------------------------------------------------------------------------
+// e32 = c64(r64)
+__global__ void __launch_bounds__(kThreads) k_demote(const Cx<double>* __restrict__ src, Cx<float>* __restrict__ dst,
+                                                     size_t n)
+{
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) {
+        const Cx<double> v = src[i];
+        dst[i] = Cx<float>{float(v.x), float(v.y)};
+    }
+}
+
+// x64 += c128(x32)
+__global__ void __launch_bounds__(kThreads) k_accum(Cx<double>* __restrict__ x, const Cx<float>* __restrict__ d,
+                                                    size_t n)
+{
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) {
+        const Cx<float> v = d[i];
+        Cx<double> a = x[i];
+        a.x += double(v.x);
+        a.y += double(v.y);
+        x[i] = a;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Green spectrum set-up (AProductCore.cpp:54-229, built on the device).
+// ---------------------------------------------------------------------------
+// Interaction tensor AProductCore.cpp:588-623 (LDR_inter).
+__device__ __forceinline__ void ldr_inter(double K, double x, double y, double z, Cx<double>* out)
+{
+    const double xs = x * x, ys = y * y, zs = z * z;
+    const double rn = sqrt(xs + ys + zs);
+    if (rn == 0.0) {
+        for (int l = 0; l < 6; l++) out[l] = Cx<double>{0.0, 0.0};
+        return;
+    }
+    const double r2 = rn * rn, r3 = rn * rn * rn;
+    double sn, cs;
+    sincos(K * rn, &sn, &cs);
+    const Cx<double> c1{cs / r3, sn / r3};
+    const Cx<double> c2{1.0 / r2, -K * rn / r2};
+    const Cx<double> c3{K * K - 3.0 * c2.x, -3.0 * c2.y};
+    const double kk = K * K;
+    Cx<double> a[6];
+    a[0] = Cx<double>{c2.x * (r2 - 3.0 * xs) - kk * (ys + zs), c2.y * (r2 - 3.0 * xs)};
+    a[1] = cscale(c3, x * y);
+    a[2] = cscale(c3, z * x);
+    a[3] = Cx<double>{c2.x * (r2 - 3.0 * ys) - kk * (zs + xs), c2.y * (r2 - 3.0 * ys)};
+    a[4] = cscale(c3, y * z);
+    a[5] = Cx<double>{c2.x * (r2 - 3.0 * zs) - kk * (xs + ys), c2.y * (r2 - 3.0 * zs)};
+    for (int l = 0; l < 6; l++) out[l] = cmul(c1, a[l]);
+}
+
+// Fills the 6-component tensor on the whole padded grid [Mx][My][Mz][6]:
+// index i < N -> offset i d; i >= M - (N - 1) -> (i - M) d; otherwise zero
+// (AProductCore.cpp:54-194 with 2N-1 replaced by M).
+__global__ void __launch_bounds__(kThreads) k_green_fill(Cx<double>* __restrict__ g, int Nx, int Ny, int Nz, int Mx,
+                                                         int My, int Mz, double d, double K)
+{
+    const size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const size_t M = (size_t)Mx * My * Mz;
+    if (idx >= M) return;
+    const int k = (int)(idx % Mz);
+    const int j = (int)((idx / Mz) % My);
+    const int i = (int)(idx / ((size_t)Mz * My));
+    const bool okx = (i <= Nx - 1) || (i >= Mx - (Nx - 1));
+    const bool oky = (j <= Ny - 1) || (j >= My - (Ny - 1));
+    const bool okz = (k <= Nz - 1) || (k >= Mz - (Nz - 1));
+    Cx<double> a[6];
+    if (okx && oky && okz) {
+        const int oi = (i <= Nx - 1) ? i : i - Mx;
+        const int oj = (j <= Ny - 1) ? j : j - My;
+        const int ok = (k <= Nz - 1) ? k : k - Mz;
+        ldr_inter(K, d * oi, d * oj, d * ok, a);
+    } else {
+        for (int l = 0; l < 6; l++) a[l] = Cx<double>{0.0, 0.0};
+    }
+    for (int l = 0; l < 6; l++) g[idx * 6 + l] = a[l];
+}
+
+// Generic in-place transform along one axis of an array [D0][D1][D2][NC]
+// viewed as pencils: element (o, a, f) at o*so + a*sa + f, f in [0, F).
+// A tile is (o, tf consecutive f).  Used for the Green spectrum (NC = 6).
+template <typename T, bool INV>
+__global__ void __launch_bounds__(kThreads) k_fft_strided(Cx<T>* __restrict__ data, int L, size_t so, size_t sa,
+                                                          int F, int tf, int ntf, int ps, FftPlan pl,
+                                                          const Cx<T>* __restrict__ tw)
+{
+    extern __shared__ __align__(16) char smem[];
+    const int o = blockIdx.x / ntf;
+    const int f0 = (blockIdx.x - o * ntf) * tf;
+    const int nf = min(tf, F - f0);
+    Cx<T>* A = reinterpret_cast<Cx<T>*>(smem);
+    Cx<T>* B = A + tf * ps;
+    Cx<T>* base = data + (size_t)o * so + f0;
+    for (int e = threadIdx.x; e < L * nf; e += blockDim.x) {
+        const int a = e / nf;
+        const int f = e - a * nf;
+        A[f * ps + a] = base[(size_t)a * sa + f];
+    }
+    __syncthreads();
+    const Cx<T>* res = lds_fft<T, INV>(A, B, nf, ps, pl, tw);
+    for (int e = threadIdx.x; e < L * nf; e += blockDim.x) {
+        const int a = e / nf;
+        const int f = e - a * nf;
+        base[(size_t)a * sa + f] = res[f * ps + a];
+    }
+}
+
+// Contiguous-axis variant: pencil q occupies data[q*L*NC ... ) as [a][NC].
+template <typename T, bool INV>
+__global__ void __launch_bounds__(kThreads) k_fft_contig(Cx<T>* __restrict__ data, int Q, int L, int NC, int P, int ps,
+                                                         FftPlan pl, const Cx<T>* __restrict__ tw)
+{
+    extern __shared__ __align__(16) char smem[];
+    const int q0 = blockIdx.x * P;
+    const int np = min(P, Q - q0);
+    Cx<T>* A = reinterpret_cast<Cx<T>*>(smem);
+    Cx<T>* B = A + P * NC * ps;
+    Cx<T>* base = data + (size_t)q0 * L * NC;
+    const int n = np * L * NC;
+    for (int e = threadIdx.x; e < n; e += blockDim.x) {
+        const int pl_ = e / (L * NC);
+        const int rem = e - pl_ * (L * NC);
+        const int a = rem / NC;
+        const int c = rem - a * NC;
+        A[(pl_ * NC + c) * ps + a] = base[e];
+    }
+    __syncthreads();
+    const Cx<T>* res = lds_fft<T, INV>(A, B, np * NC, ps, pl, tw);
+    for (int e = threadIdx.x; e < n; e += blockDim.x) {
+        const int pl_ = e / (L * NC);
+        const int rem = e - pl_ * (L * NC);
+        const int a = rem / NC;
+        const int c = rem - a * NC;
+        base[e] = res[(pl_ * NC + c) * ps + a];
+    }
+}
+
+// Octant extraction with the 1/M scale folded in: full [Mx][My][Mz][6] FP64
+// spectrum -> [Hx][Hy][Hz][6] of type T.
+template <typename T>
+__global__ void __launch_bounds__(kThreads) k_green_octant(const Cx<double>* __restrict__ full, Cx<T>* __restrict__ oct,
+                                                           int Mx, int My, int Mz, int Hx, int Hy, int Hz, double scale)
+{
+    const size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const size_t H = (size_t)Hx * Hy * Hz * 6;
+    if (idx >= H) return;
+    const int l = (int)(idx % 6);
+    const size_t pnt = idx / 6;
+    const int k = (int)(pnt % Hz);
+    const int j = (int)((pnt / Hz) % Hy);
+    const int i = (int)(pnt / ((size_t)Hz * Hy));
+    const Cx<double> v = full[(((size_t)i * My + j) * Mz + k) * 6 + l];
+    oct[idx] = Cx<T>{T(v.x * scale), T(v.y * scale)};
+}
+
+}  // namespace tdda

@@ -1,0 +1,224 @@
+"""ctypes binding of libtopodda_hip.so (C ABI declared in include/topodda.h).
+
+Used by the tests, bench.py and __graft_entry__.  Loading fails loudly when the
+HIP library has not been built: there is no CPU fallback.
+
+Class names follow the reference's call surface (Topo-DDA-forWin64):
+``AProductCore`` (AProductCore.h:51-80) owns the device operator and
+``DDAModel`` (DDAModel.h:34-74) owns device-resident solver state.
+"""
+from __future__ import annotations
+
+import ctypes
+import math
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIBPATH = os.path.join(HERE, "libtopodda_hip.so")
+
+C128 = 1
+C64 = 2
+VEC_AL, VEC_E, VEC_P, VEC_Y = 0, 1, 2, 3
+MODE_FP64, MODE_MIXED = 0, 1
+CONVERGED, BREAKDOWN, MAXIT = 1, 2, 3
+
+EXPORTS = [
+    "tdda_last_error", "tdda_version", "tdda_ctx_create", "tdda_ctx_destroy", "tdda_ctx_sync",
+    "tdda_op_create", "tdda_op_destroy", "tdda_op_info", "tdda_aproduct", "tdda_green_tensor",
+    "tdda_model_create", "tdda_model_destroy", "tdda_model_set", "tdda_model_get", "tdda_model_apply",
+    "tdda_solve_opts_default", "tdda_model_solve",
+]
+
+
+class TddaError(RuntimeError):
+    pass
+
+
+class OpDesc(ctypes.Structure):
+    _fields_ = [("N", ctypes.c_int), ("Nx", ctypes.c_int), ("Ny", ctypes.c_int), ("Nz", ctypes.c_int),
+                ("R", ctypes.POINTER(ctypes.c_int)), ("d", ctypes.c_double), ("lam", ctypes.c_double),
+                ("nback", ctypes.c_double), ("Mx", ctypes.c_int), ("My", ctypes.c_int), ("Mz", ctypes.c_int),
+                ("precisions", ctypes.c_int)]
+
+
+class SolveOpts(ctypes.Structure):
+    _fields_ = [("max_it", ctypes.c_int), ("tol", ctypes.c_double), ("mode", ctypes.c_int),
+                ("x0_zero", ctypes.c_int), ("inner_tol", ctypes.c_double), ("inner_max_it", ctypes.c_int),
+                ("max_outer", ctypes.c_int), ("check_every", ctypes.c_int), ("true_residual", ctypes.c_int)]
+
+
+class SolveInfo(ctypes.Structure):
+    _fields_ = [("status", ctypes.c_int), ("iterations", ctypes.c_int), ("error", ctypes.c_double),
+                ("true_residual", ctypes.c_double), ("outer", ctypes.c_int)]
+
+
+_lib = None
+
+
+def lib():
+    """Loads the HIP library; raises if it is missing (no fallback path)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIBPATH):
+            raise TddaError(f"{LIBPATH} is missing: build it with __graft_entry__.build()")
+        L = ctypes.CDLL(LIBPATH)
+        vp = ctypes.c_void_p
+        dp = ctypes.POINTER(ctypes.c_double)
+        L.tdda_last_error.restype = ctypes.c_char_p
+        L.tdda_ctx_create.argtypes = [ctypes.c_int, ctypes.POINTER(vp)]
+        L.tdda_ctx_destroy.argtypes = [vp]
+        L.tdda_ctx_sync.argtypes = [vp]
+        L.tdda_op_create.argtypes = [vp, ctypes.POINTER(OpDesc), ctypes.POINTER(vp)]
+        L.tdda_op_destroy.argtypes = [vp]
+        L.tdda_op_info.argtypes = [vp, ctypes.POINTER(ctypes.c_int), dp]
+        L.tdda_aproduct.argtypes = [vp, dp, dp, ctypes.c_int, dp, ctypes.c_int, ctypes.c_int]
+        L.tdda_green_tensor.argtypes = [ctypes.c_double] * 4 + [dp]
+        L.tdda_model_create.argtypes = [vp, ctypes.c_int, ctypes.POINTER(vp)]
+        L.tdda_model_destroy.argtypes = [vp]
+        L.tdda_model_set.argtypes = [vp, ctypes.c_int, ctypes.c_int, dp]
+        L.tdda_model_get.argtypes = [vp, ctypes.c_int, ctypes.c_int, dp]
+        L.tdda_model_apply.argtypes = [vp, ctypes.c_int]
+        L.tdda_solve_opts_default.argtypes = [ctypes.POINTER(SolveOpts)]
+        L.tdda_solve_opts_default.restype = None
+        L.tdda_model_solve.argtypes = [vp, ctypes.POINTER(SolveOpts), ctypes.POINTER(SolveInfo)]
+        _lib = L
+    return _lib
+
+
+def _check(rc):
+    if rc != 0:
+        raise TddaError(f"topodda error {rc}: {lib().tdda_last_error().decode()}")
+
+
+def _dptr(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+class Context:
+    def __init__(self, device: int = 0):
+        self.h = ctypes.c_void_p()
+        _check(lib().tdda_ctx_create(device, ctypes.byref(self.h)))
+
+    def sync(self):
+        _check(lib().tdda_ctx_sync(self.h))
+
+    def close(self):
+        if self.h:
+            lib().tdda_ctx_destroy(self.h)
+            self.h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def green_tensor(K, x, y, z):
+    """AProductCore::LDR_inter (AProductCore.cpp:588-623) through the C ABI."""
+    out = np.zeros(12)
+    _check(lib().tdda_green_tensor(K, x, y, z, _dptr(out)))
+    return out.view(np.complex128)
+
+
+class AProductCore:
+    """Device operator: AProductCore (AProductCore.cpp:12-249, 697-758)."""
+
+    def __init__(self, ctx: Context, R, Nx, Ny, Nz, d, lam, nback=1.0, grid=None, precisions=C128 | C64):
+        self.ctx = ctx
+        self.R = np.ascontiguousarray(R, dtype=np.int32)
+        self.N = len(self.R) // 3
+        self.Nx, self.Ny, self.Nz = Nx, Ny, Nz
+        self.d, self.lam, self.nback = d, lam, nback
+        g = grid or (0, 0, 0)
+        desc = OpDesc(self.N, Nx, Ny, Nz, self.R.ctypes.data_as(ctypes.POINTER(ctypes.c_int)), d, lam, nback,
+                      g[0], g[1], g[2], precisions)
+        self.h = ctypes.c_void_p()
+        _check(lib().tdda_op_create(ctx.h, ctypes.byref(desc), ctypes.byref(self.h)))
+        g3 = (ctypes.c_int * 3)()
+        K = ctypes.c_double()
+        _check(lib().tdda_op_info(self.h, g3, ctypes.byref(K)))
+        self.grid = tuple(g3)
+        self.K = K.value
+
+    def Aproduct(self, b, al=None, precision=C128):
+        """y = A_offdiag b (+ al o b); b is (3N,) or (nrhs, 3N) complex128."""
+        b = np.ascontiguousarray(b, dtype=np.complex128)
+        nrhs = 1 if b.ndim == 1 else b.shape[0]
+        y = np.zeros_like(b)
+        alp = None
+        per = 0
+        if al is not None:
+            al = np.ascontiguousarray(al, dtype=np.complex128)
+            per = 1 if al.ndim == 2 else 0
+            alp = _dptr(al.view(np.float64))
+        _check(lib().tdda_aproduct(self.h, _dptr(b.view(np.float64)), _dptr(y.view(np.float64)), nrhs, alp, per,
+                                   precision))
+        return y
+
+    def close(self):
+        if self.h:
+            lib().tdda_op_destroy(self.h)
+            self.h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def solve_opts(**kw) -> SolveOpts:
+    o = SolveOpts()
+    lib().tdda_solve_opts_default(ctypes.byref(o))
+    for k, v in kw.items():
+        setattr(o, k, v)
+    return o
+
+
+class DeviceModel:
+    """Device-resident state of nrhs DDAModel right-hand sides (al, E, P)."""
+
+    def __init__(self, core: AProductCore, nrhs: int = 1):
+        self.core = core
+        self.nrhs = nrhs
+        self.n3 = 3 * core.N
+        self.h = ctypes.c_void_p()
+        _check(lib().tdda_model_create(core.h, nrhs, ctypes.byref(self.h)))
+
+    def set(self, which, arr, rhs=-1):
+        a = np.ascontiguousarray(arr, dtype=np.complex128)
+        need = self.n3 * (self.nrhs if rhs < 0 else 1)
+        if a.size != need:
+            raise ValueError(f"expected {need} complex values, got {a.size}")
+        _check(lib().tdda_model_set(self.h, which, rhs, _dptr(a.view(np.float64))))
+
+    def get(self, which, rhs=-1):
+        n = self.n3 * (self.nrhs if rhs < 0 else 1)
+        out = np.zeros(n, np.complex128)
+        _check(lib().tdda_model_get(self.h, which, rhs, _dptr(out.view(np.float64))))
+        return out.reshape(self.nrhs, self.n3) if rhs < 0 else out
+
+    def apply(self, precision=C128):
+        _check(lib().tdda_model_apply(self.h, precision))
+        return self.get(VEC_Y)
+
+    def solve(self, **kw):
+        o = solve_opts(**kw)
+        info = (SolveInfo * self.nrhs)()
+        _check(lib().tdda_model_solve(self.h, ctypes.byref(o), info))
+        return [dict(status=i.status, iterations=i.iterations, error=i.error, true_residual=i.true_residual,
+                     outer=i.outer) for i in info]
+
+    def close(self):
+        if self.h:
+            lib().tdda_model_destroy(self.h)
+            self.h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
