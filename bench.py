@@ -1,0 +1,199 @@
+#!/usr/bin/env python3
+"""Benchmark: converged DDA solves/s on one topology-optimization step.
+
+Workload "topo80" (80 x 80 x 16 slab, 102,400 dipoles, d = 10 nm,
+lambda = 542 nm, cone-filtered column design, eps between 1.01 and
+2.25 + 0.01i, FFT grid 160 x 160 x 32): one step = a forward solve
+(z-incident, x-polarised plane wave) followed by the adjoint solve whose
+source is conj(E_int) at 100 seeded target dipoles -- the two solves of
+EvoDDAModel::EvoOptimization (EvoDDAModel.cpp:435-437, 596-608).  Both solves
+start from P = 0 and run until the FP64 true residual ||E - A P|| / ||E|| is
+<= 1e-8.  `value` is complex128 solves/s with the reference's BiCGSTAB on the
+device; the mixed-precision engine and the per-pass Aproduct figures are
+reported beside it.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import platform
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+PKG = os.path.join(ROOT, "topo-dda-forwin64_amd")
+sys.path.insert(0, PKG)
+
+HBM_PEAK_GBS = 8000.0
+TOL = 1e-8
+
+
+def pass_bytes(core, s):
+    """Algorithmic HBM bytes of the five passes of one application
+    Y = A P + al o P (one right-hand side), element size s bytes."""
+    Nx, Ny, Nz = core.Nx, core.Ny, core.Nz
+    Mx, My, Mz = core.grid
+    n3 = 3 * core.N
+    w1 = Nx * Ny * Mz * 3
+    w2 = Nx * My * Mz * 3
+    green = (Mx // 2 + 1) * (My // 2 + 1) * (Mz // 2 + 1) * 6
+    return np.array([n3 + w1,              # P1: read P, write w1
+                     w1 + w2,              # P2: read w1, write w2
+                     2 * w2 + green,       # P3: read/write w2, read Green octant
+                     w2 + w1,              # P4: read w2, write w1
+                     w1 + 3 * n3],         # P5: read w1, al, P; write Y
+                    dtype=np.float64) * s
+
+
+def run_step(T, F, A, targets, diel, mode, lat):
+    """One topology step: forward solve, adjoint source, adjoint solve."""
+    t0 = time.perf_counter()
+    fi = F.solve(mode=mode, tol=TOL, x0_zero=1, true_residual=1)[0]
+    t1 = time.perf_counter()
+    idx = (3 * targets[:, None] + np.arange(3)[None, :]).reshape(-1)
+    P = F.get_entries(T.VEC_P, 0, idx)
+    al_t = F.al_host[idx]
+    g = np.conj(al_t * P * 3.0 / (2.0 + diel[idx]))
+    A.set_entries(T.VEC_E, 0, idx, g)
+    ai = A.solve(mode=mode, tol=TOL, x0_zero=1, true_residual=1)[0]
+    t2 = time.perf_counter()
+    lat.append((t1 - t0) * 1e3)
+    lat.append((t2 - t1) * 1e3)
+    return fi, ai
+
+
+def cpu_baseline(prob, targets, al, E, diel, budget_s=30.0):
+    """The oracle's FP64 restatement of the same step on the host cores."""
+    sys.path.insert(0, ROOT)
+    from oracle import oracle as O
+    op = O.OrcOperator(prob.R, prob.Nx, prob.Ny, prob.Nz, prob.d, prob.K, grid=(160, 160, 32))
+    t0 = time.perf_counter()
+    P, st, it, err = op.bicgstab(al, E, max_it=10000, tol=TOL)
+    t1 = time.perf_counter()
+    solves, elapsed = 1, t1 - t0
+    sample = f"forward solve of topo80 ({it} iterations, to 1e-8)"
+    if elapsed < budget_s / 2:
+        idx = (3 * targets[:, None] + np.arange(3)[None, :]).reshape(-1)
+        g = np.zeros_like(E)
+        g[idx] = np.conj(al[idx] * P[idx] * 3.0 / (2.0 + diel[idx]))
+        P2, st2, it2, _ = op.bicgstab(al, g, max_it=10000, tol=TOL)
+        elapsed = time.perf_counter() - t0
+        solves = 2
+        sample = f"one topo80 step: forward ({it} it) + adjoint ({it2} it) solves to 1e-8"
+    op.close()
+    return {"value": solves / elapsed, "unit": "solves/s", "cores": O.lib().orc_num_threads(), "kind": "port",
+            "sample": sample, "cpu": platform.processor() or platform.machine()}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for {args.gpus}); see the sharding design notes.",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import topodda as T
+    import workloads as W
+
+    prob, targets = W.topo80(seed=2)
+    nK = np.array([0.0, 0.0, 1.0])
+    nE = np.array([1.0, 0.0, 0.0])
+    al, E, diel = prob.rhs(nK, nE)
+    ctx = T.Context(0)
+    core = T.AProductCore(ctx, prob.R, prob.Nx, prob.Ny, prob.Nz, prob.d, prob.lam, prob.nback)
+    F = T.DeviceModel(core, 1)
+    A = T.DeviceModel(core, 1)
+    for m in (F, A):
+        m.set(T.VEC_AL, al)
+        m.al_host = al
+    F.set(T.VEC_E, E)
+
+    results = {}
+    for mode, name in ((T.MODE_FP64, "fp64"), (T.MODE_MIXED, "mixed")):
+        lat = []
+        for _ in range(args.warmup):
+            run_step(T, F, A, targets, diel, mode, lat)
+        lat = []
+        ctx.sync()
+        infos = []
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            infos.append(run_step(T, F, A, targets, diel, mode, lat))
+        ctx.sync()
+        dt = time.perf_counter() - t0
+        ok = all(i["true_residual"] <= TOL and i["status"] == T.CONVERGED for pair in infos for i in pair)
+        results[name] = {
+            "solves_per_s": 2 * args.steps / dt, "ms_per_step": dt / args.steps * 1e3,
+            "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
+            "iterations_fwd": infos[-1][0]["iterations"], "iterations_adj": infos[-1][1]["iterations"],
+            "true_residual_max": max(i["true_residual"] for pair in infos for i in pair),
+            "all_converged": ok,
+        }
+
+    # per-pass timing of Y = A P + al o P on the device (HIP events)
+    F.set(T.VEC_P, E)
+    passes = {}
+    for prec, s, key in ((T.C128, 16, "c128"), (T.C64, 8, "c64")):
+        ms = F.time_passes(prec, reps=50)
+        b = pass_bytes(core, s)
+        passes[key] = {"ms": [round(float(x), 5) for x in ms],
+                       "gbs": [round(float(bb / (t * 1e-3) / 1e9), 1) for bb, t in zip(b, ms)],
+                       "op_gbs": round(float(b.sum() / (ms.sum() * 1e-3) / 1e9), 1)}
+    ms128 = np.array(passes["c128"]["ms"])
+    b128 = pass_bytes(core, 16)
+    k = int(np.argmax(ms128))
+    names = ["k_p1_zfwd", "k_py_yfwd", "k_px_xgreen", "k_py_yinv", "k_p5_zinv"]
+    traffic = None
+    tfile = os.path.join(ROOT, "profiles", "traffic_topo80.json")
+    if os.path.exists(tfile):
+        with open(tfile) as f:
+            traffic = json.load(f).get(names[k])
+    roof = {"bound": "hbm", "kernel": names[k], "achieved": round(float(b128[k] / (ms128[k] * 1e-3) / 1e9), 1),
+            "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(float(b128[k] / (ms128[k] * 1e-3) / 1e9 / HBM_PEAK_GBS), 4),
+            "traffic": traffic, "algorithmic_bytes": int(b128[k])}
+
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline(prob, targets, al, E, diel)
+
+    fp = results["fp64"]
+    line = {
+        "metric": "converged DDA solves/s (topology step: forward + adjoint, true residual <= 1e-8)",
+        "value": round(fp["solves_per_s"], 2),
+        "unit": "solves/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(fp["ms_per_step"], 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "c128",
+        "data": "synthetic",
+        "config": {"workload": "topo80: 80x80x16 slab, d=10nm, lambda=542nm, grid 160x160x32, "
+                               "forward + adjoint (100 targets) per step", "grid": list(core.grid),
+                   "dipoles": prob.N},
+        "latency_ms": {"p50": round(fp["p50_ms"], 3), "p99": round(fp["p99_ms"], 3)},
+        "fp64": fp,
+        "mixed": results["mixed"],
+        "mixed_vs_plain_fp64": round(results["mixed"]["solves_per_s"] / fp["solves_per_s"], 3),
+        "aproduct_passes": passes,
+        "roofline": roof,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line))
+
+
+if __name__ == "__main__":
+    main()

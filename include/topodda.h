@@ -113,6 +113,24 @@ int tdda_aproduct(tdda_op op, const double* b, double* y, int nrhs, const double
  * (x, y, z) for wavenumber K, as 12 doubles.                                */
 int tdda_green_tensor(double K, double x, double y, double z, double* out12);
 
+/* Host-side DDAModel set-up math (pure host code, no device needed). */
+/* Material rule of AProductCore.cpp:19-24: eps /= nback^2, +0.01 if |eps| <= 1.01
+ * (nmat complex values, in place).                                         */
+int tdda_material_rule(int nmat, double* material, double nback);
+/* Get_Alpha (tools.cpp:617-630): LDR polarizability. */
+int tdda_get_alpha(double lam, double K, double d, double diel_re, double diel_im, const double* nE3,
+                   const double* nK3, double* out2);
+/* Incident plane wave E[3i+c] = E0 nE[c] exp(i K d nK.R_i) (DDAModel.cpp:47-51). */
+int tdda_incident_field(int N, const int* R, double K, double d, double E0, const double* nK3, const double* nE3,
+                        double* E);
+/* Permittivity interpolation from the design values diel_old (3N) and the
+ * rule-adjusted material list, and al = 1/alpha (DDAModel.cpp:54-63,
+ * UpdateAlpha DDAModel.cpp:470-487).  diel may be NULL.                   */
+int tdda_polarizability(int n3, const double* diel_old, int nmat, const double* material, double lam, double K,
+                        double d, const double* nE3, const double* nK3, double* diel, double* al);
+/* Local field E_int = al P / ((2 + eps) / 3) (DDAModel.cpp:567-575). */
+int tdda_local_field(int n3, const double* al, const double* P, const double* diel, double* out);
+
 /* DDAModel device state for nrhs right-hand sides sharing one operator:
  * al, E, P per right-hand side, plus solver work space.                    */
 int tdda_model_create(tdda_op op, int nrhs, tdda_model* out);
@@ -121,8 +139,20 @@ int tdda_model_destroy(tdda_model m);
  * all nrhs vectors (nrhs * 3N complex); otherwise one right-hand side.     */
 int tdda_model_set(tdda_model m, int which, int rhs, const double* host);
 int tdda_model_get(tdda_model m, int which, int rhs, double* host);
+/* Sparse access to one right-hand side: get reads n entries (complex
+ * indices into the 3N vector) of a vector; set zeroes the vector and writes n
+ * entries.  For adjoint sources concentrated on a few dipoles, as in
+ * EvoDDAModel::devp (EvoDDAModel.cpp:353-391) feeding change_E
+ * (EvoDDAModel.cpp:596), without moving whole vectors over PCIe.          */
+int tdda_model_get_entries(tdda_model m, int which, int rhs, int n, const int* idx, double* vals);
+int tdda_model_set_entries(tdda_model m, int which, int rhs, int n, const int* idx, const double* vals);
 /* Y = A P + al o P on the device (DDAModel::Aproductwithalb). */
 int tdda_model_apply(tdda_model m, int precision);
+/* Measurement hook (no reference counterpart): runs reps device-resident
+ * applications Y = A P + al o P and returns the mean duration in ms of each
+ * of the five passes (z-forward, y-forward, x-fused Green product,
+ * y-inverse, z-inverse), timed with HIP events on the model's stream.     */
+int tdda_model_time_passes(tdda_model m, int precision, int reps, double* ms5);
 
 typedef struct {
     int max_it;        /* MAX_ITERATION of DDAModel::bicgstab             */

@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <complex>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -102,11 +103,14 @@ int env_int(const char* name, int dflt)
 // ---------------------------------------------------------------------------
 // handles
 // ---------------------------------------------------------------------------
+// Handles are reference counted so that destroying them in any order is
+// safe: a model keeps its operator alive, an operator keeps its context.
 struct tdda_ctx_s {
     int device;
     hipStream_t stream;
     size_t lds_budget;
     int ncu;
+    int refs;
 };
 
 struct TilePlan {
@@ -116,6 +120,7 @@ struct TilePlan {
 
 struct tdda_op_s {
     tdda_ctx ctx;
+    int refs = 1;
     int N, Nx, Ny, Nz, Mx, My, Mz, Hx, Hy, Hz;
     double d, lam, nback, K;
     int prec;
@@ -485,6 +490,40 @@ int solve_mixed(tdda_model_s* m, const tdda_solve_opts* o, tdda_solve_info* info
 
 }  // namespace
 
+template <typename T>
+static int time_passes_t(tdda_model m, JobTable<T>& jt, int reps, double* ms5)
+{
+    hipStream_t s = m->op->ctx->stream;
+    const int nj = m->nrhs;
+    hipEvent_t ev[6];
+    for (int k = 0; k < 6; k++) HIPCHK(hipEventCreate(&ev[k]));
+    double acc[5] = {0, 0, 0, 0, 0};
+    for (int r = 0; r < reps + 1; r++) {
+        HIPCHK(hipEventRecord(ev[0], s));
+        launch_p1<T>(jt, nj, IN_PLAIN, s);
+        HIPCHK(hipEventRecord(ev[1], s));
+        k_py<T, false><<<dim3(jt.ny), dim3(kThreads), jt.tp.lds_y, s>>>(jt.d, jt.ty, nj, 0);
+        HIPCHK(hipEventRecord(ev[2], s));
+        k_px<T><<<dim3(jt.nx), dim3(kThreads), jt.tp.lds_x, s>>>(jt.d, jt.tx, nj, 0);
+        HIPCHK(hipEventRecord(ev[3], s));
+        k_py<T, true><<<dim3(jt.ny), dim3(kThreads), jt.tp.lds_y, s>>>(jt.d, jt.ty, nj, 0);
+        HIPCHK(hipEventRecord(ev[4], s));
+        launch_p5<T>(jt, nj, OUT_PLAIN, 1, s);
+        HIPCHK(hipEventRecord(ev[5], s));
+        HIPCHK(hipEventSynchronize(ev[5]));
+        if (r == 0) continue;  // warm-up
+        for (int k = 0; k < 5; k++) {
+            float ms = 0.f;
+            HIPCHK(hipEventElapsedTime(&ms, ev[k], ev[k + 1]));
+            acc[k] += ms;
+        }
+    }
+    CHK(check_launch());
+    for (int k = 0; k < 6; k++) (void)hipEventDestroy(ev[k]);
+    for (int k = 0; k < 5; k++) ms5[k] = acc[k] / reps;
+    return TDDA_OK;
+}
+
 // ---------------------------------------------------------------------------
 // exported functions
 // ---------------------------------------------------------------------------
@@ -505,6 +544,7 @@ int tdda_ctx_create(int device, tdda_ctx* out)
     HIPCHK(hipGetDeviceProperties(&prop, device));
     tdda_ctx c = new tdda_ctx_s();
     c->device = device;
+    c->refs = 1;
     c->ncu = prop.multiProcessorCount;
     const size_t lim = std::min<size_t>(prop.sharedMemPerBlock, 65536);
     c->lds_budget = std::min<size_t>(lim, (size_t)env_int("TDDA_LDS_BUDGET", 49152));
@@ -517,13 +557,19 @@ int tdda_ctx_create(int device, tdda_ctx* out)
     return TDDA_OK;
 }
 
+static void ctx_release(tdda_ctx ctx)
+{
+    if (--ctx->refs > 0) return;
+    (void)hipSetDevice(ctx->device);
+    (void)hipStreamSynchronize(ctx->stream);
+    (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+}
+
 int tdda_ctx_destroy(tdda_ctx ctx)
 {
     if (!ctx) return TDDA_OK;
-    hipSetDevice(ctx->device);
-    hipStreamSynchronize(ctx->stream);
-    hipStreamDestroy(ctx->stream);
-    delete ctx;
+    ctx_release(ctx);
     return TDDA_OK;
 }
 
@@ -561,6 +607,95 @@ int tdda_green_tensor(double K, double x, double y, double z, double* out12)
     return TDDA_OK;
 }
 
+int tdda_material_rule(int nmat, double* material, double nback)
+{
+    if (nmat < 0 || (nmat > 0 && !material) || !(nback > 0)) return fail(TDDA_ERR_ARG, "bad material list");
+    for (int i = 0; i < nmat; i++) {
+        std::complex<double> m(material[2 * i], material[2 * i + 1]);
+        m = m / (nback * nback);
+        if (std::abs(m) <= 1.01) m = m + 0.01;
+        material[2 * i] = m.real();
+        material[2 * i + 1] = m.imag();
+    }
+    return TDDA_OK;
+}
+
+static std::complex<double> ldr_alpha(double K, double d, std::complex<double> diel, const double* nE,
+                                      const double* nK)
+{
+    const double b1 = -1.891531, b2 = 0.1648469, b3 = -1.7700004;
+    const std::complex<double> a1 = (3.0 * std::pow(d, 3) / (4.0 * M_PI)) * (diel - 1.0) / (diel + 2.0);
+    std::complex<double> res(0.0, (2.0 / 3.0) * std::pow(K * d, 3));
+    const double S = std::pow(nE[0] * nK[0], 2) + std::pow(nE[1] * nK[1], 2) + std::pow(nE[2] * nK[2], 2);
+    res = 1.0 + (a1 / std::pow(d, 3)) * ((b1 + diel * b2 + diel * b3 * S) * std::pow(K * d, 2) - res);
+    return a1 / res;
+}
+
+int tdda_get_alpha(double lam, double K, double d, double diel_re, double diel_im, const double* nE3,
+                   const double* nK3, double* out2)
+{
+    (void)lam;
+    if (!nE3 || !nK3 || !out2) return fail(TDDA_ERR_ARG, "null argument");
+    const std::complex<double> a = ldr_alpha(K, d, std::complex<double>(diel_re, diel_im), nE3, nK3);
+    out2[0] = a.real();
+    out2[1] = a.imag();
+    return TDDA_OK;
+}
+
+int tdda_incident_field(int N, const int* R, double K, double d, double E0, const double* nK3, const double* nE3,
+                        double* E)
+{
+    if (N < 0 || (N > 0 && (!R || !E)) || !nK3 || !nE3) return fail(TDDA_ERR_ARG, "null argument");
+    for (int i = 0; i < N; i++) {
+        const double ph = K * d * (nK3[0] * R[3 * i] + nK3[1] * R[3 * i + 1] + nK3[2] * R[3 * i + 2]);
+        const double c = std::cos(ph), sn = std::sin(ph);
+        for (int k = 0; k < 3; k++) {
+            E[2 * (3 * i + k)] = E0 * nE3[k] * c;
+            E[2 * (3 * i + k) + 1] = E0 * nE3[k] * sn;
+        }
+    }
+    return TDDA_OK;
+}
+
+int tdda_polarizability(int n3, const double* diel_old, int nmat, const double* material, double lam, double K,
+                        double d, const double* nE3, const double* nK3, double* diel, double* al)
+{
+    (void)lam;
+    if (n3 < 0 || (n3 > 0 && (!diel_old || !al)) || nmat <= 0 || !material || !nE3 || !nK3)
+        return fail(TDDA_ERR_ARG, "bad argument");
+    for (int i = 0; i < n3; i++) {
+        const double t = diel_old[i];
+        const int lf = (int)std::floor(t);
+        int ln = lf + 1;
+        if (lf >= 1) ln = lf;
+        if (lf < 0 || ln >= nmat) return fail(TDDA_ERR_ARG, "design value outside the material table");
+        const std::complex<double> mf(material[2 * lf], material[2 * lf + 1]);
+        const std::complex<double> mn(material[2 * ln], material[2 * ln + 1]);
+        const std::complex<double> e = mf + (t - (double)lf) * (mn - mf);
+        const std::complex<double> a = 1.0 / ldr_alpha(K, d, e, nE3, nK3);
+        if (diel) {
+            diel[2 * i] = e.real();
+            diel[2 * i + 1] = e.imag();
+        }
+        al[2 * i] = a.real();
+        al[2 * i + 1] = a.imag();
+    }
+    return TDDA_OK;
+}
+
+int tdda_local_field(int n3, const double* al, const double* P, const double* diel, double* out)
+{
+    if (n3 < 0 || (n3 > 0 && (!al || !P || !diel || !out))) return fail(TDDA_ERR_ARG, "null argument");
+    for (int i = 0; i < n3; i++) {
+        const std::complex<double> lf = (2.0 + std::complex<double>(diel[2 * i], diel[2 * i + 1])) / 3.0;
+        const std::complex<double> v =
+            std::complex<double>(al[2 * i], al[2 * i + 1]) * std::complex<double>(P[2 * i], P[2 * i + 1]) / lf;
+        out[2 * i] = v.real();
+        out[2 * i + 1] = v.imag();
+    }
+    return TDDA_OK;
+}
+
 int tdda_op_create(tdda_ctx ctx, const tdda_op_desc* desc, tdda_op* out)
 {
     if (!ctx || !desc || !out) return fail(TDDA_ERR_ARG, "null argument");
@@ -581,8 +716,24 @@ int tdda_op_create(tdda_ctx ctx, const tdda_op_desc* desc, tdda_op* out)
         if (M[a] > 4096) return fail(TDDA_ERR_UNSUPPORTED, "FFT length above 4096");
     }
     HIPCHK(hipSetDevice(ctx->device));
+    // bulk order (tools.cpp:497-510) on the full box needs no cell map
+    bool bulk = (box == D.N);
+    std::vector<int> cmap((size_t)box, -1);
+    for (int i = 0; i < D.N; i++) {
+        const int x = D.R[3 * i], y = D.R[3 * i + 1], z = D.R[3 * i + 2];
+        if (x < 0 || x >= D.Nx || y < 0 || y >= D.Ny || z < 0 || z >= D.Nz) {
+            return fail(TDDA_ERR_ARG, "site coordinate outside the Nx x Ny x Nz box");
+        }
+        const size_t cell = ((size_t)x * D.Ny + y) * D.Nz + z;
+        if (cmap[cell] != -1) {
+            return fail(TDDA_ERR_ARG, "two sites share one lattice cell");
+        }
+        cmap[cell] = i;
+        if ((size_t)i != cell) bulk = false;
+    }
     tdda_op op = new tdda_op_s();
     op->ctx = ctx;
+    ctx->refs++;
     op->N = D.N; op->Nx = D.Nx; op->Ny = D.Ny; op->Nz = D.Nz;
     op->Mx = M[0]; op->My = M[1]; op->Mz = M[2];
     op->Hx = M[0] / 2 + 1; op->Hy = M[1] / 2 + 1; op->Hz = M[2] / 2 + 1;
@@ -590,23 +741,6 @@ int tdda_op_create(tdda_ctx ctx, const tdda_op_desc* desc, tdda_op* out)
     op->K = 2.0 * M_PI / (D.lam / D.nback);  // AProductCore.cpp:17
     op->prec = prec;
     op->R.assign(D.R, D.R + 3 * (size_t)D.N);
-    // bulk order (tools.cpp:497-510) on the full box needs no cell map
-    bool bulk = (box == D.N);
-    std::vector<int> cmap((size_t)box, -1);
-    for (int i = 0; i < D.N; i++) {
-        const int x = D.R[3 * i], y = D.R[3 * i + 1], z = D.R[3 * i + 2];
-        if (x < 0 || x >= D.Nx || y < 0 || y >= D.Ny || z < 0 || z >= D.Nz) {
-            delete op;
-            return fail(TDDA_ERR_ARG, "site coordinate outside the Nx x Ny x Nz box");
-        }
-        const size_t cell = ((size_t)x * D.Ny + y) * D.Nz + z;
-        if (cmap[cell] != -1) {
-            delete op;
-            return fail(TDDA_ERR_ARG, "two sites share one lattice cell");
-        }
-        cmap[cell] = i;
-        if ((size_t)i != cell) bulk = false;
-    }
     auto cleanup = [&](int rc) {
         tdda_op_destroy(op);
         return rc;
@@ -624,8 +758,9 @@ int tdda_op_create(tdda_ctx ctx, const tdda_op_desc* desc, tdda_op* out)
         if (hipMalloc(&op->tw64[a], sizeof(Cx<double>) * M[a]) != hipSuccess ||
             hipMalloc(&op->tw32[a], sizeof(Cx<float>) * M[a]) != hipSuccess)
             return cleanup(fail(TDDA_ERR_OOM, "twiddles"));
-        hipMemcpy(op->tw64[a], t.data(), sizeof(Cx<double>) * M[a], hipMemcpyHostToDevice);
-        hipMemcpy(op->tw32[a], tf.data(), sizeof(Cx<float>) * M[a], hipMemcpyHostToDevice);
+        if (hipMemcpy(op->tw64[a], t.data(), sizeof(Cx<double>) * M[a], hipMemcpyHostToDevice) != hipSuccess ||
+            hipMemcpy(op->tw32[a], tf.data(), sizeof(Cx<float>) * M[a], hipMemcpyHostToDevice) != hipSuccess)
+            return cleanup(fail(TDDA_ERR_HIP, "twiddle upload"));
     }
     // Green tensor on the full grid, FP64, then its 3D forward transform
     // (AProductCore.cpp:54-229).
@@ -643,7 +778,7 @@ int tdda_op_create(tdda_ctx ctx, const tdda_op_desc* desc, tdda_op* out)
         int P = (int)(budget / (2 * 6 * ps * sizeof(Cx<double>)));
         P = std::max(1, std::min(P, 32));
         if (2 * 6 * (size_t)ps * sizeof(Cx<double>) > 65536) {
-            hipFree(full);
+            (void)hipFree(full);
             return cleanup(fail(TDDA_ERR_UNSUPPORTED, "z FFT length too large for the Green set-up pass"));
         }
         const int Q = M[0] * M[1];
@@ -669,7 +804,7 @@ int tdda_op_create(tdda_ctx ctx, const tdda_op_desc* desc, tdda_op* out)
     const unsigned gb = (unsigned)((Ht + kThreads - 1) / kThreads);
     if (prec & TDDA_C128) {
         if (hipMalloc(&op->green64, sizeof(Cx<double>) * Ht) != hipSuccess) {
-            hipFree(full);
+            (void)hipFree(full);
             return cleanup(fail(TDDA_ERR_OOM, "Green spectrum"));
         }
         k_green_octant<double><<<gb, kThreads, 0, s>>>(full, op->green64, M[0], M[1], M[2], op->Hx, op->Hy, op->Hz,
@@ -677,7 +812,7 @@ int tdda_op_create(tdda_ctx ctx, const tdda_op_desc* desc, tdda_op* out)
     }
     if (prec & TDDA_C64) {
         if (hipMalloc(&op->green32, sizeof(Cx<float>) * Ht) != hipSuccess) {
-            hipFree(full);
+            (void)hipFree(full);
             return cleanup(fail(TDDA_ERR_OOM, "Green spectrum"));
         }
         k_green_octant<float><<<gb, kThreads, 0, s>>>(full, op->green32, M[0], M[1], M[2], op->Hx, op->Hy, op->Hz,
@@ -685,7 +820,7 @@ int tdda_op_create(tdda_ctx ctx, const tdda_op_desc* desc, tdda_op* out)
     }
     hipError_t e = hipGetLastError();
     hipError_t e2 = hipStreamSynchronize(s);
-    hipFree(full);
+    (void)hipFree(full);
     if (e != hipSuccess || e2 != hipSuccess)
         return cleanup(fail(TDDA_ERR_HIP, std::string("Green set-up: ") + hipGetErrorString(e != hipSuccess ? e : e2)));
     op->tp64 = plan_tiles<double>(op, ctx->lds_budget);
@@ -694,20 +829,32 @@ int tdda_op_create(tdda_ctx ctx, const tdda_op_desc* desc, tdda_op* out)
     return TDDA_OK;
 }
 
+static void op_release(tdda_op op)
+{
+    if (--op->refs > 0) return;
+    (void)hipSetDevice(op->ctx->device);
+    (void)hipStreamSynchronize(op->ctx->stream);
+    (void)hipFree(op->cellmap);
+    (void)hipFree(op->green64);
+    (void)hipFree(op->green32);
+    for (int a = 0; a < 3; a++) {
+        (void)hipFree(op->tw64[a]);
+        (void)hipFree(op->tw32[a]);
+    }
+    tdda_ctx ctx = op->ctx;
+    delete op;
+    ctx_release(ctx);
+}
+
 int tdda_op_destroy(tdda_op op)
 {
     if (!op) return TDDA_OK;
-    hipSetDevice(op->ctx->device);
-    hipStreamSynchronize(op->ctx->stream);
-    if (op->scratch) tdda_model_destroy(op->scratch);
-    hipFree(op->cellmap);
-    hipFree(op->green64);
-    hipFree(op->green32);
-    for (int a = 0; a < 3; a++) {
-        hipFree(op->tw64[a]);
-        hipFree(op->tw32[a]);
+    if (op->scratch) {
+        tdda_model m = op->scratch;
+        op->scratch = nullptr;
+        tdda_model_destroy(m);
     }
-    delete op;
+    op_release(op);
     return TDDA_OK;
 }
 
@@ -730,6 +877,7 @@ int tdda_model_create(tdda_op op, int nrhs, tdda_model* out)
     HIPCHK(hipSetDevice(op->ctx->device));
     tdda_model m = new tdda_model_s();
     m->op = op;
+    op->refs++;
     m->nrhs = nrhs;
     m->n3 = 3 * (size_t)op->N;
     auto bail = [&](int rc) {
@@ -762,11 +910,13 @@ int tdda_model_create(tdda_op op, int nrhs, tdda_model* out)
 int tdda_model_destroy(tdda_model m)
 {
     if (!m) return TDDA_OK;
-    hipSetDevice(m->op->ctx->device);
-    hipStreamSynchronize(m->op->ctx->stream);
-    for (void* p : m->allocs) hipFree(p);
-    if (m->h_st) hipHostFree(m->h_st);
+    (void)hipSetDevice(m->op->ctx->device);
+    (void)hipStreamSynchronize(m->op->ctx->stream);
+    for (void* p : m->allocs) (void)hipFree(p);
+    if (m->h_st) (void)hipHostFree(m->h_st);
+    tdda_op op = m->op;
     delete m;
+    op_release(op);
     return TDDA_OK;
 }
 
@@ -818,6 +968,63 @@ int tdda_model_get(tdda_model m, int which, int rhs, double* host)
     HIPCHK(hipMemcpyAsync(host, src + off, cnt * sizeof(Cx<double>), hipMemcpyDeviceToHost, s));
     HIPCHK(hipStreamSynchronize(s));
     return TDDA_OK;
+}
+
+int tdda_model_get_entries(tdda_model m, int which, int rhs, int n, const int* idx, double* vals)
+{
+    if (!m || (n > 0 && (!idx || !vals))) return fail(TDDA_ERR_ARG, "null argument");
+    if (rhs < 0 || rhs >= m->nrhs || n < 0) return fail(TDDA_ERR_ARG, "rhs index or count out of range");
+    Cx<double>* src = vec64(m, which);
+    if (!src) return fail(TDDA_ERR_ARG, "unknown vector kind");
+    for (int k = 0; k < n; k++)
+        if (idx[k] < 0 || (size_t)idx[k] >= m->n3) return fail(TDDA_ERR_ARG, "entry index out of range");
+    HIPCHK(hipSetDevice(m->op->ctx->device));
+    hipStream_t s = m->op->ctx->stream;
+    src += (size_t)rhs * m->n3;
+    for (int k = 0; k < n; k++)
+        HIPCHK(hipMemcpyAsync(vals + 2 * k, src + idx[k], sizeof(Cx<double>), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    return TDDA_OK;
+}
+
+int tdda_model_set_entries(tdda_model m, int which, int rhs, int n, const int* idx, const double* vals)
+{
+    if (!m || (n > 0 && (!idx || !vals))) return fail(TDDA_ERR_ARG, "null argument");
+    if (rhs < 0 || rhs >= m->nrhs || n < 0) return fail(TDDA_ERR_ARG, "rhs index or count out of range");
+    if (which != TDDA_VEC_AL && which != TDDA_VEC_E && which != TDDA_VEC_P)
+        return fail(TDDA_ERR_ARG, "vector kind not settable");
+    for (int k = 0; k < n; k++)
+        if (idx[k] < 0 || (size_t)idx[k] >= m->n3) return fail(TDDA_ERR_ARG, "entry index out of range");
+    HIPCHK(hipSetDevice(m->op->ctx->device));
+    hipStream_t s = m->op->ctx->stream;
+    Cx<double>* dst = vec64(m, which) + (size_t)rhs * m->n3;
+    HIPCHK(hipMemsetAsync(dst, 0, m->n3 * sizeof(Cx<double>), s));
+    for (int k = 0; k < n; k++)
+        HIPCHK(hipMemcpyAsync(dst + idx[k], vals + 2 * k, sizeof(Cx<double>), hipMemcpyHostToDevice, s));
+    if (which == TDDA_VEC_AL && m->v32.al) {
+        const unsigned blocks = (unsigned)((m->n3 + kThreads - 1) / kThreads);
+        k_demote<<<blocks, kThreads, 0, s>>>(dst, m->v32.al + (size_t)rhs * m->n3, m->n3);
+        CHK(check_launch());
+    }
+    HIPCHK(hipStreamSynchronize(s));
+    return TDDA_OK;
+}
+
+int tdda_model_time_passes(tdda_model m, int precision, int reps, double* ms5)
+{
+    if (!m || !ms5 || reps <= 0) return fail(TDDA_ERR_ARG, "bad argument");
+    HIPCHK(hipSetDevice(m->op->ctx->device));
+    if (precision == TDDA_C128) {
+        if (!m->v64.x) return fail(TDDA_ERR_STATE, "operator built without TDDA_C128");
+        return time_passes_t<double>(m, m->j64, reps, ms5);
+    }
+    if (precision == TDDA_C64) {
+        if (!m->v32.x) return fail(TDDA_ERR_STATE, "operator built without TDDA_C64");
+        const size_t n = m->n3 * m->nrhs;
+        k_demote<<<(unsigned)((n + kThreads - 1) / kThreads), kThreads, 0, m->op->ctx->stream>>>(m->v64.x, m->v32.x, n);
+        return time_passes_t<float>(m, m->j32, reps, ms5);
+    }
+    return fail(TDDA_ERR_ARG, "precision must be TDDA_C128 or TDDA_C64");
 }
 
 static int model_apply(tdda_model m, int precision, int diag)

@@ -28,7 +28,9 @@ EXPORTS = [
     "tdda_last_error", "tdda_version", "tdda_ctx_create", "tdda_ctx_destroy", "tdda_ctx_sync",
     "tdda_op_create", "tdda_op_destroy", "tdda_op_info", "tdda_aproduct", "tdda_green_tensor",
     "tdda_model_create", "tdda_model_destroy", "tdda_model_set", "tdda_model_get", "tdda_model_apply",
-    "tdda_solve_opts_default", "tdda_model_solve",
+    "tdda_solve_opts_default", "tdda_model_solve", "tdda_model_get_entries", "tdda_model_set_entries",
+    "tdda_model_time_passes", "tdda_material_rule", "tdda_get_alpha", "tdda_incident_field",
+    "tdda_polarizability", "tdda_local_field",
 ]
 
 
@@ -80,6 +82,15 @@ def lib():
         L.tdda_model_set.argtypes = [vp, ctypes.c_int, ctypes.c_int, dp]
         L.tdda_model_get.argtypes = [vp, ctypes.c_int, ctypes.c_int, dp]
         L.tdda_model_apply.argtypes = [vp, ctypes.c_int]
+        ip = ctypes.POINTER(ctypes.c_int)
+        L.tdda_model_get_entries.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ip, dp]
+        L.tdda_model_set_entries.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ip, dp]
+        L.tdda_model_time_passes.argtypes = [vp, ctypes.c_int, ctypes.c_int, dp]
+        L.tdda_material_rule.argtypes = [ctypes.c_int, dp, ctypes.c_double]
+        L.tdda_get_alpha.argtypes = [ctypes.c_double] * 5 + [dp, dp, dp]
+        L.tdda_incident_field.argtypes = [ctypes.c_int, ip] + [ctypes.c_double] * 3 + [dp, dp, dp]
+        L.tdda_polarizability.argtypes = [ctypes.c_int, dp, ctypes.c_int, dp] + [ctypes.c_double] * 3 + [dp] * 4
+        L.tdda_local_field.argtypes = [ctypes.c_int, dp, dp, dp, dp]
         L.tdda_solve_opts_default.argtypes = [ctypes.POINTER(SolveOpts)]
         L.tdda_solve_opts_default.restype = None
         L.tdda_model_solve.argtypes = [vp, ctypes.POINTER(SolveOpts), ctypes.POINTER(SolveInfo)]
@@ -121,6 +132,76 @@ def green_tensor(K, x, y, z):
     out = np.zeros(12)
     _check(lib().tdda_green_tensor(K, x, y, z, _dptr(out)))
     return out.view(np.complex128)
+
+
+def _c(a):
+    return np.ascontiguousarray(a, dtype=np.complex128)
+
+
+def _f(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def wavenumber(lam, nback=1.0):
+    """K = 2 pi / (lam / nback) (AProductCore.cpp:17)."""
+    return 2 * math.pi / (lam / nback)
+
+
+def material_rule(material, nback=1.0):
+    """AProductCore.cpp:19-24."""
+    m = _c(material).copy()
+    _check(lib().tdda_material_rule(len(m), _dptr(m.view(np.float64)), nback))
+    return m
+
+
+def get_alpha(lam, K, d, diel, nE, nK):
+    """Get_Alpha (tools.cpp:617-630)."""
+    out = np.zeros(2)
+    _check(lib().tdda_get_alpha(lam, K, d, complex(diel).real, complex(diel).imag, _dptr(_f(nE)), _dptr(_f(nK)),
+                                _dptr(out)))
+    return complex(out[0], out[1])
+
+
+def incident_field(R, K, d, nK, nE, E0=1.0):
+    """DDAModel.cpp:47-51."""
+    R = np.ascontiguousarray(R, dtype=np.int32)
+    N = len(R) // 3
+    E = np.zeros(3 * N, np.complex128)
+    _check(lib().tdda_incident_field(N, R.ctypes.data_as(ctypes.POINTER(ctypes.c_int)), K, d, E0, _dptr(_f(nK)),
+                                     _dptr(_f(nE)), _dptr(E.view(np.float64))))
+    return E
+
+
+def polarizability(diel_old, material, lam, K, d, nE, nK):
+    """(eps, 1/alpha) per component (DDAModel.cpp:54-63)."""
+    t = _f(diel_old)
+    m = _c(material)
+    diel = np.zeros(len(t), np.complex128)
+    al = np.zeros(len(t), np.complex128)
+    _check(lib().tdda_polarizability(len(t), _dptr(t), len(m), _dptr(m.view(np.float64)), lam, K, d, _dptr(_f(nE)),
+                                     _dptr(_f(nK)), _dptr(diel.view(np.float64)), _dptr(al.view(np.float64))))
+    return diel, al
+
+
+def local_field(al, P, diel):
+    """DDAModel.cpp:567-575."""
+    al, P, diel = _c(al), _c(P), _c(diel)
+    out = np.zeros_like(P)
+    _check(lib().tdda_local_field(len(P), _dptr(al.view(np.float64)), _dptr(P.view(np.float64)),
+                                  _dptr(diel.view(np.float64)), _dptr(out.view(np.float64))))
+    return out
+
+
+def build_a_bulk(Nx, Ny, Nz):
+    """Lattice sites in x-major, then y, then z order (tools.cpp:497-510)."""
+    x, y, z = np.meshgrid(np.arange(Nx), np.arange(Ny), np.arange(Nz), indexing="ij")
+    return np.stack([x.ravel(), y.ravel(), z.ravel()], axis=1).astype(np.int32).reshape(-1)
+
+
+def ne_perp_in_xz(theta):
+    """p-polarisation perpendicular to k = (sin t, 0, cos t) in the xz plane
+    with non-negative x component (nEPerpinXZ, tools.cpp:686-709, phi = 0)."""
+    return np.array([math.cos(theta), 0.0, -math.sin(theta)])
 
 
 class AProductCore:
@@ -200,6 +281,24 @@ class DeviceModel:
         out = np.zeros(n, np.complex128)
         _check(lib().tdda_model_get(self.h, which, rhs, _dptr(out.view(np.float64))))
         return out.reshape(self.nrhs, self.n3) if rhs < 0 else out
+
+    def get_entries(self, which, rhs, idx):
+        idx = np.ascontiguousarray(idx, dtype=np.int32)
+        out = np.zeros(len(idx), np.complex128)
+        _check(lib().tdda_model_get_entries(self.h, which, rhs, len(idx), idx.ctypes.data_as(ctypes.POINTER(ctypes.c_int)),
+                                            _dptr(out.view(np.float64))))
+        return out
+
+    def set_entries(self, which, rhs, idx, vals):
+        idx = np.ascontiguousarray(idx, dtype=np.int32)
+        v = np.ascontiguousarray(vals, dtype=np.complex128)
+        _check(lib().tdda_model_set_entries(self.h, which, rhs, len(idx), idx.ctypes.data_as(ctypes.POINTER(ctypes.c_int)),
+                                            _dptr(v.view(np.float64))))
+
+    def time_passes(self, precision=C128, reps=10):
+        out = np.zeros(5)
+        _check(lib().tdda_model_time_passes(self.h, precision, reps, _dptr(out)))
+        return out
 
     def apply(self, precision=C128):
         _check(lib().tdda_model_apply(self.h, precision))
