@@ -1,0 +1,43 @@
+"""GPU test of the C++ host classes (AProductCore, DDAModel, EvoDDAModel over
+the C ABI): the dda_host_check driver runs a forward solve, an adjoint solve
+(EvoDDAModel.cpp:596-608 order) and Aproductwithalb; results are compared with
+the oracle's restatement of the same steps."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+EXE = os.path.join(ROOT, "topo-dda-forwin64_amd", "dda_host_check")
+
+
+def test_host_classes_forward_adjoint(tmp_path):
+    Nx, Ny, Nz, d, lam = 10, 8, 6, 10.0, 542.0
+    out = tmp_path / "host.bin"
+    print("dda_host_check 10x8x6", flush=True)
+    r = subprocess.run([EXE, str(out), str(Nx), str(Ny), str(Nz), str(d), str(lam), "5"], capture_output=True,
+                       text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    raw = out.read_bytes()
+    N, it_f, it_a = np.frombuffer(raw[:12], np.int32)
+    n3 = 3 * N
+    vecs = np.frombuffer(raw[12:12 + 6 * n3 * 16], np.complex128).reshape(6, n3)
+    Pf, lam_adj, y, al, E, diel = vecs
+    nt = int(np.frombuffer(raw[12 + 6 * n3 * 16:16 + 6 * n3 * 16], np.int32)[0])
+    targets = np.frombuffer(raw[16 + 6 * n3 * 16:], np.int32)[:nt]
+    R = O.build_a_bulk(Nx, Ny, Nz)
+    K = O.wavenumber(lam)
+    op = O.OrcOperator(R, Nx, Ny, Nz, d, K, grid=(20, 16, 12))
+    assert O.rel_l2(y, op.apply(E, al)) < 1e-12
+    Pc, st, it, _ = op.bicgstab(al, E, max_it=1000, tol=1e-10)
+    assert st == 0 and it == it_f and O.rel_l2(Pf, Pc) < 1e-10
+    Eint = O.local_field(al, Pc, diel)
+    devp = np.zeros(n3, complex)
+    for t in targets:
+        devp[3 * t:3 * t + 3] = np.conj(Eint[3 * t:3 * t + 3])
+    Lc, st, it, _ = op.bicgstab(al, devp, max_it=1000, tol=1e-10)
+    assert st == 0 and it == it_a and O.rel_l2(lam_adj, Lc) < 1e-9

@@ -1,0 +1,44 @@
+// CoreStructure.h -- the geometry/design data contract the solve path reads
+// (CoreStructure.h:12-50 of the reference): dipole count N, box extents
+// Nx/Ny/Nz, lattice spacing d, integer lattice coordinates R (3N) and the
+// per-component design values diel_old (3N, t in [0, 1] or material labels).
+//
+// The reference builds it from Space/SpacePara (geometry files, symmetry,
+// parameter maps); that preprocessing is not part of the solve path, so this
+// class is built directly from R and diel_old.
+#pragma once
+#include <stdexcept>
+
+#include "dda_types.h"
+
+namespace topodda {
+
+class CoreStructure {
+public:
+    CoreStructure(int Nx, int Ny, int Nz, double d, const VectorXi& R, const VectorXd& diel_old);
+    // Full Nx x Ny x Nz box in build_a_bulk order (tools.cpp:497-510).
+    static CoreStructure Bulk(int Nx, int Ny, int Nz, double d, const VectorXd& diel_old);
+    static VectorXi build_a_bulk(int Nx, int Ny, int Nz);
+
+    // Replace the design (the effect of UpdateStr, CoreStructure.cpp:99-109).
+    void UpdateStr(const VectorXd& diel_old_new);
+    void UpdateStrSingle(int idx, double value);  // sets the 3 components of dipole idx
+
+    int get_N() const { return N; }
+    int get_Nx() const { return Nx; }
+    int get_Ny() const { return Ny; }
+    int get_Nz() const { return Nz; }
+    double get_d() const { return d; }
+    VectorXi* get_R() { return &R; }
+    VectorXd* get_diel_old() { return &diel_old; }
+    VectorXd* get_diel_old_max() { return &diel_old_max; }
+
+private:
+    int N, Nx, Ny, Nz;
+    double d;
+    VectorXi R;
+    VectorXd diel_old;
+    VectorXd diel_old_max;
+};
+
+}  // namespace topodda

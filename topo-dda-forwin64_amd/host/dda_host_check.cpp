@@ -1,0 +1,66 @@
+// dda_host_check -- drives the host classes (CoreStructure, AProductCore,
+// DDAModel, EvoDDAModel) through one forward and one adjoint solve and one
+// Aproductwithalb, writing the results for comparison with the oracle.
+//
+// usage: dda_host_check OUT.bin Nx Ny Nz d lam seed
+// OUT.bin layout (little endian): int32 N, int32 it_fwd, int32 it_adj,
+// then complex128 vectors P_fwd, lambda_adj, Aproductwithalb(E), al, E, diel (3N each),
+// then int32 adjoint target count and the targets.
+#include <cstdio>
+#include <cstdlib>
+#include <random>
+
+#include "EvoDDAModel.h"
+
+using namespace topodda;
+
+int main(int argc, char** argv)
+{
+    if (argc < 8) {
+        std::fprintf(stderr, "usage: %s OUT.bin Nx Ny Nz d lam seed\n", argv[0]);
+        return 2;
+    }
+    const int Nx = atoi(argv[2]), Ny = atoi(argv[3]), Nz = atoi(argv[4]);
+    const double d = atof(argv[5]), lam = atof(argv[6]);
+    const unsigned seed = (unsigned)atoi(argv[7]);
+    const int N = Nx * Ny * Nz;
+    std::mt19937_64 rng(seed);
+    std::uniform_real_distribution<double> U(0.0, 1.0);
+    VectorXd diel_old = VectorXd::Zero(3 * (std::size_t)N);
+    for (int i = 0; i < N; i++) {
+        const double t = U(rng);
+        for (int c = 0; c < 3; c++) diel_old(3 * (std::size_t)i + c) = t;
+    }
+    CoreStructure CStr = CoreStructure::Bulk(Nx, Ny, Nz, d, diel_old);
+    VectorXcd material = VectorXcd::Zero(2);
+    material(0) = cd(1.0, 0.0);
+    material(1) = cd(2.25, 0.01);
+    AProductCore Core(&CStr, lam, material, 1.0, "LDR");
+    DDAModel model(&Core, Vector3d(0, 0, 1), 1.0, Vector3d(1, 0, 0), false);
+    EvoDDAModel evo(&CStr, {&model});
+    evo.ForwardSolve(1000, 1e-10);
+    const int it_fwd = model.get_ITERATION();
+    VectorXcd Pf = *model.get_P();
+    // adjoint source: conj(E_int) at 8 targets
+    VectorXcd devp = VectorXcd::Zero(3 * (std::size_t)N);
+    std::vector<int> targets;
+    for (int k = 0; k < 8; k++) targets.push_back((int)(rng() % (unsigned)N));
+    for (int t : targets)
+        for (int c = 0; c < 3; c++) devp(3 * (std::size_t)t + c) = std::conj((*model.get_Einternal())(3 * (std::size_t)t + c));
+    VectorXcd lambdaT = evo.AdjointSolve(0, devp, 1000, 1e-10);
+    const int it_adj = model.get_ITERATION();
+    VectorXcd E = *model.get_E();
+    VectorXcd y = model.Aproductwithalb(E);
+    FILE* f = std::fopen(argv[1], "wb");
+    if (!f) return 3;
+    const int hdr[3] = {N, it_fwd, it_adj};
+    std::fwrite(hdr, sizeof(int), 3, f);
+    for (const VectorXcd* v : {&Pf, &lambdaT, &y, model.get_al(), &E, model.get_diel()})
+        std::fwrite(v->data(), sizeof(cd), v->size(), f);
+    const int nt = (int)targets.size();
+    std::fwrite(&nt, sizeof(int), 1, f);
+    std::fwrite(targets.data(), sizeof(int), nt, f);
+    std::fclose(f);
+    std::printf("{\"N\": %d, \"it_fwd\": %d, \"it_adj\": %d}\n", N, it_fwd, it_adj);
+    return 0;
+}
