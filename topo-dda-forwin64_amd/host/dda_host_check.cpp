@@ -3,7 +3,7 @@
 // Aproductwithalb, writing the results for comparison with the oracle.
 //
 // usage: dda_host_check OUT.bin Nx Ny Nz d lam seed
-// OUT.bin layout (little endian): int32 N, int32 it_fwd, int32 it_adj,
+// OUT.bin layout (little endian): int32 N, int32 it_fwd, int32 it_adj, int32 grid[3],
 // then complex128 vectors P_fwd, lambda_adj, Aproductwithalb(E), al, E, diel (3N each),
 // then int32 adjoint target count and the targets.
 #include <cstdio>
@@ -53,8 +53,9 @@ int main(int argc, char** argv)
     VectorXcd y = model.Aproductwithalb(E);
     FILE* f = std::fopen(argv[1], "wb");
     if (!f) return 3;
-    const int hdr[3] = {N, it_fwd, it_adj};
-    std::fwrite(hdr, sizeof(int), 3, f);
+    int hdr[6] = {N, it_fwd, it_adj, 0, 0, 0};
+    check(tdda_op_info(Core.handle(), hdr + 3, nullptr));
+    std::fwrite(hdr, sizeof(int), 6, f);
     for (const VectorXcd* v : {&Pf, &lambdaT, &y, model.get_al(), &E, model.get_diel()})
         std::fwrite(v->data(), sizeof(cd), v->size(), f);
     const int nt = (int)targets.size();
