@@ -165,7 +165,15 @@ struct tdda_model_s {
     JobTable<double> j64;  // FP64 solver and operator applications
     JobTable<double> jo;   // FP64 outer residual of the mixed solver
     JobTable<float> j32;   // complex64 inner solver
-    SolverState* h_st = nullptr;  // pinned host mirror, 3 * nrhs
+    SolverState* h_st = nullptr;    // pinned host mirror, 3 * nrhs
+    SolverState* h_poll = nullptr;  // pinned status slots for asynchronous polling, 2 * nrhs
+    hipEvent_t ev[2] = {nullptr, nullptr};
+    // staging for sparse entry access
+    int cap_entries = 0;
+    int* d_idx = nullptr;
+    Cx<double>* d_vals = nullptr;
+    int* h_idx = nullptr;
+    Cx<double>* h_vals = nullptr;
     std::vector<void*> allocs;
 };
 
@@ -356,25 +364,73 @@ int check_launch()
 
 // Runs BiCGSTAB iterations for every running job until each has stopped.
 // States must already be initialised (first = 1) and r, r0, x, rho set.
+// Chunks of iterations are queued ahead of the status they depend on: while
+// one chunk runs, the host reads the status copied after the previous chunk,
+// so the device does not idle between chunks.  The size of the next chunk
+// comes from each job's observed residual decay; chunks queued after every
+// job has stopped run as no-ops (each kernel returns on the stop flag).
 template <typename T>
 int run_iterations(tdda_model_s* m, JobTable<T>& jt, SolverState* hst, int max_total, int check_every)
 {
     hipStream_t s = m->op->ctx->stream;
     const int nj = m->nrhs;
-    int done = 0;
-    int chunk = check_every > 0 ? check_every : 4;
-    while (done <= max_total + 1) {
-        for (int k = 0; k < chunk; k++) launch_iteration<T>(jt, nj, s);
-        CHK(check_launch());
-        done += chunk;
-        HIPCHK(hipMemcpyAsync(hst, jt.st, sizeof(SolverState) * nj, hipMemcpyDeviceToHost, s));
-        HIPCHK(hipStreamSynchronize(s));
+    SolverState* slot[2] = {m->h_poll, m->h_poll + nj};
+    bool busy[2] = {false, false};
+    int upto[2] = {0, 0};  // iterations (including the first step) issued when slot k was filled
+    int issued = 0;
+    const int limit = max_total + 1;
+    auto issue = [&](int k, int chunk) -> int {
+        chunk = std::max(1, std::min(chunk, limit - issued));
+        for (int i = 0; i < chunk; i++) launch_iteration<T>(jt, nj, s);
+        issued += chunk;
+        HIPCHK(hipMemcpyAsync(slot[k], jt.st, sizeof(SolverState) * nj, hipMemcpyDeviceToHost, s));
+        HIPCHK(hipEventRecord(m->ev[k], s));
+        busy[k] = true;
+        upto[k] = issued;
+        return check_launch();
+    };
+    std::vector<double> prev_err(nj, -1.0);
+    std::vector<int> prev_it(nj, 0);
+    int cur = 0;
+    CHK(issue(0, check_every > 0 ? check_every : 4));
+    if (issued < limit) CHK(issue(1, check_every > 0 ? check_every : 4));
+    while (true) {
+        HIPCHK(hipEventSynchronize(m->ev[cur]));
+        busy[cur] = false;
+        const SolverState* S = slot[cur];
         bool running = false;
-        for (int j = 0; j < nj; j++) running |= (hst[j].status == ST_RUNNING);
-        if (!running) break;
-        if (check_every <= 0) chunk = std::min(32, std::max(4, done / 3));
+        int need = 0;
+        for (int j = 0; j < nj; j++) {
+            if (S[j].status != ST_RUNNING) continue;
+            running = true;
+            int rem = 4;
+            if (prev_err[j] > 0 && S[j].iter > prev_it[j] && S[j].err < prev_err[j] && S[j].err > 0) {
+                const double rate = std::pow(S[j].err / prev_err[j], 1.0 / (S[j].iter - prev_it[j]));
+                if (rate < 0.999) rem = (int)std::ceil(std::log(S[j].tol / S[j].err) / std::log(rate));
+            }
+            rem = std::max(1, std::min(rem, 64));
+            prev_err[j] = S[j].err;
+            prev_it[j] = S[j].iter;
+            need = std::max(need, rem);
+        }
+        if (!running || issued >= limit) {
+            if (!running || !busy[1 - cur]) {
+                memcpy(hst, S, sizeof(SolverState) * nj);
+                if (!running) return TDDA_OK;
+                // iteration cap reached with jobs still running: the device
+                // marks them ST_MAXIT in the last queued chunk
+                HIPCHK(hipStreamSynchronize(s));
+                HIPCHK(hipMemcpy(hst, jt.st, sizeof(SolverState) * nj, hipMemcpyDeviceToHost));
+                return TDDA_OK;
+            }
+            cur = 1 - cur;
+            continue;
+        }
+        const int in_flight = busy[1 - cur] ? upto[1 - cur] - upto[cur] : 0;
+        const int more = check_every > 0 ? check_every : std::min(32, need - in_flight);
+        if (more > 0 || !busy[1 - cur]) CHK(issue(cur, std::max(more, 1)));
+        if (busy[1 - cur]) cur = 1 - cur;
     }
-    return TDDA_OK;
 }
 
 template <typename T>
@@ -900,8 +956,12 @@ int tdda_model_create(tdda_op op, int nrhs, tdda_model* out)
         if ((rc = build_table<float>(m, m->j32, m->v32, op->green32, tw, op->tp32, m->v32.r, m->v32.r0)) != TDDA_OK)
             return bail(rc);
     }
-    if (hipHostMalloc((void**)&m->h_st, sizeof(SolverState) * 3 * nrhs, hipHostMallocDefault) != hipSuccess)
+    if (hipHostMalloc((void**)&m->h_st, sizeof(SolverState) * 3 * nrhs, hipHostMallocDefault) != hipSuccess ||
+        hipHostMalloc((void**)&m->h_poll, sizeof(SolverState) * 2 * nrhs, hipHostMallocDefault) != hipSuccess)
         return bail(fail(TDDA_ERR_OOM, "pinned host state"));
+    for (int k = 0; k < 2; k++)
+        if (hipEventCreateWithFlags(&m->ev[k], hipEventDisableTiming) != hipSuccess)
+            return bail(fail(TDDA_ERR_HIP, "event creation"));
     if (hipStreamSynchronize(op->ctx->stream) != hipSuccess) return bail(fail(TDDA_ERR_HIP, "model set-up"));
     *out = m;
     return TDDA_OK;
@@ -914,6 +974,13 @@ int tdda_model_destroy(tdda_model m)
     (void)hipStreamSynchronize(m->op->ctx->stream);
     for (void* p : m->allocs) (void)hipFree(p);
     if (m->h_st) (void)hipHostFree(m->h_st);
+    if (m->h_poll) (void)hipHostFree(m->h_poll);
+    for (int k = 0; k < 2; k++)
+        if (m->ev[k]) (void)hipEventDestroy(m->ev[k]);
+    if (m->d_idx) (void)hipFree(m->d_idx);
+    if (m->d_vals) (void)hipFree(m->d_vals);
+    if (m->h_idx) (void)hipHostFree(m->h_idx);
+    if (m->h_vals) (void)hipHostFree(m->h_vals);
     tdda_op op = m->op;
     delete m;
     op_release(op);
@@ -970,7 +1037,28 @@ int tdda_model_get(tdda_model m, int which, int rhs, double* host)
     return TDDA_OK;
 }
 
-int tdda_model_get_entries(tdda_model m, int which, int rhs, int n, const int* idx, double* vals)
+static int ensure_entries(tdda_model m, int n)
+{
+    if (n <= m->cap_entries) return TDDA_OK;
+    int cap = std::max(n, 2 * m->cap_entries);
+    if (m->d_idx) (void)hipFree(m->d_idx);
+    if (m->d_vals) (void)hipFree(m->d_vals);
+    if (m->h_idx) (void)hipHostFree(m->h_idx);
+    if (m->h_vals) (void)hipHostFree(m->h_vals);
+    m->d_idx = nullptr;
+    m->d_vals = nullptr;
+    m->h_idx = nullptr;
+    m->h_vals = nullptr;
+    m->cap_entries = 0;
+    HIPCHK(hipMalloc(&m->d_idx, sizeof(int) * cap));
+    HIPCHK(hipMalloc(&m->d_vals, sizeof(Cx<double>) * cap));
+    HIPCHK(hipHostMalloc((void**)&m->h_idx, sizeof(int) * cap, hipHostMallocDefault));
+    HIPCHK(hipHostMalloc((void**)&m->h_vals, sizeof(Cx<double>) * cap, hipHostMallocDefault));
+    m->cap_entries = cap;
+    return TDDA_OK;
+}
+
+extern "C" int tdda_model_get_entries(tdda_model m, int which, int rhs, int n, const int* idx, double* vals)
 {
     if (!m || (n > 0 && (!idx || !vals))) return fail(TDDA_ERR_ARG, "null argument");
     if (rhs < 0 || rhs >= m->nrhs || n < 0) return fail(TDDA_ERR_ARG, "rhs index or count out of range");
@@ -978,16 +1066,21 @@ int tdda_model_get_entries(tdda_model m, int which, int rhs, int n, const int* i
     if (!src) return fail(TDDA_ERR_ARG, "unknown vector kind");
     for (int k = 0; k < n; k++)
         if (idx[k] < 0 || (size_t)idx[k] >= m->n3) return fail(TDDA_ERR_ARG, "entry index out of range");
+    if (n == 0) return TDDA_OK;
     HIPCHK(hipSetDevice(m->op->ctx->device));
+    CHK(ensure_entries(m, n));
     hipStream_t s = m->op->ctx->stream;
-    src += (size_t)rhs * m->n3;
-    for (int k = 0; k < n; k++)
-        HIPCHK(hipMemcpyAsync(vals + 2 * k, src + idx[k], sizeof(Cx<double>), hipMemcpyDeviceToHost, s));
+    memcpy(m->h_idx, idx, sizeof(int) * n);
+    HIPCHK(hipMemcpyAsync(m->d_idx, m->h_idx, sizeof(int) * n, hipMemcpyHostToDevice, s));
+    k_gather<<<(n + kThreads - 1) / kThreads, kThreads, 0, s>>>(src + (size_t)rhs * m->n3, m->d_idx, n, m->d_vals);
+    CHK(check_launch());
+    HIPCHK(hipMemcpyAsync(m->h_vals, m->d_vals, sizeof(Cx<double>) * n, hipMemcpyDeviceToHost, s));
     HIPCHK(hipStreamSynchronize(s));
+    memcpy(vals, m->h_vals, sizeof(Cx<double>) * n);
     return TDDA_OK;
 }
 
-int tdda_model_set_entries(tdda_model m, int which, int rhs, int n, const int* idx, const double* vals)
+extern "C" int tdda_model_set_entries(tdda_model m, int which, int rhs, int n, const int* idx, const double* vals)
 {
     if (!m || (n > 0 && (!idx || !vals))) return fail(TDDA_ERR_ARG, "null argument");
     if (rhs < 0 || rhs >= m->nrhs || n < 0) return fail(TDDA_ERR_ARG, "rhs index or count out of range");
@@ -996,11 +1089,18 @@ int tdda_model_set_entries(tdda_model m, int which, int rhs, int n, const int* i
     for (int k = 0; k < n; k++)
         if (idx[k] < 0 || (size_t)idx[k] >= m->n3) return fail(TDDA_ERR_ARG, "entry index out of range");
     HIPCHK(hipSetDevice(m->op->ctx->device));
+    CHK(ensure_entries(m, std::max(n, 1)));
     hipStream_t s = m->op->ctx->stream;
     Cx<double>* dst = vec64(m, which) + (size_t)rhs * m->n3;
     HIPCHK(hipMemsetAsync(dst, 0, m->n3 * sizeof(Cx<double>), s));
-    for (int k = 0; k < n; k++)
-        HIPCHK(hipMemcpyAsync(dst + idx[k], vals + 2 * k, sizeof(Cx<double>), hipMemcpyHostToDevice, s));
+    if (n > 0) {
+        memcpy(m->h_idx, idx, sizeof(int) * n);
+        memcpy(m->h_vals, vals, sizeof(Cx<double>) * n);
+        HIPCHK(hipMemcpyAsync(m->d_idx, m->h_idx, sizeof(int) * n, hipMemcpyHostToDevice, s));
+        HIPCHK(hipMemcpyAsync(m->d_vals, m->h_vals, sizeof(Cx<double>) * n, hipMemcpyHostToDevice, s));
+        k_scatter<<<(n + kThreads - 1) / kThreads, kThreads, 0, s>>>(dst, m->d_idx, n, m->d_vals);
+        CHK(check_launch());
+    }
     if (which == TDDA_VEC_AL && m->v32.al) {
         const unsigned blocks = (unsigned)((m->n3 + kThreads - 1) / kThreads);
         k_demote<<<blocks, kThreads, 0, s>>>(dst, m->v32.al + (size_t)rhs * m->n3, m->n3);
@@ -1009,6 +1109,7 @@ int tdda_model_set_entries(tdda_model m, int which, int rhs, int n, const int* i
     HIPCHK(hipStreamSynchronize(s));
     return TDDA_OK;
 }
+
 
 int tdda_model_time_passes(tdda_model m, int precision, int reps, double* ms5)
 {

@@ -632,6 +632,21 @@ __global__ void __launch_bounds__(kThreads) k_accum(Cx<double>* __restrict__ x, 
     }
 }
 
+// Sparse entry access: out[k] = v[idx[k]] and v[idx[k]] = vals[k].
+__global__ void __launch_bounds__(kThreads) k_gather(const Cx<double>* __restrict__ v, const int* __restrict__ idx,
+                                                     int n, Cx<double>* __restrict__ out)
+{
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < n) out[k] = v[idx[k]];
+}
+
+__global__ void __launch_bounds__(kThreads) k_scatter(Cx<double>* __restrict__ v, const int* __restrict__ idx, int n,
+                                                      const Cx<double>* __restrict__ vals)
+{
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < n) v[idx[k]] = vals[k];
+}
+
 // ---------------------------------------------------------------------------
 // Green spectrum set-up (AProductCore.cpp:54-229, built on the device).
 // ---------------------------------------------------------------------------

@@ -1,0 +1,64 @@
+#!/usr/bin/env python3
+"""Profiling driver: runs the operator passes of one workload a fixed number
+of times, plus one device-to-device copy of known size, so that a
+`rocprofv3 --pmc` run yields per-dispatch counters for each pass and a
+calibration point for the counters (the copy moves exactly --copy-mib MiB
+in and out).
+
+usage: python3 topo-dda-forwin64_amd/profile_passes.py [--workload topo80|mixed128]
+       [--precision c128|c64] [--reps 20] [--copy-mib 256]
+"""
+import argparse
+import ctypes
+import json
+import os
+import sys
+
+import numpy as np
+
+sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
+import topodda as T  # noqa: E402
+import workloads as W  # noqa: E402
+
+
+def copy_calibration(mib):
+    hip = ctypes.CDLL("libamdhip64.so")
+    n = mib << 20
+    a, b = ctypes.c_void_p(), ctypes.c_void_p()
+    assert hip.hipMalloc(ctypes.byref(a), ctypes.c_size_t(n)) == 0
+    assert hip.hipMalloc(ctypes.byref(b), ctypes.c_size_t(n)) == 0
+    assert hip.hipMemset(a, 1, ctypes.c_size_t(n)) == 0
+    assert hip.hipMemcpy(b, a, ctypes.c_size_t(n), 3) == 0  # hipMemcpyDeviceToDevice
+    assert hip.hipDeviceSynchronize() == 0
+    hip.hipFree(a)
+    hip.hipFree(b)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--workload", default="topo80")
+    ap.add_argument("--precision", default="c128")
+    ap.add_argument("--reps", type=int, default=20)
+    ap.add_argument("--copy-mib", type=int, default=256)
+    a = ap.parse_args()
+    if a.workload == "topo80":
+        prob, _ = W.topo80()
+    elif a.workload == "mixed128":
+        prob = W.mixed128()
+    else:
+        raise SystemExit("unknown workload")
+    al, E, _ = prob.rhs(np.array([0.0, 0.0, 1.0]), np.array([1.0, 0.0, 0.0]))
+    ctx = T.Context(0)
+    core = T.AProductCore(ctx, prob.R, prob.Nx, prob.Ny, prob.Nz, prob.d, prob.lam)
+    m = T.DeviceModel(core, 1)
+    m.set(T.VEC_AL, al)
+    m.set(T.VEC_P, E)
+    prec = T.C128 if a.precision == "c128" else T.C64
+    ms = m.time_passes(prec, reps=a.reps)
+    copy_calibration(a.copy_mib)
+    print(json.dumps({"workload": a.workload, "precision": a.precision, "grid": core.grid,
+                      "pass_ms": [float(x) for x in ms], "copy_mib": a.copy_mib}))
+
+
+if __name__ == "__main__":
+    main()
