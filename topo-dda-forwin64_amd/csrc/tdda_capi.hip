@@ -77,7 +77,26 @@ FftPlan make_plan(int L)
         p.rad[p.nst++] = 5;
         n /= 5;
     }
+    p.tb = 1;
+    while (p.tb * p.tb < L) p.tb++;
     return p;
+}
+
+// two-level table: T0[q0] = w^q0 (q0 < tb), T1[q1] = w^(q1 tb) (q1 < ceil(L / tb))
+std::vector<Cx<double>> twiddles2(int L, int tb)
+{
+    const int n1 = (L + tb - 1) / tb;
+    std::vector<Cx<double>> t(tb + n1);
+    const long double pi = 3.141592653589793238462643383279502884L;
+    for (int q = 0; q < tb; q++) {
+        const long double a = -2.0L * pi * (long double)q / (long double)L;
+        t[q] = Cx<double>{(double)cosl(a), (double)sinl(a)};
+    }
+    for (int q = 0; q < n1; q++) {
+        const long double a = -2.0L * pi * (long double)(q * tb) / (long double)L;
+        t[tb + q] = Cx<double>{(double)cosl(a), (double)sinl(a)};
+    }
+    return t;
 }
 
 // roots of unity exp(-2 pi i q / L) in long double
@@ -128,8 +147,10 @@ struct tdda_op_s {
     int* cellmap = nullptr;
     Cx<double>* green64 = nullptr;
     Cx<float>* green32 = nullptr;
-    Cx<double>*tw64[3] = {nullptr, nullptr, nullptr};
+    Cx<double>*tw64[3] = {nullptr, nullptr, nullptr};   // full tables (Green set-up)
     Cx<float>*tw32[3] = {nullptr, nullptr, nullptr};
+    Cx<double>*tws64[3] = {nullptr, nullptr, nullptr};  // two-level tables (operator passes)
+    Cx<float>*tws32[3] = {nullptr, nullptr, nullptr};
     FftPlan pl[3];
     TilePlan tp64, tp32;
     tdda_model scratch = nullptr;
@@ -179,25 +200,37 @@ struct tdda_model_s {
 
 namespace {
 
+// Tile widths of the passes for element type T: a tile holds at most
+// `cap` elements (pencils x length, two LDS buffers of it), rows of the
+// strided passes are kept to whole 128-byte lines where the budget allows,
+// and the Green product of the x pass covers at most 8 points per thread.
 template <typename T>
-TilePlan plan_tiles(const tdda_op_s* op, size_t budget)
+int plan_tiles(const tdda_op_s* op, TilePlan& t)
 {
-    TilePlan t{};
     const size_t s = sizeof(Cx<T>);
-    t.ps_z = op->Mz + 1;
-    t.ps_y = op->My + 1;
-    t.ps_x = op->Mx + 1;
-    const int maxz = (int)(budget / (2 * 3 * t.ps_z * s));
-    t.pz = std::max(1, std::min({maxz, 64, op->Nx * op->Ny}));
-    const int maxy = (int)(budget / (2 * t.ps_y * s));
-    t.tfy = std::max(1, std::min({maxy, 48, 3 * op->Mz}));
-    int maxx = (int)(budget / (2 * t.ps_x * s));
-    maxx = std::min({maxx, 48, 3 * op->Mz});
-    t.tfx = std::max(3, (maxx / 3) * 3);
-    t.lds_z = 2 * 3 * (size_t)t.pz * t.ps_z * s;
-    t.lds_y = 2 * (size_t)t.tfy * t.ps_y * s;
-    t.lds_x = 2 * (size_t)t.tfx * t.ps_x * s;
-    return t;
+    const int dflt = sizeof(T) == 8 ? 1536 : 3072;
+    const int cap = env_int(sizeof(T) == 8 ? "TDDA_TILE_C128" : "TDDA_TILE_C64", dflt);
+    const int unit = (int)(128 / s);
+    t.ps_z = lds_pencil_stride(op->Mz);
+    t.ps_y = lds_pencil_stride(op->My);
+    t.ps_x = lds_pencil_stride(op->Mx);
+    if (3 * op->Mz > cap || op->My > cap || 3 * op->Mx > cap)
+        return fail(TDDA_ERR_UNSUPPORTED, "FFT length too long for the pass tiles of this precision");
+    t.pz = std::max(1, std::min({cap / (3 * op->Mz), 64, op->Nx * op->Ny}));
+    int ty = std::min(cap / op->My, 3 * op->Mz);
+    if (ty >= unit && ty < 3 * op->Mz) ty = (ty / unit) * unit;
+    t.tfy = std::max(1, ty);
+    int tx = std::min({cap / op->Mx, 3 * op->Mz, 3 * (8 * kThreads / op->Mx)});
+    tx = (tx / 3) * 3;
+    if (tx >= 3 * unit && tx < 3 * op->Mz) tx = (tx / (3 * unit)) * (3 * unit);
+    t.tfx = std::max(3, tx);
+    if ((t.tfx / 3) * op->Mx > 8 * kThreads) return fail(TDDA_ERR_UNSUPPORTED, "x pass tile too wide");
+    t.lds_z = (size_t)(tw2_slots(op->Mz, op->pl[2].tb) + 2 * 3 * t.pz * t.ps_z) * s;
+    t.lds_y = (size_t)(tw2_slots(op->My, op->pl[1].tb) + 2 * t.tfy * t.ps_y) * s;
+    t.lds_x = (size_t)(tw2_slots(op->Mx, op->pl[0].tb) + 2 * t.tfx * t.ps_x) * s;
+    if (std::max({t.lds_z, t.lds_y, t.lds_x}) > op->ctx->lds_budget)
+        return fail(TDDA_ERR_UNSUPPORTED, "pass tile exceeds the LDS budget");
+    return TDDA_OK;
 }
 
 template <typename T>
@@ -271,7 +304,9 @@ int build_table(tdda_model_s* m, JobTable<T>& jt, const VecSet<T>& v, const Cx<T
         J.ps_z = tp.ps_z; J.ps_y = tp.ps_y; J.ps_x = tp.ps_x;
         J.cellmap = op->cellmap;
         J.green = green;
-        J.twx = tw[0]; J.twy = tw[1]; J.twz = tw[2];
+        J.twx = tw[0];
+        J.twy = tw[1];
+        J.twz = tw[2];
         J.plx = op->pl[0]; J.ply = op->pl[1]; J.plz = op->pl[2];
         J.w1 = v.w1 + j * w1n;
         J.w2 = v.w2 + j * w2n;
@@ -602,8 +637,7 @@ int tdda_ctx_create(int device, tdda_ctx* out)
     c->device = device;
     c->refs = 1;
     c->ncu = prop.multiProcessorCount;
-    const size_t lim = std::min<size_t>(prop.sharedMemPerBlock, 65536);
-    c->lds_budget = std::min<size_t>(lim, (size_t)env_int("TDDA_LDS_BUDGET", 49152));
+    c->lds_budget = std::min<size_t>(prop.sharedMemPerBlock, 65536);
     hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
     if (e != hipSuccess) {
         delete c;
@@ -817,6 +851,15 @@ int tdda_op_create(tdda_ctx ctx, const tdda_op_desc* desc, tdda_op* out)
         if (hipMemcpy(op->tw64[a], t.data(), sizeof(Cx<double>) * M[a], hipMemcpyHostToDevice) != hipSuccess ||
             hipMemcpy(op->tw32[a], tf.data(), sizeof(Cx<float>) * M[a], hipMemcpyHostToDevice) != hipSuccess)
             return cleanup(fail(TDDA_ERR_HIP, "twiddle upload"));
+        std::vector<Cx<double>> t2 = twiddles2(M[a], op->pl[a].tb);
+        std::vector<Cx<float>> t2f(t2.size());
+        for (size_t q = 0; q < t2.size(); q++) t2f[q] = Cx<float>{(float)t2[q].x, (float)t2[q].y};
+        if (hipMalloc(&op->tws64[a], sizeof(Cx<double>) * t2.size()) != hipSuccess ||
+            hipMalloc(&op->tws32[a], sizeof(Cx<float>) * t2.size()) != hipSuccess)
+            return cleanup(fail(TDDA_ERR_OOM, "twiddles"));
+        if (hipMemcpy(op->tws64[a], t2.data(), sizeof(Cx<double>) * t2.size(), hipMemcpyHostToDevice) != hipSuccess ||
+            hipMemcpy(op->tws32[a], t2f.data(), sizeof(Cx<float>) * t2.size(), hipMemcpyHostToDevice) != hipSuccess)
+            return cleanup(fail(TDDA_ERR_HIP, "twiddle upload"));
     }
     // Green tensor on the full grid, FP64, then its 3D forward transform
     // (AProductCore.cpp:54-229).
@@ -879,8 +922,11 @@ int tdda_op_create(tdda_ctx ctx, const tdda_op_desc* desc, tdda_op* out)
     (void)hipFree(full);
     if (e != hipSuccess || e2 != hipSuccess)
         return cleanup(fail(TDDA_ERR_HIP, std::string("Green set-up: ") + hipGetErrorString(e != hipSuccess ? e : e2)));
-    op->tp64 = plan_tiles<double>(op, ctx->lds_budget);
-    op->tp32 = plan_tiles<float>(op, ctx->lds_budget);
+    {
+        int rc = TDDA_OK;
+        if ((prec & TDDA_C128) && (rc = plan_tiles<double>(op, op->tp64)) != TDDA_OK) return cleanup(rc);
+        if ((prec & TDDA_C64) && (rc = plan_tiles<float>(op, op->tp32)) != TDDA_OK) return cleanup(rc);
+    }
     *out = op;
     return TDDA_OK;
 }
@@ -896,6 +942,8 @@ static void op_release(tdda_op op)
     for (int a = 0; a < 3; a++) {
         (void)hipFree(op->tw64[a]);
         (void)hipFree(op->tw32[a]);
+        (void)hipFree(op->tws64[a]);
+        (void)hipFree(op->tws32[a]);
     }
     tdda_ctx ctx = op->ctx;
     delete op;
@@ -943,7 +991,7 @@ int tdda_model_create(tdda_op op, int nrhs, tdda_model* out)
     int rc;
     if (op->green64) {
         if ((rc = alloc_vecs<double>(m, m->v64)) != TDDA_OK) return bail(rc);
-        Cx<double>* tw[3] = {op->tw64[0], op->tw64[1], op->tw64[2]};
+        Cx<double>* tw[3] = {op->tws64[0], op->tws64[1], op->tws64[2]};
         if ((rc = build_table<double>(m, m->j64, m->v64, op->green64, tw, op->tp64, m->v64.r, m->v64.r0)) != TDDA_OK)
             return bail(rc);
         // outer residual of the mixed solver: r -> p, its r0 copy -> t (scratch)
@@ -952,7 +1000,7 @@ int tdda_model_create(tdda_op op, int nrhs, tdda_model* out)
     }
     if (op->green32) {
         if ((rc = alloc_vecs<float>(m, m->v32)) != TDDA_OK) return bail(rc);
-        Cx<float>* tw[3] = {op->tw32[0], op->tw32[1], op->tw32[2]};
+        Cx<float>* tw[3] = {op->tws32[0], op->tws32[1], op->tws32[2]};
         if ((rc = build_table<float>(m, m->j32, m->v32, op->green32, tw, op->tp32, m->v32.r, m->v32.r0)) != TDDA_OK)
             return bail(rc);
     }

@@ -43,11 +43,17 @@ __device__ __forceinline__ Cx<T> rot90(Cx<T> a)
     return {a.y, -a.x};
 }
 
+// Threads per workgroup of every operator pass.
+constexpr int kThreadsFft = 256;
+
 // Plan of one 1D transform: length and its radix sequence.
+// tb: base of the two-level twiddle table, exp(-2 pi i q / L) =
+// T0[q % tb] * T1[q / tb] with T0 of tb entries and T1 of ceil(L / tb).
 struct FftPlan {
     int L;
     int nst;
     int rad[12];
+    int tb;
 };
 
 // ---------------------------------------------------------------------------
@@ -202,6 +208,117 @@ __device__ Cx<T>* lds_fft(Cx<T>* a, Cx<T>* b, int NP, int ps, const FftPlan& pl,
         Ns *= R;
     }
     return a;
+}
+
+}  // namespace tdda
+
+namespace tdda {
+
+// ---------------------------------------------------------------------------
+// LDS transform used by the operator passes.
+//
+// Pencil p, element i lives at p * ps + pad(i) with pad(i) = i + (i >> 5):
+// one spare slot per 32 elements spreads the strided Stockham writes of the
+// early stages over the LDS banks.  Twiddles come from an LDS copy of a
+// two-level table of the L-th roots of unity (tb + ceil(L / tb) entries,
+// tb ~ sqrt(L)): one table product per butterfly, then powers by recurrence.
+// Index arithmetic uses float reciprocals instead of integer division.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ int lpad(int i) { return i + (i >> 5); }
+__host__ __device__ __forceinline__ int lds_pencil_stride(int L) { return L + (L >> 5) + 1; }
+
+// q = n / d, r = n % d for 0 <= n < 2^24 via a float reciprocal and one
+// correction step (exact in that range).
+__device__ __forceinline__ void fdivmod(int n, int d, float rd, int& q, int& r)
+{
+    q = (int)((float)n * rd);
+    r = n - q * d;
+    if (r < 0) {
+        q -= 1;
+        r += d;
+    } else if (r >= d) {
+        q += 1;
+        r -= d;
+    }
+}
+
+// One Stockham stage of the operator passes: NP pencils, src -> dst (LDS,
+// padded layout), twiddles from the two-level LDS table by recurrence.
+template <typename T, int R, bool INV>
+__device__ __forceinline__ void pp_stage(const Cx<T>* __restrict__ src, Cx<T>* __restrict__ dst, int NP, int ps,
+                                         int L, int Ns, const Cx<T>* __restrict__ tw, int tb)
+{
+    const int LR = L / R;
+    const int total = NP * LR;
+    const int twstep = L / (Ns * R);
+    const float rLR = 1.0f / (float)LR;
+    const float rNs = 1.0f / (float)Ns;
+    const float rtb = 1.0f / (float)tb;
+    for (int b = threadIdx.x; b < total; b += kThreadsFft) {
+        int p, j, q, kk;
+        fdivmod(b, LR, rLR, p, j);
+        fdivmod(j, Ns, rNs, q, kk);
+        const Cx<T>* sp = src + p * ps;
+        Cx<T> v[R];
+#pragma unroll
+        for (int m = 0; m < R; m++) v[m] = sp[lpad(j + m * LR)];
+        if (Ns > 1) {
+            // w^m by recurrence from w = exp(-+2 pi i kk twstep / L)
+            int q1, q0;
+            fdivmod(kk * twstep, tb, rtb, q1, q0);
+            const Cx<T> w1 = cmul(tw[q0], tw[tb + q1]);
+            Cx<T> w = w1;
+#pragma unroll
+            for (int m = 1; m < R; m++) {
+                v[m] = INV ? cmulc(v[m], w) : cmul(v[m], w);
+                if (m + 1 < R) w = cmul(w, w1);
+            }
+        }
+        dft<T, R, INV>(v);
+        Cx<T>* dp = dst + p * ps;
+        const int base = (j - kk) * R + kk;
+#pragma unroll
+        for (int m = 0; m < R; m++) dp[lpad(base + m * Ns)] = v[m];
+    }
+}
+
+// Transform of NP pencils held in a (padded layout); b is scratch of the
+// same size.  Returns the buffer holding the result; ends with a barrier.
+template <typename T, bool INV>
+__device__ Cx<T>* pp_fft(Cx<T>* a, Cx<T>* b, int NP, int ps, const FftPlan& pl, const Cx<T>* tw)
+{
+    int Ns = 1;
+    const int L = pl.L;
+    for (int s = 0; s < pl.nst; s++) {
+        const int R = pl.rad[s];
+        switch (R) {
+        case 8: pp_stage<T, 8, INV>(a, b, NP, ps, L, Ns, tw, pl.tb); break;
+        case 4: pp_stage<T, 4, INV>(a, b, NP, ps, L, Ns, tw, pl.tb); break;
+        case 2: pp_stage<T, 2, INV>(a, b, NP, ps, L, Ns, tw, pl.tb); break;
+        case 3: pp_stage<T, 3, INV>(a, b, NP, ps, L, Ns, tw, pl.tb); break;
+        default: pp_stage<T, 5, INV>(a, b, NP, ps, L, Ns, tw, pl.tb); break;
+        }
+        __syncthreads();
+        Cx<T>* t = a;
+        a = b;
+        b = t;
+        Ns *= R;
+    }
+    return a;
+}
+
+// Entries of the two-level twiddle table of a plan.
+__host__ __device__ __forceinline__ int tw2_size(int L, int tb) { return tb + (L + tb - 1) / tb; }
+// LDS slots reserved for it (even, so the data after it stays 16-byte aligned).
+__host__ __device__ __forceinline__ int tw2_slots(int L, int tb) { return (tw2_size(L, tb) + 1) & ~1; }
+
+// Cooperative copy of the two-level twiddle table into LDS.
+template <typename T>
+__device__ __forceinline__ void load_twiddles(Cx<T>* __restrict__ dst, const Cx<T>* __restrict__ src,
+                                              const FftPlan& pl)
+{
+    const int n = tw2_size(pl.L, pl.tb);
+    for (int i = threadIdx.x; i < n; i += blockDim.x) dst[i] = src[i];
 }
 
 }  // namespace tdda

@@ -30,7 +30,17 @@
 
 namespace tdda {
 
-constexpr int kThreads = 256;
+constexpr int kThreads = kThreadsFft;
+
+// Operator passes ask for at least TDDA_PASS_WAVES waves per SIMD so the
+// compiler keeps their register footprint small enough for several
+// workgroups per CU (latency hiding of the tile loads).
+#ifndef TDDA_PASS_WAVES
+#define TDDA_PASS_WAVES 4
+#endif
+#define TDDA_PASS_BOUNDS __launch_bounds__(kThreads, TDDA_PASS_WAVES)
+
+
 
 enum SolveStatus : int {
     ST_RUNNING = 0,
@@ -201,7 +211,7 @@ __device__ __forceinline__ bool job_inactive(const SolverState* st)
 // P1: z-forward.  Tile = pz consecutive (x, y) pencils.
 // ---------------------------------------------------------------------------
 template <typename T, int MODE>
-__global__ void __launch_bounds__(kThreads) k_p1(const JobDesc<T>* __restrict__ jobs, const int* __restrict__ tstart,
+__global__ void TDDA_PASS_BOUNDS k_p1(const JobDesc<T>* __restrict__ jobs, const int* __restrict__ tstart,
                                                  int njobs)
 {
     extern __shared__ __align__(16) char smem[];
@@ -214,8 +224,10 @@ __global__ void __launch_bounds__(kThreads) k_p1(const JobDesc<T>* __restrict__ 
     const int q0 = tile * J.pz;
     const int np = min(J.pz, Q - q0);
     const int ps = J.ps_z;
-    Cx<T>* A = reinterpret_cast<Cx<T>*>(smem);
+    Cx<T>* twl = reinterpret_cast<Cx<T>*>(smem);
+    Cx<T>* A = twl + tw2_slots(Mz, J.plz.tb);
     Cx<T>* B = A + 3 * J.pz * ps;
+    load_twiddles(twl, J.twz, J.plz);
 
     // scalars for the fused vector updates
     Cx<T> sa{T(0), T(0)}, sb{T(0), T(0)};
@@ -237,11 +249,11 @@ __global__ void __launch_bounds__(kThreads) k_p1(const JobDesc<T>* __restrict__ 
     }
 
     const int nin = np * Nz * 3;
+    const float r3 = 1.0f / 3.0f, rNz3 = 1.0f / (float)(Nz * 3);
     for (int e = threadIdx.x; e < nin; e += blockDim.x) {
-        const int pl = e / (Nz * 3);
-        const int rem = e - pl * (Nz * 3);
-        const int z = rem / 3;
-        const int c = rem - z * 3;
+        int pl, rem, z, c;
+        fdivmod(e, Nz * 3, rNz3, pl, rem);
+        fdivmod(rem, 3, r3, z, c);
         const int cell = (q0 + pl) * Nz + z;
         const int site = J.cellmap ? J.cellmap[cell] : cell;
         Cx<T> v{T(0), T(0)};
@@ -266,24 +278,27 @@ __global__ void __launch_bounds__(kThreads) k_p1(const JobDesc<T>* __restrict__ 
                 J.t[i] = v;
             }
         }
-        A[(pl * 3 + c) * ps + z] = v;
+        A[(pl * 3 + c) * ps + lpad(z)] = v;
     }
     const int npad = Mz - Nz;
-    for (int e = threadIdx.x; e < np * 3 * npad; e += blockDim.x) {
-        const int pc = e / npad;
-        const int z = Nz + (e - pc * npad);
-        A[pc * ps + z] = Cx<T>{T(0), T(0)};
+    if (npad > 0) {
+        const float rpad = 1.0f / (float)npad;
+        for (int e = threadIdx.x; e < np * 3 * npad; e += blockDim.x) {
+            int pc, zz;
+            fdivmod(e, npad, rpad, pc, zz);
+            A[pc * ps + lpad(Nz + zz)] = Cx<T>{T(0), T(0)};
+        }
     }
     __syncthreads();
-    const Cx<T>* res = lds_fft<T, false>(A, B, np * 3, ps, J.plz, J.twz);
+    const Cx<T>* res = pp_fft<T, false>(A, B, np * 3, ps, J.plz, twl);
     const int nout = np * Mz * 3;
     Cx<T>* w1 = J.w1 + (size_t)q0 * Mz * 3;
+    const float rMz3 = 1.0f / (float)(Mz * 3);
     for (int e = threadIdx.x; e < nout; e += blockDim.x) {
-        const int pl = e / (Mz * 3);
-        const int rem = e - pl * (Mz * 3);
-        const int kz = rem / 3;
-        const int c = rem - kz * 3;
-        w1[e] = res[(pl * 3 + c) * ps + kz];
+        int pl, rem, kz, c;
+        fdivmod(e, Mz * 3, rMz3, pl, rem);
+        fdivmod(rem, 3, r3, kz, c);
+        w1[e] = res[(pl * 3 + c) * ps + lpad(kz)];
     }
 }
 
@@ -292,7 +307,7 @@ __global__ void __launch_bounds__(kThreads) k_p1(const JobDesc<T>* __restrict__ 
 // FWD: w1[x][y<Ny][.] -> w2[x][ky<My][.];  INV: w2[x][ky][.] -> w1[x][y<Ny][.]
 // ---------------------------------------------------------------------------
 template <typename T, bool INV>
-__global__ void __launch_bounds__(kThreads) k_py(const JobDesc<T>* __restrict__ jobs, const int* __restrict__ tstart,
+__global__ void TDDA_PASS_BOUNDS k_py(const JobDesc<T>* __restrict__ jobs, const int* __restrict__ tstart,
                                                  int njobs, int solver)
 {
     extern __shared__ __align__(16) char smem[];
@@ -306,30 +321,33 @@ __global__ void __launch_bounds__(kThreads) k_py(const JobDesc<T>* __restrict__ 
     const int nf = min(J.tfy, F - f0);
     const int Ny = J.Ny, My = J.My;
     const int ps = J.ps_y;
-    Cx<T>* A = reinterpret_cast<Cx<T>*>(smem);
+    Cx<T>* twl = reinterpret_cast<Cx<T>*>(smem);
+    Cx<T>* A = twl + tw2_slots(My, J.ply.tb);
     Cx<T>* B = A + J.tfy * ps;
+    load_twiddles(twl, J.twy, J.ply);
     const int nin = INV ? My : Ny;
     const int nout = INV ? Ny : My;
     const Cx<T>* src = (INV ? J.w2 + (size_t)x * My * F : J.w1 + (size_t)x * Ny * F) + f0;
     Cx<T>* dst = (INV ? J.w1 + (size_t)x * Ny * F : J.w2 + (size_t)x * My * F) + f0;
+    const float rnf = 1.0f / (float)nf;
     for (int e = threadIdx.x; e < nin * nf; e += blockDim.x) {
-        const int y = e / nf;
-        const int f = e - y * nf;
-        A[f * ps + y] = src[(size_t)y * F + f];
+        int y, f;
+        fdivmod(e, nf, rnf, y, f);
+        A[f * ps + lpad(y)] = src[(size_t)y * F + f];
     }
     if (!INV) {
         for (int e = threadIdx.x; e < (My - Ny) * nf; e += blockDim.x) {
-            const int y = Ny + e / nf;
-            const int f = e % nf;
-            A[f * ps + y] = Cx<T>{T(0), T(0)};
+            int y, f;
+            fdivmod(e, nf, rnf, y, f);
+            A[f * ps + lpad(Ny + y)] = Cx<T>{T(0), T(0)};
         }
     }
     __syncthreads();
-    const Cx<T>* res = lds_fft<T, INV>(A, B, nf, ps, J.ply, J.twy);
+    const Cx<T>* res = pp_fft<T, INV>(A, B, nf, ps, J.ply, twl);
     for (int e = threadIdx.x; e < nout * nf; e += blockDim.x) {
-        const int y = e / nf;
-        const int f = e - y * nf;
-        dst[(size_t)y * F + f] = res[f * ps + y];
+        int y, f;
+        fdivmod(e, nf, rnf, y, f);
+        dst[(size_t)y * F + f] = res[f * ps + lpad(y)];
     }
 }
 
@@ -338,7 +356,7 @@ __global__ void __launch_bounds__(kThreads) k_py(const JobDesc<T>* __restrict__ 
 // Tile = (ky, tfx consecutive flattened (kz, c) columns, tfx % 3 == 0).
 // ---------------------------------------------------------------------------
 template <typename T>
-__global__ void __launch_bounds__(kThreads) k_px(const JobDesc<T>* __restrict__ jobs, const int* __restrict__ tstart,
+__global__ void TDDA_PASS_BOUNDS k_px(const JobDesc<T>* __restrict__ jobs, const int* __restrict__ tstart,
                                                  int njobs, int solver)
 {
     extern __shared__ __align__(16) char smem[];
@@ -352,55 +370,59 @@ __global__ void __launch_bounds__(kThreads) k_px(const JobDesc<T>* __restrict__ 
     const int nf = min(J.tfx, F - f0);
     const int Nx = J.Nx, Mx = J.Mx, My = J.My, Mz = J.Mz;
     const int ps = J.ps_x;
-    Cx<T>* A = reinterpret_cast<Cx<T>*>(smem);
+    Cx<T>* twl = reinterpret_cast<Cx<T>*>(smem);
+    Cx<T>* A = twl + tw2_slots(Mx, J.plx.tb);
     Cx<T>* B = A + J.tfx * ps;
+    load_twiddles(twl, J.twx, J.plx);
     const size_t sx = (size_t)My * F;
     Cx<T>* base = J.w2 + (size_t)ky * F + f0;
+    const float rnf = 1.0f / (float)nf;
     for (int e = threadIdx.x; e < Nx * nf; e += blockDim.x) {
-        const int x = e / nf;
-        const int f = e - x * nf;
-        A[f * ps + x] = base[x * sx + f];
+        int x, f;
+        fdivmod(e, nf, rnf, x, f);
+        A[f * ps + lpad(x)] = base[x * sx + f];
     }
     for (int e = threadIdx.x; e < (Mx - Nx) * nf; e += blockDim.x) {
-        const int x = Nx + e / nf;
-        const int f = e % nf;
-        A[f * ps + x] = Cx<T>{T(0), T(0)};
+        int x, f;
+        fdivmod(e, nf, rnf, x, f);
+        A[f * ps + lpad(Nx + x)] = Cx<T>{T(0), T(0)};
     }
-    __syncthreads();
-    Cx<T>* res = lds_fft<T, false>(A, B, nf, ps, J.plx, J.twx);
-    Cx<T>* other = (res == A) ? B : A;
 
-    // Green product.  Octant fold per axis: k' = min(k, M - k); odd components
-    // flip sign when k was folded.
+    // Green product.  Octant fold per axis: k' = min(k, M - k); odd
+    // components flip sign when k was folded.
     const int kyf = (ky > My / 2) ? My - ky : ky;
     const T sy = (ky > My / 2) ? T(-1) : T(1);
     const int nkz = nf / 3;
     const int Hy = J.Hy, Hz = J.Hz;
-    for (int e = threadIdx.x; e < Mx * nkz; e += blockDim.x) {
-        const int kx = e / nkz;
-        const int kl = e - kx * nkz;
+    const float rnkz = 1.0f / (float)nkz;
+    __syncthreads();
+    Cx<T>* res = pp_fft<T, false>(A, B, nf, ps, J.plx, twl);
+    Cx<T>* other = (res == A) ? B : A;
+    for (int e = threadIdx.x; e < Mx * nkz; e += kThreads) {
+        int kx, kl;
+        fdivmod(e, nkz, rnkz, kx, kl);
         const int kz = f0 / 3 + kl;
         const int kxf = (kx > Mx / 2) ? Mx - kx : kx;
         const T sxv = (kx > Mx / 2) ? T(-1) : T(1);
         const int kzf = (kz > Mz / 2) ? Mz - kz : kz;
         const T szv = (kz > Mz / 2) ? T(-1) : T(1);
-        const Cx<T>* g = J.green + ((size_t)(kxf * Hy + kyf) * Hz + kzf) * 6;
-        const Cx<T> gxx = g[0], gyy = g[3], gzz = g[5];
-        const Cx<T> gxy = cscale(g[1], sxv * sy);
-        const Cx<T> gxz = cscale(g[2], sxv * szv);
-        const Cx<T> gyz = cscale(g[4], sy * szv);
-        Cx<T>* v = res + (kl * 3) * ps + kx;
+        const Cx<T>* gp = J.green + ((size_t)(kxf * Hy + kyf) * Hz + kzf) * 6;
+        const Cx<T> gxx = gp[0], gyy = gp[3], gzz = gp[5];
+        const Cx<T> gxy = cscale(gp[1], sxv * sy);
+        const Cx<T> gxz = cscale(gp[2], sxv * szv);
+        const Cx<T> gyz = cscale(gp[4], sy * szv);
+        Cx<T>* v = res + (kl * 3) * ps + lpad(kx);
         const Cx<T> bx = v[0], by = v[ps], bz = v[2 * ps];
         v[0] = cadd(cadd(cmul(gxx, bx), cmul(gxy, by)), cmul(gxz, bz));
         v[ps] = cadd(cadd(cmul(gxy, bx), cmul(gyy, by)), cmul(gyz, bz));
         v[2 * ps] = cadd(cadd(cmul(gxz, bx), cmul(gyz, by)), cmul(gzz, bz));
     }
     __syncthreads();
-    const Cx<T>* res2 = lds_fft<T, true>(res, other, nf, ps, J.plx, J.twx);
+    const Cx<T>* res2 = pp_fft<T, true>(res, other, nf, ps, J.plx, twl);
     for (int e = threadIdx.x; e < Nx * nf; e += blockDim.x) {
-        const int x = e / nf;
-        const int f = e - x * nf;
-        base[x * sx + f] = res2[f * ps + x];
+        int x, f;
+        fdivmod(e, nf, rnf, x, f);
+        base[x * sx + f] = res2[f * ps + lpad(x)];
     }
 }
 
@@ -408,7 +430,7 @@ __global__ void __launch_bounds__(kThreads) k_px(const JobDesc<T>* __restrict__ 
 // P5: z-inverse, crop to the box, diagonal term and solver epilogues.
 // ---------------------------------------------------------------------------
 template <typename T, int MODE>
-__global__ void __launch_bounds__(kThreads) k_p5(const JobDesc<T>* __restrict__ jobs, const int* __restrict__ tstart,
+__global__ void TDDA_PASS_BOUNDS k_p5(const JobDesc<T>* __restrict__ jobs, const int* __restrict__ tstart,
                                                  int njobs, int diag)
 {
     extern __shared__ __align__(16) char smem[];
@@ -424,32 +446,33 @@ __global__ void __launch_bounds__(kThreads) k_p5(const JobDesc<T>* __restrict__ 
     const int q0 = tile * J.pz;
     const int np = min(J.pz, Q - q0);
     const int ps = J.ps_z;
-    Cx<T>* A = reinterpret_cast<Cx<T>*>(smem);
+    Cx<T>* twl = reinterpret_cast<Cx<T>*>(smem);
+    Cx<T>* A = twl + tw2_slots(Mz, J.plz.tb);
     Cx<T>* B = A + 3 * J.pz * ps;
+    load_twiddles(twl, J.twz, J.plz);
     const Cx<T>* w1 = J.w1 + (size_t)q0 * Mz * 3;
     const int nin = np * Mz * 3;
+    const float r3 = 1.0f / 3.0f, rMz3 = 1.0f / (float)(Mz * 3), rNz3 = 1.0f / (float)(Nz * 3);
     for (int e = threadIdx.x; e < nin; e += blockDim.x) {
-        const int pl = e / (Mz * 3);
-        const int rem = e - pl * (Mz * 3);
-        const int kz = rem / 3;
-        const int c = rem - kz * 3;
-        A[(pl * 3 + c) * ps + kz] = w1[e];
+        int pl, rem, kz, c;
+        fdivmod(e, Mz * 3, rMz3, pl, rem);
+        fdivmod(rem, 3, r3, kz, c);
+        A[(pl * 3 + c) * ps + lpad(kz)] = w1[e];
     }
     __syncthreads();
-    const Cx<T>* res = lds_fft<T, true>(A, B, np * 3, ps, J.plz, J.twz);
+    const Cx<T>* res = pp_fft<T, true>(A, B, np * 3, ps, J.plz, twl);
 
     double acc[4] = {0.0, 0.0, 0.0, 0.0};
     const int nout = np * Nz * 3;
     for (int e = threadIdx.x; e < nout; e += blockDim.x) {
-        const int pl = e / (Nz * 3);
-        const int rem = e - pl * (Nz * 3);
-        const int z = rem / 3;
-        const int c = rem - z * 3;
+        int pl, rem, z, c;
+        fdivmod(e, Nz * 3, rNz3, pl, rem);
+        fdivmod(rem, 3, r3, z, c);
         const int cell = (q0 + pl) * Nz + z;
         const int site = J.cellmap ? J.cellmap[cell] : cell;
         if (site < 0) continue;
         const int i = site * 3 + c;
-        const Cx<T> conv = res[(pl * 3 + c) * ps + z];
+        const Cx<T> conv = res[(pl * 3 + c) * ps + lpad(z)];
         if (MODE == OUT_PLAIN) {
             Cx<T> y = conv;
             if (diag) y = cadd(y, cmul(J.al[i], J.in[i]));
