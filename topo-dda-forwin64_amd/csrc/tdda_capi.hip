@@ -127,14 +127,15 @@ int env_int(const char* name, int dflt)
 struct tdda_ctx_s {
     int device;
     hipStream_t stream;
-    size_t lds_budget;
+    size_t lds_budget;  // per workgroup
+    size_t lds_cu;      // per CU
     int ncu;
     int refs;
 };
 
 struct TilePlan {
-    int pz, tfy, tfx, ps_z, ps_y, ps_x;
-    size_t lds_z, lds_y, lds_x;
+    int pz, pz1, tfy, tfx, ps_z, ps_y, ps_x;
+    size_t lds_z, lds_z1, lds_y, lds_x;
 };
 
 struct tdda_op_s {
@@ -166,11 +167,13 @@ template <typename T>
 struct JobTable {
     std::vector<JobDesc<T>> h;
     JobDesc<T>* d = nullptr;
-    int* tz = nullptr;  // tile starts, njobs+1 each
+    int* tz = nullptr;  // tile starts, njobs+1 each (P5)
+    int* tz1 = nullptr;  // P1
     int* ty = nullptr;
     int* tx = nullptr;
     int* tv = nullptr;
-    int nz = 0, ny = 0, nx = 0, nv = 0;
+    int nz = 0, nz1 = 0, ny = 0, nx = 0, nv = 0;
+    int gz = 0, gz1 = 0, gy = 0, gx = 0;  // persistent grid sizes
     SolverState* st = nullptr;
     double* part = nullptr;
     unsigned* cnt = nullptr;
@@ -208,8 +211,9 @@ template <typename T>
 int plan_tiles(const tdda_op_s* op, TilePlan& t)
 {
     const size_t s = sizeof(Cx<T>);
-    const int dflt = sizeof(T) == 8 ? 1536 : 3072;
-    const int cap = env_int(sizeof(T) == 8 ? "TDDA_TILE_C128" : "TDDA_TILE_C64", dflt);
+    const int full = kThreads * Ept<T>::v;  // prefetch capacity of a workgroup
+    const int cap = std::min(full, env_int(sizeof(T) == 8 ? "TDDA_TILE_C128" : "TDDA_TILE_C64", full));
+    const int cap1 = kThreads * (Ept<T>::v / 3);  // P1: up to three input vectors
     const int unit = (int)(128 / s);
     t.ps_z = lds_pencil_stride(op->Mz);
     t.ps_y = lds_pencil_stride(op->My);
@@ -217,6 +221,8 @@ int plan_tiles(const tdda_op_s* op, TilePlan& t)
     if (3 * op->Mz > cap || op->My > cap || 3 * op->Mx > cap)
         return fail(TDDA_ERR_UNSUPPORTED, "FFT length too long for the pass tiles of this precision");
     t.pz = std::max(1, std::min({cap / (3 * op->Mz), 64, op->Nx * op->Ny}));
+    if (3 * op->Nz > cap1) return fail(TDDA_ERR_UNSUPPORTED, "Nz too large for the z-pass tiles of this precision");
+    t.pz1 = std::max(1, std::min({cap1 / (3 * op->Nz), cap / (3 * op->Mz), 64, op->Nx * op->Ny}));
     int ty = std::min(cap / op->My, 3 * op->Mz);
     if (ty >= unit && ty < 3 * op->Mz) ty = (ty / unit) * unit;
     t.tfy = std::max(1, ty);
@@ -226,9 +232,10 @@ int plan_tiles(const tdda_op_s* op, TilePlan& t)
     t.tfx = std::max(3, tx);
     if ((t.tfx / 3) * op->Mx > 8 * kThreads) return fail(TDDA_ERR_UNSUPPORTED, "x pass tile too wide");
     t.lds_z = (size_t)(tw2_slots(op->Mz, op->pl[2].tb) + 2 * 3 * t.pz * t.ps_z) * s;
+    t.lds_z1 = (size_t)(tw2_slots(op->Mz, op->pl[2].tb) + 2 * 3 * t.pz1 * t.ps_z) * s;
     t.lds_y = (size_t)(tw2_slots(op->My, op->pl[1].tb) + 2 * t.tfy * t.ps_y) * s;
     t.lds_x = (size_t)(tw2_slots(op->Mx, op->pl[0].tb) + 2 * t.tfx * t.ps_x) * s;
-    if (std::max({t.lds_z, t.lds_y, t.lds_x}) > op->ctx->lds_budget)
+    if (std::max({t.lds_z, t.lds_z1, t.lds_y, t.lds_x}) > op->ctx->lds_budget)
         return fail(TDDA_ERR_UNSUPPORTED, "pass tile exceeds the LDS budget");
     return TDDA_OK;
 }
@@ -278,6 +285,7 @@ int build_table(tdda_model_s* m, JobTable<T>& jt, const VecSet<T>& v, const Cx<T
     const size_t n3 = m->n3;
     jt.tp = tp;
     const int tiles_z = (op->Nx * op->Ny + tp.pz - 1) / tp.pz;
+    const int tiles_z1 = (op->Nx * op->Ny + tp.pz1 - 1) / tp.pz1;
     const int nfy = (3 * op->Mz + tp.tfy - 1) / tp.tfy;
     const int nfx = (3 * op->Mz + tp.tfx - 1) / tp.tfx;
     const int tiles_y = op->Nx * nfy;
@@ -299,7 +307,10 @@ int build_table(tdda_model_s* m, JobTable<T>& jt, const VecSet<T>& v, const Cx<T
         J.Mx = op->Mx; J.My = op->My; J.Mz = op->Mz;
         J.Hx = op->Hx; J.Hy = op->Hy; J.Hz = op->Hz;
         J.N = op->N;
-        J.pz = tp.pz; J.tfy = tp.tfy; J.tfx = tp.tfx;
+        J.pz = tp.pz;
+        J.pz1 = tp.pz1;
+        J.tfy = tp.tfy;
+        J.tfx = tp.tfx;
         J.nfy = nfy; J.nfx = nfx;
         J.ps_z = tp.ps_z; J.ps_y = tp.ps_y; J.ps_x = tp.ps_x;
         J.cellmap = op->cellmap;
@@ -327,14 +338,32 @@ int build_table(tdda_model_s* m, JobTable<T>& jt, const VecSet<T>& v, const Cx<T
     }
     CHK(dev_alloc(m, &jt.d, nj));
     HIPCHK(hipMemcpy(jt.d, jt.h.data(), sizeof(JobDesc<T>) * nj, hipMemcpyHostToDevice));
-    std::vector<int> hz(nj + 1), hy(nj + 1), hx(nj + 1), hv(nj + 1);
+    std::vector<int> hz(nj + 1), hz1(nj + 1), hy(nj + 1), hx(nj + 1), hv(nj + 1);
     for (int j = 0; j <= nj; j++) {
         hz[j] = j * tiles_z;
+        hz1[j] = j * tiles_z1;
         hy[j] = j * tiles_y;
         hx[j] = j * tiles_x;
         hv[j] = j * tiles_v;
     }
-    jt.nz = hz[nj]; jt.ny = hy[nj]; jt.nx = hx[nj]; jt.nv = hv[nj];
+    jt.nz = hz[nj];
+    jt.nz1 = hz1[nj];
+    jt.ny = hy[nj];
+    jt.nx = hx[nj];
+    jt.nv = hv[nj];
+    // persistent grids: as many workgroups as stay resident (4 per CU by the
+    // register budget of the passes, fewer if their LDS does not fit)
+    const tdda_ctx_s* c = op->ctx;
+    auto grid = [&](int ntiles, size_t lds) {
+        const int per_cu = (int)std::max<size_t>(1, std::min<size_t>(8, c->lds_cu / std::max<size_t>(lds, 1)));
+        return std::max(1, std::min(ntiles, c->ncu * per_cu));
+    };
+    jt.gz = grid(jt.nz, tp.lds_z);
+    jt.gz1 = grid(jt.nz1, tp.lds_z1);
+    jt.gy = grid(jt.ny, tp.lds_y);
+    jt.gx = grid(jt.nx, tp.lds_x);
+    CHK(dev_alloc(m, &jt.tz1, nj + 1));
+    HIPCHK(hipMemcpy(jt.tz1, hz1.data(), sizeof(int) * (nj + 1), hipMemcpyHostToDevice));
     CHK(dev_alloc(m, &jt.tz, nj + 1));
     CHK(dev_alloc(m, &jt.ty, nj + 1));
     CHK(dev_alloc(m, &jt.tx, nj + 1));
@@ -349,19 +378,19 @@ int build_table(tdda_model_s* m, JobTable<T>& jt, const VecSet<T>& v, const Cx<T
 template <typename T>
 void launch_p1(const JobTable<T>& jt, int nj, int mode, hipStream_t s)
 {
-    const dim3 g(jt.nz), b(kThreads);
+    const dim3 g(jt.gz1), b(kThreads);
     switch (mode) {
-    case IN_PLAIN: k_p1<T, IN_PLAIN><<<g, b, jt.tp.lds_z, s>>>(jt.d, jt.tz, nj); break;
-    case IN_P: k_p1<T, IN_P><<<g, b, jt.tp.lds_z, s>>>(jt.d, jt.tz, nj); break;
-    case IN_T: k_p1<T, IN_T><<<g, b, jt.tp.lds_z, s>>>(jt.d, jt.tz, nj); break;
-    default: k_p1<T, IN_X><<<g, b, jt.tp.lds_z, s>>>(jt.d, jt.tz, nj); break;
+    case IN_PLAIN: k_p1<T, IN_PLAIN><<<g, b, jt.tp.lds_z1, s>>>(jt.d, jt.tz1, nj); break;
+    case IN_P: k_p1<T, IN_P><<<g, b, jt.tp.lds_z1, s>>>(jt.d, jt.tz1, nj); break;
+    case IN_T: k_p1<T, IN_T><<<g, b, jt.tp.lds_z1, s>>>(jt.d, jt.tz1, nj); break;
+    default: k_p1<T, IN_X><<<g, b, jt.tp.lds_z1, s>>>(jt.d, jt.tz1, nj); break;
     }
 }
 
 template <typename T>
 void launch_p5(const JobTable<T>& jt, int nj, int mode, int diag, hipStream_t s)
 {
-    const dim3 g(jt.nz), b(kThreads);
+    const dim3 g(jt.gz), b(kThreads);
     switch (mode) {
     case OUT_PLAIN: k_p5<T, OUT_PLAIN><<<g, b, jt.tp.lds_z, s>>>(jt.d, jt.tz, nj, diag); break;
     case OUT_RES: k_p5<T, OUT_RES><<<g, b, jt.tp.lds_z, s>>>(jt.d, jt.tz, nj, 1); break;
@@ -375,9 +404,9 @@ template <typename T>
 void launch_apply(const JobTable<T>& jt, int nj, int inmode, int outmode, int diag, int solver, hipStream_t s)
 {
     launch_p1<T>(jt, nj, inmode, s);
-    k_py<T, false><<<dim3(jt.ny), dim3(kThreads), jt.tp.lds_y, s>>>(jt.d, jt.ty, nj, solver);
-    k_px<T><<<dim3(jt.nx), dim3(kThreads), jt.tp.lds_x, s>>>(jt.d, jt.tx, nj, solver);
-    k_py<T, true><<<dim3(jt.ny), dim3(kThreads), jt.tp.lds_y, s>>>(jt.d, jt.ty, nj, solver);
+    k_py<T, false><<<dim3(jt.gy), dim3(kThreads), jt.tp.lds_y, s>>>(jt.d, jt.ty, nj, solver);
+    k_px<T><<<dim3(jt.gx), dim3(kThreads), jt.tp.lds_x, s>>>(jt.d, jt.tx, nj, solver);
+    k_py<T, true><<<dim3(jt.gy), dim3(kThreads), jt.tp.lds_y, s>>>(jt.d, jt.ty, nj, solver);
     launch_p5<T>(jt, nj, outmode, diag, s);
 }
 
@@ -593,11 +622,11 @@ static int time_passes_t(tdda_model m, JobTable<T>& jt, int reps, double* ms5)
         HIPCHK(hipEventRecord(ev[0], s));
         launch_p1<T>(jt, nj, IN_PLAIN, s);
         HIPCHK(hipEventRecord(ev[1], s));
-        k_py<T, false><<<dim3(jt.ny), dim3(kThreads), jt.tp.lds_y, s>>>(jt.d, jt.ty, nj, 0);
+        k_py<T, false><<<dim3(jt.gy), dim3(kThreads), jt.tp.lds_y, s>>>(jt.d, jt.ty, nj, 0);
         HIPCHK(hipEventRecord(ev[2], s));
-        k_px<T><<<dim3(jt.nx), dim3(kThreads), jt.tp.lds_x, s>>>(jt.d, jt.tx, nj, 0);
+        k_px<T><<<dim3(jt.gx), dim3(kThreads), jt.tp.lds_x, s>>>(jt.d, jt.tx, nj, 0);
         HIPCHK(hipEventRecord(ev[3], s));
-        k_py<T, true><<<dim3(jt.ny), dim3(kThreads), jt.tp.lds_y, s>>>(jt.d, jt.ty, nj, 0);
+        k_py<T, true><<<dim3(jt.gy), dim3(kThreads), jt.tp.lds_y, s>>>(jt.d, jt.ty, nj, 0);
         HIPCHK(hipEventRecord(ev[4], s));
         launch_p5<T>(jt, nj, OUT_PLAIN, 1, s);
         HIPCHK(hipEventRecord(ev[5], s));
@@ -638,6 +667,7 @@ int tdda_ctx_create(int device, tdda_ctx* out)
     c->refs = 1;
     c->ncu = prop.multiProcessorCount;
     c->lds_budget = std::min<size_t>(prop.sharedMemPerBlock, 65536);
+    c->lds_cu = prop.maxSharedMemoryPerMultiProcessor ? prop.maxSharedMemoryPerMultiProcessor : 65536;
     hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
     if (e != hipSuccess) {
         delete c;

@@ -285,7 +285,7 @@ __device__ __forceinline__ void pp_stage(const Cx<T>* __restrict__ src, Cx<T>* _
 // Transform of NP pencils held in a (padded layout); b is scratch of the
 // same size.  Returns the buffer holding the result; ends with a barrier.
 template <typename T, bool INV>
-__device__ Cx<T>* pp_fft(Cx<T>* a, Cx<T>* b, int NP, int ps, const FftPlan& pl, const Cx<T>* tw)
+__device__ __forceinline__ Cx<T>* pp_fft(Cx<T>* a, Cx<T>* b, int NP, int ps, const FftPlan& pl, const Cx<T>* tw)
 {
     int Ns = 1;
     const int L = pl.L;

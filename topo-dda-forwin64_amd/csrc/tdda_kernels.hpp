@@ -39,6 +39,9 @@ constexpr int kThreads = kThreadsFft;
 #define TDDA_PASS_WAVES 4
 #endif
 #define TDDA_PASS_BOUNDS __launch_bounds__(kThreads, TDDA_PASS_WAVES)
+#ifndef TDDA_EPT_C128
+#define TDDA_EPT_C128 6
+#endif
 
 
 
@@ -61,7 +64,8 @@ template <typename T>
 struct JobDesc {
     int Nx, Ny, Nz, Mx, My, Mz, Hx, Hy, Hz;
     int N;       // sites
-    int pz;      // pencils per tile in P1/P5
+    int pz;      // pencils per tile in P5
+    int pz1;     // pencils per tile in P1
     int tfy;     // flattened (kz, c) columns per tile in P2/P4
     int tfx;     // flattened (kz, c) columns per tile in P3 (multiple of 3)
     int nfy;     // tiles per x-slab in P2/P4: ceil(3 Mz / tfy)
@@ -208,316 +212,549 @@ __device__ __forceinline__ bool job_inactive(const SolverState* st)
 }
 
 // ---------------------------------------------------------------------------
-// P1: z-forward.  Tile = pz consecutive (x, y) pencils.
-// ---------------------------------------------------------------------------
-template <typename T, int MODE>
-__global__ void TDDA_PASS_BOUNDS k_p1(const JobDesc<T>* __restrict__ jobs, const int* __restrict__ tstart,
-                                                 int njobs)
-{
-    extern __shared__ __align__(16) char smem[];
-    const int job = find_job(tstart, njobs, blockIdx.x);
-    const JobDesc<T>& J = jobs[job];
-    if (MODE != IN_PLAIN && job_inactive(J.st)) return;
-    const int tile = blockIdx.x - tstart[job];
-    const int Ny = J.Ny, Nz = J.Nz, Mz = J.Mz;
-    const int Q = J.Nx * Ny;
-    const int q0 = tile * J.pz;
-    const int np = min(J.pz, Q - q0);
-    const int ps = J.ps_z;
-    Cx<T>* twl = reinterpret_cast<Cx<T>*>(smem);
-    Cx<T>* A = twl + tw2_slots(Mz, J.plz.tb);
-    Cx<T>* B = A + 3 * J.pz * ps;
-    load_twiddles(twl, J.twz, J.plz);
-
-    // scalars for the fused vector updates
-    Cx<T> sa{T(0), T(0)}, sb{T(0), T(0)};
-    bool first = false;
-    if (MODE == IN_P) {
-        const SolverState* st = J.st;
-        first = st->first != 0;
-        if (!first) {
-            // beta = (alpha / zeta) * rho / rho_prev   (DDAModel.cpp:218)
-            double q1r, q1i, q2r, q2i;
-            cdiv(st->alpha[0], st->alpha[1], st->zeta[0], st->zeta[1], q1r, q1i);
-            cdiv(st->rho[0], st->rho[1], st->rho_prev[0], st->rho_prev[1], q2r, q2i);
-            const double br = q1r * q2r - q1i * q2i, bi = q1r * q2i + q1i * q2r;
-            sa = from_d<T>(br, bi);
-            sb = from_d<T>(st->zeta[0], st->zeta[1]);
-        }
-    } else if (MODE == IN_T) {
-        sa = from_d<T>(J.st->alpha[0], J.st->alpha[1]);
-    }
-
-    const int nin = np * Nz * 3;
-    const float r3 = 1.0f / 3.0f, rNz3 = 1.0f / (float)(Nz * 3);
-    for (int e = threadIdx.x; e < nin; e += blockDim.x) {
-        int pl, rem, z, c;
-        fdivmod(e, Nz * 3, rNz3, pl, rem);
-        fdivmod(rem, 3, r3, z, c);
-        const int cell = (q0 + pl) * Nz + z;
-        const int site = J.cellmap ? J.cellmap[cell] : cell;
-        Cx<T> v{T(0), T(0)};
-        if (site >= 0) {
-            const int i = site * 3 + c;
-            if (MODE == IN_PLAIN) {
-                v = J.in[i];
-            } else if (MODE == IN_X) {
-                v = J.x[i];
-            } else if (MODE == IN_P) {
-                // p = r + beta (p - u), u = zeta Ap   (DDAModel.cpp:219)
-                const Cx<T> r = J.r[i];
-                if (first) {
-                    v = r;
-                } else {
-                    const Cx<T> u = cmul(sb, J.ap[i]);
-                    v = cadd(r, cmul(sa, csub(J.p[i], u)));
-                }
-                J.p[i] = v;
-            } else {  // IN_T: t = r - alpha Ap   (DDAModel.cpp:222)
-                v = csub(J.r[i], cmul(sa, J.ap[i]));
-                J.t[i] = v;
-            }
-        }
-        A[(pl * 3 + c) * ps + lpad(z)] = v;
-    }
-    const int npad = Mz - Nz;
-    if (npad > 0) {
-        const float rpad = 1.0f / (float)npad;
-        for (int e = threadIdx.x; e < np * 3 * npad; e += blockDim.x) {
-            int pc, zz;
-            fdivmod(e, npad, rpad, pc, zz);
-            A[pc * ps + lpad(Nz + zz)] = Cx<T>{T(0), T(0)};
-        }
-    }
-    __syncthreads();
-    const Cx<T>* res = pp_fft<T, false>(A, B, np * 3, ps, J.plz, twl);
-    const int nout = np * Mz * 3;
-    Cx<T>* w1 = J.w1 + (size_t)q0 * Mz * 3;
-    const float rMz3 = 1.0f / (float)(Mz * 3);
-    for (int e = threadIdx.x; e < nout; e += blockDim.x) {
-        int pl, rem, kz, c;
-        fdivmod(e, Mz * 3, rMz3, pl, rem);
-        fdivmod(rem, 3, r3, kz, c);
-        w1[e] = res[(pl * 3 + c) * ps + lpad(kz)];
-    }
-}
-
-// ---------------------------------------------------------------------------
-// P2 / P4: y pass.  Tile = (x, tfy consecutive flattened (kz, c) columns).
-// FWD: w1[x][y<Ny][.] -> w2[x][ky<My][.];  INV: w2[x][ky][.] -> w1[x][y<Ny][.]
-// ---------------------------------------------------------------------------
-template <typename T, bool INV>
-__global__ void TDDA_PASS_BOUNDS k_py(const JobDesc<T>* __restrict__ jobs, const int* __restrict__ tstart,
-                                                 int njobs, int solver)
-{
-    extern __shared__ __align__(16) char smem[];
-    const int job = find_job(tstart, njobs, blockIdx.x);
-    const JobDesc<T>& J = jobs[job];
-    if (solver && job_inactive(J.st)) return;
-    const int tile = blockIdx.x - tstart[job];
-    const int x = tile / J.nfy;
-    const int f0 = (tile - x * J.nfy) * J.tfy;
-    const int F = J.Mz * 3;
-    const int nf = min(J.tfy, F - f0);
-    const int Ny = J.Ny, My = J.My;
-    const int ps = J.ps_y;
-    Cx<T>* twl = reinterpret_cast<Cx<T>*>(smem);
-    Cx<T>* A = twl + tw2_slots(My, J.ply.tb);
-    Cx<T>* B = A + J.tfy * ps;
-    load_twiddles(twl, J.twy, J.ply);
-    const int nin = INV ? My : Ny;
-    const int nout = INV ? Ny : My;
-    const Cx<T>* src = (INV ? J.w2 + (size_t)x * My * F : J.w1 + (size_t)x * Ny * F) + f0;
-    Cx<T>* dst = (INV ? J.w1 + (size_t)x * Ny * F : J.w2 + (size_t)x * My * F) + f0;
-    const float rnf = 1.0f / (float)nf;
-    for (int e = threadIdx.x; e < nin * nf; e += blockDim.x) {
-        int y, f;
-        fdivmod(e, nf, rnf, y, f);
-        A[f * ps + lpad(y)] = src[(size_t)y * F + f];
-    }
-    if (!INV) {
-        for (int e = threadIdx.x; e < (My - Ny) * nf; e += blockDim.x) {
-            int y, f;
-            fdivmod(e, nf, rnf, y, f);
-            A[f * ps + lpad(Ny + y)] = Cx<T>{T(0), T(0)};
-        }
-    }
-    __syncthreads();
-    const Cx<T>* res = pp_fft<T, INV>(A, B, nf, ps, J.ply, twl);
-    for (int e = threadIdx.x; e < nout * nf; e += blockDim.x) {
-        int y, f;
-        fdivmod(e, nf, rnf, y, f);
-        dst[(size_t)y * F + f] = res[f * ps + lpad(y)];
-    }
-}
-
-// ---------------------------------------------------------------------------
-// P3: x pass fused with the Green product (kernel.cu:74-95).
-// Tile = (ky, tfx consecutive flattened (kz, c) columns, tfx % 3 == 0).
+// Operator passes.  Every pass is a persistent kernel: a workgroup walks the
+// flattened tile list (all jobs) with stride gridDim.x and software-pipelines
+// it -- iteration i moves the prefetched inputs of tile i-1 into LDS, issues
+// the global loads of tile i into registers, then transforms tile i-1, so the
+// loads overlap the transform.  Ept<T> is the number of prefetched elements
+// per thread; tiles hold at most kThreads * Ept<T>::v elements (P1: a third
+// of that per input vector).
 // ---------------------------------------------------------------------------
 template <typename T>
-__global__ void TDDA_PASS_BOUNDS k_px(const JobDesc<T>* __restrict__ jobs, const int* __restrict__ tstart,
-                                                 int njobs, int solver)
+struct Ept;
+template <>
+struct Ept<double> {
+    static constexpr int v = TDDA_EPT_C128;
+};
+template <>
+struct Ept<float> {
+    static constexpr int v = 12;
+};
+
+// Prefetch registers: real and imaginary parts in separate scalar arrays
+// (an array of 16-byte complex structs is left in scratch memory by the
+// compiler; scalar arrays are promoted to registers).
+template <typename T, int N>
+struct Pre {
+    T re[N], im[N];
+    __device__ __forceinline__ Cx<T> get(int k) const { return Cx<T>{re[k], im[k]}; }
+    __device__ __forceinline__ void set(int k, Cx<T> v)
+    {
+        re[k] = v.x;
+        im[k] = v.y;
+    }
+};
+
+// Geometry of a z-pass tile (P1 / P5): pencils [q0, q0 + np) of job `job`.
+struct ZTile {
+    int job, jt, q0, np;
+    bool live;
+};
+
+template <typename T>
+__device__ __forceinline__ ZTile z_tile(const JobDesc<T>* __restrict__ jobs, const int* __restrict__ tstart,
+                                        int njobs, int tl, int pz_sel, bool check_state)
+{
+    ZTile z;
+    z.job = find_job(tstart, njobs, tl);
+    const JobDesc<T>* J = jobs + z.job;
+    const int pz = pz_sel ? J->pz1 : J->pz;
+    z.jt = tl - tstart[z.job];
+    z.q0 = z.jt * pz;
+    z.np = min(pz, J->Nx * J->Ny - z.q0);
+    z.live = !(check_state && job_inactive(J->st));
+    return z;
+}
+
+// Geometry of a strided-pass tile (P2/P4: outer = x; P3: outer = ky).
+struct STile {
+    int job, outer, f0, nf;
+    bool live;
+};
+
+template <typename T>
+__device__ __forceinline__ STile s_tile(const JobDesc<T>* __restrict__ jobs, const int* __restrict__ tstart,
+                                        int njobs, int tl, bool xpass, bool check_state)
+{
+    STile t;
+    t.job = find_job(tstart, njobs, tl);
+    const JobDesc<T>* J = jobs + t.job;
+    const int nfb = xpass ? J->nfx : J->nfy;
+    const int tf = xpass ? J->tfx : J->tfy;
+    const int r = tl - tstart[t.job];
+    t.outer = r / nfb;
+    t.f0 = (r - t.outer * nfb) * tf;
+    t.nf = min(tf, J->Mz * 3 - t.f0);
+    t.live = !(check_state && job_inactive(J->st));
+    return t;
+}
+
+// P1: z-forward.  Tile = pz1 consecutive (x, y) pencils.
+template <typename T, int MODE>
+__global__ void TDDA_PASS_BOUNDS k_p1(const JobDesc<T>* __restrict__ jobs, const int* __restrict__ tstart,
+                                      int njobs)
 {
     extern __shared__ __align__(16) char smem[];
-    const int job = find_job(tstart, njobs, blockIdx.x);
-    const JobDesc<T>& J = jobs[job];
-    if (solver && job_inactive(J.st)) return;
-    const int tile = blockIdx.x - tstart[job];
-    const int ky = tile / J.nfx;
-    const int f0 = (tile - ky * J.nfx) * J.tfx;
-    const int F = J.Mz * 3;
-    const int nf = min(J.tfx, F - f0);
-    const int Nx = J.Nx, Mx = J.Mx, My = J.My, Mz = J.Mz;
-    const int ps = J.ps_x;
-    Cx<T>* twl = reinterpret_cast<Cx<T>*>(smem);
-    Cx<T>* A = twl + tw2_slots(Mx, J.plx.tb);
-    Cx<T>* B = A + J.tfx * ps;
-    load_twiddles(twl, J.twx, J.plx);
-    const size_t sx = (size_t)My * F;
-    Cx<T>* base = J.w2 + (size_t)ky * F + f0;
-    const float rnf = 1.0f / (float)nf;
-    for (int e = threadIdx.x; e < Nx * nf; e += blockDim.x) {
-        int x, f;
-        fdivmod(e, nf, rnf, x, f);
-        A[f * ps + lpad(x)] = base[x * sx + f];
-    }
-    for (int e = threadIdx.x; e < (Mx - Nx) * nf; e += blockDim.x) {
-        int x, f;
-        fdivmod(e, nf, rnf, x, f);
-        A[f * ps + lpad(Nx + x)] = Cx<T>{T(0), T(0)};
-    }
-
-    // Green product.  Octant fold per axis: k' = min(k, M - k); odd
-    // components flip sign when k was folded.
-    const int kyf = (ky > My / 2) ? My - ky : ky;
-    const T sy = (ky > My / 2) ? T(-1) : T(1);
-    const int nkz = nf / 3;
-    const int Hy = J.Hy, Hz = J.Hz;
-    const float rnkz = 1.0f / (float)nkz;
-    __syncthreads();
-    Cx<T>* res = pp_fft<T, false>(A, B, nf, ps, J.plx, twl);
-    Cx<T>* other = (res == A) ? B : A;
-    for (int e = threadIdx.x; e < Mx * nkz; e += kThreads) {
-        int kx, kl;
-        fdivmod(e, nkz, rnkz, kx, kl);
-        const int kz = f0 / 3 + kl;
-        const int kxf = (kx > Mx / 2) ? Mx - kx : kx;
-        const T sxv = (kx > Mx / 2) ? T(-1) : T(1);
-        const int kzf = (kz > Mz / 2) ? Mz - kz : kz;
-        const T szv = (kz > Mz / 2) ? T(-1) : T(1);
-        const Cx<T>* gp = J.green + ((size_t)(kxf * Hy + kyf) * Hz + kzf) * 6;
-        const Cx<T> gxx = gp[0], gyy = gp[3], gzz = gp[5];
-        const Cx<T> gxy = cscale(gp[1], sxv * sy);
-        const Cx<T> gxz = cscale(gp[2], sxv * szv);
-        const Cx<T> gyz = cscale(gp[4], sy * szv);
-        Cx<T>* v = res + (kl * 3) * ps + lpad(kx);
-        const Cx<T> bx = v[0], by = v[ps], bz = v[2 * ps];
-        v[0] = cadd(cadd(cmul(gxx, bx), cmul(gxy, by)), cmul(gxz, bz));
-        v[ps] = cadd(cadd(cmul(gxy, bx), cmul(gyy, by)), cmul(gyz, bz));
-        v[2 * ps] = cadd(cadd(cmul(gxz, bx), cmul(gyz, by)), cmul(gzz, bz));
-    }
-    __syncthreads();
-    const Cx<T>* res2 = pp_fft<T, true>(res, other, nf, ps, J.plx, twl);
-    for (int e = threadIdx.x; e < Nx * nf; e += blockDim.x) {
-        int x, f;
-        fdivmod(e, nf, rnf, x, f);
-        base[x * sx + f] = res2[f * ps + lpad(x)];
+    constexpr int NV = (MODE == IN_P) ? 3 : ((MODE == IN_T) ? 2 : 1);
+    constexpr int EP = Ept<T>::v / 3;
+    const int ntot = tstart[njobs];
+    const float r3 = 1.0f / 3.0f;
+    Pre<T, EP> pr[NV];
+    ZTile cur;
+    cur.live = false;
+    for (int tl = blockIdx.x;; tl += gridDim.x) {
+        // 1. inputs of the current tile -> LDS (fused BiCGSTAB vector updates)
+        if (cur.live) {
+            const JobDesc<T>* J = jobs + cur.job;
+            const int Nz = J->Nz, Mz = J->Mz, ps = J->ps_z;
+            Cx<T>* twl = reinterpret_cast<Cx<T>*>(smem);
+            Cx<T>* A = twl + tw2_slots(Mz, J->plz.tb);
+            load_twiddles(twl, J->twz, J->plz);
+            Cx<T> sa{T(0), T(0)}, sb{T(0), T(0)};
+            bool first = false;
+            if (MODE == IN_P) {
+                const SolverState* st = J->st;
+                first = st->first != 0;
+                if (!first) {
+                    // beta = (alpha / zeta) * rho / rho_prev   (DDAModel.cpp:218)
+                    double q1r, q1i, q2r, q2i;
+                    cdiv(st->alpha[0], st->alpha[1], st->zeta[0], st->zeta[1], q1r, q1i);
+                    cdiv(st->rho[0], st->rho[1], st->rho_prev[0], st->rho_prev[1], q2r, q2i);
+                    sa = from_d<T>(q1r * q2r - q1i * q2i, q1r * q2i + q1i * q2r);
+                    sb = from_d<T>(st->zeta[0], st->zeta[1]);
+                }
+            } else if (MODE == IN_T) {
+                sa = from_d<T>(J->st->alpha[0], J->st->alpha[1]);
+            }
+            const int nin = cur.np * Nz * 3;
+            const int base = cur.q0 * Nz;
+            const float rNz3 = 1.0f / (float)(Nz * 3);
+#pragma unroll
+            for (int k = 0; k < EP; k++) {
+                const int e = threadIdx.x + k * kThreads;
+                if (e >= nin) continue;
+                int pl, rem, z, c;
+                fdivmod(e, Nz * 3, rNz3, pl, rem);
+                fdivmod(rem, 3, r3, z, c);
+                Cx<T> v = pr[0].get(k);
+                if (MODE == IN_P || MODE == IN_T) {
+                    int idx = -1;
+                    if (J->cellmap) {
+                        const int site = J->cellmap[base + pl * Nz + z];
+                        if (site >= 0) idx = site * 3 + c;
+                    } else {
+                        idx = base * 3 + e;
+                    }
+                    if (MODE == IN_P) {
+                        // p = r + beta (p - u), u = zeta Ap   (DDAModel.cpp:219)
+                        if (!first) v = cadd(pr[0].get(k), cmul(sa, csub(pr[1].get(k), cmul(sb, pr[2].get(k)))));
+                        if (idx >= 0) J->p[idx] = v;
+                    } else {
+                        // t = r - alpha Ap   (DDAModel.cpp:222)
+                        v = csub(pr[0].get(k), cmul(sa, pr[1].get(k)));
+                        if (idx >= 0) J->t[idx] = v;
+                    }
+                }
+                A[(pl * 3 + c) * ps + lpad(z)] = v;
+            }
+            const int npad = Mz - Nz;
+            if (npad > 0) {
+                const float rpad = 1.0f / (float)npad;
+                for (int e = threadIdx.x; e < cur.np * 3 * npad; e += kThreads) {
+                    int pc, zz;
+                    fdivmod(e, npad, rpad, pc, zz);
+                    A[pc * ps + lpad(Nz + zz)] = Cx<T>{T(0), T(0)};
+                }
+            }
+        }
+        // 2. prefetch of the next tile
+        ZTile nxt;
+        nxt.live = false;
+        if (tl < ntot) {
+            nxt = z_tile(jobs, tstart, njobs, tl, 1, MODE != IN_PLAIN);
+            if (nxt.live) {
+                const JobDesc<T>* J = jobs + nxt.job;
+                const int Nz = J->Nz;
+                const int nin = nxt.np * Nz * 3;
+                const int base = nxt.q0 * Nz;
+                const float rNz3 = 1.0f / (float)(Nz * 3);
+#pragma unroll
+                for (int k = 0; k < EP; k++) {
+                    const int e = threadIdx.x + k * kThreads;
+                    if (e >= nin) continue;
+                    int idx;
+                    if (J->cellmap) {
+                        int pl, rem, z, c;
+                        fdivmod(e, Nz * 3, rNz3, pl, rem);
+                        fdivmod(rem, 3, r3, z, c);
+                        const int site = J->cellmap[base + pl * Nz + z];
+                        idx = site >= 0 ? site * 3 + c : -1;
+                    } else {
+                        idx = base * 3 + e;  // bulk order: the tile is one contiguous run
+                    }
+                    if (idx < 0) {
+#pragma unroll
+                        for (int v = 0; v < NV; v++) pr[v].set(k, Cx<T>{T(0), T(0)});
+                        continue;
+                    }
+                    if (MODE == IN_PLAIN) pr[0].set(k, J->in[idx]);
+                    else if (MODE == IN_X) pr[0].set(k, J->x[idx]);
+                    else if (MODE == IN_P) {
+                        pr[0].set(k, J->r[idx]);
+                        pr[1].set(k, J->p[idx]);
+                        pr[2].set(k, J->ap[idx]);
+                    } else {
+                        pr[0].set(k, J->r[idx]);
+                        pr[1].set(k, J->ap[idx]);
+                    }
+                }
+            }
+        }
+        // 3. transform of the current tile
+        if (cur.live) {
+            const JobDesc<T>* J = jobs + cur.job;
+            const int Mz = J->Mz, ps = J->ps_z;
+            Cx<T>* twl = reinterpret_cast<Cx<T>*>(smem);
+            Cx<T>* A = twl + tw2_slots(Mz, J->plz.tb);
+            Cx<T>* B = A + 3 * J->pz1 * ps;
+            __syncthreads();
+            const Cx<T>* res = pp_fft<T, false>(A, B, cur.np * 3, ps, J->plz, twl);
+            const int nout = cur.np * Mz * 3;
+            Cx<T>* w1 = J->w1 + (size_t)cur.q0 * Mz * 3;
+            const float rMz3 = 1.0f / (float)(Mz * 3);
+            for (int e = threadIdx.x; e < nout; e += kThreads) {
+                int pl, rem, kz, c;
+                fdivmod(e, Mz * 3, rMz3, pl, rem);
+                fdivmod(rem, 3, r3, kz, c);
+                w1[e] = res[(pl * 3 + c) * ps + lpad(kz)];
+            }
+            __syncthreads();
+        }
+        if (tl >= ntot) break;
+        cur = nxt;
     }
 }
 
-// ---------------------------------------------------------------------------
+// P2 / P4: y pass.  Tile = (x, tfy consecutive flattened (kz, c) columns).
+// FWD: w1[x][y<Ny][.] -> w2[x][ky<My][.];  INV: w2[x][ky][.] -> w1[x][y<Ny][.]
+template <typename T, bool INV>
+__global__ void TDDA_PASS_BOUNDS k_py(const JobDesc<T>* __restrict__ jobs, const int* __restrict__ tstart,
+                                      int njobs, int solver)
+{
+    extern __shared__ __align__(16) char smem[];
+    constexpr int EP = Ept<T>::v;
+    const int ntot = tstart[njobs];
+    Pre<T, EP> pr;
+    STile cur;
+    cur.live = false;
+    for (int tl = blockIdx.x;; tl += gridDim.x) {
+        if (cur.live) {
+            const JobDesc<T>* J = jobs + cur.job;
+            const int Ny = J->Ny, My = J->My, ps = J->ps_y, nf = cur.nf;
+            Cx<T>* twl = reinterpret_cast<Cx<T>*>(smem);
+            Cx<T>* A = twl + tw2_slots(My, J->ply.tb);
+            load_twiddles(twl, J->twy, J->ply);
+            const int nin = INV ? My : Ny;
+            const float rnf = 1.0f / (float)nf;
+#pragma unroll
+            for (int k = 0; k < EP; k++) {
+                const int e = threadIdx.x + k * kThreads;
+                if (e < nin * nf) {
+                    int y, f;
+                    fdivmod(e, nf, rnf, y, f);
+                    A[f * ps + lpad(y)] = pr.get(k);
+                }
+            }
+            if (!INV) {
+                for (int e = threadIdx.x; e < (My - Ny) * nf; e += kThreads) {
+                    int y, f;
+                    fdivmod(e, nf, rnf, y, f);
+                    A[f * ps + lpad(Ny + y)] = Cx<T>{T(0), T(0)};
+                }
+            }
+        }
+        STile nxt;
+        nxt.live = false;
+        if (tl < ntot) {
+            nxt = s_tile(jobs, tstart, njobs, tl, false, solver != 0);
+            if (nxt.live) {
+                const JobDesc<T>* J = jobs + nxt.job;
+                const int F = J->Mz * 3, nf = nxt.nf;
+                const int nin = INV ? J->My : J->Ny;
+                const Cx<T>* src =
+                    (INV ? J->w2 + (size_t)nxt.outer * J->My * F : J->w1 + (size_t)nxt.outer * J->Ny * F) + nxt.f0;
+                const float rnf = 1.0f / (float)nf;
+#pragma unroll
+                for (int k = 0; k < EP; k++) {
+                    const int e = threadIdx.x + k * kThreads;
+                    if (e < nin * nf) {
+                        int y, f;
+                        fdivmod(e, nf, rnf, y, f);
+                        pr.set(k, src[(size_t)y * F + f]);
+                    }
+                }
+            }
+        }
+        if (cur.live) {
+            const JobDesc<T>* J = jobs + cur.job;
+            const int Ny = J->Ny, My = J->My, ps = J->ps_y, nf = cur.nf, F = J->Mz * 3;
+            Cx<T>* twl = reinterpret_cast<Cx<T>*>(smem);
+            Cx<T>* A = twl + tw2_slots(My, J->ply.tb);
+            Cx<T>* B = A + J->tfy * ps;
+            __syncthreads();
+            const Cx<T>* res = pp_fft<T, INV>(A, B, nf, ps, J->ply, twl);
+            const int nout = INV ? Ny : My;
+            Cx<T>* dst =
+                (INV ? J->w1 + (size_t)cur.outer * Ny * F : J->w2 + (size_t)cur.outer * My * F) + cur.f0;
+            const float rnf = 1.0f / (float)nf;
+            for (int e = threadIdx.x; e < nout * nf; e += kThreads) {
+                int y, f;
+                fdivmod(e, nf, rnf, y, f);
+                dst[(size_t)y * F + f] = res[f * ps + lpad(y)];
+            }
+            __syncthreads();
+        }
+        if (tl >= ntot) break;
+        cur = nxt;
+    }
+}
+
+// P3: x pass fused with the Green product (kernel.cu:74-95).
+// Tile = (ky, tfx consecutive flattened (kz, c) columns, tfx % 3 == 0).
+template <typename T>
+__global__ void TDDA_PASS_BOUNDS k_px(const JobDesc<T>* __restrict__ jobs, const int* __restrict__ tstart,
+                                      int njobs, int solver)
+{
+    extern __shared__ __align__(16) char smem[];
+    constexpr int EP = Ept<T>::v;
+    const int ntot = tstart[njobs];
+    Pre<T, EP> pr;
+    STile cur;
+    cur.live = false;
+    for (int tl = blockIdx.x;; tl += gridDim.x) {
+        if (cur.live) {
+            const JobDesc<T>* J = jobs + cur.job;
+            const int Nx = J->Nx, Mx = J->Mx, ps = J->ps_x, nf = cur.nf;
+            Cx<T>* twl = reinterpret_cast<Cx<T>*>(smem);
+            Cx<T>* A = twl + tw2_slots(Mx, J->plx.tb);
+            load_twiddles(twl, J->twx, J->plx);
+            const float rnf = 1.0f / (float)nf;
+#pragma unroll
+            for (int k = 0; k < EP; k++) {
+                const int e = threadIdx.x + k * kThreads;
+                if (e < Nx * nf) {
+                    int x, f;
+                    fdivmod(e, nf, rnf, x, f);
+                    A[f * ps + lpad(x)] = pr.get(k);
+                }
+            }
+            for (int e = threadIdx.x; e < (Mx - Nx) * nf; e += kThreads) {
+                int x, f;
+                fdivmod(e, nf, rnf, x, f);
+                A[f * ps + lpad(Nx + x)] = Cx<T>{T(0), T(0)};
+            }
+        }
+        STile nxt;
+        nxt.live = false;
+        if (tl < ntot) {
+            nxt = s_tile(jobs, tstart, njobs, tl, true, solver != 0);
+            if (nxt.live) {
+                const JobDesc<T>* J = jobs + nxt.job;
+                const int F = J->Mz * 3, nf = nxt.nf;
+                const size_t sx = (size_t)J->My * F;
+                const Cx<T>* base = J->w2 + (size_t)nxt.outer * F + nxt.f0;
+                const float rnf = 1.0f / (float)nf;
+#pragma unroll
+                for (int k = 0; k < EP; k++) {
+                    const int e = threadIdx.x + k * kThreads;
+                    if (e < J->Nx * nf) {
+                        int x, f;
+                        fdivmod(e, nf, rnf, x, f);
+                        pr.set(k, base[x * sx + f]);
+                    }
+                }
+            }
+        }
+        if (cur.live) {
+            const JobDesc<T>* J = jobs + cur.job;
+            const int Nx = J->Nx, Mx = J->Mx, My = J->My, Mz = J->Mz, F = Mz * 3;
+            const int ps = J->ps_x, nf = cur.nf, ky = cur.outer, f0 = cur.f0;
+            Cx<T>* twl = reinterpret_cast<Cx<T>*>(smem);
+            Cx<T>* A = twl + tw2_slots(Mx, J->plx.tb);
+            Cx<T>* B = A + J->tfx * ps;
+            __syncthreads();
+            Cx<T>* res = pp_fft<T, false>(A, B, nf, ps, J->plx, twl);
+            Cx<T>* other = (res == A) ? B : A;
+            // Green product.  Octant fold per axis: k' = min(k, M - k); odd
+            // components flip sign when k was folded.
+            const int kyf = (ky > My / 2) ? My - ky : ky;
+            const T sy = (ky > My / 2) ? T(-1) : T(1);
+            const int nkz = nf / 3;
+            const int Hy = J->Hy, Hz = J->Hz;
+            const float rnkz = 1.0f / (float)nkz;
+            for (int e = threadIdx.x; e < Mx * nkz; e += kThreads) {
+                int kx, kl;
+                fdivmod(e, nkz, rnkz, kx, kl);
+                const int kz = f0 / 3 + kl;
+                const int kxf = (kx > Mx / 2) ? Mx - kx : kx;
+                const T sxv = (kx > Mx / 2) ? T(-1) : T(1);
+                const int kzf = (kz > Mz / 2) ? Mz - kz : kz;
+                const T szv = (kz > Mz / 2) ? T(-1) : T(1);
+                const Cx<T>* gp = J->green + ((size_t)(kxf * Hy + kyf) * Hz + kzf) * 6;
+                const Cx<T> gxx = gp[0], gyy = gp[3], gzz = gp[5];
+                const Cx<T> gxy = cscale(gp[1], sxv * sy);
+                const Cx<T> gxz = cscale(gp[2], sxv * szv);
+                const Cx<T> gyz = cscale(gp[4], sy * szv);
+                Cx<T>* v = res + (kl * 3) * ps + lpad(kx);
+                const Cx<T> bx = v[0], by = v[ps], bz = v[2 * ps];
+                v[0] = cadd(cadd(cmul(gxx, bx), cmul(gxy, by)), cmul(gxz, bz));
+                v[ps] = cadd(cadd(cmul(gxy, bx), cmul(gyy, by)), cmul(gyz, bz));
+                v[2 * ps] = cadd(cadd(cmul(gxz, bx), cmul(gyz, by)), cmul(gzz, bz));
+            }
+            __syncthreads();
+            const Cx<T>* res2 = pp_fft<T, true>(res, other, nf, ps, J->plx, twl);
+            const size_t sx = (size_t)My * F;
+            Cx<T>* base = J->w2 + (size_t)ky * F + f0;
+            const float rnf = 1.0f / (float)nf;
+            for (int e = threadIdx.x; e < Nx * nf; e += kThreads) {
+                int x, f;
+                fdivmod(e, nf, rnf, x, f);
+                base[x * sx + f] = res2[f * ps + lpad(x)];
+            }
+            __syncthreads();
+        }
+        if (tl >= ntot) break;
+        cur = nxt;
+    }
+}
+
 // P5: z-inverse, crop to the box, diagonal term and solver epilogues.
-// ---------------------------------------------------------------------------
+// Tile = pz consecutive (x, y) pencils.
 template <typename T, int MODE>
 __global__ void TDDA_PASS_BOUNDS k_p5(const JobDesc<T>* __restrict__ jobs, const int* __restrict__ tstart,
-                                                 int njobs, int diag)
+                                      int njobs, int diag)
 {
     extern __shared__ __align__(16) char smem[];
     __shared__ double red[16 * 4];
     __shared__ int s_last;
-    const int job = find_job(tstart, njobs, blockIdx.x);
-    const JobDesc<T>& J = jobs[job];
-    if (MODE != OUT_PLAIN && job_inactive(J.st)) return;
-    const int tile = blockIdx.x - tstart[job];
-    const int ntiles = tstart[job + 1] - tstart[job];
-    const int Ny = J.Ny, Nz = J.Nz, Mz = J.Mz;
-    const int Q = J.Nx * Ny;
-    const int q0 = tile * J.pz;
-    const int np = min(J.pz, Q - q0);
-    const int ps = J.ps_z;
-    Cx<T>* twl = reinterpret_cast<Cx<T>*>(smem);
-    Cx<T>* A = twl + tw2_slots(Mz, J.plz.tb);
-    Cx<T>* B = A + 3 * J.pz * ps;
-    load_twiddles(twl, J.twz, J.plz);
-    const Cx<T>* w1 = J.w1 + (size_t)q0 * Mz * 3;
-    const int nin = np * Mz * 3;
-    const float r3 = 1.0f / 3.0f, rMz3 = 1.0f / (float)(Mz * 3), rNz3 = 1.0f / (float)(Nz * 3);
-    for (int e = threadIdx.x; e < nin; e += blockDim.x) {
-        int pl, rem, kz, c;
-        fdivmod(e, Mz * 3, rMz3, pl, rem);
-        fdivmod(rem, 3, r3, kz, c);
-        A[(pl * 3 + c) * ps + lpad(kz)] = w1[e];
-    }
-    __syncthreads();
-    const Cx<T>* res = pp_fft<T, true>(A, B, np * 3, ps, J.plz, twl);
-
-    double acc[4] = {0.0, 0.0, 0.0, 0.0};
-    const int nout = np * Nz * 3;
-    for (int e = threadIdx.x; e < nout; e += blockDim.x) {
-        int pl, rem, z, c;
-        fdivmod(e, Nz * 3, rNz3, pl, rem);
-        fdivmod(rem, 3, r3, z, c);
-        const int cell = (q0 + pl) * Nz + z;
-        const int site = J.cellmap ? J.cellmap[cell] : cell;
-        if (site < 0) continue;
-        const int i = site * 3 + c;
-        const Cx<T> conv = res[(pl * 3 + c) * ps + lpad(z)];
-        if (MODE == OUT_PLAIN) {
-            Cx<T> y = conv;
-            if (diag) y = cadd(y, cmul(J.al[i], J.in[i]));
-            J.out[i] = y;
-        } else if (MODE == OUT_RES) {
-            // r = E - A x ; r0 = r   (DDAModel.cpp:163-166)
-            const Cx<T> ax = cadd(conv, cmul(J.al[i], J.x[i]));
-            const Cx<T> ev = J.e[i];
-            const Cx<T> r = csub(ev, ax);
-            J.r[i] = r;
-            J.r0[i] = r;
-            const double2 rd = to_d(r), ed = to_d(ev);
-            acc[0] += rd.x * rd.x + rd.y * rd.y;
-            acc[1] += ed.x * ed.x + ed.y * ed.y;
-        } else if (MODE == OUT_AP) {
-            const Cx<T> ap = cadd(conv, cmul(J.al[i], J.p[i]));
-            J.ap[i] = ap;
-            const double2 a = to_d(J.r0[i]), b = to_d(ap);
-            acc[0] += a.x * b.x + a.y * b.y;  // conj(r0) Ap
-            acc[1] += a.x * b.y - a.y * b.x;
-        } else {  // OUT_AT
-            const Cx<T> at = cadd(conv, cmul(J.al[i], J.t[i]));
-            J.at[i] = at;
-            const double2 a = to_d(at), b = to_d(J.t[i]);
-            acc[0] += a.x * b.x + a.y * b.y;  // conj(At) t
-            acc[1] += a.x * b.y - a.y * b.x;
-            acc[2] += a.x * a.x + a.y * a.y;  // conj(At) At
+    constexpr int EP = Ept<T>::v;
+    const int ntot = tstart[njobs];
+    const float r3 = 1.0f / 3.0f;
+    Pre<T, EP> pr;
+    ZTile cur;
+    cur.live = false;
+    for (int tl = blockIdx.x;; tl += gridDim.x) {
+        if (cur.live) {
+            const JobDesc<T>* J = jobs + cur.job;
+            const int Mz = J->Mz, ps = J->ps_z;
+            Cx<T>* twl = reinterpret_cast<Cx<T>*>(smem);
+            Cx<T>* A = twl + tw2_slots(Mz, J->plz.tb);
+            load_twiddles(twl, J->twz, J->plz);
+            const float rMz3 = 1.0f / (float)(Mz * 3);
+#pragma unroll
+            for (int k = 0; k < EP; k++) {
+                const int e = threadIdx.x + k * kThreads;
+                if (e < cur.np * Mz * 3) {
+                    int pl, rem, kz, c;
+                    fdivmod(e, Mz * 3, rMz3, pl, rem);
+                    fdivmod(rem, 3, r3, kz, c);
+                    A[(pl * 3 + c) * ps + lpad(kz)] = pr.get(k);
+                }
+            }
         }
-    }
-    if (MODE == OUT_PLAIN) return;
-    block_sum<4>(acc, red);
-    double tot[4];
-    if (!publish_partials<4>(J.part, J.cnt, tile, ntiles, acc, tot, &s_last)) return;
-    SolverState* st = J.st;
-    if (MODE == OUT_RES) {
-        st->rho[0] = tot[0];  // r0.r = |r|^2
-        st->rho[1] = 0.0;
-        st->enorm = sqrt(tot[1]);
-        st->rnorm = sqrt(tot[0]);
-    } else if (MODE == OUT_AP) {
-        // alpha = r0.r / r0.Ap   (DDAModel.cpp:168, 221)
-        cdiv(st->rho[0], st->rho[1], tot[0], tot[1], st->alpha[0], st->alpha[1]);
-    } else {
-        // zeta = At.t / At.At   (DDAModel.cpp:172, 225)
-        cdiv(tot[0], tot[1], tot[2], 0.0, st->zeta[0], st->zeta[1]);
+        ZTile nxt;
+        nxt.live = false;
+        if (tl < ntot) {
+            nxt = z_tile(jobs, tstart, njobs, tl, 0, MODE != OUT_PLAIN);
+            if (nxt.live) {
+                const JobDesc<T>* J = jobs + nxt.job;
+                const int Mz = J->Mz;
+                const Cx<T>* w1 = J->w1 + (size_t)nxt.q0 * Mz * 3;
+#pragma unroll
+                for (int k = 0; k < EP; k++) {
+                    const int e = threadIdx.x + k * kThreads;
+                    if (e < nxt.np * Mz * 3) pr.set(k, w1[e]);
+                }
+            }
+        }
+        if (cur.live) {
+            const JobDesc<T>* J = jobs + cur.job;
+            const int Nz = J->Nz, Mz = J->Mz, ps = J->ps_z;
+            Cx<T>* twl = reinterpret_cast<Cx<T>*>(smem);
+            Cx<T>* A = twl + tw2_slots(Mz, J->plz.tb);
+            Cx<T>* B = A + 3 * J->pz * ps;
+            __syncthreads();
+            const Cx<T>* res = pp_fft<T, true>(A, B, cur.np * 3, ps, J->plz, twl);
+            double acc[4] = {0.0, 0.0, 0.0, 0.0};
+            const int nout = cur.np * Nz * 3;
+            const int base = cur.q0 * Nz;
+            const float rNz3 = 1.0f / (float)(Nz * 3);
+            for (int e = threadIdx.x; e < nout; e += kThreads) {
+                int pl, rem, z, c;
+                fdivmod(e, Nz * 3, rNz3, pl, rem);
+                fdivmod(rem, 3, r3, z, c);
+                int i;
+                if (J->cellmap) {
+                    const int site = J->cellmap[base + pl * Nz + z];
+                    if (site < 0) continue;
+                    i = site * 3 + c;
+                } else {
+                    i = base * 3 + e;
+                }
+                const Cx<T> conv = res[(pl * 3 + c) * ps + lpad(z)];
+                if (MODE == OUT_PLAIN) {
+                    Cx<T> y = conv;
+                    if (diag) y = cadd(y, cmul(J->al[i], J->in[i]));
+                    J->out[i] = y;
+                } else if (MODE == OUT_RES) {
+                    // r = E - A x ; r0 = r   (DDAModel.cpp:163-166)
+                    const Cx<T> ax = cadd(conv, cmul(J->al[i], J->x[i]));
+                    const Cx<T> ev = J->e[i];
+                    const Cx<T> r = csub(ev, ax);
+                    J->r[i] = r;
+                    J->r0[i] = r;
+                    const double2 rd = to_d(r), ed = to_d(ev);
+                    acc[0] += rd.x * rd.x + rd.y * rd.y;
+                    acc[1] += ed.x * ed.x + ed.y * ed.y;
+                } else if (MODE == OUT_AP) {
+                    const Cx<T> ap = cadd(conv, cmul(J->al[i], J->p[i]));
+                    J->ap[i] = ap;
+                    const double2 a = to_d(J->r0[i]), b = to_d(ap);
+                    acc[0] += a.x * b.x + a.y * b.y;  // conj(r0) Ap
+                    acc[1] += a.x * b.y - a.y * b.x;
+                } else {  // OUT_AT
+                    const Cx<T> at = cadd(conv, cmul(J->al[i], J->t[i]));
+                    J->at[i] = at;
+                    const double2 a = to_d(at), b = to_d(J->t[i]);
+                    acc[0] += a.x * b.x + a.y * b.y;  // conj(At) t
+                    acc[1] += a.x * b.y - a.y * b.x;
+                    acc[2] += a.x * a.x + a.y * a.y;  // conj(At) At
+                }
+            }
+            if (MODE != OUT_PLAIN) {
+                block_sum<4>(acc, red);
+                double tot[4];
+                const int ntiles = tstart[cur.job + 1] - tstart[cur.job];
+                if (publish_partials<4>(J->part, J->cnt, cur.jt, ntiles, acc, tot, &s_last)) {
+                    SolverState* st = J->st;
+                    if (MODE == OUT_RES) {
+                        st->rho[0] = tot[0];  // r0.r = |r|^2
+                        st->rho[1] = 0.0;
+                        st->enorm = sqrt(tot[1]);
+                        st->rnorm = sqrt(tot[0]);
+                    } else if (MODE == OUT_AP) {
+                        // alpha = r0.r / r0.Ap   (DDAModel.cpp:168, 221)
+                        cdiv(st->rho[0], st->rho[1], tot[0], tot[1], st->alpha[0], st->alpha[1]);
+                    } else {
+                        // zeta = At.t / At.At   (DDAModel.cpp:172, 225)
+                        cdiv(tot[0], tot[1], tot[2], 0.0, st->zeta[0], st->zeta[1]);
+                    }
+                }
+            }
+            __syncthreads();
+        }
+        if (tl >= ntot) break;
+        cur = nxt;
     }
 }
 
