@@ -14,6 +14,7 @@
 
 #include "topodda.h"
 #include "tdda_kernels.hpp"
+#include "tdda_launch.hpp"
 
 using namespace tdda;
 
@@ -174,6 +175,7 @@ struct JobTable {
     int* tv = nullptr;
     int nz = 0, nz1 = 0, ny = 0, nx = 0, nv = 0;
     int gz = 0, gz1 = 0, gy = 0, gx = 0;  // persistent grid sizes
+    int Lx = 0, Ly = 0, Lz = 0;           // FFT lengths of the passes
     SolverState* st = nullptr;
     double* part = nullptr;
     unsigned* cnt = nullptr;
@@ -346,6 +348,9 @@ int build_table(tdda_model_s* m, JobTable<T>& jt, const VecSet<T>& v, const Cx<T
         hx[j] = j * tiles_x;
         hv[j] = j * tiles_v;
     }
+    jt.Lx = op->Mx;
+    jt.Ly = op->My;
+    jt.Lz = op->Mz;
     jt.nz = hz[nj];
     jt.nz1 = hz1[nj];
     jt.ny = hy[nj];
@@ -376,38 +381,38 @@ int build_table(tdda_model_s* m, JobTable<T>& jt, const VecSet<T>& v, const Cx<T
 }
 
 template <typename T>
-void launch_p1(const JobTable<T>& jt, int nj, int mode, hipStream_t s)
+void run_p1(const JobTable<T>& jt, int nj, int mode, hipStream_t s)
 {
-    const dim3 g(jt.gz1), b(kThreads);
-    switch (mode) {
-    case IN_PLAIN: k_p1<T, IN_PLAIN><<<g, b, jt.tp.lds_z1, s>>>(jt.d, jt.tz1, nj); break;
-    case IN_P: k_p1<T, IN_P><<<g, b, jt.tp.lds_z1, s>>>(jt.d, jt.tz1, nj); break;
-    case IN_T: k_p1<T, IN_T><<<g, b, jt.tp.lds_z1, s>>>(jt.d, jt.tz1, nj); break;
-    default: k_p1<T, IN_X><<<g, b, jt.tp.lds_z1, s>>>(jt.d, jt.tz1, nj); break;
-    }
+    launch_p1<T>(jt.Lz, mode, PassLaunch{dim3(jt.gz1), jt.tp.lds_z1, s}, jt.d, jt.tz1, nj);
 }
 
 template <typename T>
-void launch_p5(const JobTable<T>& jt, int nj, int mode, int diag, hipStream_t s)
+void run_p5(const JobTable<T>& jt, int nj, int mode, int diag, hipStream_t s)
 {
-    const dim3 g(jt.gz), b(kThreads);
-    switch (mode) {
-    case OUT_PLAIN: k_p5<T, OUT_PLAIN><<<g, b, jt.tp.lds_z, s>>>(jt.d, jt.tz, nj, diag); break;
-    case OUT_RES: k_p5<T, OUT_RES><<<g, b, jt.tp.lds_z, s>>>(jt.d, jt.tz, nj, 1); break;
-    case OUT_AP: k_p5<T, OUT_AP><<<g, b, jt.tp.lds_z, s>>>(jt.d, jt.tz, nj, 1); break;
-    default: k_p5<T, OUT_AT><<<g, b, jt.tp.lds_z, s>>>(jt.d, jt.tz, nj, 1); break;
-    }
+    launch_p5<T>(jt.Lz, mode, diag, PassLaunch{dim3(jt.gz), jt.tp.lds_z, s}, jt.d, jt.tz, nj);
+}
+
+template <typename T>
+void run_py(const JobTable<T>& jt, int nj, bool inv, int solver, hipStream_t s)
+{
+    launch_py<T>(jt.Ly, inv, solver, PassLaunch{dim3(jt.gy), jt.tp.lds_y, s}, jt.d, jt.ty, nj);
+}
+
+template <typename T>
+void run_px(const JobTable<T>& jt, int nj, int solver, hipStream_t s)
+{
+    launch_px<T>(jt.Lx, solver, PassLaunch{dim3(jt.gx), jt.tp.lds_x, s}, jt.d, jt.tx, nj);
 }
 
 // one operator application: P1 .. P5
 template <typename T>
 void launch_apply(const JobTable<T>& jt, int nj, int inmode, int outmode, int diag, int solver, hipStream_t s)
 {
-    launch_p1<T>(jt, nj, inmode, s);
-    k_py<T, false><<<dim3(jt.gy), dim3(kThreads), jt.tp.lds_y, s>>>(jt.d, jt.ty, nj, solver);
-    k_px<T><<<dim3(jt.gx), dim3(kThreads), jt.tp.lds_x, s>>>(jt.d, jt.tx, nj, solver);
-    k_py<T, true><<<dim3(jt.gy), dim3(kThreads), jt.tp.lds_y, s>>>(jt.d, jt.ty, nj, solver);
-    launch_p5<T>(jt, nj, outmode, diag, s);
+    run_p1<T>(jt, nj, inmode, s);
+    run_py<T>(jt, nj, false, solver, s);
+    run_px<T>(jt, nj, solver, s);
+    run_py<T>(jt, nj, true, solver, s);
+    run_p5<T>(jt, nj, outmode, diag, s);
 }
 
 // one BiCGSTAB iteration (or the unrolled first step when state.first)
@@ -620,15 +625,15 @@ static int time_passes_t(tdda_model m, JobTable<T>& jt, int reps, double* ms5)
     double acc[5] = {0, 0, 0, 0, 0};
     for (int r = 0; r < reps + 1; r++) {
         HIPCHK(hipEventRecord(ev[0], s));
-        launch_p1<T>(jt, nj, IN_PLAIN, s);
+        run_p1<T>(jt, nj, IN_PLAIN, s);
         HIPCHK(hipEventRecord(ev[1], s));
-        k_py<T, false><<<dim3(jt.gy), dim3(kThreads), jt.tp.lds_y, s>>>(jt.d, jt.ty, nj, 0);
+        run_py<T>(jt, nj, false, 0, s);
         HIPCHK(hipEventRecord(ev[2], s));
-        k_px<T><<<dim3(jt.gx), dim3(kThreads), jt.tp.lds_x, s>>>(jt.d, jt.tx, nj, 0);
+        run_px<T>(jt, nj, 0, s);
         HIPCHK(hipEventRecord(ev[3], s));
-        k_py<T, true><<<dim3(jt.gy), dim3(kThreads), jt.tp.lds_y, s>>>(jt.d, jt.ty, nj, 0);
+        run_py<T>(jt, nj, true, 0, s);
         HIPCHK(hipEventRecord(ev[4], s));
-        launch_p5<T>(jt, nj, OUT_PLAIN, 1, s);
+        run_p5<T>(jt, nj, OUT_PLAIN, 1, s);
         HIPCHK(hipEventRecord(ev[5], s));
         HIPCHK(hipEventSynchronize(ev[5]));
         if (r == 0) continue;  // warm-up

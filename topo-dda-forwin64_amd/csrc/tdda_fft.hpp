@@ -322,3 +322,124 @@ __device__ __forceinline__ void load_twiddles(Cx<T>* __restrict__ dst, const Cx<
 }
 
 }  // namespace tdda
+
+namespace tdda {
+
+// ---------------------------------------------------------------------------
+// Compile-time specialisation of the pass transform for a fixed length L:
+// radix sequence, strides, twiddle steps and pencil stride are constants, so
+// every index division becomes a multiply-shift and the stage loops unroll.
+// Same data layout and twiddle table as pp_fft.
+// ---------------------------------------------------------------------------
+constexpr int ct_nst(int L)
+{
+    int n = L, e2 = 0, c = 0;
+    while (n % 2 == 0) {
+        n /= 2;
+        e2++;
+    }
+    c = e2 / 3 + (e2 % 3 ? 1 : 0);
+    while (n % 3 == 0) {
+        n /= 3;
+        c++;
+    }
+    while (n % 5 == 0) {
+        n /= 5;
+        c++;
+    }
+    return c;
+}
+
+constexpr int ct_rad(int L, int s)
+{
+    int n = L, e2 = 0, k = 0;
+    while (n % 2 == 0) {
+        n /= 2;
+        e2++;
+    }
+    while (e2 >= 3) {
+        if (k++ == s) return 8;
+        e2 -= 3;
+    }
+    if (e2 == 2 && k++ == s) return 4;
+    if (e2 == 1 && k++ == s) return 2;
+    while (n % 3 == 0) {
+        n /= 3;
+        if (k++ == s) return 3;
+    }
+    while (n % 5 == 0) {
+        n /= 5;
+        if (k++ == s) return 5;
+    }
+    return 1;
+}
+
+constexpr int ct_tb(int L)
+{
+    int t = 1;
+    while (t * t < L) t++;
+    return t;
+}
+
+template <typename T, int L, int R, int NS, bool INV, int NPMAX>
+__device__ __forceinline__ void ct_stage(const Cx<T>* __restrict__ src, Cx<T>* __restrict__ dst, int np,
+                                         const Cx<T>* __restrict__ tw)
+{
+    constexpr int LR = L / R;
+    constexpr int PS = L + (L >> 5) + 1;
+    constexpr int TWSTEP = L / (NS * R);
+    constexpr int TB = ct_tb(L);
+    constexpr int NBMAX = NPMAX * LR;
+    const int nb = np * LR;
+#pragma unroll
+    for (int it = 0; it < (NBMAX + kThreadsFft - 1) / kThreadsFft; it++) {
+        const int b = threadIdx.x + it * kThreadsFft;
+        if (b < nb) {
+            const int p = b / LR;
+            const int j = b - p * LR;
+            const int kk = j % NS;
+            const Cx<T>* sp = src + p * PS;
+            Cx<T> v[R];
+#pragma unroll
+            for (int m = 0; m < R; m++) v[m] = sp[lpad(j + m * LR)];
+            if constexpr (NS > 1) {
+                const int q = kk * TWSTEP;
+                const Cx<T> w1 = cmul(tw[q % TB], tw[TB + q / TB]);
+                Cx<T> w = w1;
+#pragma unroll
+                for (int m = 1; m < R; m++) {
+                    v[m] = INV ? cmulc(v[m], w) : cmul(v[m], w);
+                    if (m + 1 < R) w = cmul(w, w1);
+                }
+            }
+            dft<T, R, INV>(v);
+            Cx<T>* dp = dst + p * PS;
+            const int base = (j - kk) * R + kk;
+#pragma unroll
+            for (int m = 0; m < R; m++) dp[lpad(base + m * NS)] = v[m];
+        }
+    }
+}
+
+template <typename T, int L, int S, int NS, bool INV, int NPMAX>
+__device__ __forceinline__ Cx<T>* ct_stages(Cx<T>* a, Cx<T>* b, int np, const Cx<T>* tw)
+{
+    if constexpr (S == ct_nst(L)) {
+        return a;
+    } else {
+        constexpr int R = ct_rad(L, S);
+        ct_stage<T, L, R, NS, INV, NPMAX>(a, b, np, tw);
+        __syncthreads();
+        return ct_stages<T, L, S + 1, NS * R, INV, NPMAX>(b, a, np, tw);
+    }
+}
+
+// Transform of np <= NPMAX pencils of the compile-time length L; same
+// contract as pp_fft.
+template <typename T, int L, bool INV, int NPMAX>
+__device__ __forceinline__ Cx<T>* ct_fft(Cx<T>* a, Cx<T>* b, int np, const Cx<T>* tw)
+{
+    return ct_stages<T, L, 0, 1, INV, NPMAX>(a, b, np, tw);
+}
+
+}  // namespace tdda

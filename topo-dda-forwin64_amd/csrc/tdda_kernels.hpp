@@ -245,6 +245,24 @@ struct Pre {
     }
 };
 
+// Largest pencil count of a tile of length L (compile-time lengths).
+template <typename T>
+__host__ __device__ constexpr int np_max(int L)
+{
+    return L > 0 ? (kThreads * Ept<T>::v) / L : 0;
+}
+
+// Transform of a pass tile: compile-time length LC > 0, or the runtime plan.
+template <typename T, int LC, bool INV>
+__device__ __forceinline__ Cx<T>* pass_fft(Cx<T>* a, Cx<T>* b, int np, int ps, const FftPlan& pl, const Cx<T>* tw)
+{
+    if constexpr (LC > 0) {
+        return ct_fft<T, LC, INV, np_max<T>(LC)>(a, b, np, tw);
+    } else {
+        return pp_fft<T, INV>(a, b, np, ps, pl, tw);
+    }
+}
+
 // Geometry of a z-pass tile (P1 / P5): pencils [q0, q0 + np) of job `job`.
 struct ZTile {
     int job, jt, q0, np;
@@ -290,7 +308,7 @@ __device__ __forceinline__ STile s_tile(const JobDesc<T>* __restrict__ jobs, con
 }
 
 // P1: z-forward.  Tile = pz1 consecutive (x, y) pencils.
-template <typename T, int MODE>
+template <typename T, int MODE, int LC>
 __global__ void TDDA_PASS_BOUNDS k_p1(const JobDesc<T>* __restrict__ jobs, const int* __restrict__ tstart,
                                       int njobs)
 {
@@ -418,7 +436,7 @@ __global__ void TDDA_PASS_BOUNDS k_p1(const JobDesc<T>* __restrict__ jobs, const
             Cx<T>* A = twl + tw2_slots(Mz, J->plz.tb);
             Cx<T>* B = A + 3 * J->pz1 * ps;
             __syncthreads();
-            const Cx<T>* res = pp_fft<T, false>(A, B, cur.np * 3, ps, J->plz, twl);
+            const Cx<T>* res = pass_fft<T, LC, false>(A, B, cur.np * 3, ps, J->plz, twl);
             const int nout = cur.np * Mz * 3;
             Cx<T>* w1 = J->w1 + (size_t)cur.q0 * Mz * 3;
             const float rMz3 = 1.0f / (float)(Mz * 3);
@@ -437,7 +455,7 @@ __global__ void TDDA_PASS_BOUNDS k_p1(const JobDesc<T>* __restrict__ jobs, const
 
 // P2 / P4: y pass.  Tile = (x, tfy consecutive flattened (kz, c) columns).
 // FWD: w1[x][y<Ny][.] -> w2[x][ky<My][.];  INV: w2[x][ky][.] -> w1[x][y<Ny][.]
-template <typename T, bool INV>
+template <typename T, bool INV, int LC>
 __global__ void TDDA_PASS_BOUNDS k_py(const JobDesc<T>* __restrict__ jobs, const int* __restrict__ tstart,
                                       int njobs, int solver)
 {
@@ -502,7 +520,7 @@ __global__ void TDDA_PASS_BOUNDS k_py(const JobDesc<T>* __restrict__ jobs, const
             Cx<T>* A = twl + tw2_slots(My, J->ply.tb);
             Cx<T>* B = A + J->tfy * ps;
             __syncthreads();
-            const Cx<T>* res = pp_fft<T, INV>(A, B, nf, ps, J->ply, twl);
+            const Cx<T>* res = pass_fft<T, LC, INV>(A, B, nf, ps, J->ply, twl);
             const int nout = INV ? Ny : My;
             Cx<T>* dst =
                 (INV ? J->w1 + (size_t)cur.outer * Ny * F : J->w2 + (size_t)cur.outer * My * F) + cur.f0;
@@ -521,7 +539,7 @@ __global__ void TDDA_PASS_BOUNDS k_py(const JobDesc<T>* __restrict__ jobs, const
 
 // P3: x pass fused with the Green product (kernel.cu:74-95).
 // Tile = (ky, tfx consecutive flattened (kz, c) columns, tfx % 3 == 0).
-template <typename T>
+template <typename T, int LC>
 __global__ void TDDA_PASS_BOUNDS k_px(const JobDesc<T>* __restrict__ jobs, const int* __restrict__ tstart,
                                       int njobs, int solver)
 {
@@ -583,7 +601,7 @@ __global__ void TDDA_PASS_BOUNDS k_px(const JobDesc<T>* __restrict__ jobs, const
             Cx<T>* A = twl + tw2_slots(Mx, J->plx.tb);
             Cx<T>* B = A + J->tfx * ps;
             __syncthreads();
-            Cx<T>* res = pp_fft<T, false>(A, B, nf, ps, J->plx, twl);
+            Cx<T>* res = pass_fft<T, LC, false>(A, B, nf, ps, J->plx, twl);
             Cx<T>* other = (res == A) ? B : A;
             // Green product.  Octant fold per axis: k' = min(k, M - k); odd
             // components flip sign when k was folded.
@@ -612,7 +630,7 @@ __global__ void TDDA_PASS_BOUNDS k_px(const JobDesc<T>* __restrict__ jobs, const
                 v[2 * ps] = cadd(cadd(cmul(gxz, bx), cmul(gyz, by)), cmul(gzz, bz));
             }
             __syncthreads();
-            const Cx<T>* res2 = pp_fft<T, true>(res, other, nf, ps, J->plx, twl);
+            const Cx<T>* res2 = pass_fft<T, LC, true>(res, other, nf, ps, J->plx, twl);
             const size_t sx = (size_t)My * F;
             Cx<T>* base = J->w2 + (size_t)ky * F + f0;
             const float rnf = 1.0f / (float)nf;
@@ -630,7 +648,7 @@ __global__ void TDDA_PASS_BOUNDS k_px(const JobDesc<T>* __restrict__ jobs, const
 
 // P5: z-inverse, crop to the box, diagonal term and solver epilogues.
 // Tile = pz consecutive (x, y) pencils.
-template <typename T, int MODE>
+template <typename T, int MODE, int LC>
 __global__ void TDDA_PASS_BOUNDS k_p5(const JobDesc<T>* __restrict__ jobs, const int* __restrict__ tstart,
                                       int njobs, int diag)
 {
@@ -684,7 +702,7 @@ __global__ void TDDA_PASS_BOUNDS k_p5(const JobDesc<T>* __restrict__ jobs, const
             Cx<T>* A = twl + tw2_slots(Mz, J->plz.tb);
             Cx<T>* B = A + 3 * J->pz * ps;
             __syncthreads();
-            const Cx<T>* res = pp_fft<T, true>(A, B, cur.np * 3, ps, J->plz, twl);
+            const Cx<T>* res = pass_fft<T, LC, true>(A, B, cur.np * 3, ps, J->plz, twl);
             double acc[4] = {0.0, 0.0, 0.0, 0.0};
             const int nout = cur.np * Nz * 3;
             const int base = cur.q0 * Nz;
@@ -868,7 +886,7 @@ __global__ void __launch_bounds__(kThreads) k_update(const JobDesc<T>* __restric
 // Mixed precision glue: inner right-hand side and solution accumulation.
 // ---------------------------------------------------------------------------
 // e32 = c64(r64)
-__global__ void __launch_bounds__(kThreads) k_demote(const Cx<double>* __restrict__ src, Cx<float>* __restrict__ dst,
+static __global__ void __launch_bounds__(kThreads) k_demote(const Cx<double>* __restrict__ src, Cx<float>* __restrict__ dst,
                                                      size_t n)
 {
     const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -879,7 +897,7 @@ __global__ void __launch_bounds__(kThreads) k_demote(const Cx<double>* __restric
 }
 
 // x64 += c128(x32)
-__global__ void __launch_bounds__(kThreads) k_accum(Cx<double>* __restrict__ x, const Cx<float>* __restrict__ d,
+static __global__ void __launch_bounds__(kThreads) k_accum(Cx<double>* __restrict__ x, const Cx<float>* __restrict__ d,
                                                     size_t n)
 {
     const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -893,14 +911,14 @@ __global__ void __launch_bounds__(kThreads) k_accum(Cx<double>* __restrict__ x, 
 }
 
 // Sparse entry access: out[k] = v[idx[k]] and v[idx[k]] = vals[k].
-__global__ void __launch_bounds__(kThreads) k_gather(const Cx<double>* __restrict__ v, const int* __restrict__ idx,
+static __global__ void __launch_bounds__(kThreads) k_gather(const Cx<double>* __restrict__ v, const int* __restrict__ idx,
                                                      int n, Cx<double>* __restrict__ out)
 {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k < n) out[k] = v[idx[k]];
 }
 
-__global__ void __launch_bounds__(kThreads) k_scatter(Cx<double>* __restrict__ v, const int* __restrict__ idx, int n,
+static __global__ void __launch_bounds__(kThreads) k_scatter(Cx<double>* __restrict__ v, const int* __restrict__ idx, int n,
                                                       const Cx<double>* __restrict__ vals)
 {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
@@ -939,7 +957,7 @@ __device__ __forceinline__ void ldr_inter(double K, double x, double y, double z
 // Fills the 6-component tensor on the whole padded grid [Mx][My][Mz][6]:
 // index i < N -> offset i d; i >= M - (N - 1) -> (i - M) d; otherwise zero
 // (AProductCore.cpp:54-194 with 2N-1 replaced by M).
-__global__ void __launch_bounds__(kThreads) k_green_fill(Cx<double>* __restrict__ g, int Nx, int Ny, int Nz, int Mx,
+static __global__ void __launch_bounds__(kThreads) k_green_fill(Cx<double>* __restrict__ g, int Nx, int Ny, int Nz, int Mx,
                                                          int My, int Mz, double d, double K)
 {
     const size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;

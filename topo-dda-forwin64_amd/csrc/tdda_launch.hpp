@@ -1,0 +1,32 @@
+// tdda_launch.hpp -- host launchers of the operator passes.  Each launcher
+// picks the kernel specialised for the pass's FFT length when one is
+// compiled (lengths 2^a, 3 * 2^a, 5 * 2^a from 8 to 1024; z passes up to
+// 128) and the runtime-plan kernel otherwise.  Definitions live in
+// tdda_pass_f64.hip / tdda_pass_f32.hip so both precisions build in parallel.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "tdda_kernels.hpp"
+
+namespace tdda {
+
+struct PassLaunch {
+    dim3 grid;
+    size_t lds;
+    hipStream_t stream;
+};
+
+template <typename T>
+void launch_p1(int L, int mode, const PassLaunch& pl, const JobDesc<T>* jobs, const int* tstart, int nj);
+template <typename T>
+void launch_p5(int L, int mode, int diag, const PassLaunch& pl, const JobDesc<T>* jobs, const int* tstart, int nj);
+template <typename T>
+void launch_py(int L, bool inv, int solver, const PassLaunch& pl, const JobDesc<T>* jobs, const int* tstart, int nj);
+template <typename T>
+void launch_px(int L, int solver, const PassLaunch& pl, const JobDesc<T>* jobs, const int* tstart, int nj);
+// true when a specialised kernel exists for this length and pass kind
+// (0: z pass, 1: y pass, 2: x pass)
+template <typename T>
+bool pass_length_compiled(int kind, int L);
+
+}  // namespace tdda
