@@ -381,6 +381,12 @@ constexpr int ct_tb(int L)
     return t;
 }
 
+// Wave-local stage: pencil p belongs to wavefront p % 4, so a stage only
+// reads pencils its own wavefront wrote.  LDS instructions of one wavefront
+// complete in order, so consecutive stages need no workgroup barrier, only
+// a compiler-ordering barrier.
+constexpr int kWaves = kThreadsFft / 64;
+
 template <typename T, int L, int R, int NS, bool INV, int NPMAX>
 __device__ __forceinline__ void ct_stage(const Cx<T>* __restrict__ src, Cx<T>* __restrict__ dst, int np,
                                          const Cx<T>* __restrict__ tw)
@@ -389,15 +395,19 @@ __device__ __forceinline__ void ct_stage(const Cx<T>* __restrict__ src, Cx<T>* _
     constexpr int PS = L + (L >> 5) + 1;
     constexpr int TWSTEP = L / (NS * R);
     constexpr int TB = ct_tb(L);
-    constexpr int NBMAX = NPMAX * LR;
-    const int nb = np * LR;
+    constexpr int NPW = (NPMAX + kWaves - 1) / kWaves;  // pencils per wavefront, at most
+    constexpr int NBMAX = NPW * LR;
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int npw = (np - wave + kWaves - 1) / kWaves;  // this wavefront's pencils
+    const int nb = npw * LR;
 #pragma unroll
-    for (int it = 0; it < (NBMAX + kThreadsFft - 1) / kThreadsFft; it++) {
-        const int b = threadIdx.x + it * kThreadsFft;
+    for (int it = 0; it < (NBMAX + 63) / 64; it++) {
+        const int b = lane + it * 64;
         if (b < nb) {
-            const int p = b / LR;
-            const int j = b - p * LR;
+            const int pi = b / LR;
+            const int j = b - pi * LR;
             const int kk = j % NS;
+            const int p = wave + pi * kWaves;
             const Cx<T>* sp = src + p * PS;
             Cx<T> v[R];
 #pragma unroll
@@ -429,17 +439,22 @@ __device__ __forceinline__ Cx<T>* ct_stages(Cx<T>* a, Cx<T>* b, int np, const Cx
     } else {
         constexpr int R = ct_rad(L, S);
         ct_stage<T, L, R, NS, INV, NPMAX>(a, b, np, tw);
-        __syncthreads();
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
         return ct_stages<T, L, S + 1, NS * R, INV, NPMAX>(b, a, np, tw);
     }
 }
 
 // Transform of np <= NPMAX pencils of the compile-time length L; same
-// contract as pp_fft.
+// contract as pp_fft (input visible to all threads, ends with a workgroup
+// barrier).
 template <typename T, int L, bool INV, int NPMAX>
 __device__ __forceinline__ Cx<T>* ct_fft(Cx<T>* a, Cx<T>* b, int np, const Cx<T>* tw)
 {
-    return ct_stages<T, L, 0, 1, INV, NPMAX>(a, b, np, tw);
+    Cx<T>* r = ct_stages<T, L, 0, 1, INV, NPMAX>(a, b, np, tw);
+    __syncthreads();
+    return r;
 }
 
 }  // namespace tdda
