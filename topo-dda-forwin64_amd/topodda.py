@@ -16,7 +16,7 @@ import os
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIBPATH = os.path.join(HERE, "libtopodda_hip.so")
+LIBPATH = os.environ.get("TDDA_LIB", os.path.join(HERE, "libtopodda_hip.so"))
 
 C128 = 1
 C64 = 2
