@@ -63,6 +63,7 @@ extern "C" {
 typedef struct tdda_ctx_s* tdda_ctx;
 typedef struct tdda_op_s* tdda_op;
 typedef struct tdda_model_s* tdda_model;
+typedef struct tdda_batch_s* tdda_batch;
 
 /* Last error message of the calling thread ("" if none). */
 const char* tdda_last_error(void);
@@ -182,6 +183,24 @@ void tdda_solve_opts_default(tdda_solve_opts* o);
  * an FP64 residual-refinement loop whose stop test uses the true residual.
  * info must hold nrhs entries.  P is updated in device memory.            */
 int tdda_model_solve(tdda_model m, const tdda_solve_opts* opts, tdda_solve_info* info);
+
+/* Batches (no counterpart in the reference, whose solves run one at a time:
+ * T/main.cpp:584-598 loops DDAModel::bicgstab over structures, and
+ * EvoDDAModel.cpp:433-441 over models).  A batch gathers the right-hand
+ * sides of several models -- different structures, grids, wavelengths and
+ * incidences -- and runs every operator pass as one launch over all of
+ * them; solver states stay per right-hand side, so each result equals its
+ * solve alone.  The batch references the models' device state (models must
+ * share one context and stay alive; the batch keeps them referenced).     */
+int tdda_batch_create(tdda_model* models, int nmodels, tdda_batch* out);
+int tdda_batch_destroy(tdda_batch b);
+/* Number of right-hand sides (sum of the models' nrhs, in model order). */
+int tdda_batch_size(tdda_batch b);
+/* DDAModel::bicgstab for every right-hand side of the batch; info holds
+ * tdda_batch_size(b) entries in model order.  P is updated in each model. */
+int tdda_batch_solve(tdda_batch b, const tdda_solve_opts* opts, tdda_solve_info* info);
+/* Y = A P + al o P for every right-hand side (tdda_model_apply per model). */
+int tdda_batch_apply(tdda_batch b, int precision);
 
 #ifdef __cplusplus
 }

@@ -165,25 +165,34 @@ struct VecSet {
 };
 
 template <typename T>
+struct PassGroup {
+    int L = 0;                   // FFT length of the group's kernel (0: runtime plan)
+    const JobDesc<T>* d = nullptr;  // descriptors of the group's jobs
+    const int* ts = nullptr;     // tile starts of the group, nj + 1
+    int nj = 0;
+    int grid = 0;                // persistent grid
+    size_t lds = 0;
+};
+
+// Jobs of one set of launches.  Each pass runs one launch per group of jobs
+// that share the pass's kernel (same compiled FFT length, or the runtime-plan
+// kernel); a job's tiles, partial sums and kernel are the same in every
+// table it belongs to, so its results do not depend on the other jobs.
+template <typename T>
 struct JobTable {
     std::vector<JobDesc<T>> h;
     JobDesc<T>* d = nullptr;
-    int* tz = nullptr;  // tile starts, njobs+1 each (P5)
-    int* tz1 = nullptr;  // P1
-    int* ty = nullptr;
-    int* tx = nullptr;
-    int* tv = nullptr;
-    int nz = 0, nz1 = 0, ny = 0, nx = 0, nv = 0;
-    int gz = 0, gz1 = 0, gy = 0, gx = 0;  // persistent grid sizes
-    int Lx = 0, Ly = 0, Lz = 0;           // FFT lengths of the passes
+    int* tv = nullptr;  // vector-tile starts, njobs+1
+    int nv = 0;
+    std::vector<PassGroup<T>> gz, gz1, gy, gx;  // P5, P1, P2/P4, P3
     SolverState* st = nullptr;
     double* part = nullptr;
     unsigned* cnt = nullptr;
-    TilePlan tp;
 };
 
 struct tdda_model_s {
     tdda_op op;
+    int refs = 1;  // batches hold references
     int nrhs;
     size_t n3;
     VecSet<double> v64;
@@ -200,6 +209,19 @@ struct tdda_model_s {
     Cx<double>* d_vals = nullptr;
     int* h_idx = nullptr;
     Cx<double>* h_vals = nullptr;
+    std::vector<void*> allocs;
+};
+
+struct tdda_batch_s {
+    tdda_ctx ctx;
+    int nj = 0;
+    bool has32 = false;
+    std::vector<tdda_model> models;
+    JobTable<double> j64, jo;
+    JobTable<float> j32;
+    SolverState* h_st = nullptr;
+    SolverState* h_poll = nullptr;
+    hipEvent_t ev[2] = {nullptr, nullptr};
     std::vector<void*> allocs;
 };
 
@@ -243,13 +265,13 @@ int plan_tiles(const tdda_op_s* op, TilePlan& t)
 }
 
 template <typename T>
-int dev_alloc(tdda_model_s* m, T** p, size_t count)
+int dev_alloc(std::vector<void*>& arena, T** p, size_t count)
 {
     void* q = nullptr;
     if (count == 0) count = 1;
     hipError_t e = hipMalloc(&q, count * sizeof(T));
     if (e != hipSuccess) return fail(TDDA_ERR_OOM, std::string("hipMalloc failed: ") + hipGetErrorString(e));
-    m->allocs.push_back(q);
+    arena.push_back(q);
     *p = reinterpret_cast<T*>(q);
     return TDDA_OK;
 }
@@ -259,68 +281,65 @@ int alloc_vecs(tdda_model_s* m, VecSet<T>& v)
 {
     const size_t n = m->n3 * m->nrhs;
     const tdda_op_s* op = m->op;
-    CHK(dev_alloc(m, &v.al, n));
-    CHK(dev_alloc(m, &v.e, n));
-    CHK(dev_alloc(m, &v.x, n));
-    CHK(dev_alloc(m, &v.r, n));
-    CHK(dev_alloc(m, &v.r0, n));
-    CHK(dev_alloc(m, &v.p, n));
-    CHK(dev_alloc(m, &v.ap, n));
-    CHK(dev_alloc(m, &v.t, n));
-    CHK(dev_alloc(m, &v.at, n));
-    CHK(dev_alloc(m, &v.w1, (size_t)op->Nx * op->Ny * op->Mz * 3 * m->nrhs));
-    CHK(dev_alloc(m, &v.w2, (size_t)op->Nx * op->My * op->Mz * 3 * m->nrhs));
+    std::vector<void*>& a = m->allocs;
+    CHK(dev_alloc(a, &v.al, n));
+    CHK(dev_alloc(a, &v.e, n));
+    CHK(dev_alloc(a, &v.x, n));
+    CHK(dev_alloc(a, &v.r, n));
+    CHK(dev_alloc(a, &v.r0, n));
+    CHK(dev_alloc(a, &v.p, n));
+    CHK(dev_alloc(a, &v.ap, n));
+    CHK(dev_alloc(a, &v.t, n));
+    CHK(dev_alloc(a, &v.at, n));
+    CHK(dev_alloc(a, &v.w1, (size_t)op->Nx * op->Ny * op->Mz * 3 * m->nrhs));
+    CHK(dev_alloc(a, &v.w2, (size_t)op->Nx * op->My * op->Mz * 3 * m->nrhs));
     HIPCHK(hipMemsetAsync(v.x, 0, n * sizeof(Cx<T>), op->ctx->stream));
     HIPCHK(hipMemsetAsync(v.al, 0, n * sizeof(Cx<T>), op->ctx->stream));
     HIPCHK(hipMemsetAsync(v.e, 0, n * sizeof(Cx<T>), op->ctx->stream));
     return TDDA_OK;
 }
 
-// Builds the job table of one model for precision T.  r0_override lets the
-// outer residual table write its r0 copy into a scratch vector.
+// Host job descriptors of one model for precision T (one job per right-hand
+// side).  r_vec / r0_vec let the outer residual table of the mixed solver
+// write its residual and r0 copy into scratch vectors.
 template <typename T>
-int build_table(tdda_model_s* m, JobTable<T>& jt, const VecSet<T>& v, const Cx<T>* green, Cx<T>* const* tw,
-                const TilePlan& tp, Cx<T>* r_vec, Cx<T>* r0_vec)
+void fill_jobs(const tdda_model_s* m, std::vector<JobDesc<T>>& out, const VecSet<T>& v, const Cx<T>* green,
+               Cx<T>* const* tw, const TilePlan& tp, Cx<T>* r_vec, Cx<T>* r0_vec)
 {
     const tdda_op_s* op = m->op;
-    const int nj = m->nrhs;
     const size_t n3 = m->n3;
-    jt.tp = tp;
-    const int tiles_z = (op->Nx * op->Ny + tp.pz - 1) / tp.pz;
-    const int tiles_z1 = (op->Nx * op->Ny + tp.pz1 - 1) / tp.pz1;
-    const int nfy = (3 * op->Mz + tp.tfy - 1) / tp.tfy;
-    const int nfx = (3 * op->Mz + tp.tfx - 1) / tp.tfx;
-    const int tiles_y = op->Nx * nfy;
-    const int tiles_x = op->My * nfx;
-    const int tiles_v = (int)((n3 + kThreads * kVecPer - 1) / (kThreads * kVecPer));
-    const int maxparts = std::max(tiles_z, tiles_v);
-    CHK(dev_alloc(m, &jt.st, nj));
-    CHK(dev_alloc(m, &jt.part, (size_t)maxparts * 4 * nj));
-    CHK(dev_alloc(m, &jt.cnt, nj));
-    HIPCHK(hipMemsetAsync(jt.cnt, 0, sizeof(unsigned) * nj, op->ctx->stream));
-    HIPCHK(hipMemsetAsync(jt.st, 0, sizeof(SolverState) * nj, op->ctx->stream));
-    jt.h.resize(nj);
     const size_t w1n = (size_t)op->Nx * op->Ny * op->Mz * 3;
     const size_t w2n = (size_t)op->Nx * op->My * op->Mz * 3;
-    for (int j = 0; j < nj; j++) {
-        JobDesc<T>& J = jt.h[j];
+    for (int j = 0; j < m->nrhs; j++) {
+        JobDesc<T> J;
         memset(&J, 0, sizeof(J));
-        J.Nx = op->Nx; J.Ny = op->Ny; J.Nz = op->Nz;
-        J.Mx = op->Mx; J.My = op->My; J.Mz = op->Mz;
-        J.Hx = op->Hx; J.Hy = op->Hy; J.Hz = op->Hz;
+        J.Nx = op->Nx;
+        J.Ny = op->Ny;
+        J.Nz = op->Nz;
+        J.Mx = op->Mx;
+        J.My = op->My;
+        J.Mz = op->Mz;
+        J.Hx = op->Hx;
+        J.Hy = op->Hy;
+        J.Hz = op->Hz;
         J.N = op->N;
         J.pz = tp.pz;
         J.pz1 = tp.pz1;
         J.tfy = tp.tfy;
         J.tfx = tp.tfx;
-        J.nfy = nfy; J.nfx = nfx;
-        J.ps_z = tp.ps_z; J.ps_y = tp.ps_y; J.ps_x = tp.ps_x;
+        J.nfy = (3 * op->Mz + tp.tfy - 1) / tp.tfy;
+        J.nfx = (3 * op->Mz + tp.tfx - 1) / tp.tfx;
+        J.ps_z = tp.ps_z;
+        J.ps_y = tp.ps_y;
+        J.ps_x = tp.ps_x;
         J.cellmap = op->cellmap;
         J.green = green;
         J.twx = tw[0];
         J.twy = tw[1];
         J.twz = tw[2];
-        J.plx = op->pl[0]; J.ply = op->pl[1]; J.plz = op->pl[2];
+        J.plx = op->pl[0];
+        J.ply = op->pl[1];
+        J.plz = op->pl[2];
         J.w1 = v.w1 + j * w1n;
         J.w2 = v.w2 + j * w2n;
         J.in = v.x + j * n3;
@@ -334,85 +353,150 @@ int build_table(tdda_model_s* m, JobTable<T>& jt, const VecSet<T>& v, const Cx<T
         J.t = v.t + j * n3;
         J.at = v.at + j * n3;
         J.e = v.e + j * n3;
-        J.st = jt.st + j;
-        J.part = jt.part + (size_t)j * maxparts * 4;
-        J.cnt = jt.cnt + j;
+        out.push_back(J);
     }
-    CHK(dev_alloc(m, &jt.d, nj));
+}
+
+struct JobTiles {
+    int z, z1, y, x, v;
+};
+
+template <typename T>
+JobTiles job_tiles(const JobDesc<T>& J)
+{
+    JobTiles t;
+    t.z = (J.Nx * J.Ny + J.pz - 1) / J.pz;
+    t.z1 = (J.Nx * J.Ny + J.pz1 - 1) / J.pz1;
+    t.y = J.Nx * J.nfy;
+    t.x = J.My * J.nfx;
+    t.v = (int)((3 * (size_t)J.N + kThreads * kVecPer - 1) / (kThreads * kVecPer));
+    return t;
+}
+
+// Completes a job table from its host descriptors: per-job solver state,
+// partial-sum buffers and counters, tile prefix sums, persistent grid sizes,
+// LDS sizes (largest over the jobs) and the pass FFT lengths (0 when the
+// jobs differ, which selects the runtime-plan kernels).
+template <typename T>
+int finalize_table(std::vector<void*>& arena, const tdda_ctx_s* c, JobTable<T>& jt)
+{
+    const int nj = (int)jt.h.size();
+    const size_t s = sizeof(Cx<T>);
+    std::vector<JobTiles> tiles(nj);
+    std::vector<int> hv(nj + 1, 0);
+    std::vector<size_t> poff(nj + 1, 0);
+    for (int j = 0; j < nj; j++) {
+        tiles[j] = job_tiles(jt.h[j]);
+        hv[j + 1] = hv[j] + tiles[j].v;
+        poff[j + 1] = poff[j] + 4 * (size_t)std::max(tiles[j].z, tiles[j].v);
+    }
+    CHK(dev_alloc(arena, &jt.st, nj));
+    CHK(dev_alloc(arena, &jt.part, poff[nj]));
+    CHK(dev_alloc(arena, &jt.cnt, nj));
+    HIPCHK(hipMemsetAsync(jt.cnt, 0, sizeof(unsigned) * nj, c->stream));
+    HIPCHK(hipMemsetAsync(jt.st, 0, sizeof(SolverState) * nj, c->stream));
+    for (int j = 0; j < nj; j++) {
+        jt.h[j].st = jt.st + j;
+        jt.h[j].part = jt.part + poff[j];
+        jt.h[j].cnt = jt.cnt + j;
+    }
+    CHK(dev_alloc(arena, &jt.d, nj));
     HIPCHK(hipMemcpy(jt.d, jt.h.data(), sizeof(JobDesc<T>) * nj, hipMemcpyHostToDevice));
-    std::vector<int> hz(nj + 1), hz1(nj + 1), hy(nj + 1), hx(nj + 1), hv(nj + 1);
-    for (int j = 0; j <= nj; j++) {
-        hz[j] = j * tiles_z;
-        hz1[j] = j * tiles_z1;
-        hy[j] = j * tiles_y;
-        hx[j] = j * tiles_x;
-        hv[j] = j * tiles_v;
-    }
-    jt.Lx = op->Mx;
-    jt.Ly = op->My;
-    jt.Lz = op->Mz;
-    jt.nz = hz[nj];
-    jt.nz1 = hz1[nj];
-    jt.ny = hy[nj];
-    jt.nx = hx[nj];
     jt.nv = hv[nj];
-    // persistent grids: as many workgroups as stay resident (4 per CU by the
-    // register budget of the passes, fewer if their LDS does not fit)
-    const tdda_ctx_s* c = op->ctx;
+    CHK(dev_alloc(arena, &jt.tv, nj + 1));
+    HIPCHK(hipMemcpy(jt.tv, hv.data(), sizeof(int) * (nj + 1), hipMemcpyHostToDevice));
+    // persistent grids: as many workgroups as stay resident (LDS-limited,
+    // at most 8 per CU)
     auto grid = [&](int ntiles, size_t lds) {
         const int per_cu = (int)std::max<size_t>(1, std::min<size_t>(8, c->lds_cu / std::max<size_t>(lds, 1)));
         return std::max(1, std::min(ntiles, c->ncu * per_cu));
     };
-    jt.gz = grid(jt.nz, tp.lds_z);
-    jt.gz1 = grid(jt.nz1, tp.lds_z1);
-    jt.gy = grid(jt.ny, tp.lds_y);
-    jt.gx = grid(jt.nx, tp.lds_x);
-    CHK(dev_alloc(m, &jt.tz1, nj + 1));
-    HIPCHK(hipMemcpy(jt.tz1, hz1.data(), sizeof(int) * (nj + 1), hipMemcpyHostToDevice));
-    CHK(dev_alloc(m, &jt.tz, nj + 1));
-    CHK(dev_alloc(m, &jt.ty, nj + 1));
-    CHK(dev_alloc(m, &jt.tx, nj + 1));
-    CHK(dev_alloc(m, &jt.tv, nj + 1));
-    HIPCHK(hipMemcpy(jt.tz, hz.data(), sizeof(int) * (nj + 1), hipMemcpyHostToDevice));
-    HIPCHK(hipMemcpy(jt.ty, hy.data(), sizeof(int) * (nj + 1), hipMemcpyHostToDevice));
-    HIPCHK(hipMemcpy(jt.tx, hx.data(), sizeof(int) * (nj + 1), hipMemcpyHostToDevice));
-    HIPCHK(hipMemcpy(jt.tv, hv.data(), sizeof(int) * (nj + 1), hipMemcpyHostToDevice));
+    // per pass kind: FFT length, pass kind of pass_length_compiled, tiles, LDS bytes
+    auto build = [&](std::vector<PassGroup<T>>& groups, int kind, auto len, auto ntiles, auto lds) -> int {
+        groups.clear();
+        std::vector<int> key(nj);
+        for (int j = 0; j < nj; j++) key[j] = pass_length_compiled<T>(kind, len(jt.h[j])) ? len(jt.h[j]) : 0;
+        std::vector<int> keys;
+        for (int j = 0; j < nj; j++)
+            if (std::find(keys.begin(), keys.end(), key[j]) == keys.end()) keys.push_back(key[j]);
+        for (int k : keys) {
+            PassGroup<T> g;
+            g.L = k;
+            std::vector<JobDesc<T>> sub;
+            std::vector<int> ts(1, 0);
+            for (int j = 0; j < nj; j++) {
+                if (key[j] != k) continue;
+                sub.push_back(jt.h[j]);
+                ts.push_back(ts.back() + ntiles(tiles[j]));
+                g.lds = std::max(g.lds, lds(jt.h[j]));
+            }
+            if (g.lds > c->lds_budget) return fail(TDDA_ERR_UNSUPPORTED, "pass tile exceeds the LDS budget");
+            g.nj = (int)sub.size();
+            g.grid = grid(ts.back(), g.lds);
+            if (g.nj == nj) {
+                g.d = jt.d;
+            } else {
+                JobDesc<T>* d;
+                CHK(dev_alloc(arena, &d, sub.size()));
+                HIPCHK(hipMemcpy(d, sub.data(), sizeof(JobDesc<T>) * sub.size(), hipMemcpyHostToDevice));
+                g.d = d;
+            }
+            int* dts;
+            CHK(dev_alloc(arena, &dts, ts.size()));
+            HIPCHK(hipMemcpy(dts, ts.data(), sizeof(int) * ts.size(), hipMemcpyHostToDevice));
+            g.ts = dts;
+            groups.push_back(g);
+        }
+        return TDDA_OK;
+    };
+    auto Lz = [](const JobDesc<T>& J) { return J.Mz; };
+    CHK(build(jt.gz, 0, Lz, [](const JobTiles& t) { return t.z; },
+              [&](const JobDesc<T>& J) { return (size_t)(tw2_slots(J.Mz, J.plz.tb) + 2 * 3 * J.pz * J.ps_z) * s; }));
+    CHK(build(jt.gz1, 0, Lz, [](const JobTiles& t) { return t.z1; },
+              [&](const JobDesc<T>& J) { return (size_t)(tw2_slots(J.Mz, J.plz.tb) + 2 * 3 * J.pz1 * J.ps_z) * s; }));
+    CHK(build(jt.gy, 1, [](const JobDesc<T>& J) { return J.My; }, [](const JobTiles& t) { return t.y; },
+              [&](const JobDesc<T>& J) { return (size_t)(tw2_slots(J.My, J.ply.tb) + 2 * J.tfy * J.ps_y) * s; }));
+    CHK(build(jt.gx, 2, [](const JobDesc<T>& J) { return J.Mx; }, [](const JobTiles& t) { return t.x; },
+              [&](const JobDesc<T>& J) { return (size_t)(tw2_slots(J.Mx, J.plx.tb) + 2 * J.tfx * J.ps_x) * s; }));
     return TDDA_OK;
 }
 
 template <typename T>
-void run_p1(const JobTable<T>& jt, int nj, int mode, hipStream_t s)
+void run_p1(const JobTable<T>& jt, int mode, hipStream_t s)
 {
-    launch_p1<T>(jt.Lz, mode, PassLaunch{dim3(jt.gz1), jt.tp.lds_z1, s}, jt.d, jt.tz1, nj);
+    for (const PassGroup<T>& g : jt.gz1) launch_p1<T>(g.L, mode, PassLaunch{dim3(g.grid), g.lds, s}, g.d, g.ts, g.nj);
 }
 
 template <typename T>
-void run_p5(const JobTable<T>& jt, int nj, int mode, int diag, hipStream_t s)
+void run_p5(const JobTable<T>& jt, int mode, int diag, hipStream_t s)
 {
-    launch_p5<T>(jt.Lz, mode, diag, PassLaunch{dim3(jt.gz), jt.tp.lds_z, s}, jt.d, jt.tz, nj);
+    for (const PassGroup<T>& g : jt.gz)
+        launch_p5<T>(g.L, mode, diag, PassLaunch{dim3(g.grid), g.lds, s}, g.d, g.ts, g.nj);
 }
 
 template <typename T>
-void run_py(const JobTable<T>& jt, int nj, bool inv, int solver, hipStream_t s)
+void run_py(const JobTable<T>& jt, bool inv, int solver, hipStream_t s)
 {
-    launch_py<T>(jt.Ly, inv, solver, PassLaunch{dim3(jt.gy), jt.tp.lds_y, s}, jt.d, jt.ty, nj);
+    for (const PassGroup<T>& g : jt.gy)
+        launch_py<T>(g.L, inv, solver, PassLaunch{dim3(g.grid), g.lds, s}, g.d, g.ts, g.nj);
 }
 
 template <typename T>
-void run_px(const JobTable<T>& jt, int nj, int solver, hipStream_t s)
+void run_px(const JobTable<T>& jt, int solver, hipStream_t s)
 {
-    launch_px<T>(jt.Lx, solver, PassLaunch{dim3(jt.gx), jt.tp.lds_x, s}, jt.d, jt.tx, nj);
+    for (const PassGroup<T>& g : jt.gx) launch_px<T>(g.L, solver, PassLaunch{dim3(g.grid), g.lds, s}, g.d, g.ts, g.nj);
 }
 
 // one operator application: P1 .. P5
 template <typename T>
 void launch_apply(const JobTable<T>& jt, int nj, int inmode, int outmode, int diag, int solver, hipStream_t s)
 {
-    run_p1<T>(jt, nj, inmode, s);
-    run_py<T>(jt, nj, false, solver, s);
-    run_px<T>(jt, nj, solver, s);
-    run_py<T>(jt, nj, true, solver, s);
-    run_p5<T>(jt, nj, outmode, diag, s);
+    (void)nj;
+    run_p1<T>(jt, inmode, s);
+    run_py<T>(jt, false, solver, s);
+    run_px<T>(jt, solver, s);
+    run_py<T>(jt, true, solver, s);
+    run_p5<T>(jt, outmode, diag, s);
 }
 
 // one BiCGSTAB iteration (or the unrolled first step when state.first)
@@ -423,6 +507,19 @@ void launch_iteration(const JobTable<T>& jt, int nj, hipStream_t s)
     launch_apply<T>(jt, nj, IN_T, OUT_AT, 1, 1, s);
     k_update<T><<<dim3(jt.nv), dim3(kThreads), 0, s>>>(jt.d, jt.tv, nj);
 }
+
+// The job tables and host buffers one solve runs on: a model's own, or the
+// concatenation of several models' (a batch).
+struct SolveSet {
+    hipStream_t s;
+    int nj;
+    JobTable<double>* j64;
+    JobTable<double>* jo;
+    JobTable<float>* j32;  // nullptr without complex64 storage
+    SolverState* h_st;     // pinned, 3 * nj
+    SolverState* h_poll;   // pinned, 2 * nj
+    hipEvent_t* ev;        // 2 events
+};
 
 int check_launch()
 {
@@ -439,11 +536,11 @@ int check_launch()
 // comes from each job's observed residual decay; chunks queued after every
 // job has stopped run as no-ops (each kernel returns on the stop flag).
 template <typename T>
-int run_iterations(tdda_model_s* m, JobTable<T>& jt, SolverState* hst, int max_total, int check_every)
+int run_iterations(SolveSet& S, JobTable<T>& jt, SolverState* hst, int max_total, int check_every)
 {
-    hipStream_t s = m->op->ctx->stream;
-    const int nj = m->nrhs;
-    SolverState* slot[2] = {m->h_poll, m->h_poll + nj};
+    hipStream_t s = S.s;
+    const int nj = S.nj;
+    SolverState* slot[2] = {S.h_poll, S.h_poll + nj};
     bool busy[2] = {false, false};
     int upto[2] = {0, 0};  // iterations (including the first step) issued when slot k was filled
     int issued = 0;
@@ -453,7 +550,7 @@ int run_iterations(tdda_model_s* m, JobTable<T>& jt, SolverState* hst, int max_t
         for (int i = 0; i < chunk; i++) launch_iteration<T>(jt, nj, s);
         issued += chunk;
         HIPCHK(hipMemcpyAsync(slot[k], jt.st, sizeof(SolverState) * nj, hipMemcpyDeviceToHost, s));
-        HIPCHK(hipEventRecord(m->ev[k], s));
+        HIPCHK(hipEventRecord(S.ev[k], s));
         busy[k] = true;
         upto[k] = issued;
         return check_launch();
@@ -464,27 +561,27 @@ int run_iterations(tdda_model_s* m, JobTable<T>& jt, SolverState* hst, int max_t
     CHK(issue(0, check_every > 0 ? check_every : 4));
     if (issued < limit) CHK(issue(1, check_every > 0 ? check_every : 4));
     while (true) {
-        HIPCHK(hipEventSynchronize(m->ev[cur]));
+        HIPCHK(hipEventSynchronize(S.ev[cur]));
         busy[cur] = false;
-        const SolverState* S = slot[cur];
+        const SolverState* St = slot[cur];
         bool running = false;
         int need = 0;
         for (int j = 0; j < nj; j++) {
-            if (S[j].status != ST_RUNNING) continue;
+            if (St[j].status != ST_RUNNING) continue;
             running = true;
             int rem = 4;
-            if (prev_err[j] > 0 && S[j].iter > prev_it[j] && S[j].err < prev_err[j] && S[j].err > 0) {
-                const double rate = std::pow(S[j].err / prev_err[j], 1.0 / (S[j].iter - prev_it[j]));
-                if (rate < 0.999) rem = (int)std::ceil(std::log(S[j].tol / S[j].err) / std::log(rate));
+            if (prev_err[j] > 0 && St[j].iter > prev_it[j] && St[j].err < prev_err[j] && St[j].err > 0) {
+                const double rate = std::pow(St[j].err / prev_err[j], 1.0 / (St[j].iter - prev_it[j]));
+                if (rate < 0.999) rem = (int)std::ceil(std::log(St[j].tol / St[j].err) / std::log(rate));
             }
             rem = std::max(1, std::min(rem, 64));
-            prev_err[j] = S[j].err;
-            prev_it[j] = S[j].iter;
+            prev_err[j] = St[j].err;
+            prev_it[j] = St[j].iter;
             need = std::max(need, rem);
         }
         if (!running || issued >= limit) {
             if (!running || !busy[1 - cur]) {
-                memcpy(hst, S, sizeof(SolverState) * nj);
+                memcpy(hst, St, sizeof(SolverState) * nj);
                 if (!running) return TDDA_OK;
                 // iteration cap reached with jobs still running: the device
                 // marks them ST_MAXIT in the last queued chunk
@@ -503,19 +600,18 @@ int run_iterations(tdda_model_s* m, JobTable<T>& jt, SolverState* hst, int max_t
 }
 
 template <typename T>
-int init_states(tdda_model_s* m, JobTable<T>& jt, SolverState* hst, const std::vector<int>& active, double tol,
+int init_states(SolveSet& S, JobTable<T>& jt, SolverState* hst, const std::vector<int>& active, double tol,
                 int maxit)
 {
-    const int nj = m->nrhs;
-    for (int j = 0; j < nj; j++) {
-        SolverState& S = hst[j];
-        memset(&S, 0, sizeof(S));
-        S.tol = tol;
-        S.maxit = maxit;
-        S.first = 1;
-        S.status = active[j] ? ST_RUNNING : ST_CONVERGED;
+    for (int j = 0; j < S.nj; j++) {
+        SolverState& St = hst[j];
+        memset(&St, 0, sizeof(St));
+        St.tol = tol;
+        St.maxit = maxit;
+        St.first = 1;
+        St.status = active[j] ? ST_RUNNING : ST_CONVERGED;
     }
-    HIPCHK(hipMemcpyAsync(jt.st, hst, sizeof(SolverState) * nj, hipMemcpyHostToDevice, m->op->ctx->stream));
+    HIPCHK(hipMemcpyAsync(jt.st, hst, sizeof(SolverState) * S.nj, hipMemcpyHostToDevice, S.s));
     return TDDA_OK;
 }
 
@@ -525,20 +621,19 @@ void launch_init_zero(const JobTable<T>& jt, int nj, hipStream_t s)
     k_init_zero<T><<<dim3(jt.nv), dim3(kThreads), 0, s>>>(jt.d, jt.tv, nj);
 }
 
-int solve_fp64(tdda_model_s* m, const tdda_solve_opts* o, tdda_solve_info* info)
+int solve_fp64(SolveSet& S, const tdda_solve_opts* o, tdda_solve_info* info)
 {
-    hipStream_t s = m->op->ctx->stream;
-    const int nj = m->nrhs;
-    SolverState* hst = m->h_st;
+    const int nj = S.nj;
+    SolverState* hst = S.h_st;
     std::vector<int> active(nj, 1);
-    CHK(init_states<double>(m, m->j64, hst, active, o->tol, o->max_it));
+    CHK(init_states<double>(S, *S.j64, hst, active, o->tol, o->max_it));
     if (o->x0_zero) {
-        launch_init_zero<double>(m->j64, nj, s);
+        launch_init_zero<double>(*S.j64, nj, S.s);
     } else {
-        launch_apply<double>(m->j64, nj, IN_X, OUT_RES, 1, 1, s);
+        launch_apply<double>(*S.j64, nj, IN_X, OUT_RES, 1, 1, S.s);
     }
     CHK(check_launch());
-    CHK(run_iterations<double>(m, m->j64, hst, o->max_it, o->check_every));
+    CHK(run_iterations<double>(S, *S.j64, hst, o->max_it, o->check_every));
     for (int j = 0; j < nj; j++) {
         info[j].status = hst[j].status;
         info[j].iterations = hst[j].iter;
@@ -549,36 +644,37 @@ int solve_fp64(tdda_model_s* m, const tdda_solve_opts* o, tdda_solve_info* info)
     return TDDA_OK;
 }
 
-// FP64 true residual r = E - A x into jo's r vector; returns ||r||/||E||.
-int true_residual(tdda_model_s* m, std::vector<double>& rel, const std::vector<int>& active)
+// FP64 true residual r = E - A x into jo's r vector; rel <- ||r|| / ||E|| for
+// the active jobs.
+int true_residual(SolveSet& S, std::vector<double>& rel, const std::vector<int>& active)
 {
-    hipStream_t s = m->op->ctx->stream;
-    const int nj = m->nrhs;
-    SolverState* hst = m->h_st + nj;
-    std::vector<int> act = active;
-    CHK(init_states<double>(m, m->jo, hst, act, 0.0, 0));
-    launch_apply<double>(m->jo, nj, IN_X, OUT_RES, 1, 1, s);
+    const int nj = S.nj;
+    SolverState* hst = S.h_st + nj;
+    CHK(init_states<double>(S, *S.jo, hst, active, 0.0, 0));
+    launch_apply<double>(*S.jo, nj, IN_X, OUT_RES, 1, 1, S.s);
     CHK(check_launch());
-    HIPCHK(hipMemcpyAsync(hst, m->jo.st, sizeof(SolverState) * nj, hipMemcpyDeviceToHost, s));
-    HIPCHK(hipStreamSynchronize(s));
+    HIPCHK(hipMemcpyAsync(hst, S.jo->st, sizeof(SolverState) * nj, hipMemcpyDeviceToHost, S.s));
+    HIPCHK(hipStreamSynchronize(S.s));
     rel.resize(nj);
-    for (int j = 0; j < nj; j++) rel[j] = active[j] ? hst[j].rnorm / hst[j].enorm : rel[j];
+    for (int j = 0; j < nj; j++)
+        if (active[j]) rel[j] = hst[j].rnorm / hst[j].enorm;
     return TDDA_OK;
 }
 
-int solve_mixed(tdda_model_s* m, const tdda_solve_opts* o, tdda_solve_info* info)
+int solve_mixed(SolveSet& S, const tdda_solve_opts* o, tdda_solve_info* info)
 {
-    hipStream_t s = m->op->ctx->stream;
-    const int nj = m->nrhs;
-    const size_t n3 = m->n3;
-    if (!m->op->green32) return fail(TDDA_ERR_STATE, "mixed solve needs an operator built with TDDA_C64");
-    if (o->x0_zero) HIPCHK(hipMemsetAsync(m->v64.x, 0, sizeof(Cx<double>) * n3 * nj, s));
+    hipStream_t s = S.s;
+    const int nj = S.nj;
+    if (!S.j32) return fail(TDDA_ERR_STATE, "mixed solve needs operators built with TDDA_C64");
+    auto n3of = [&](int j) { return 3 * (size_t)S.j64->h[j].N; };
+    auto blocks = [&](int j) { return (unsigned)((n3of(j) + kThreads - 1) / kThreads); };
+    if (o->x0_zero)
+        for (int j = 0; j < nj; j++) HIPCHK(hipMemsetAsync(S.j64->h[j].x, 0, sizeof(Cx<double>) * n3of(j), s));
     std::vector<int> active(nj, 1);
     std::vector<double> rel(nj, 0.0);
     std::vector<int> inner_it(nj, 0), outer(nj, 0);
-    CHK(true_residual(m, rel, active));
-    SolverState* hin = m->h_st + 2 * nj;
-    const int blocks = (int)((n3 + kThreads - 1) / kThreads);
+    CHK(true_residual(S, rel, active));
+    SolverState* hin = S.h_st + 2 * nj;
     for (int c = 0; c < o->max_outer; c++) {
         bool any = false;
         for (int j = 0; j < nj; j++) {
@@ -589,19 +685,19 @@ int solve_mixed(tdda_model_s* m, const tdda_solve_opts* o, tdda_solve_info* info
         // inner right-hand side = r (complex64)
         for (int j = 0; j < nj; j++)
             if (active[j])
-                k_demote<<<blocks, kThreads, 0, s>>>(m->jo.h[j].r, m->v32.e + j * n3, n3);
-        CHK(init_states<float>(m, m->j32, hin, active, o->inner_tol, o->inner_max_it));
-        launch_init_zero<float>(m->j32, nj, s);
+                k_demote<<<blocks(j), kThreads, 0, s>>>(S.jo->h[j].r, const_cast<Cx<float>*>(S.j32->h[j].e), n3of(j));
+        CHK(init_states<float>(S, *S.j32, hin, active, o->inner_tol, o->inner_max_it));
+        launch_init_zero<float>(*S.j32, nj, s);
         CHK(check_launch());
-        CHK(run_iterations<float>(m, m->j32, hin, o->inner_max_it, o->check_every));
+        CHK(run_iterations<float>(S, *S.j32, hin, o->inner_max_it, o->check_every));
         for (int j = 0; j < nj; j++) {
             if (!active[j]) continue;
             inner_it[j] += hin[j].iter;
             outer[j] += 1;
-            k_accum<<<blocks, kThreads, 0, s>>>(m->v64.x + j * n3, m->v32.x + j * n3, n3);
+            k_accum<<<blocks(j), kThreads, 0, s>>>(S.j64->h[j].x, S.j32->h[j].x, n3of(j));
         }
         CHK(check_launch());
-        CHK(true_residual(m, rel, active));
+        CHK(true_residual(S, rel, active));
     }
     for (int j = 0; j < nj; j++) {
         info[j].status = (rel[j] <= o->tol) ? TDDA_SOLVE_CONVERGED : TDDA_SOLVE_MAXIT;
@@ -613,27 +709,46 @@ int solve_mixed(tdda_model_s* m, const tdda_solve_opts* o, tdda_solve_info* info
     return TDDA_OK;
 }
 
+int solve_set(SolveSet& S, const tdda_solve_opts* opts, tdda_solve_info* info)
+{
+    if (opts->max_it < 0 || !(opts->tol >= 0)) return fail(TDDA_ERR_ARG, "bad iteration limit or tolerance");
+    if (opts->mode == TDDA_MODE_FP64) {
+        CHK(solve_fp64(S, opts, info));
+        if (opts->true_residual) {
+            std::vector<double> rel(S.nj, 0.0);
+            std::vector<int> act(S.nj, 1);
+            CHK(true_residual(S, rel, act));
+            for (int j = 0; j < S.nj; j++) info[j].true_residual = rel[j];
+        }
+        return TDDA_OK;
+    }
+    if (opts->mode == TDDA_MODE_MIXED) {
+        if (opts->inner_max_it <= 0 || opts->max_outer <= 0) return fail(TDDA_ERR_ARG, "bad inner/outer limits");
+        return solve_mixed(S, opts, info);
+    }
+    return fail(TDDA_ERR_ARG, "unknown solve mode");
+}
+
 }  // namespace
 
 template <typename T>
 static int time_passes_t(tdda_model m, JobTable<T>& jt, int reps, double* ms5)
 {
     hipStream_t s = m->op->ctx->stream;
-    const int nj = m->nrhs;
     hipEvent_t ev[6];
     for (int k = 0; k < 6; k++) HIPCHK(hipEventCreate(&ev[k]));
     double acc[5] = {0, 0, 0, 0, 0};
     for (int r = 0; r < reps + 1; r++) {
         HIPCHK(hipEventRecord(ev[0], s));
-        run_p1<T>(jt, nj, IN_PLAIN, s);
+        run_p1<T>(jt, IN_PLAIN, s);
         HIPCHK(hipEventRecord(ev[1], s));
-        run_py<T>(jt, nj, false, 0, s);
+        run_py<T>(jt, false, 0, s);
         HIPCHK(hipEventRecord(ev[2], s));
-        run_px<T>(jt, nj, 0, s);
+        run_px<T>(jt, 0, s);
         HIPCHK(hipEventRecord(ev[3], s));
-        run_py<T>(jt, nj, true, 0, s);
+        run_py<T>(jt, true, 0, s);
         HIPCHK(hipEventRecord(ev[4], s));
-        run_p5<T>(jt, nj, OUT_PLAIN, 1, s);
+        run_p5<T>(jt, OUT_PLAIN, 1, s);
         HIPCHK(hipEventRecord(ev[5], s));
         HIPCHK(hipEventSynchronize(ev[5]));
         if (r == 0) continue;  // warm-up
@@ -1027,17 +1142,17 @@ int tdda_model_create(tdda_op op, int nrhs, tdda_model* out)
     if (op->green64) {
         if ((rc = alloc_vecs<double>(m, m->v64)) != TDDA_OK) return bail(rc);
         Cx<double>* tw[3] = {op->tws64[0], op->tws64[1], op->tws64[2]};
-        if ((rc = build_table<double>(m, m->j64, m->v64, op->green64, tw, op->tp64, m->v64.r, m->v64.r0)) != TDDA_OK)
-            return bail(rc);
+        fill_jobs<double>(m, m->j64.h, m->v64, op->green64, tw, op->tp64, m->v64.r, m->v64.r0);
         // outer residual of the mixed solver: r -> p, its r0 copy -> t (scratch)
-        if ((rc = build_table<double>(m, m->jo, m->v64, op->green64, tw, op->tp64, m->v64.p, m->v64.t)) != TDDA_OK)
-            return bail(rc);
+        fill_jobs<double>(m, m->jo.h, m->v64, op->green64, tw, op->tp64, m->v64.p, m->v64.t);
+        if ((rc = finalize_table<double>(m->allocs, op->ctx, m->j64)) != TDDA_OK) return bail(rc);
+        if ((rc = finalize_table<double>(m->allocs, op->ctx, m->jo)) != TDDA_OK) return bail(rc);
     }
     if (op->green32) {
         if ((rc = alloc_vecs<float>(m, m->v32)) != TDDA_OK) return bail(rc);
         Cx<float>* tw[3] = {op->tws32[0], op->tws32[1], op->tws32[2]};
-        if ((rc = build_table<float>(m, m->j32, m->v32, op->green32, tw, op->tp32, m->v32.r, m->v32.r0)) != TDDA_OK)
-            return bail(rc);
+        fill_jobs<float>(m, m->j32.h, m->v32, op->green32, tw, op->tp32, m->v32.r, m->v32.r0);
+        if ((rc = finalize_table<float>(m->allocs, op->ctx, m->j32)) != TDDA_OK) return bail(rc);
     }
     if (hipHostMalloc((void**)&m->h_st, sizeof(SolverState) * 3 * nrhs, hipHostMallocDefault) != hipSuccess ||
         hipHostMalloc((void**)&m->h_poll, sizeof(SolverState) * 2 * nrhs, hipHostMallocDefault) != hipSuccess)
@@ -1050,9 +1165,18 @@ int tdda_model_create(tdda_op op, int nrhs, tdda_model* out)
     return TDDA_OK;
 }
 
+static void model_release(tdda_model m);
+
 int tdda_model_destroy(tdda_model m)
 {
     if (!m) return TDDA_OK;
+    model_release(m);
+    return TDDA_OK;
+}
+
+static void model_release(tdda_model m)
+{
+    if (--m->refs > 0) return;
     (void)hipSetDevice(m->op->ctx->device);
     (void)hipStreamSynchronize(m->op->ctx->stream);
     for (void* p : m->allocs) (void)hipFree(p);
@@ -1067,7 +1191,6 @@ int tdda_model_destroy(tdda_model m)
     tdda_op op = m->op;
     delete m;
     op_release(op);
-    return TDDA_OK;
 }
 
 static Cx<double>* vec64(tdda_model m, int which)
@@ -1285,25 +1408,118 @@ void tdda_solve_opts_default(tdda_solve_opts* o)
 int tdda_model_solve(tdda_model m, const tdda_solve_opts* opts, tdda_solve_info* info)
 {
     if (!m || !opts || !info) return fail(TDDA_ERR_ARG, "null argument");
-    if (opts->max_it < 0 || !(opts->tol >= 0)) return fail(TDDA_ERR_ARG, "bad iteration limit or tolerance");
     HIPCHK(hipSetDevice(m->op->ctx->device));
     if (!m->v64.x) return fail(TDDA_ERR_STATE, "solves need an operator built with TDDA_C128");
-    int rc;
-    if (opts->mode == TDDA_MODE_FP64) {
-        rc = solve_fp64(m, opts, info);
-        if (rc == TDDA_OK && opts->true_residual) {
-            std::vector<double> rel(m->nrhs, 0.0);
-            std::vector<int> act(m->nrhs, 1);
-            rc = true_residual(m, rel, act);
-            for (int j = 0; j < m->nrhs; j++) info[j].true_residual = rel[j];
-        }
-    } else if (opts->mode == TDDA_MODE_MIXED) {
-        if (opts->inner_max_it <= 0 || opts->max_outer <= 0) return fail(TDDA_ERR_ARG, "bad inner/outer limits");
-        rc = solve_mixed(m, opts, info);
-    } else {
-        return fail(TDDA_ERR_ARG, "unknown solve mode");
+    SolveSet S{m->op->ctx->stream, m->nrhs, &m->j64, &m->jo, m->v32.x ? &m->j32 : nullptr, m->h_st, m->h_poll, m->ev};
+    return solve_set(S, opts, info);
+}
+
+// ---------------------------------------------------------------------------
+// Batches: the right-hand sides of several models (possibly different
+// structures, grids and wavelengths) solved together -- every pass is one
+// launch over the tiles of all of them.
+// ---------------------------------------------------------------------------
+int tdda_batch_create(tdda_model* models, int nmodels, tdda_batch* out)
+{
+    if (!models || !out || nmodels <= 0) return fail(TDDA_ERR_ARG, "bad model list");
+    tdda_ctx ctx = models[0] ? models[0]->op->ctx : nullptr;
+    bool all32 = true;
+    int nj = 0;
+    for (int i = 0; i < nmodels; i++) {
+        if (!models[i]) return fail(TDDA_ERR_ARG, "null model");
+        if (models[i]->op->ctx != ctx) return fail(TDDA_ERR_ARG, "models of one batch must share a context");
+        if (!models[i]->v64.x) return fail(TDDA_ERR_STATE, "batched models need complex128 storage");
+        all32 = all32 && models[i]->v32.x != nullptr;
+        nj += models[i]->nrhs;
     }
-    return rc;
+    HIPCHK(hipSetDevice(ctx->device));
+    tdda_batch b = new tdda_batch_s();
+    b->ctx = ctx;
+    ctx->refs++;
+    b->nj = nj;
+    for (int i = 0; i < nmodels; i++) {
+        models[i]->refs++;
+        b->models.push_back(models[i]);
+        for (int j = 0; j < models[i]->nrhs; j++) {
+            b->j64.h.push_back(models[i]->j64.h[j]);
+            b->jo.h.push_back(models[i]->jo.h[j]);
+            if (all32) b->j32.h.push_back(models[i]->j32.h[j]);
+        }
+    }
+    b->has32 = all32;
+    auto bail = [&](int rc) {
+        tdda_batch_destroy(b);
+        return rc;
+    };
+    int rc;
+    if ((rc = finalize_table<double>(b->allocs, ctx, b->j64)) != TDDA_OK) return bail(rc);
+    if ((rc = finalize_table<double>(b->allocs, ctx, b->jo)) != TDDA_OK) return bail(rc);
+    if (all32 && (rc = finalize_table<float>(b->allocs, ctx, b->j32)) != TDDA_OK) return bail(rc);
+    if (hipHostMalloc((void**)&b->h_st, sizeof(SolverState) * 3 * nj, hipHostMallocDefault) != hipSuccess ||
+        hipHostMalloc((void**)&b->h_poll, sizeof(SolverState) * 2 * nj, hipHostMallocDefault) != hipSuccess)
+        return bail(fail(TDDA_ERR_OOM, "pinned host state"));
+    for (int k = 0; k < 2; k++)
+        if (hipEventCreateWithFlags(&b->ev[k], hipEventDisableTiming) != hipSuccess)
+            return bail(fail(TDDA_ERR_HIP, "event creation"));
+    if (hipStreamSynchronize(ctx->stream) != hipSuccess) return bail(fail(TDDA_ERR_HIP, "batch set-up"));
+    *out = b;
+    return TDDA_OK;
+}
+
+int tdda_batch_destroy(tdda_batch b)
+{
+    if (!b) return TDDA_OK;
+    (void)hipSetDevice(b->ctx->device);
+    (void)hipStreamSynchronize(b->ctx->stream);
+    for (void* p : b->allocs) (void)hipFree(p);
+    if (b->h_st) (void)hipHostFree(b->h_st);
+    if (b->h_poll) (void)hipHostFree(b->h_poll);
+    for (int k = 0; k < 2; k++)
+        if (b->ev[k]) (void)hipEventDestroy(b->ev[k]);
+    for (tdda_model m : b->models) model_release(m);
+    tdda_ctx ctx = b->ctx;
+    delete b;
+    ctx_release(ctx);
+    return TDDA_OK;
+}
+
+int tdda_batch_size(tdda_batch b) { return b ? b->nj : 0; }
+
+int tdda_batch_solve(tdda_batch b, const tdda_solve_opts* opts, tdda_solve_info* info)
+{
+    if (!b || !opts || !info) return fail(TDDA_ERR_ARG, "null argument");
+    HIPCHK(hipSetDevice(b->ctx->device));
+    SolveSet S{b->ctx->stream, b->nj, &b->j64, &b->jo, b->has32 ? &b->j32 : nullptr, b->h_st, b->h_poll, b->ev};
+    return solve_set(S, opts, info);
+}
+
+int tdda_batch_apply(tdda_batch b, int precision)
+{
+    if (!b) return fail(TDDA_ERR_ARG, "null batch");
+    HIPCHK(hipSetDevice(b->ctx->device));
+    hipStream_t s = b->ctx->stream;
+    if (precision == TDDA_C128) {
+        launch_apply<double>(b->j64, b->nj, IN_PLAIN, OUT_PLAIN, 1, 0, s);
+    } else if (precision == TDDA_C64) {
+        if (!b->has32) return fail(TDDA_ERR_STATE, "batch has no complex64 storage");
+        for (int j = 0; j < b->nj; j++) {
+            const size_t n3 = 3 * (size_t)b->j64.h[j].N;
+            const unsigned blocks = (unsigned)((n3 + kThreads - 1) / kThreads);
+            k_demote<<<blocks, kThreads, 0, s>>>(b->j64.h[j].x, b->j32.h[j].x, n3);
+        }
+        launch_apply<float>(b->j32, b->nj, IN_PLAIN, OUT_PLAIN, 1, 0, s);
+        for (int j = 0; j < b->nj; j++) {
+            const size_t n3 = 3 * (size_t)b->j64.h[j].N;
+            const unsigned blocks = (unsigned)((n3 + kThreads - 1) / kThreads);
+            HIPCHK(hipMemsetAsync(b->j64.h[j].ap, 0, n3 * sizeof(Cx<double>), s));
+            k_accum<<<blocks, kThreads, 0, s>>>(b->j64.h[j].ap, b->j32.h[j].ap, n3);
+        }
+    } else {
+        return fail(TDDA_ERR_ARG, "precision must be TDDA_C128 or TDDA_C64");
+    }
+    CHK(check_launch());
+    HIPCHK(hipStreamSynchronize(s));
+    return TDDA_OK;
 }
 
 }  // extern "C"

@@ -30,7 +30,8 @@ EXPORTS = [
     "tdda_model_create", "tdda_model_destroy", "tdda_model_set", "tdda_model_get", "tdda_model_apply",
     "tdda_solve_opts_default", "tdda_model_solve", "tdda_model_get_entries", "tdda_model_set_entries",
     "tdda_model_time_passes", "tdda_material_rule", "tdda_get_alpha", "tdda_incident_field",
-    "tdda_polarizability", "tdda_local_field",
+    "tdda_polarizability", "tdda_local_field", "tdda_batch_create", "tdda_batch_destroy", "tdda_batch_size",
+    "tdda_batch_solve", "tdda_batch_apply",
 ]
 
 
@@ -94,6 +95,11 @@ def lib():
         L.tdda_solve_opts_default.argtypes = [ctypes.POINTER(SolveOpts)]
         L.tdda_solve_opts_default.restype = None
         L.tdda_model_solve.argtypes = [vp, ctypes.POINTER(SolveOpts), ctypes.POINTER(SolveInfo)]
+        L.tdda_batch_create.argtypes = [ctypes.POINTER(vp), ctypes.c_int, ctypes.POINTER(vp)]
+        L.tdda_batch_destroy.argtypes = [vp]
+        L.tdda_batch_size.argtypes = [vp]
+        L.tdda_batch_solve.argtypes = [vp, ctypes.POINTER(SolveOpts), ctypes.POINTER(SolveInfo)]
+        L.tdda_batch_apply.argtypes = [vp, ctypes.c_int]
         _lib = L
     return _lib
 
@@ -251,6 +257,11 @@ class AProductCore:
             pass
 
 
+def _infos(info):
+    return [dict(status=i.status, iterations=i.iterations, error=i.error, true_residual=i.true_residual,
+                 outer=i.outer) for i in info]
+
+
 def solve_opts(**kw) -> SolveOpts:
     o = SolveOpts()
     lib().tdda_solve_opts_default(ctypes.byref(o))
@@ -308,12 +319,44 @@ class DeviceModel:
         o = solve_opts(**kw)
         info = (SolveInfo * self.nrhs)()
         _check(lib().tdda_model_solve(self.h, ctypes.byref(o), info))
-        return [dict(status=i.status, iterations=i.iterations, error=i.error, true_residual=i.true_residual,
-                     outer=i.outer) for i in info]
+        return _infos(info)
 
     def close(self):
         if self.h:
             lib().tdda_model_destroy(self.h)
+            self.h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Batch:
+    """Right-hand sides of several DeviceModels (any structures, grids and
+    wavelengths on one context) solved together: each operator pass is one
+    launch over all of them.  Results equal the models' solves alone."""
+
+    def __init__(self, models):
+        self.models = list(models)
+        arr = (ctypes.c_void_p * len(self.models))(*[m.h.value for m in self.models])
+        self.h = ctypes.c_void_p()
+        _check(lib().tdda_batch_create(arr, len(self.models), ctypes.byref(self.h)))
+        self.size = lib().tdda_batch_size(self.h)
+
+    def solve(self, **kw):
+        o = solve_opts(**kw)
+        info = (SolveInfo * self.size)()
+        _check(lib().tdda_batch_solve(self.h, ctypes.byref(o), info))
+        return _infos(info)
+
+    def apply(self, precision=C128):
+        _check(lib().tdda_batch_apply(self.h, precision))
+
+    def close(self):
+        if self.h:
+            lib().tdda_batch_destroy(self.h)
             self.h = ctypes.c_void_p()
 
     def __del__(self):
