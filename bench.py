@@ -67,6 +67,130 @@ def run_step(T, F, A, targets, diel, mode, lat):
     return fi, ai
 
 
+def batch_bytes(cores_rhs, s):
+    """Algorithmic bytes of one application over a batch: every right-hand
+    side streams its vectors and intermediates, each structure's Green
+    octant is read once for all its right-hand sides."""
+    tot = np.zeros(5)
+    for core, R in cores_rhs:
+        b = pass_bytes(core, s)
+        Mx, My, Mz = core.grid
+        green = (Mx // 2 + 1) * (My // 2 + 1) * (Mz // 2 + 1) * 6 * s
+        b = b * R
+        b[2] -= (R - 1) * green
+        tot += b
+    return tot
+
+
+def pass_report(ms, b):
+    ms = np.asarray(ms, dtype=np.float64)
+    return {"ms": [round(float(x), 5) for x in ms],
+            "gbs": [round(float(bb / (t * 1e-3) / 1e9), 1) for bb, t in zip(b, ms)],
+            "op_gbs": round(float(b.sum() / (ms.sum() * 1e-3) / 1e9), 1),
+            "op_frac": round(float(b.sum() / (ms.sum() * 1e-3) / 1e9 / HBM_PEAK_GBS), 4)}
+
+
+def bench_mixed128(T, W, ctx, reps):
+    """128 x 128 x 32, 50 % occupancy binary design: mixed-precision solve
+    (complex64 inner, FP64 refinement) against the plain FP64 solver; field
+    error against the FP64 device solver converged to 1e-11."""
+    prob = W.mixed128(seed=3)
+    al, E, _ = prob.rhs(np.array([0.0, 0.0, 1.0]), np.array([1.0, 0.0, 0.0]))
+    core = T.AProductCore(ctx, prob.R, prob.Nx, prob.Ny, prob.Nz, prob.d, prob.lam, prob.nback)
+    m = T.DeviceModel(core, 1)
+    m.set(T.VEC_AL, al)
+    m.set(T.VEC_E, E)
+    ref = m.solve(mode=T.MODE_FP64, tol=1e-11, x0_zero=1, true_residual=1)[0]
+    Pref = m.get(T.VEC_P)[0]
+    out = {"workload": "mixed128: 128x128x32, d=8nm, lambda=600nm, 50% occupancy, eps=(2+0.01i)^2",
+           "grid": list(core.grid), "dipoles": prob.N,
+           "reference_solve": {"iterations": ref["iterations"], "true_residual": ref["true_residual"]}}
+    for mode, name in ((T.MODE_FP64, "fp64"), (T.MODE_MIXED, "mixed")):
+        m.solve(mode=mode, tol=TOL, x0_zero=1, true_residual=1)
+        lat, infos = [], []
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            infos.append(m.solve(mode=mode, tol=TOL, x0_zero=1, true_residual=1)[0])
+            lat.append(time.perf_counter() - t0)
+        P = m.get(T.VEC_P)[0]
+        out[name] = {"solves_per_s": reps / sum(lat), "p50_ms": float(np.percentile(lat, 50) * 1e3),
+                     "p99_ms": float(np.percentile(lat, 99) * 1e3), "iterations": infos[-1]["iterations"],
+                     "outer": infos[-1]["outer"],
+                     "true_residual_max": max(i["true_residual"] for i in infos),
+                     "all_converged": all(i["status"] == T.CONVERGED and i["true_residual"] <= TOL for i in infos),
+                     "field_error": float(np.linalg.norm(P - Pref) / np.linalg.norm(Pref))}
+    out["mixed_vs_plain_fp64"] = round(out["mixed"]["solves_per_s"] / out["fp64"]["solves_per_s"], 3)
+    m.set(T.VEC_P, E)
+    out["aproduct_passes"] = {"c128": pass_report(m.time_passes(T.C128, reps=20), pass_bytes(core, 16)),
+                              "c64": pass_report(m.time_passes(T.C64, reps=20), pass_bytes(core, 8))}
+    m.close()
+    core.close()
+    return out
+
+
+def bench_batch64(T, W, ctx, reps):
+    """64 uneven structures x 8 incidences (512 right-hand sides): one
+    batched mixed-precision solve of all of them against plain FP64 solves
+    of one right-hand side at a time (and against the batched FP64 solver)."""
+    probs = W.batch64(seed=4)
+    incs = W.incidences()
+    models = []
+    for p in probs:
+        core = T.AProductCore(ctx, p.R, p.Nx, p.Ny, p.Nz, p.d, p.lam, p.nback)
+        rhs = [p.rhs(nK, nE)[:2] for nK, nE in incs]
+        m = T.DeviceModel(core, len(incs))
+        m.set(T.VEC_AL, np.concatenate([a for a, _ in rhs]))
+        m.set(T.VEC_E, np.concatenate([e for _, e in rhs]))
+        models.append((core, m, rhs))
+    batch = T.Batch([m for _, m, _ in models])
+    nrhs = batch.size
+    out = {"workload": "batch64: 64 structures Nx,Ny in 16..64, Nz in 8..16, d=10nm, lambda=500nm, "
+                       "2 polarisations x 4 angles", "rhs": nrhs,
+           "dipoles_total": int(sum(p.N for p in probs)) * len(incs)}
+    for mode, name in ((T.MODE_MIXED, "batched_mixed"), (T.MODE_FP64, "batched_fp64")):
+        batch.solve(mode=mode, tol=TOL, x0_zero=1, true_residual=1)
+        ts, infos = [], None
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            infos = batch.solve(mode=mode, tol=TOL, x0_zero=1, true_residual=1)
+            ts.append(time.perf_counter() - t0)
+        out[name] = {"solves_per_s": nrhs * reps / sum(ts), "batch_ms": float(np.median(ts) * 1e3),
+                     "iterations_mean": float(np.mean([i["iterations"] for i in infos])),
+                     "true_residual_max": max(i["true_residual"] for i in infos),
+                     "all_converged": all(i["status"] == T.CONVERGED and i["true_residual"] <= TOL
+                                          for i in infos)}
+    # plain FP64: one right-hand side at a time (device-resident inputs; the
+    # uploads between solves are outside the timed calls)
+    lat, infos = [], []
+    for core, _, rhs in models:
+        one = T.DeviceModel(core, 1)
+        for a, e in rhs:
+            one.set(T.VEC_AL, a)
+            one.set(T.VEC_E, e)
+            t0 = time.perf_counter()
+            infos.append(one.solve(mode=T.MODE_FP64, tol=TOL, x0_zero=1, true_residual=1)[0])
+            lat.append(time.perf_counter() - t0)
+        one.close()
+    out["plain_fp64"] = {"solves_per_s": len(lat) / sum(lat), "p50_ms": float(np.percentile(lat, 50) * 1e3),
+                         "p99_ms": float(np.percentile(lat, 99) * 1e3),
+                         "iterations_mean": float(np.mean([i["iterations"] for i in infos])),
+                         "true_residual_max": max(i["true_residual"] for i in infos),
+                         "all_converged": all(i["status"] == T.CONVERGED and i["true_residual"] <= TOL
+                                              for i in infos)}
+    out["batched_mixed_vs_plain_fp64"] = round(out["batched_mixed"]["solves_per_s"]
+                                               / out["plain_fp64"]["solves_per_s"], 3)
+    cr = [(core, m.nrhs) for core, m, _ in models]
+    for _, m, _ in models:
+        m.set(T.VEC_P, m.get(T.VEC_E))
+    out["aproduct_passes"] = {"c128": pass_report(batch.time_passes(T.C128, reps=10), batch_bytes(cr, 16)),
+                              "c64": pass_report(batch.time_passes(T.C64, reps=10), batch_bytes(cr, 8))}
+    batch.close()
+    for core, m, _ in models:
+        m.close()
+        core.close()
+    return out
+
+
 def cpu_baseline(prob, targets, al, E, diel, budget_s=30.0):
     """The oracle's FP64 restatement of the same step on the host cores."""
     sys.path.insert(0, ROOT)
@@ -96,6 +220,9 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--workloads", default="mixed128,batch64",
+                    help="extra workloads reported beside the headline (comma list, or none)")
+    ap.add_argument("--extra-reps", type=int, default=3)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); see the sharding design notes.",
@@ -145,10 +272,7 @@ def main():
     passes = {}
     for prec, s, key in ((T.C128, 16, "c128"), (T.C64, 8, "c64")):
         ms = F.time_passes(prec, reps=50)
-        b = pass_bytes(core, s)
-        passes[key] = {"ms": [round(float(x), 5) for x in ms],
-                       "gbs": [round(float(bb / (t * 1e-3) / 1e9), 1) for bb, t in zip(b, ms)],
-                       "op_gbs": round(float(b.sum() / (ms.sum() * 1e-3) / 1e9), 1)}
+        passes[key] = pass_report(ms, pass_bytes(core, s))
     ms128 = np.array(passes["c128"]["ms"])
     b128 = pass_bytes(core, 16)
     k = int(np.argmax(ms128))
@@ -162,6 +286,11 @@ def main():
             "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(float(b128[k] / (ms128[k] * 1e-3) / 1e9 / HBM_PEAK_GBS), 4),
             "traffic": traffic, "algorithmic_bytes": int(b128[k])}
+
+    extras = {}
+    for w in [x for x in args.workloads.split(",") if x and x != "none"]:
+        fn = {"mixed128": bench_mixed128, "batch64": bench_batch64}[w]
+        extras[w] = fn(T, W, ctx, args.extra_reps)
 
     cpu = None
     if not args.no_cpu:
@@ -191,6 +320,7 @@ def main():
         "aproduct_passes": passes,
         "roofline": roof,
         "cpu_baseline": cpu,
+        "workloads": extras,
     }
     print(json.dumps(line))
 

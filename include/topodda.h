@@ -201,6 +201,9 @@ int tdda_batch_size(tdda_batch b);
 int tdda_batch_solve(tdda_batch b, const tdda_solve_opts* opts, tdda_solve_info* info);
 /* Y = A P + al o P for every right-hand side (tdda_model_apply per model). */
 int tdda_batch_apply(tdda_batch b, int precision);
+/* Measurement hook (no reference counterpart): tdda_model_time_passes over
+ * every right-hand side of the batch (each pass timed as all its launches). */
+int tdda_batch_time_passes(tdda_batch b, int precision, int reps, double* ms5);
 
 #ifdef __cplusplus
 }

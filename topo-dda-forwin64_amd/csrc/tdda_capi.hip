@@ -14,6 +14,7 @@
 
 #include "topodda.h"
 #include "tdda_kernels.hpp"
+#include "tdda_setup.hpp"
 #include "tdda_launch.hpp"
 
 using namespace tdda;
@@ -732,9 +733,8 @@ int solve_set(SolveSet& S, const tdda_solve_opts* opts, tdda_solve_info* info)
 }  // namespace
 
 template <typename T>
-static int time_passes_t(tdda_model m, JobTable<T>& jt, int reps, double* ms5)
+static int time_passes_t(hipStream_t s, JobTable<T>& jt, int reps, double* ms5)
 {
-    hipStream_t s = m->op->ctx->stream;
     hipEvent_t ev[6];
     for (int k = 0; k < 6; k++) HIPCHK(hipEventCreate(&ev[k]));
     double acc[5] = {0, 0, 0, 0, 0};
@@ -1323,13 +1323,13 @@ int tdda_model_time_passes(tdda_model m, int precision, int reps, double* ms5)
     HIPCHK(hipSetDevice(m->op->ctx->device));
     if (precision == TDDA_C128) {
         if (!m->v64.x) return fail(TDDA_ERR_STATE, "operator built without TDDA_C128");
-        return time_passes_t<double>(m, m->j64, reps, ms5);
+        return time_passes_t<double>(m->op->ctx->stream, m->j64, reps, ms5);
     }
     if (precision == TDDA_C64) {
         if (!m->v32.x) return fail(TDDA_ERR_STATE, "operator built without TDDA_C64");
         const size_t n = m->n3 * m->nrhs;
         k_demote<<<(unsigned)((n + kThreads - 1) / kThreads), kThreads, 0, m->op->ctx->stream>>>(m->v64.x, m->v32.x, n);
-        return time_passes_t<float>(m, m->j32, reps, ms5);
+        return time_passes_t<float>(m->op->ctx->stream, m->j32, reps, ms5);
     }
     return fail(TDDA_ERR_ARG, "precision must be TDDA_C128 or TDDA_C64");
 }
@@ -1491,6 +1491,23 @@ int tdda_batch_solve(tdda_batch b, const tdda_solve_opts* opts, tdda_solve_info*
     HIPCHK(hipSetDevice(b->ctx->device));
     SolveSet S{b->ctx->stream, b->nj, &b->j64, &b->jo, b->has32 ? &b->j32 : nullptr, b->h_st, b->h_poll, b->ev};
     return solve_set(S, opts, info);
+}
+
+int tdda_batch_time_passes(tdda_batch b, int precision, int reps, double* ms5)
+{
+    if (!b || !ms5 || reps <= 0) return fail(TDDA_ERR_ARG, "bad argument");
+    HIPCHK(hipSetDevice(b->ctx->device));
+    hipStream_t s = b->ctx->stream;
+    if (precision == TDDA_C128) return time_passes_t<double>(s, b->j64, reps, ms5);
+    if (precision == TDDA_C64) {
+        if (!b->has32) return fail(TDDA_ERR_STATE, "batch has no complex64 storage");
+        for (int j = 0; j < b->nj; j++) {
+            const size_t n3 = 3 * (size_t)b->j64.h[j].N;
+            k_demote<<<(unsigned)((n3 + kThreads - 1) / kThreads), kThreads, 0, s>>>(b->j64.h[j].x, b->j32.h[j].x, n3);
+        }
+        return time_passes_t<float>(s, b->j32, reps, ms5);
+    }
+    return fail(TDDA_ERR_ARG, "precision must be TDDA_C128 or TDDA_C64");
 }
 
 int tdda_batch_apply(tdda_batch b, int precision)

@@ -1,0 +1,186 @@
+// tdda_setup.hpp -- host-library-only kernels: precision conversion for the
+// mixed solver, sparse entry access, and the operator set-up (interaction
+// tensor on the padded grid, its 3D FFT, octant extraction).  Included by
+// tdda_capi.hip only.
+#pragma once
+#include "tdda_kernels.hpp"
+
+namespace tdda {
+
+// ---------------------------------------------------------------------------
+// Mixed precision glue: inner right-hand side and solution accumulation.
+// ---------------------------------------------------------------------------
+// e32 = c64(r64)
+static __global__ void __launch_bounds__(kThreads) k_demote(const Cx<double>* __restrict__ src, Cx<float>* __restrict__ dst,
+                                                     size_t n)
+{
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) {
+        const Cx<double> v = src[i];
+        dst[i] = Cx<float>{float(v.x), float(v.y)};
+    }
+}
+
+// x64 += c128(x32)
+static __global__ void __launch_bounds__(kThreads) k_accum(Cx<double>* __restrict__ x, const Cx<float>* __restrict__ d,
+                                                    size_t n)
+{
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) {
+        const Cx<float> v = d[i];
+        Cx<double> a = x[i];
+        a.x += double(v.x);
+        a.y += double(v.y);
+        x[i] = a;
+    }
+}
+
+// Sparse entry access: out[k] = v[idx[k]] and v[idx[k]] = vals[k].
+static __global__ void __launch_bounds__(kThreads) k_gather(const Cx<double>* __restrict__ v, const int* __restrict__ idx,
+                                                     int n, Cx<double>* __restrict__ out)
+{
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < n) out[k] = v[idx[k]];
+}
+
+static __global__ void __launch_bounds__(kThreads) k_scatter(Cx<double>* __restrict__ v, const int* __restrict__ idx, int n,
+                                                      const Cx<double>* __restrict__ vals)
+{
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < n) v[idx[k]] = vals[k];
+}
+
+// ---------------------------------------------------------------------------
+// Green spectrum set-up (AProductCore.cpp:54-229, built on the device).
+// ---------------------------------------------------------------------------
+// Interaction tensor AProductCore.cpp:588-623 (LDR_inter).
+__device__ __forceinline__ void ldr_inter(double K, double x, double y, double z, Cx<double>* out)
+{
+    const double xs = x * x, ys = y * y, zs = z * z;
+    const double rn = sqrt(xs + ys + zs);
+    if (rn == 0.0) {
+        for (int l = 0; l < 6; l++) out[l] = Cx<double>{0.0, 0.0};
+        return;
+    }
+    const double r2 = rn * rn, r3 = rn * rn * rn;
+    double sn, cs;
+    sincos(K * rn, &sn, &cs);
+    const Cx<double> c1{cs / r3, sn / r3};
+    const Cx<double> c2{1.0 / r2, -K * rn / r2};
+    const Cx<double> c3{K * K - 3.0 * c2.x, -3.0 * c2.y};
+    const double kk = K * K;
+    Cx<double> a[6];
+    a[0] = Cx<double>{c2.x * (r2 - 3.0 * xs) - kk * (ys + zs), c2.y * (r2 - 3.0 * xs)};
+    a[1] = cscale(c3, x * y);
+    a[2] = cscale(c3, z * x);
+    a[3] = Cx<double>{c2.x * (r2 - 3.0 * ys) - kk * (zs + xs), c2.y * (r2 - 3.0 * ys)};
+    a[4] = cscale(c3, y * z);
+    a[5] = Cx<double>{c2.x * (r2 - 3.0 * zs) - kk * (xs + ys), c2.y * (r2 - 3.0 * zs)};
+    for (int l = 0; l < 6; l++) out[l] = cmul(c1, a[l]);
+}
+
+// Fills the 6-component tensor on the whole padded grid [Mx][My][Mz][6]:
+// index i < N -> offset i d; i >= M - (N - 1) -> (i - M) d; otherwise zero
+// (AProductCore.cpp:54-194 with 2N-1 replaced by M).
+static __global__ void __launch_bounds__(kThreads) k_green_fill(Cx<double>* __restrict__ g, int Nx, int Ny, int Nz, int Mx,
+                                                         int My, int Mz, double d, double K)
+{
+    const size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const size_t M = (size_t)Mx * My * Mz;
+    if (idx >= M) return;
+    const int k = (int)(idx % Mz);
+    const int j = (int)((idx / Mz) % My);
+    const int i = (int)(idx / ((size_t)Mz * My));
+    const bool okx = (i <= Nx - 1) || (i >= Mx - (Nx - 1));
+    const bool oky = (j <= Ny - 1) || (j >= My - (Ny - 1));
+    const bool okz = (k <= Nz - 1) || (k >= Mz - (Nz - 1));
+    Cx<double> a[6];
+    if (okx && oky && okz) {
+        const int oi = (i <= Nx - 1) ? i : i - Mx;
+        const int oj = (j <= Ny - 1) ? j : j - My;
+        const int ok = (k <= Nz - 1) ? k : k - Mz;
+        ldr_inter(K, d * oi, d * oj, d * ok, a);
+    } else {
+        for (int l = 0; l < 6; l++) a[l] = Cx<double>{0.0, 0.0};
+    }
+    for (int l = 0; l < 6; l++) g[idx * 6 + l] = a[l];
+}
+
+// Generic in-place transform along one axis of an array [D0][D1][D2][NC]
+// viewed as pencils: element (o, a, f) at o*so + a*sa + f, f in [0, F).
+// A tile is (o, tf consecutive f).  Used for the Green spectrum (NC = 6).
+template <typename T, bool INV>
+__global__ void __launch_bounds__(kThreads) k_fft_strided(Cx<T>* __restrict__ data, int L, size_t so, size_t sa,
+                                                          int F, int tf, int ntf, int ps, FftPlan pl,
+                                                          const Cx<T>* __restrict__ tw)
+{
+    extern __shared__ __align__(16) char smem[];
+    const int o = blockIdx.x / ntf;
+    const int f0 = (blockIdx.x - o * ntf) * tf;
+    const int nf = min(tf, F - f0);
+    Cx<T>* A = reinterpret_cast<Cx<T>*>(smem);
+    Cx<T>* B = A + tf * ps;
+    Cx<T>* base = data + (size_t)o * so + f0;
+    for (int e = threadIdx.x; e < L * nf; e += blockDim.x) {
+        const int a = e / nf;
+        const int f = e - a * nf;
+        A[f * ps + a] = base[(size_t)a * sa + f];
+    }
+    __syncthreads();
+    const Cx<T>* res = lds_fft<T, INV>(A, B, nf, ps, pl, tw);
+    for (int e = threadIdx.x; e < L * nf; e += blockDim.x) {
+        const int a = e / nf;
+        const int f = e - a * nf;
+        base[(size_t)a * sa + f] = res[f * ps + a];
+    }
+}
+
+// Contiguous-axis variant: pencil q occupies data[q*L*NC ... ) as [a][NC].
+template <typename T, bool INV>
+__global__ void __launch_bounds__(kThreads) k_fft_contig(Cx<T>* __restrict__ data, int Q, int L, int NC, int P, int ps,
+                                                         FftPlan pl, const Cx<T>* __restrict__ tw)
+{
+    extern __shared__ __align__(16) char smem[];
+    const int q0 = blockIdx.x * P;
+    const int np = min(P, Q - q0);
+    Cx<T>* A = reinterpret_cast<Cx<T>*>(smem);
+    Cx<T>* B = A + P * NC * ps;
+    Cx<T>* base = data + (size_t)q0 * L * NC;
+    const int n = np * L * NC;
+    for (int e = threadIdx.x; e < n; e += blockDim.x) {
+        const int pl_ = e / (L * NC);
+        const int rem = e - pl_ * (L * NC);
+        const int a = rem / NC;
+        const int c = rem - a * NC;
+        A[(pl_ * NC + c) * ps + a] = base[e];
+    }
+    __syncthreads();
+    const Cx<T>* res = lds_fft<T, INV>(A, B, np * NC, ps, pl, tw);
+    for (int e = threadIdx.x; e < n; e += blockDim.x) {
+        const int pl_ = e / (L * NC);
+        const int rem = e - pl_ * (L * NC);
+        const int a = rem / NC;
+        const int c = rem - a * NC;
+        base[e] = res[(pl_ * NC + c) * ps + a];
+    }
+}
+
+// Octant extraction with the 1/M scale folded in: full [Mx][My][Mz][6] FP64
+// spectrum -> [Hx][Hy][Hz][6] of type T.
+template <typename T>
+__global__ void __launch_bounds__(kThreads) k_green_octant(const Cx<double>* __restrict__ full, Cx<T>* __restrict__ oct,
+                                                           int Mx, int My, int Mz, int Hx, int Hy, int Hz, double scale)
+{
+    const size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const size_t H = (size_t)Hx * Hy * Hz * 6;
+    if (idx >= H) return;
+    const int l = (int)(idx % 6);
+    const size_t pnt = idx / 6;
+    const int k = (int)(pnt % Hz);
+    const int j = (int)((pnt / Hz) % Hy);
+    const int i = (int)(pnt / ((size_t)Hz * Hy));
+    const Cx<double> v = full[(((size_t)i * My + j) * Mz + k) * 6 + l];
+    oct[idx] = Cx<T>{T(v.x * scale), T(v.y * scale)};
+}
+
+}  // namespace tdda

@@ -31,7 +31,7 @@ EXPORTS = [
     "tdda_solve_opts_default", "tdda_model_solve", "tdda_model_get_entries", "tdda_model_set_entries",
     "tdda_model_time_passes", "tdda_material_rule", "tdda_get_alpha", "tdda_incident_field",
     "tdda_polarizability", "tdda_local_field", "tdda_batch_create", "tdda_batch_destroy", "tdda_batch_size",
-    "tdda_batch_solve", "tdda_batch_apply",
+    "tdda_batch_solve", "tdda_batch_apply", "tdda_batch_time_passes",
 ]
 
 
@@ -100,6 +100,7 @@ def lib():
         L.tdda_batch_size.argtypes = [vp]
         L.tdda_batch_solve.argtypes = [vp, ctypes.POINTER(SolveOpts), ctypes.POINTER(SolveInfo)]
         L.tdda_batch_apply.argtypes = [vp, ctypes.c_int]
+        L.tdda_batch_time_passes.argtypes = [vp, ctypes.c_int, ctypes.c_int, dp]
         _lib = L
     return _lib
 
@@ -353,6 +354,11 @@ class Batch:
 
     def apply(self, precision=C128):
         _check(lib().tdda_batch_apply(self.h, precision))
+
+    def time_passes(self, precision=C128, reps=10):
+        out = np.zeros(5)
+        _check(lib().tdda_batch_time_passes(self.h, precision, reps, _dptr(out)))
+        return out
 
     def close(self):
         if self.h:
