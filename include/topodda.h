@@ -64,6 +64,7 @@ typedef struct tdda_ctx_s* tdda_ctx;
 typedef struct tdda_op_s* tdda_op;
 typedef struct tdda_model_s* tdda_model;
 typedef struct tdda_batch_s* tdda_batch;
+typedef struct tdda_server_s* tdda_server;
 
 /* Last error message of the calling thread ("" if none). */
 const char* tdda_last_error(void);
@@ -77,7 +78,8 @@ int tdda_ctx_sync(tdda_ctx ctx);
 /* Operator description.  AProductCore.cpp:12-44 takes these from
  * CoreStructure (N, Nx, Ny, Nz, R, d) and its own arguments (lam, nback).
  * R holds 3N integer lattice coordinates with 0 <= R[3i+a] < N_a; sites must
- * be distinct.  grid = (Mx, My, Mz) chooses the zero-padded FFT grid; 0 lets
+ * be distinct.  R may be NULL for a full box (N = Nx Ny Nz) in build_a_bulk
+ * order (tools.cpp:497-510).  grid = (Mx, My, Mz) chooses the zero-padded FFT grid; 0 lets
  * the planner pick the smallest {2,3,5}-smooth sizes >= 2N_a - 1.  Any grid
  * >= 2N_a - 1 gives the same linear convolution as the reference's 2N_a - 1.
  * K = 2 pi nback / lam (AProductCore.cpp:17).                               */
@@ -204,6 +206,68 @@ int tdda_batch_apply(tdda_batch b, int precision);
 /* Measurement hook (no reference counterpart): tdda_model_time_passes over
  * every right-hand side of the batch (each pass timed as all its launches). */
 int tdda_batch_time_passes(tdda_batch b, int precision, int reps, double* ms5);
+
+/* Request loop (no counterpart in the reference, which runs one task's
+ * solves in sequence: T/main.cpp:584-598, EvoDDAModel.cpp:433-441).  A
+ * server owns a worker thread and its own stream on `device`.  Submitted
+ * requests -- one structure with nrhs incidences -- wait in a FIFO queue
+ * until their device footprint fits the memory budget and nrhs solver
+ * slots are free; admitted right-hand sides run together (continuous
+ * batching: every tick is one launch group per pass over all running
+ * right-hand sides, whatever their structure and phase) and finished ones
+ * leave at once.  Each request's al and E are built on the device exactly
+ * as the DDAModel constructor builds them (DDAModel.cpp:12-69), or E is a
+ * sparse source (adjoint, EvoDDAModel.cpp:596).  Solves start from P = 0
+ * and follow tdda_model_solve (solve.mode, tolerances, limits), so a
+ * result does not depend on the other requests in flight.                  */
+typedef struct {
+    double hbm_budget_bytes; /* device memory the server may hold; <= 0: 85% of free memory */
+    int max_rhs;             /* right-hand sides in flight (solver slots); <= 0: 4096 */
+    tdda_solve_opts solve;   /* x0_zero is implied */
+} tdda_server_opts;
+
+typedef struct {
+    tdda_op_desc op;          /* structure; R may be NULL for a full box          */
+    const double* diel_old;   /* 3N design values t (CoreStructure::diel_old)      */
+    int nmat;                 /* material table (complex, after tdda_material_rule) */
+    const double* material;
+    int nrhs;
+    const double* nK;         /* 3 per right-hand side: propagation direction      */
+    const double* nE;         /* 3 per right-hand side: polarisation               */
+    const double* E0;         /* amplitude per right-hand side, or NULL for 1      */
+    int nsrc;                 /* > 0: E = sparse source instead of the plane wave  */
+    const int* src_idx;       /* nrhs * nsrc distinct entries of the 3N vector      */
+    const double* src_vals;   /* nrhs * nsrc complex values                        */
+} tdda_request;
+
+typedef struct {
+    double t_submit, t_admit, t_done; /* seconds, steady clock of the server */
+    double footprint_bytes;           /* device memory charged to the request */
+} tdda_request_times;
+
+typedef struct {
+    long long submitted, completed, failed, ticks;
+    int queued, active_requests, active_rhs;
+    double used_bytes, peak_bytes, budget_bytes;
+} tdda_server_stats;
+
+/* Defaults: budget 85% of free memory, 4096 slots, mixed precision to a
+ * true residual of 1e-8. */
+void tdda_server_opts_default(tdda_server_opts* o);
+int tdda_server_create(int device, const tdda_server_opts* o, tdda_server* out);
+/* Stops the loop; requests not yet finished fail. */
+int tdda_server_destroy(tdda_server s);
+/* Copies the request's inputs; returns at once with a ticket. */
+int tdda_server_submit(tdda_server s, const tdda_request* r, long long* ticket);
+/* 0: queued or running, 1: done, 2: failed (negative: error code). */
+int tdda_server_poll(tdda_server s, long long ticket);
+/* Blocks until the request is done or failed, or timeout_s passes (< 0:
+ * no limit); returns as tdda_server_poll. */
+int tdda_server_wait(tdda_server s, long long ticket, double timeout_s);
+/* Copies P (nrhs * 3N complex, right-hand side major) and the per-RHS solve
+ * information of a finished request and releases its ticket. */
+int tdda_server_result(tdda_server s, long long ticket, double* P, tdda_solve_info* info, tdda_request_times* t);
+int tdda_server_get_stats(tdda_server s, tdda_server_stats* out);
 
 #ifdef __cplusplus
 }

@@ -4,12 +4,20 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
+#include <chrono>
 #include <complex>
+#include <condition_variable>
+#include <deque>
+#include <map>
+#include <mutex>
+#include <thread>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "topodda.h"
@@ -146,7 +154,6 @@ struct tdda_op_s {
     int N, Nx, Ny, Nz, Mx, My, Mz, Hx, Hy, Hz;
     double d, lam, nback, K;
     int prec;
-    std::vector<int> R;
     int* cellmap = nullptr;
     Cx<double>* green64 = nullptr;
     Cx<float>* green32 = nullptr;
@@ -236,22 +243,26 @@ template <typename T>
 int plan_tiles(const tdda_op_s* op, TilePlan& t)
 {
     const size_t s = sizeof(Cx<T>);
-    const int full = kThreads * Ept<T>::v;  // prefetch capacity of a workgroup
-    const int cap = std::min(full, env_int(sizeof(T) == 8 ? "TDDA_TILE_C128" : "TDDA_TILE_C64", full));
-    const int cap1 = kThreads * (Ept<T>::v / 3);  // P1: up to three input vectors
+    // prefetch capacity of a workgroup in each pass (elements), as compiled
+    // into the pass kernel that will run: ept_len of its FFT length
+    auto full = [&](int kind, int L) { return kThreads * ept_len<T>(pass_length_compiled<T>(kind, L) ? L : 0); };
+    const int fz = full(0, op->Mz), fy = full(1, op->My), fx = full(2, op->Mx);
+    const int env = env_int(sizeof(T) == 8 ? "TDDA_TILE_C128" : "TDDA_TILE_C64", 1 << 30);
+    const int capz = std::min(fz, env), capy = std::min(fy, env), capx = std::min(fx, env);
+    const int cap1 = fz / 3;  // P1: up to three input vectors
     const int unit = (int)(128 / s);
     t.ps_z = lds_pencil_stride(op->Mz);
     t.ps_y = lds_pencil_stride(op->My);
     t.ps_x = lds_pencil_stride(op->Mx);
-    if (3 * op->Mz > cap || op->My > cap || 3 * op->Mx > cap)
+    if (3 * op->Mz > capz || op->My > capy || 3 * op->Mx > capx)
         return fail(TDDA_ERR_UNSUPPORTED, "FFT length too long for the pass tiles of this precision");
-    t.pz = std::max(1, std::min({cap / (3 * op->Mz), 64, op->Nx * op->Ny}));
+    t.pz = std::max(1, std::min({capz / (3 * op->Mz), 64, op->Nx * op->Ny}));
     if (3 * op->Nz > cap1) return fail(TDDA_ERR_UNSUPPORTED, "Nz too large for the z-pass tiles of this precision");
-    t.pz1 = std::max(1, std::min({cap1 / (3 * op->Nz), cap / (3 * op->Mz), 64, op->Nx * op->Ny}));
-    int ty = std::min(cap / op->My, 3 * op->Mz);
+    t.pz1 = std::max(1, std::min({cap1 / (3 * op->Nz), capz / (3 * op->Mz), 64, op->Nx * op->Ny}));
+    int ty = std::min(capy / op->My, 3 * op->Mz);
     if (ty >= unit && ty < 3 * op->Mz) ty = (ty / unit) * unit;
     t.tfy = std::max(1, ty);
-    int tx = std::min({cap / op->Mx, 3 * op->Mz, 3 * (8 * kThreads / op->Mx)});
+    int tx = std::min({capx / op->Mx, 3 * op->Mz, 3 * (8 * kThreads / op->Mx)});
     tx = (tx / 3) * 3;
     if (tx >= 3 * unit && tx < 3 * op->Mz) tx = (tx / (3 * unit)) * (3 * unit);
     t.tfx = std::max(3, tx);
@@ -374,38 +385,27 @@ JobTiles job_tiles(const JobDesc<T>& J)
     return t;
 }
 
-// Completes a job table from its host descriptors: per-job solver state,
-// partial-sum buffers and counters, tile prefix sums, persistent grid sizes,
-// LDS sizes (largest over the jobs) and the pass FFT lengths (0 when the
-// jobs differ, which selects the runtime-plan kernels).
-template <typename T>
-int finalize_table(std::vector<void*>& arena, const tdda_ctx_s* c, JobTable<T>& jt)
+// Per-pass launch groups of a job table whose host descriptors are final:
+// device descriptor array, vector-tile prefix sums and, for every pass kind,
+// one group per kernel (compiled FFT length, or 0 for the runtime-plan
+// kernel) with its own descriptor subset, tile prefix sums, LDS size and
+// persistent grid.  `put(host, count)` copies an array to device memory and
+// returns its device address (nullptr on failure).
+template <typename T, typename Put>
+int build_groups(const tdda_ctx_s* c, JobTable<T>& jt, Put&& put)
 {
     const int nj = (int)jt.h.size();
     const size_t s = sizeof(Cx<T>);
     std::vector<JobTiles> tiles(nj);
     std::vector<int> hv(nj + 1, 0);
-    std::vector<size_t> poff(nj + 1, 0);
     for (int j = 0; j < nj; j++) {
         tiles[j] = job_tiles(jt.h[j]);
         hv[j + 1] = hv[j] + tiles[j].v;
-        poff[j + 1] = poff[j] + 4 * (size_t)std::max(tiles[j].z, tiles[j].v);
     }
-    CHK(dev_alloc(arena, &jt.st, nj));
-    CHK(dev_alloc(arena, &jt.part, poff[nj]));
-    CHK(dev_alloc(arena, &jt.cnt, nj));
-    HIPCHK(hipMemsetAsync(jt.cnt, 0, sizeof(unsigned) * nj, c->stream));
-    HIPCHK(hipMemsetAsync(jt.st, 0, sizeof(SolverState) * nj, c->stream));
-    for (int j = 0; j < nj; j++) {
-        jt.h[j].st = jt.st + j;
-        jt.h[j].part = jt.part + poff[j];
-        jt.h[j].cnt = jt.cnt + j;
-    }
-    CHK(dev_alloc(arena, &jt.d, nj));
-    HIPCHK(hipMemcpy(jt.d, jt.h.data(), sizeof(JobDesc<T>) * nj, hipMemcpyHostToDevice));
+    jt.d = const_cast<JobDesc<T>*>(put(jt.h.data(), (size_t)nj));
+    jt.tv = const_cast<int*>(put(hv.data(), (size_t)nj + 1));
+    if (!jt.d || !jt.tv) return fail(TDDA_ERR_OOM, "job table upload");
     jt.nv = hv[nj];
-    CHK(dev_alloc(arena, &jt.tv, nj + 1));
-    HIPCHK(hipMemcpy(jt.tv, hv.data(), sizeof(int) * (nj + 1), hipMemcpyHostToDevice));
     // persistent grids: as many workgroups as stay resident (LDS-limited,
     // at most 8 per CU)
     auto grid = [&](int ntiles, size_t lds) {
@@ -434,18 +434,9 @@ int finalize_table(std::vector<void*>& arena, const tdda_ctx_s* c, JobTable<T>& 
             if (g.lds > c->lds_budget) return fail(TDDA_ERR_UNSUPPORTED, "pass tile exceeds the LDS budget");
             g.nj = (int)sub.size();
             g.grid = grid(ts.back(), g.lds);
-            if (g.nj == nj) {
-                g.d = jt.d;
-            } else {
-                JobDesc<T>* d;
-                CHK(dev_alloc(arena, &d, sub.size()));
-                HIPCHK(hipMemcpy(d, sub.data(), sizeof(JobDesc<T>) * sub.size(), hipMemcpyHostToDevice));
-                g.d = d;
-            }
-            int* dts;
-            CHK(dev_alloc(arena, &dts, ts.size()));
-            HIPCHK(hipMemcpy(dts, ts.data(), sizeof(int) * ts.size(), hipMemcpyHostToDevice));
-            g.ts = dts;
+            g.d = (g.nj == nj) ? jt.d : put(sub.data(), sub.size());
+            g.ts = put(ts.data(), ts.size());
+            if (!g.d || !g.ts) return fail(TDDA_ERR_OOM, "job table upload");
             groups.push_back(g);
         }
         return TDDA_OK;
@@ -460,6 +451,44 @@ int finalize_table(std::vector<void*>& arena, const tdda_ctx_s* c, JobTable<T>& 
     CHK(build(jt.gx, 2, [](const JobDesc<T>& J) { return J.Mx; }, [](const JobTiles& t) { return t.x; },
               [&](const JobDesc<T>& J) { return (size_t)(tw2_slots(J.Mx, J.plx.tb) + 2 * J.tfx * J.ps_x) * s; }));
     return TDDA_OK;
+}
+
+// Partial-sum slots of one job (deterministic reductions: one slot of 4
+// values per tile of its largest reducing pass).
+template <typename T>
+size_t job_part_size(const JobDesc<T>& J)
+{
+    const JobTiles t = job_tiles(J);
+    return 4 * (size_t)std::max(t.z, t.v);
+}
+
+// Completes a job table from its host descriptors: per-job solver state,
+// partial-sum buffers and counters, then the launch groups, all in device
+// memory owned by `arena`.
+template <typename T>
+int finalize_table(std::vector<void*>& arena, const tdda_ctx_s* c, JobTable<T>& jt)
+{
+    const int nj = (int)jt.h.size();
+    std::vector<size_t> poff(nj + 1, 0);
+    for (int j = 0; j < nj; j++) poff[j + 1] = poff[j] + job_part_size(jt.h[j]);
+    CHK(dev_alloc(arena, &jt.st, nj));
+    CHK(dev_alloc(arena, &jt.part, poff[nj]));
+    CHK(dev_alloc(arena, &jt.cnt, nj));
+    HIPCHK(hipMemsetAsync(jt.cnt, 0, sizeof(unsigned) * nj, c->stream));
+    HIPCHK(hipMemsetAsync(jt.st, 0, sizeof(SolverState) * nj, c->stream));
+    for (int j = 0; j < nj; j++) {
+        jt.h[j].st = jt.st + j;
+        jt.h[j].part = jt.part + poff[j];
+        jt.h[j].cnt = jt.cnt + j;
+    }
+    auto put = [&](const auto* host, size_t n) {
+        using U = std::remove_const_t<std::remove_pointer_t<decltype(host)>>;
+        U* d = nullptr;
+        if (dev_alloc(arena, &d, n) != TDDA_OK) return (const U*)nullptr;
+        if (hipMemcpy(d, host, sizeof(U) * n, hipMemcpyHostToDevice) != hipSuccess) return (const U*)nullptr;
+        return (const U*)d;
+    };
+    return build_groups(c, jt, put);
 }
 
 template <typename T>
@@ -940,8 +969,7 @@ int tdda_op_create(tdda_ctx ctx, const tdda_op_desc* desc, tdda_op* out)
 {
     if (!ctx || !desc || !out) return fail(TDDA_ERR_ARG, "null argument");
     const tdda_op_desc& D = *desc;
-    if (D.N <= 0 || D.Nx <= 0 || D.Ny <= 0 || D.Nz <= 0 || !D.R)
-        return fail(TDDA_ERR_ARG, "lattice sizes must be positive and R non-null");
+    if (D.N <= 0 || D.Nx <= 0 || D.Ny <= 0 || D.Nz <= 0) return fail(TDDA_ERR_ARG, "lattice sizes must be positive");
     if (!(D.d > 0) || !(D.lam > 0) || !(D.nback > 0)) return fail(TDDA_ERR_ARG, "d, lam and nback must be positive");
     const long long box = (long long)D.Nx * D.Ny * D.Nz;
     if (box > (1LL << 30) || D.N > box) return fail(TDDA_ERR_ARG, "lattice too large or N exceeds the box");
@@ -956,10 +984,12 @@ int tdda_op_create(tdda_ctx ctx, const tdda_op_desc* desc, tdda_op* out)
         if (M[a] > 4096) return fail(TDDA_ERR_UNSUPPORTED, "FFT length above 4096");
     }
     HIPCHK(hipSetDevice(ctx->device));
-    // bulk order (tools.cpp:497-510) on the full box needs no cell map
+    // bulk order (tools.cpp:497-510) on the full box needs no cell map;
+    // R == NULL stands for that order
+    if (!D.R && box != D.N) return fail(TDDA_ERR_ARG, "R may be NULL only for a full box");
     bool bulk = (box == D.N);
-    std::vector<int> cmap((size_t)box, -1);
-    for (int i = 0; i < D.N; i++) {
+    std::vector<int> cmap(D.R ? (size_t)box : 0, -1);
+    for (int i = 0; D.R && i < D.N; i++) {
         const int x = D.R[3 * i], y = D.R[3 * i + 1], z = D.R[3 * i + 2];
         if (x < 0 || x >= D.Nx || y < 0 || y >= D.Ny || z < 0 || z >= D.Nz) {
             return fail(TDDA_ERR_ARG, "site coordinate outside the Nx x Ny x Nz box");
@@ -980,7 +1010,6 @@ int tdda_op_create(tdda_ctx ctx, const tdda_op_desc* desc, tdda_op* out)
     op->d = D.d; op->lam = D.lam; op->nback = D.nback;
     op->K = 2.0 * M_PI / (D.lam / D.nback);  // AProductCore.cpp:17
     op->prec = prec;
-    op->R.assign(D.R, D.R + 3 * (size_t)D.N);
     auto cleanup = [&](int rc) {
         tdda_op_destroy(op);
         return rc;
@@ -1540,3 +1569,5 @@ int tdda_batch_apply(tdda_batch b, int precision)
 }
 
 }  // extern "C"
+
+#include "tdda_serve.inc"

@@ -400,7 +400,7 @@ __device__ __forceinline__ void ct_stage(const Cx<T>* __restrict__ src, Cx<T>* _
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int npw = (np - wave + kWaves - 1) / kWaves;  // this wavefront's pencils
     const int nb = npw * LR;
-#pragma unroll
+#pragma unroll 1
     for (int it = 0; it < (NBMAX + 63) / 64; it++) {
         const int b = lane + it * 64;
         if (b < nb) {

@@ -39,6 +39,18 @@ constexpr int kThreads = kThreadsFft;
 #define TDDA_PASS_WAVES 4
 #endif
 #define TDDA_PASS_BOUNDS __launch_bounds__(kThreads, TDDA_PASS_WAVES)
+#ifndef TDDA_EPT_C64
+#define TDDA_EPT_C64 12
+#endif
+#ifndef TDDA_EPT_C64_SMALL
+#define TDDA_EPT_C64_SMALL 6
+#endif
+#ifndef TDDA_EPT_C128_SMALL
+#define TDDA_EPT_C128_SMALL 6
+#endif
+#ifndef TDDA_EPT_SPLIT
+#define TDDA_EPT_SPLIT 256
+#endif
 #ifndef TDDA_EPT_C128
 #define TDDA_EPT_C128 6
 #endif
@@ -216,20 +228,33 @@ __device__ __forceinline__ bool job_inactive(const SolverState* st)
 // flattened tile list (all jobs) with stride gridDim.x and software-pipelines
 // it -- iteration i moves the prefetched inputs of tile i-1 into LDS, issues
 // the global loads of tile i into registers, then transforms tile i-1, so the
-// loads overlap the transform.  Ept<T> is the number of prefetched elements
-// per thread; tiles hold at most kThreads * Ept<T>::v elements (P1: a third
-// of that per input vector).
+// loads overlap the transform.  EP = ept_len<T>(L) is the number of
+// prefetched elements per thread; tiles hold at most kThreads * EP elements
+// (P1: a third of that per input vector).
 // ---------------------------------------------------------------------------
 template <typename T>
 struct Ept;
 template <>
 struct Ept<double> {
-    static constexpr int v = TDDA_EPT_C128;
+    static constexpr int big = TDDA_EPT_C128;
+    static constexpr int small = TDDA_EPT_C128_SMALL;
 };
 template <>
 struct Ept<float> {
-    static constexpr int v = 12;
+    static constexpr int big = TDDA_EPT_C64;
+    static constexpr int small = TDDA_EPT_C64_SMALL;
 };
+
+// Prefetched elements per thread of a pass whose FFT length is L (0: the
+// runtime-plan kernel, which takes the larger setting).  Pencils shorter
+// than TDDA_EPT_SPLIT points use smaller tiles (less LDS per workgroup, more
+// resident workgroups); longer pencils keep larger tiles so that the x pass
+// still reads rows of several consecutive columns.
+template <typename T>
+__host__ __device__ constexpr int ept_len(int L)
+{
+    return (L == 0 || L >= TDDA_EPT_SPLIT) ? Ept<T>::big : Ept<T>::small;
+}
 
 // Prefetch registers: real and imaginary parts in separate scalar arrays
 // (an array of 16-byte complex structs is left in scratch memory by the
@@ -249,7 +274,7 @@ struct Pre {
 template <typename T>
 __host__ __device__ constexpr int np_max(int L)
 {
-    return L > 0 ? (kThreads * Ept<T>::v) / L : 0;
+    return L > 0 ? (kThreads * ept_len<T>(L)) / L : 0;
 }
 
 // Transform of a pass tile: compile-time length LC > 0, or the runtime plan.
@@ -314,7 +339,7 @@ __global__ void TDDA_PASS_BOUNDS k_p1(const JobDesc<T>* __restrict__ jobs, const
 {
     extern __shared__ __align__(16) char smem[];
     constexpr int NV = (MODE == IN_P) ? 3 : ((MODE == IN_T) ? 2 : 1);
-    constexpr int EP = Ept<T>::v / 3;
+    constexpr int EP = ept_len<T>(LC) / 3;
     const int ntot = tstart[njobs];
     const float r3 = 1.0f / 3.0f;
     Pre<T, EP> pr[NV];
@@ -460,7 +485,7 @@ __global__ void TDDA_PASS_BOUNDS k_py(const JobDesc<T>* __restrict__ jobs, const
                                       int njobs, int solver)
 {
     extern __shared__ __align__(16) char smem[];
-    constexpr int EP = Ept<T>::v;
+    constexpr int EP = ept_len<T>(LC);
     const int ntot = tstart[njobs];
     Pre<T, EP> pr;
     STile cur;
@@ -544,7 +569,7 @@ __global__ void TDDA_PASS_BOUNDS k_px(const JobDesc<T>* __restrict__ jobs, const
                                       int njobs, int solver)
 {
     extern __shared__ __align__(16) char smem[];
-    constexpr int EP = Ept<T>::v;
+    constexpr int EP = ept_len<T>(LC);
     const int ntot = tstart[njobs];
     Pre<T, EP> pr;
     STile cur;
@@ -655,7 +680,7 @@ __global__ void TDDA_PASS_BOUNDS k_p5(const JobDesc<T>* __restrict__ jobs, const
     extern __shared__ __align__(16) char smem[];
     __shared__ double red[16 * 4];
     __shared__ int s_last;
-    constexpr int EP = Ept<T>::v;
+    constexpr int EP = ept_len<T>(LC);
     const int ntot = tstart[njobs];
     const float r3 = 1.0f / 3.0f;
     Pre<T, EP> pr;

@@ -1,0 +1,4 @@
+// complex128 operator passes: y and x passes (P2, P3, P4)
+#define TDDA_T double
+#define TDDA_PASS_PART 2
+#include "tdda_pass.inc"

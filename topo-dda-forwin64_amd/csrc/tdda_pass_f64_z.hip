@@ -1,0 +1,4 @@
+// complex128 operator passes: z passes (P1, P5)
+#define TDDA_T double
+#define TDDA_PASS_PART 1
+#include "tdda_pass.inc"

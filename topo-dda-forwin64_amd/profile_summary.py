@@ -1,0 +1,124 @@
+#!/usr/bin/env python3
+"""Summarises rocprofv3 CSV output of the operator passes.
+
+  python3 topo-dda-forwin64_amd/profile_summary.py stats  <kernel_stats.csv>  [--top 20]
+  python3 topo-dda-forwin64_amd/profile_summary.py traffic <fetch counter_collection.csv>
+          <write counter_collection.csv> --precision c128|c64 --workload topo80 [--json out.json]
+
+`stats` prints the per-kernel duration table (markdown).  `traffic` prints the
+average FETCH_SIZE / WRITE_SIZE per dispatch of each pass kernel, scaled by
+the calibration that the plain device copy in the same run gives (the copy
+of --copy-mib MiB run by profile_passes.py reads and writes exactly that
+many bytes), next to the algorithmic bytes of the pass; with --json it
+writes {pass name: measured HBM bytes per dispatch}.
+"""
+import argparse
+import collections
+import csv
+import json
+import os
+import sys
+
+sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
+
+PASS_NAMES = {"k_p1<": "k_p1_zfwd", "k_py<{t}, false": "k_py_yfwd", "k_px<": "k_px_xgreen",
+              "k_py<{t}, true": "k_py_yinv", "k_p5<": "k_p5_zinv"}
+
+
+def stats(path, top):
+    rows = list(csv.DictReader(open(path)))
+    rows.sort(key=lambda r: -float(r["TotalDurationNs"]))
+    print("| kernel | calls | avg us | min us | max us | % time |")
+    print("|---|---|---|---|---|---|")
+    for r in rows[:top]:
+        name = r["Name"].split("(")[0].replace("void ", "")
+        print(f"| `{name}` | {r['Calls']} | {float(r['AverageNs']) / 1e3:.2f} | {float(r['MinNs']) / 1e3:.2f} | "
+              f"{float(r['MaxNs']) / 1e3:.2f} | {float(r['Percentage']):.1f} |")
+
+
+def per_kernel(path, counter):
+    agg = collections.defaultdict(list)
+    for r in csv.DictReader(open(path)):
+        if r["Counter_Name"] == counter:
+            agg[r["Kernel_Name"]].append(float(r["Counter_Value"]) * 1024.0)
+    return agg
+
+
+def traffic(fetch_csv, write_csv, precision, workload, copy_mib, out_json):
+    import workloads as W
+    t = "double" if precision == "c128" else "float"
+    s = 16 if precision == "c128" else 8
+    fetch = per_kernel(fetch_csv, "FETCH_SIZE")
+    write = per_kernel(write_csv, "WRITE_SIZE")
+    copy_bytes = copy_mib * 1024.0 * 1024.0
+    # the copy may be split into several dispatches: take the largest ones
+    # that together move copy_bytes of writes
+    cw = sorted(write.get("__amd_rocclr_copyBuffer", []), reverse=True)
+    cf = sorted(fetch.get("__amd_rocclr_copyBuffer", []), reverse=True)
+    acc, n = 0.0, 0
+    while n < len(cw) and acc < 0.999 * copy_bytes:
+        acc += cw[n]
+        n += 1
+    wfac = copy_bytes / sum(cw[:n])
+    ffac = copy_bytes / sum(cf[:n])
+    prob = W.topo80()[0] if workload == "topo80" else W.mixed128()
+
+    class Core:
+        pass
+    core = Core()
+    core.Nx, core.Ny, core.Nz, core.N = prob.Nx, prob.Ny, prob.Nz, prob.N
+    core.grid = tuple(next_smooth(2 * n - 1) for n in (prob.Nx, prob.Ny, prob.Nz))
+    sys.path.insert(0, os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+    from bench import pass_bytes
+    alg = pass_bytes(core, s)
+    print(f"calibration: device copy of {copy_mib} MiB in {n} dispatches; FETCH_SIZE x {ffac:.3f}, "
+          f"WRITE_SIZE x {wfac:.3f}")
+    print("| pass | kernel | dispatches | fetch MB | write MB | measured MB | algorithmic MB | ratio |")
+    print("|---|---|---|---|---|---|---|---|")
+    res = {}
+    for k, (pat, pname) in enumerate(PASS_NAMES.items()):
+        pat = pat.format(t=t)
+        names = [kn for kn in fetch if f"tdda::{pat}" in kn and f"<{t}" in kn]
+        if not names:
+            continue
+        kn = names[0]
+        f = sum(fetch[kn]) / len(fetch[kn]) * ffac
+        w = sum(write.get(kn, [0.0])) / max(1, len(write.get(kn, []))) * wfac
+        res[pname] = int(f + w)
+        print(f"| {pname} | `{kn.split('(')[0].replace('void ', '')}` | {len(fetch[kn])} | {f / 1e6:.2f} | "
+              f"{w / 1e6:.2f} | {(f + w) / 1e6:.2f} | {alg[k] / 1e6:.2f} | {(f + w) / alg[k]:.2f} |")
+    if out_json:
+        with open(out_json, "w") as fh:
+            json.dump(res, fh, indent=1)
+
+
+def next_smooth(n):
+    m = n
+    while True:
+        k = m
+        for p in (2, 3, 5):
+            while k % p == 0:
+                k //= p
+        if k == 1:
+            return m
+        m += 1
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("mode", choices=["stats", "traffic"])
+    ap.add_argument("files", nargs="+")
+    ap.add_argument("--top", type=int, default=20)
+    ap.add_argument("--precision", default="c128")
+    ap.add_argument("--workload", default="topo80")
+    ap.add_argument("--copy-mib", type=int, default=256)
+    ap.add_argument("--json", default=None)
+    a = ap.parse_args()
+    if a.mode == "stats":
+        stats(a.files[0], a.top)
+    else:
+        traffic(a.files[0], a.files[1], a.precision, a.workload, a.copy_mib, a.json)
+
+
+if __name__ == "__main__":
+    main()

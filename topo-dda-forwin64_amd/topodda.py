@@ -31,7 +31,9 @@ EXPORTS = [
     "tdda_solve_opts_default", "tdda_model_solve", "tdda_model_get_entries", "tdda_model_set_entries",
     "tdda_model_time_passes", "tdda_material_rule", "tdda_get_alpha", "tdda_incident_field",
     "tdda_polarizability", "tdda_local_field", "tdda_batch_create", "tdda_batch_destroy", "tdda_batch_size",
-    "tdda_batch_solve", "tdda_batch_apply", "tdda_batch_time_passes",
+    "tdda_batch_solve", "tdda_batch_apply", "tdda_batch_time_passes", "tdda_server_opts_default",
+    "tdda_server_create", "tdda_server_destroy", "tdda_server_submit", "tdda_server_poll", "tdda_server_wait",
+    "tdda_server_result", "tdda_server_get_stats",
 ]
 
 
@@ -55,6 +57,29 @@ class SolveOpts(ctypes.Structure):
 class SolveInfo(ctypes.Structure):
     _fields_ = [("status", ctypes.c_int), ("iterations", ctypes.c_int), ("error", ctypes.c_double),
                 ("true_residual", ctypes.c_double), ("outer", ctypes.c_int)]
+
+
+class ServerOpts(ctypes.Structure):
+    _fields_ = [("hbm_budget_bytes", ctypes.c_double), ("max_rhs", ctypes.c_int), ("solve", SolveOpts)]
+
+
+class Request(ctypes.Structure):
+    _dp = ctypes.POINTER(ctypes.c_double)
+    _fields_ = [("op", OpDesc), ("diel_old", _dp), ("nmat", ctypes.c_int), ("material", _dp),
+                ("nrhs", ctypes.c_int), ("nK", _dp), ("nE", _dp), ("E0", _dp), ("nsrc", ctypes.c_int),
+                ("src_idx", ctypes.POINTER(ctypes.c_int)), ("src_vals", _dp)]
+
+
+class RequestTimes(ctypes.Structure):
+    _fields_ = [("t_submit", ctypes.c_double), ("t_admit", ctypes.c_double), ("t_done", ctypes.c_double),
+                ("footprint_bytes", ctypes.c_double)]
+
+
+class ServerStats(ctypes.Structure):
+    _fields_ = [("submitted", ctypes.c_longlong), ("completed", ctypes.c_longlong), ("failed", ctypes.c_longlong),
+                ("ticks", ctypes.c_longlong), ("queued", ctypes.c_int), ("active_requests", ctypes.c_int),
+                ("active_rhs", ctypes.c_int), ("used_bytes", ctypes.c_double), ("peak_bytes", ctypes.c_double),
+                ("budget_bytes", ctypes.c_double)]
 
 
 _lib = None
@@ -101,6 +126,16 @@ def lib():
         L.tdda_batch_solve.argtypes = [vp, ctypes.POINTER(SolveOpts), ctypes.POINTER(SolveInfo)]
         L.tdda_batch_apply.argtypes = [vp, ctypes.c_int]
         L.tdda_batch_time_passes.argtypes = [vp, ctypes.c_int, ctypes.c_int, dp]
+        L.tdda_server_opts_default.argtypes = [ctypes.POINTER(ServerOpts)]
+        L.tdda_server_opts_default.restype = None
+        L.tdda_server_create.argtypes = [ctypes.c_int, ctypes.POINTER(ServerOpts), ctypes.POINTER(vp)]
+        L.tdda_server_destroy.argtypes = [vp]
+        L.tdda_server_submit.argtypes = [vp, ctypes.POINTER(Request), ctypes.POINTER(ctypes.c_longlong)]
+        L.tdda_server_poll.argtypes = [vp, ctypes.c_longlong]
+        L.tdda_server_wait.argtypes = [vp, ctypes.c_longlong, ctypes.c_double]
+        L.tdda_server_result.argtypes = [vp, ctypes.c_longlong, dp, ctypes.POINTER(SolveInfo),
+                                         ctypes.POINTER(RequestTimes)]
+        L.tdda_server_get_stats.argtypes = [vp, ctypes.POINTER(ServerStats)]
         _lib = L
     return _lib
 
@@ -363,6 +398,100 @@ class Batch:
     def close(self):
         if self.h:
             lib().tdda_batch_destroy(self.h)
+            self.h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Server:
+    """Request loop with continuous batching on one device (own worker
+    thread and stream).  submit() copies the request and returns a ticket;
+    result() returns (P[nrhs, 3N], per-RHS solve info, times)."""
+
+    def __init__(self, device: int = 0, budget_bytes: float = 0.0, max_rhs: int = 0, **solve_kw):
+        o = ServerOpts()
+        lib().tdda_server_opts_default(ctypes.byref(o))
+        o.hbm_budget_bytes = budget_bytes
+        o.max_rhs = max_rhs
+        for k, v in solve_kw.items():
+            setattr(o.solve, k, v)
+        self.h = ctypes.c_void_p()
+        _check(lib().tdda_server_create(device, ctypes.byref(o), ctypes.byref(self.h)))
+        self.shape = {}
+
+    def submit(self, Nx, Ny, Nz, d, lam, t, material, incidences, E0=None, R=None, sources=None, nback=1.0,
+               grid=None):
+        """t: 3N design values; material: complex list after material_rule;
+        incidences: [(nK, nE), ...]; sources: None or [(idx, vals), ...] per
+        incidence (sparse right-hand sides, same count each)."""
+        t = _f(t)
+        N = len(t) // 3
+        g = grid or (0, 0, 0)
+        Rp = None
+        if R is not None:
+            R = np.ascontiguousarray(R, dtype=np.int32)
+            Rp = R.ctypes.data_as(ctypes.POINTER(ctypes.c_int))
+        m = _c(material)
+        nrhs = len(incidences)
+        nK = _f(np.array([i[0] for i in incidences]).reshape(-1))
+        nE = _f(np.array([i[1] for i in incidences]).reshape(-1))
+        e0 = None if E0 is None else _f(E0)
+        q = Request()
+        q.op = OpDesc(N, Nx, Ny, Nz, Rp, d, lam, nback, g[0], g[1], g[2], 0)
+        q.diel_old = _dptr(t)
+        q.nmat = len(m)
+        q.material = _dptr(m.view(np.float64))
+        q.nrhs = nrhs
+        q.nK = _dptr(nK)
+        q.nE = _dptr(nE)
+        q.E0 = _dptr(e0) if e0 is not None else None
+        keep = []
+        if sources:
+            idx = np.ascontiguousarray(np.concatenate([np.asarray(i, np.int32) for i, _ in sources]), dtype=np.int32)
+            val = _c(np.concatenate([np.asarray(v) for _, v in sources]))
+            q.nsrc = len(sources[0][0])
+            q.src_idx = idx.ctypes.data_as(ctypes.POINTER(ctypes.c_int))
+            q.src_vals = _dptr(val.view(np.float64))
+            keep = [idx, val]
+        tk = ctypes.c_longlong()
+        _check(lib().tdda_server_submit(self.h, ctypes.byref(q), ctypes.byref(tk)))
+        del keep
+        self.shape[tk.value] = (N, nrhs)
+        return tk.value
+
+    def poll(self, ticket):
+        rc = lib().tdda_server_poll(self.h, ticket)
+        if rc < 0:
+            _check(rc)
+        return rc
+
+    def wait(self, ticket, timeout=-1.0):
+        rc = lib().tdda_server_wait(self.h, ticket, float(timeout))
+        if rc < 0:
+            _check(rc)
+        return rc
+
+    def result(self, ticket):
+        N, nrhs = self.shape.pop(ticket)
+        P = np.zeros((nrhs, 3 * N), np.complex128)
+        info = (SolveInfo * nrhs)()
+        tm = RequestTimes()
+        _check(lib().tdda_server_result(self.h, ticket, _dptr(P.view(np.float64)), info, ctypes.byref(tm)))
+        return P, _infos(info), dict(t_submit=tm.t_submit, t_admit=tm.t_admit, t_done=tm.t_done,
+                                     footprint_bytes=tm.footprint_bytes)
+
+    def stats(self):
+        st = ServerStats()
+        _check(lib().tdda_server_get_stats(self.h, ctypes.byref(st)))
+        return {k: getattr(st, k) for k, _ in ServerStats._fields_}
+
+    def close(self):
+        if self.h:
+            lib().tdda_server_destroy(self.h)
             self.h = ctypes.c_void_p()
 
     def __del__(self):
