@@ -18,6 +18,7 @@ from __future__ import annotations
 
 import argparse
 import json
+import math
 import os
 import platform
 import sys
@@ -191,6 +192,104 @@ def bench_batch64(T, W, ctx, reps):
     return out
 
 
+def bench_serve(T, seconds, rate=200.0, seed=5):
+    """Request stream: Poisson arrivals at `rate`/s for `seconds`; dipole
+    count log-uniform in [8k, 2M], Nz in {8, 16, 32}, Nx = Ny =
+    round(sqrt(n / Nz)); design values U[0, 1) (eps between 1.01 and
+    2.25 + 0.01i); lambda U[450, 700] nm, d = 10 nm; 1-4 right-hand sides;
+    half forward (z-incident plane wave, x or y polarised), half adjoint
+    (sources at 100 seeded target dipoles).  Mixed precision to a true
+    residual of 1e-8.  Latency = result ready - scheduled arrival (both on
+    the monotonic clock)."""
+    import threading
+    rng = np.random.default_rng(seed)
+    pool = rng.random(3 * 2_100_000)
+    mat = T.material_rule(np.array([1.0, 2.25 + 0.01j]))
+    reqs = []
+    t = 0.0
+    while True:
+        t += rng.exponential(1.0 / rate)
+        if t > seconds:
+            break
+        n = math.exp(rng.uniform(math.log(8192), math.log(2_097_152)))
+        Nz = int(rng.choice([8, 16, 32]))
+        Nx = max(1, int(round(math.sqrt(n / Nz))))
+        N = Nx * Nx * Nz
+        nrhs = int(rng.integers(1, 5))
+        adjoint = bool(rng.random() < 0.5)
+        incs = [(np.array([0.0, 0.0, 1.0]), np.array([1.0, 0.0, 0.0]) if rng.random() < 0.5 else
+                 np.array([0.0, 1.0, 0.0])) for _ in range(nrhs)]
+        src = None
+        if adjoint:
+            src = []
+            for _ in range(nrhs):
+                sites = rng.choice(N, 100, replace=False)
+                idx = np.sort((3 * sites[:, None] + np.arange(3)[None, :]).reshape(-1)).astype(np.int32)
+                src.append((idx, rng.standard_normal(300) + 1j * rng.standard_normal(300)))
+        off = int(rng.integers(0, len(pool) - 3 * N + 1))
+        reqs.append(dict(at=t, Nx=Nx, Nz=Nz, N=N, lam=float(rng.uniform(450.0, 700.0)), incs=incs, src=src,
+                         off=off))
+    srv = T.Server(0)
+    done = []
+    tickets = []
+    lock = threading.Condition()
+
+    def collect():
+        k = 0
+        while True:
+            with lock:
+                while k >= len(tickets):
+                    lock.wait()
+                tk, at = tickets[k]
+            k += 1
+            if tk is None:
+                return
+            srv.wait(tk, -1.0)
+            try:
+                _, infos, tm = srv.result(tk, want_P=False)
+                done.append((at, tm, infos))
+            except T.TddaError as e:
+                done.append((at, None, str(e)))
+
+    th = threading.Thread(target=collect)
+    th.start()
+    t0 = time.monotonic()
+    lag = []
+    for r in reqs:
+        now = time.monotonic()
+        if t0 + r["at"] > now:
+            time.sleep(t0 + r["at"] - now)
+        lag.append(time.monotonic() - (t0 + r["at"]))
+        tk = srv.submit(r["Nx"], r["Nx"], r["Nz"], 10.0, r["lam"], pool[r["off"]:r["off"] + 3 * r["N"]], mat,
+                        r["incs"], sources=r["src"])
+        with lock:
+            tickets.append((tk, t0 + r["at"]))
+            lock.notify()
+    with lock:
+        tickets.append((None, None))
+        lock.notify()
+    th.join()
+    t_end = max(tm["t_done"] for _, tm, _ in done if tm)
+    st = srv.stats()
+    srv.close()
+    ok = [(at, tm, infos) for at, tm, infos in done if tm]
+    lat = np.array([tm["t_done"] - at for at, tm, _ in ok])
+    nrhs = sum(len(infos) for _, _, infos in ok)
+    tr = [i["true_residual"] for _, _, infos in ok for i in infos]
+    return {"workload": f"serve: Poisson {rate:g}/s for {seconds:g} s, dipoles log-U[8k, 2M], Nz in {{8,16,32}}, "
+                        "1-4 RHS, half adjoint, mixed precision to 1e-8",
+            "requests": len(reqs), "completed": len(ok), "failed": len(done) - len(ok),
+            "rhs_solved": nrhs, "solves_per_s": nrhs / (t_end - t0),
+            "requests_per_s": len(ok) / (t_end - t0), "wall_s": t_end - t0,
+            "latency_ms": {"p50": float(np.percentile(lat, 50) * 1e3), "p99": float(np.percentile(lat, 99) * 1e3)},
+            "submit_lag_ms_max": float(max(lag) * 1e3),
+            "true_residual_max": max(tr), "all_converged": all(i["status"] == T.CONVERGED
+                                                              for _, _, infos in ok for i in infos),
+            "dipoles_mean": float(np.mean([r["N"] for r in reqs])),
+            "peak_hbm_gb": st["peak_bytes"] / 1e9, "budget_gb": st["budget_bytes"] / 1e9, "ticks": st["ticks"],
+            "worker_s": {"setup": st["setup_s"], "ticks": st["tick_s"], "release": st["release_s"]}}
+
+
 def cpu_baseline(prob, targets, al, E, diel, budget_s=30.0):
     """The oracle's FP64 restatement of the same step on the host cores."""
     sys.path.insert(0, ROOT)
@@ -220,9 +319,10 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--workloads", default="mixed128,batch64",
+    ap.add_argument("--workloads", default="mixed128,batch64,serve",
                     help="extra workloads reported beside the headline (comma list, or none)")
     ap.add_argument("--extra-reps", type=int, default=3)
+    ap.add_argument("--serve-seconds", type=float, default=1.0, help="arrival window of the request stream")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); see the sharding design notes.",
@@ -289,6 +389,9 @@ def main():
 
     extras = {}
     for w in [x for x in args.workloads.split(",") if x and x != "none"]:
+        if w == "serve":
+            extras[w] = bench_serve(T, args.serve_seconds)
+            continue
         fn = {"mixed128": bench_mixed128, "batch64": bench_batch64}[w]
         extras[w] = fn(T, W, ctx, args.extra_reps)
 

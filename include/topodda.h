@@ -249,6 +249,7 @@ typedef struct {
     long long submitted, completed, failed, ticks;
     int queued, active_requests, active_rhs;
     double used_bytes, peak_bytes, budget_bytes;
+    double setup_s, tick_s, release_s; /* worker time: request set-up, ticks, releases */
 } tdda_server_stats;
 
 /* Defaults: budget 85% of free memory, 4096 slots, mixed precision to a

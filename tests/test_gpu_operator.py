@@ -68,6 +68,20 @@ def test_aproduct_config2_size(ctx):
     assert O.rel_l2(core.Aproduct(b), ref) < TOL64
 
 
+@pytest.mark.parametrize("dims", [(512, 3, 2), (300, 4, 3)])
+def test_aproduct_long_x_pencils(ctx, dims):
+    # Mx = 1024 / 600: the complex128 x pass holds 3 columns of Mx points
+    # (more than 64 KB of LDS for 1024); full box given as R = NULL
+    Nx, Ny, Nz = dims
+    print(f"aproduct long x {dims}", flush=True)
+    R, N, b, al = problem(Nx, Ny, Nz, 10.0, 500.0, seed=Nx)
+    core = T.AProductCore(ctx, None, Nx, Ny, Nz, 10.0, 500.0)
+    assert core.grid[0] == (1024 if Nx == 512 else 600)
+    ref = O.aproduct_numpy_fft(R, Nx, Ny, Nz, 10.0, O.wavenumber(500.0), b, grid=core.grid)
+    assert O.rel_l2(core.Aproduct(b), ref) < TOL64
+    assert O.rel_l2(core.Aproduct(b, precision=T.C64), ref) < TOL32
+
+
 def test_grid_size_independence(ctx):
     # any FFT grid >= 2N-1 gives the same linear convolution (the reference uses 2N-1)
     R, N, b, _ = problem(6, 5, 4, 10.0, 500.0, seed=4)

@@ -254,7 +254,10 @@ int plan_tiles(const tdda_op_s* op, TilePlan& t)
     t.ps_z = lds_pencil_stride(op->Mz);
     t.ps_y = lds_pencil_stride(op->My);
     t.ps_x = lds_pencil_stride(op->Mx);
-    if (3 * op->Mz > capz || op->My > capy || 3 * op->Mx > capx)
+    // one tile must hold: 3 z pencils (P5 reads all kz), one y column of My
+    // rows (P4), and the Nx loaded rows of 3 x columns (P3; the Mx - Nx
+    // padding rows are zero-filled in LDS, bounded by the LDS check below)
+    if (3 * op->Mz > capz || op->My > capy || 3 * op->Nx > capx)
         return fail(TDDA_ERR_UNSUPPORTED, "FFT length too long for the pass tiles of this precision");
     t.pz = std::max(1, std::min({capz / (3 * op->Mz), 64, op->Nx * op->Ny}));
     if (3 * op->Nz > cap1) return fail(TDDA_ERR_UNSUPPORTED, "Nz too large for the z-pass tiles of this precision");
@@ -815,7 +818,9 @@ int tdda_ctx_create(int device, tdda_ctx* out)
     c->device = device;
     c->refs = 1;
     c->ncu = prop.multiProcessorCount;
-    c->lds_budget = std::min<size_t>(prop.sharedMemPerBlock, 65536);
+    // gfx950: 160 KB of LDS per CU, all of it available to one workgroup;
+    // only tiles of very long x pencils (c128, Mx > 512) need more than 64 KB
+    c->lds_budget = std::min<size_t>(prop.sharedMemPerBlock, 163840);
     c->lds_cu = prop.maxSharedMemoryPerMultiProcessor ? prop.maxSharedMemoryPerMultiProcessor : 65536;
     hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
     if (e != hipSuccess) {

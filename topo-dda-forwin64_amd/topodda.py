@@ -79,7 +79,8 @@ class ServerStats(ctypes.Structure):
     _fields_ = [("submitted", ctypes.c_longlong), ("completed", ctypes.c_longlong), ("failed", ctypes.c_longlong),
                 ("ticks", ctypes.c_longlong), ("queued", ctypes.c_int), ("active_requests", ctypes.c_int),
                 ("active_rhs", ctypes.c_int), ("used_bytes", ctypes.c_double), ("peak_bytes", ctypes.c_double),
-                ("budget_bytes", ctypes.c_double)]
+                ("budget_bytes", ctypes.c_double), ("setup_s", ctypes.c_double), ("tick_s", ctypes.c_double),
+                ("release_s", ctypes.c_double)]
 
 
 _lib = None
@@ -251,13 +252,14 @@ class AProductCore:
 
     def __init__(self, ctx: Context, R, Nx, Ny, Nz, d, lam, nback=1.0, grid=None, precisions=C128 | C64):
         self.ctx = ctx
-        self.R = np.ascontiguousarray(R, dtype=np.int32)
-        self.N = len(self.R) // 3
+        # R = None: the full box in build_a_bulk order
+        self.R = None if R is None else np.ascontiguousarray(R, dtype=np.int32)
+        self.N = Nx * Ny * Nz if R is None else len(self.R) // 3
         self.Nx, self.Ny, self.Nz = Nx, Ny, Nz
         self.d, self.lam, self.nback = d, lam, nback
         g = grid or (0, 0, 0)
-        desc = OpDesc(self.N, Nx, Ny, Nz, self.R.ctypes.data_as(ctypes.POINTER(ctypes.c_int)), d, lam, nback,
-                      g[0], g[1], g[2], precisions)
+        Rp = None if R is None else self.R.ctypes.data_as(ctypes.POINTER(ctypes.c_int))
+        desc = OpDesc(self.N, Nx, Ny, Nz, Rp, d, lam, nback, g[0], g[1], g[2], precisions)
         self.h = ctypes.c_void_p()
         _check(lib().tdda_op_create(ctx.h, ctypes.byref(desc), ctypes.byref(self.h)))
         g3 = (ctypes.c_int * 3)()
@@ -475,12 +477,13 @@ class Server:
             _check(rc)
         return rc
 
-    def result(self, ticket):
+    def result(self, ticket, want_P=True):
         N, nrhs = self.shape.pop(ticket)
-        P = np.zeros((nrhs, 3 * N), np.complex128)
+        P = np.zeros((nrhs, 3 * N), np.complex128) if want_P else None
         info = (SolveInfo * nrhs)()
         tm = RequestTimes()
-        _check(lib().tdda_server_result(self.h, ticket, _dptr(P.view(np.float64)), info, ctypes.byref(tm)))
+        _check(lib().tdda_server_result(self.h, ticket, _dptr(P.view(np.float64)) if want_P else None, info,
+                                        ctypes.byref(tm)))
         return P, _infos(info), dict(t_submit=tm.t_submit, t_admit=tm.t_admit, t_done=tm.t_done,
                                      footprint_bytes=tm.footprint_bytes)
 
