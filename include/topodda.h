@@ -156,13 +156,17 @@ int tdda_model_apply(tdda_model m, int precision);
  * of the five passes (z-forward, y-forward, x-fused Green product,
  * y-inverse, z-inverse), timed with HIP events on the model's stream.     */
 int tdda_model_time_passes(tdda_model m, int precision, int reps, double* ms5);
+/* Measurement hook (no reference counterpart): pass `pass` (0..4, order as
+ * above) launched reps times back to back between two HIP events; *ms is
+ * the mean per launch.  Overwrites the model's pass intermediates and Y.   */
+int tdda_model_time_pass(tdda_model m, int precision, int pass, int reps, double* ms);
 
 typedef struct {
     int max_it;        /* MAX_ITERATION of DDAModel::bicgstab             */
     double tol;        /* MAX_ERROR: stop when ||r|| / ||E|| <= tol       */
     int mode;          /* TDDA_MODE_FP64 or TDDA_MODE_MIXED               */
     int x0_zero;       /* 1: start from P = 0 (skips the A P product)     */
-    double inner_tol;  /* mixed: relative tolerance of each inner solve   */
+    double inner_tol;  /* mixed: tolerance of an inner solve (loosened to tol / (2 rel) when the residual rel is near tol) */
     int inner_max_it;  /* mixed: iteration cap of each inner solve        */
     int max_outer;     /* mixed: refinement cycles                        */
     int check_every;   /* iterations between host status polls (0: auto)  */

@@ -634,12 +634,12 @@ int run_iterations(SolveSet& S, JobTable<T>& jt, SolverState* hst, int max_total
 
 template <typename T>
 int init_states(SolveSet& S, JobTable<T>& jt, SolverState* hst, const std::vector<int>& active, double tol,
-                int maxit)
+                int maxit, const std::vector<double>* tols = nullptr)
 {
     for (int j = 0; j < S.nj; j++) {
         SolverState& St = hst[j];
         memset(&St, 0, sizeof(St));
-        St.tol = tol;
+        St.tol = tols ? (*tols)[j] : tol;
         St.maxit = maxit;
         St.first = 1;
         St.status = active[j] ? ST_RUNNING : ST_CONVERGED;
@@ -694,6 +694,15 @@ int true_residual(SolveSet& S, std::vector<double>& rel, const std::vector<int>&
     return TDDA_OK;
 }
 
+// Stop test of an inner complex64 solve whose right-hand side is the
+// residual r (||r|| / ||E|| = rel): inner_tol, or looser when rel is
+// already close to tol -- the correction then only needs to gain the
+// remaining factor rel / tol (with a margin of 2).
+double inner_tolerance(const tdda_solve_opts* o, double rel)
+{
+    return std::max(o->inner_tol, 0.5 * o->tol / rel);
+}
+
 int solve_mixed(SolveSet& S, const tdda_solve_opts* o, tdda_solve_info* info)
 {
     hipStream_t s = S.s;
@@ -719,7 +728,9 @@ int solve_mixed(SolveSet& S, const tdda_solve_opts* o, tdda_solve_info* info)
         for (int j = 0; j < nj; j++)
             if (active[j])
                 k_demote<<<blocks(j), kThreads, 0, s>>>(S.jo->h[j].r, const_cast<Cx<float>*>(S.j32->h[j].e), n3of(j));
-        CHK(init_states<float>(S, *S.j32, hin, active, o->inner_tol, o->inner_max_it));
+        std::vector<double> itol(nj);
+        for (int j = 0; j < nj; j++) itol[j] = inner_tolerance(o, rel[j]);
+        CHK(init_states<float>(S, *S.j32, hin, active, o->inner_tol, o->inner_max_it, &itol));
         launch_init_zero<float>(*S.j32, nj, s);
         CHK(check_launch());
         CHK(run_iterations<float>(S, *S.j32, hin, o->inner_max_it, o->check_every));
@@ -793,6 +804,37 @@ static int time_passes_t(hipStream_t s, JobTable<T>& jt, int reps, double* ms5)
     CHK(check_launch());
     for (int k = 0; k < 6; k++) (void)hipEventDestroy(ev[k]);
     for (int k = 0; k < 5; k++) ms5[k] = acc[k] / reps;
+    return TDDA_OK;
+}
+
+// One pass kind launched reps times back to back between two events (the
+// mean is the kernel's duration plus the dispatch gap of a full queue).
+template <typename T>
+static int time_pass_t(hipStream_t s, JobTable<T>& jt, int pass, int reps, double* ms)
+{
+    hipEvent_t e0, e1;
+    HIPCHK(hipEventCreate(&e0));
+    HIPCHK(hipEventCreate(&e1));
+    auto one = [&]() {
+        switch (pass) {
+        case 0: run_p1<T>(jt, IN_PLAIN, s); break;
+        case 1: run_py<T>(jt, false, 0, s); break;
+        case 2: run_px<T>(jt, 0, s); break;
+        case 3: run_py<T>(jt, true, 0, s); break;
+        default: run_p5<T>(jt, OUT_PLAIN, 1, s); break;
+        }
+    };
+    for (int k = 0; k < 3; k++) one();  // warm-up
+    HIPCHK(hipEventRecord(e0, s));
+    for (int r = 0; r < reps; r++) one();
+    HIPCHK(hipEventRecord(e1, s));
+    HIPCHK(hipEventSynchronize(e1));
+    CHK(check_launch());
+    float t = 0.f;
+    HIPCHK(hipEventElapsedTime(&t, e0, e1));
+    *ms = t / reps;
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
     return TDDA_OK;
 }
 
@@ -1364,6 +1406,23 @@ int tdda_model_time_passes(tdda_model m, int precision, int reps, double* ms5)
         const size_t n = m->n3 * m->nrhs;
         k_demote<<<(unsigned)((n + kThreads - 1) / kThreads), kThreads, 0, m->op->ctx->stream>>>(m->v64.x, m->v32.x, n);
         return time_passes_t<float>(m->op->ctx->stream, m->j32, reps, ms5);
+    }
+    return fail(TDDA_ERR_ARG, "precision must be TDDA_C128 or TDDA_C64");
+}
+
+int tdda_model_time_pass(tdda_model m, int precision, int pass, int reps, double* ms)
+{
+    if (!m || !ms || reps <= 0 || pass < 0 || pass > 4) return fail(TDDA_ERR_ARG, "bad argument");
+    HIPCHK(hipSetDevice(m->op->ctx->device));
+    if (precision == TDDA_C128) {
+        if (!m->v64.x) return fail(TDDA_ERR_STATE, "operator built without TDDA_C128");
+        return time_pass_t<double>(m->op->ctx->stream, m->j64, pass, reps, ms);
+    }
+    if (precision == TDDA_C64) {
+        if (!m->v32.x) return fail(TDDA_ERR_STATE, "operator built without TDDA_C64");
+        const size_t n = m->n3 * m->nrhs;
+        k_demote<<<(unsigned)((n + kThreads - 1) / kThreads), kThreads, 0, m->op->ctx->stream>>>(m->v64.x, m->v32.x, n);
+        return time_pass_t<float>(m->op->ctx->stream, m->j32, pass, reps, ms);
     }
     return fail(TDDA_ERR_ARG, "precision must be TDDA_C128 or TDDA_C64");
 }

@@ -29,7 +29,7 @@ EXPORTS = [
     "tdda_op_create", "tdda_op_destroy", "tdda_op_info", "tdda_aproduct", "tdda_green_tensor",
     "tdda_model_create", "tdda_model_destroy", "tdda_model_set", "tdda_model_get", "tdda_model_apply",
     "tdda_solve_opts_default", "tdda_model_solve", "tdda_model_get_entries", "tdda_model_set_entries",
-    "tdda_model_time_passes", "tdda_material_rule", "tdda_get_alpha", "tdda_incident_field",
+    "tdda_model_time_passes", "tdda_model_time_pass", "tdda_material_rule", "tdda_get_alpha", "tdda_incident_field",
     "tdda_polarizability", "tdda_local_field", "tdda_batch_create", "tdda_batch_destroy", "tdda_batch_size",
     "tdda_batch_solve", "tdda_batch_apply", "tdda_batch_time_passes", "tdda_server_opts_default",
     "tdda_server_create", "tdda_server_destroy", "tdda_server_submit", "tdda_server_poll", "tdda_server_wait",
@@ -113,6 +113,7 @@ def lib():
         L.tdda_model_get_entries.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ip, dp]
         L.tdda_model_set_entries.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ip, dp]
         L.tdda_model_time_passes.argtypes = [vp, ctypes.c_int, ctypes.c_int, dp]
+        L.tdda_model_time_pass.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, dp]
         L.tdda_material_rule.argtypes = [ctypes.c_int, dp, ctypes.c_double]
         L.tdda_get_alpha.argtypes = [ctypes.c_double] * 5 + [dp, dp, dp]
         L.tdda_incident_field.argtypes = [ctypes.c_int, ip] + [ctypes.c_double] * 3 + [dp, dp, dp]
@@ -348,6 +349,12 @@ class DeviceModel:
         out = np.zeros(5)
         _check(lib().tdda_model_time_passes(self.h, precision, reps, _dptr(out)))
         return out
+
+    def time_pass(self, pass_index, precision=C128, reps=50):
+        """Mean ms of one pass kind launched reps times back to back."""
+        out = ctypes.c_double()
+        _check(lib().tdda_model_time_pass(self.h, precision, pass_index, reps, ctypes.byref(out)))
+        return out.value
 
     def apply(self, precision=C128):
         _check(lib().tdda_model_apply(self.h, precision))
