@@ -1491,7 +1491,7 @@ void tdda_solve_opts_default(tdda_solve_opts* o)
     o->tol = 1e-8;
     o->mode = TDDA_MODE_FP64;
     o->x0_zero = 0;
-    o->inner_tol = 1e-5;
+    o->inner_tol = 1e-4;
     o->inner_max_it = 2000;
     o->max_outer = 8;
     o->check_every = 0;
