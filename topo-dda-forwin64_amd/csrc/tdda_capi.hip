@@ -176,8 +176,9 @@ template <typename T>
 struct PassGroup {
     int L = 0;                   // FFT length of the group's kernel (0: runtime plan)
     const JobDesc<T>* d = nullptr;  // descriptors of the group's jobs
-    const int* ts = nullptr;     // tile starts of the group, nj + 1
-    int nj = 0;
+    const int* ts = nullptr;     // tile starts, nj + 1 (x pass: per Green-sharing job run)
+    int nj = 0;                  // jobs (x pass: Green-sharing job runs)
+    const int* gj = nullptr;     // x pass: first job of each run, nj + 1
     int grid = 0;                // persistent grid
     size_t lds = 0;
 };
@@ -427,17 +428,37 @@ int build_groups(const tdda_ctx_s* c, JobTable<T>& jt, Put&& put)
             PassGroup<T> g;
             g.L = k;
             std::vector<JobDesc<T>> sub;
-            std::vector<int> ts(1, 0);
+            std::vector<int> subt;
             for (int j = 0; j < nj; j++) {
                 if (key[j] != k) continue;
                 sub.push_back(jt.h[j]);
-                ts.push_back(ts.back() + ntiles(tiles[j]));
+                subt.push_back(ntiles(tiles[j]));
                 g.lds = std::max(g.lds, lds(jt.h[j]));
             }
             if (g.lds > c->lds_budget) return fail(TDDA_ERR_UNSUPPORTED, "pass tile exceeds the LDS budget");
-            g.nj = (int)sub.size();
+            std::vector<int> ts(1, 0);
+            if (kind == 2) {
+                // runs of consecutive jobs that share a Green spectrum (same
+                // structure); a run of R > 1 jobs takes R * ceil(T / 8) * 8
+                // tile slots (x_tile interleaves them by 8)
+                std::vector<int> gj(1, 0);
+                for (size_t a = 0; a < sub.size();) {
+                    size_t b = a + 1;
+                    while (b < sub.size() && sub[b].green == sub[a].green && subt[b] == subt[a]) b++;
+                    const int R = (int)(b - a), Tt = subt[a];
+                    ts.push_back(ts.back() + (R > 1 ? R * ((Tt + 7) / 8) * 8 : Tt));
+                    gj.push_back((int)b);
+                    a = b;
+                }
+                g.nj = (int)gj.size() - 1;
+                g.gj = put(gj.data(), gj.size());
+                if (!g.gj) return fail(TDDA_ERR_OOM, "job table upload");
+            } else {
+                for (int t : subt) ts.push_back(ts.back() + t);
+                g.nj = (int)sub.size();
+            }
             g.grid = grid(ts.back(), g.lds);
-            g.d = (g.nj == nj) ? jt.d : put(sub.data(), sub.size());
+            g.d = (sub.size() == (size_t)nj) ? jt.d : put(sub.data(), sub.size());
             g.ts = put(ts.data(), ts.size());
             if (!g.d || !g.ts) return fail(TDDA_ERR_OOM, "job table upload");
             groups.push_back(g);
@@ -517,19 +538,25 @@ void run_py(const JobTable<T>& jt, bool inv, int solver, hipStream_t s)
 template <typename T>
 void run_px(const JobTable<T>& jt, int solver, hipStream_t s)
 {
-    for (const PassGroup<T>& g : jt.gx) launch_px<T>(g.L, solver, PassLaunch{dim3(g.grid), g.lds, s}, g.d, g.ts, g.nj);
+    for (const PassGroup<T>& g : jt.gx)
+        launch_px<T>(g.L, solver, PassLaunch{dim3(g.grid), g.lds, s}, g.d, g.ts, g.nj, g.gj);
 }
 
 // one operator application: P1 .. P5
 template <typename T>
 void launch_apply(const JobTable<T>& jt, int nj, int inmode, int outmode, int diag, int solver, hipStream_t s)
 {
-    (void)nj;
     run_p1<T>(jt, inmode, s);
     run_py<T>(jt, false, solver, s);
     run_px<T>(jt, solver, s);
     run_py<T>(jt, true, solver, s);
     run_p5<T>(jt, outmode, diag, s);
+    switch (outmode) {
+    case OUT_RES: k_finish<T, FIN_RES><<<dim3(nj), dim3(kThreads), 0, s>>>(jt.d, nj); break;
+    case OUT_AP: k_finish<T, FIN_AP><<<dim3(nj), dim3(kThreads), 0, s>>>(jt.d, nj); break;
+    case OUT_AT: k_finish<T, FIN_AT><<<dim3(nj), dim3(kThreads), 0, s>>>(jt.d, nj); break;
+    default: break;
+    }
 }
 
 // one BiCGSTAB iteration (or the unrolled first step when state.first)
@@ -539,6 +566,7 @@ void launch_iteration(const JobTable<T>& jt, int nj, hipStream_t s)
     launch_apply<T>(jt, nj, IN_P, OUT_AP, 1, 1, s);
     launch_apply<T>(jt, nj, IN_T, OUT_AT, 1, 1, s);
     k_update<T><<<dim3(jt.nv), dim3(kThreads), 0, s>>>(jt.d, jt.tv, nj);
+    k_finish<T, FIN_UPDATE><<<dim3(nj), dim3(kThreads), 0, s>>>(jt.d, nj);
 }
 
 // The job tables and host buffers one solve runs on: a model's own, or the
@@ -652,6 +680,7 @@ template <typename T>
 void launch_init_zero(const JobTable<T>& jt, int nj, hipStream_t s)
 {
     k_init_zero<T><<<dim3(jt.nv), dim3(kThreads), 0, s>>>(jt.d, jt.tv, nj);
+    k_finish<T, FIN_INIT><<<dim3(nj), dim3(kThreads), 0, s>>>(jt.d, nj);
 }
 
 int solve_fp64(SolveSet& S, const tdda_solve_opts* o, tdda_solve_info* info)

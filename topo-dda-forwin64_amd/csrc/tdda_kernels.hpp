@@ -183,39 +183,16 @@ __device__ __forceinline__ void block_sum(double* v, double* red)
     }
 }
 
-// Publishes this tile's partial sums; returns true in thread 0 of the last
-// tile of the job to finish, after which the totals are in tot[].
-template <int NV>
-__device__ __forceinline__ bool publish_partials(double* part, unsigned* cnt, int tile, int ntiles,
-                                                 const double* v, double* tot, int* s_last)
+// Stores this tile's four partial sums (valid in thread 0 after block_sum).
+// The totals are formed by k_finish in a separate launch: a last-workgroup
+// ticket inside the pass would need a device-scope release fence per tile,
+// which on a multi-XCD GPU writes back the XCD's L2 each time.
+__device__ __forceinline__ void store_partials(double* part, int tile, const double* v)
 {
     if (threadIdx.x == 0) {
 #pragma unroll
-        for (int k = 0; k < NV; k++) part[tile * NV + k] = v[k];
-        __threadfence();
-        const unsigned prev = atomicAdd(cnt, 1u);
-        *s_last = (prev == (unsigned)(ntiles - 1));
+        for (int k = 0; k < 4; k++) part[tile * 4 + k] = v[k];
     }
-    __syncthreads();
-    if (!*s_last) return false;
-    __threadfence();
-    // deterministic final sum: fixed per-thread strided order, then block sum
-    double acc[NV];
-#pragma unroll
-    for (int k = 0; k < NV; k++) acc[k] = 0.0;
-    for (int i = threadIdx.x; i < ntiles; i += blockDim.x) {
-#pragma unroll
-        for (int k = 0; k < NV; k++)
-            acc[k] += __hip_atomic_load(&part[i * NV + k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    }
-    __shared__ double red2[16 * NV];
-    block_sum<NV>(acc, red2);
-    if (threadIdx.x == 0) {
-#pragma unroll
-        for (int k = 0; k < NV; k++) tot[k] = acc[k];
-        *cnt = 0u;
-    }
-    return threadIdx.x == 0;
 }
 
 __device__ __forceinline__ bool job_inactive(const SolverState* st)
@@ -328,6 +305,41 @@ __device__ __forceinline__ STile s_tile(const JobDesc<T>* __restrict__ jobs, con
     t.outer = r / nfb;
     t.f0 = (r - t.outer * nfb) * tf;
     t.nf = min(tf, J->Mz * 3 - t.f0);
+    t.live = !(check_state && job_inactive(J->st));
+    return t;
+}
+
+// Geometry of a P3 tile.  tstart runs over groups of consecutive jobs that
+// share one Green spectrum (the right-hand sides of one structure); gjob
+// holds each group's first job.  Inside a group of R > 1 jobs, tiles come
+// in blocks of 8R: slot w of a block is tile 8 * block + w % 8 of job w / 8,
+// so the R tiles that read the same Green entries are 8 apart in the
+// launch order -- on the same XCD (workgroups go round-robin over the 8
+// XCDs and persistent grids are multiples of 8) and in the same round --
+// and share its L2.
+template <typename T>
+__device__ __forceinline__ STile x_tile(const JobDesc<T>* __restrict__ jobs, const int* __restrict__ tstart,
+                                        const int* __restrict__ gjob, int ngroups, int tl, bool check_state)
+{
+    STile t;
+    const int g = find_job(tstart, ngroups, tl);
+    const int j0 = gjob[g], R = gjob[g + 1] - j0;
+    const int u = tl - tstart[g];
+    int r = u;
+    t.job = j0;
+    if (R > 1) {
+        const int blk = u / (8 * R), w = u - blk * 8 * R;
+        t.job = j0 + (w >> 3);
+        r = blk * 8 + (w & 7);
+    }
+    const JobDesc<T>* J = jobs + t.job;
+    if (r >= J->My * J->nfx) {
+        t.live = false;
+        return t;
+    }
+    t.outer = r / J->nfx;
+    t.f0 = (r - t.outer * J->nfx) * J->tfx;
+    t.nf = min(J->tfx, J->Mz * 3 - t.f0);
     t.live = !(check_state && job_inactive(J->st));
     return t;
 }
@@ -566,11 +578,11 @@ __global__ void TDDA_PASS_BOUNDS k_py(const JobDesc<T>* __restrict__ jobs, const
 // Tile = (ky, tfx consecutive flattened (kz, c) columns, tfx % 3 == 0).
 template <typename T, int LC>
 __global__ void TDDA_PASS_BOUNDS k_px(const JobDesc<T>* __restrict__ jobs, const int* __restrict__ tstart,
-                                      int njobs, int solver)
+                                      int ngroups, const int* __restrict__ gjob, int solver)
 {
     extern __shared__ __align__(16) char smem[];
     constexpr int EP = ept_len<T>(LC);
-    const int ntot = tstart[njobs];
+    const int ntot = tstart[ngroups];
     Pre<T, EP> pr;
     STile cur;
     cur.live = false;
@@ -600,7 +612,7 @@ __global__ void TDDA_PASS_BOUNDS k_px(const JobDesc<T>* __restrict__ jobs, const
         STile nxt;
         nxt.live = false;
         if (tl < ntot) {
-            nxt = s_tile(jobs, tstart, njobs, tl, true, solver != 0);
+            nxt = x_tile(jobs, tstart, gjob, ngroups, tl, solver != 0);
             if (nxt.live) {
                 const JobDesc<T>* J = jobs + nxt.job;
                 const int F = J->Mz * 3, nf = nxt.nf;
@@ -679,7 +691,6 @@ __global__ void TDDA_PASS_BOUNDS k_p5(const JobDesc<T>* __restrict__ jobs, const
 {
     extern __shared__ __align__(16) char smem[];
     __shared__ double red[16 * 4];
-    __shared__ int s_last;
     constexpr int EP = ept_len<T>(LC);
     const int ntot = tstart[njobs];
     const float r3 = 1.0f / 3.0f;
@@ -776,23 +787,7 @@ __global__ void TDDA_PASS_BOUNDS k_p5(const JobDesc<T>* __restrict__ jobs, const
             }
             if (MODE != OUT_PLAIN) {
                 block_sum<4>(acc, red);
-                double tot[4];
-                const int ntiles = tstart[cur.job + 1] - tstart[cur.job];
-                if (publish_partials<4>(J->part, J->cnt, cur.jt, ntiles, acc, tot, &s_last)) {
-                    SolverState* st = J->st;
-                    if (MODE == OUT_RES) {
-                        st->rho[0] = tot[0];  // r0.r = |r|^2
-                        st->rho[1] = 0.0;
-                        st->enorm = sqrt(tot[1]);
-                        st->rnorm = sqrt(tot[0]);
-                    } else if (MODE == OUT_AP) {
-                        // alpha = r0.r / r0.Ap   (DDAModel.cpp:168, 221)
-                        cdiv(st->rho[0], st->rho[1], tot[0], tot[1], st->alpha[0], st->alpha[1]);
-                    } else {
-                        // zeta = At.t / At.At   (DDAModel.cpp:172, 225)
-                        cdiv(tot[0], tot[1], tot[2], 0.0, st->zeta[0], st->zeta[1]);
-                    }
-                }
+                store_partials(J->part, cur.jt, acc);
             }
             __syncthreads();
         }
@@ -812,15 +807,13 @@ template <typename T>
 __global__ void __launch_bounds__(kThreads) k_init_zero(const JobDesc<T>* __restrict__ jobs,
                                                         const int* __restrict__ tstart, int njobs)
 {
-    __shared__ double red[16 * 2];
-    __shared__ int s_last;
+    __shared__ double red[16 * 4];
     const int job = find_job(tstart, njobs, blockIdx.x);
     const JobDesc<T>& J = jobs[job];
     if (job_inactive(J.st)) return;
     const int tile = blockIdx.x - tstart[job];
-    const int ntiles = tstart[job + 1] - tstart[job];
     const int n = J.N * 3;
-    double acc[2] = {0.0, 0.0};
+    double acc[4] = {0.0, 0.0, 0.0, 0.0};
     const int i0 = tile * kThreads * kVecPer;
     for (int k = 0; k < kVecPer; k++) {
         const int i = i0 + k * kThreads + threadIdx.x;
@@ -833,14 +826,8 @@ __global__ void __launch_bounds__(kThreads) k_init_zero(const JobDesc<T>* __rest
             acc[0] += d.x * d.x + d.y * d.y;
         }
     }
-    block_sum<2>(acc, red);
-    double tot[2];
-    if (!publish_partials<2>(J.part, J.cnt, tile, ntiles, acc, tot, &s_last)) return;
-    SolverState* st = J.st;
-    st->rho[0] = tot[0];
-    st->rho[1] = 0.0;
-    st->enorm = sqrt(tot[0]);
-    st->rnorm = sqrt(tot[0]);
+    block_sum<4>(acc, red);
+    store_partials(J.part, tile, acc);
 }
 
 // u = zeta Ap; z = zeta r - alpha u; P += alpha p + z; rl = r; r = t - zeta At
@@ -851,12 +838,10 @@ __global__ void __launch_bounds__(kThreads) k_update(const JobDesc<T>* __restric
                                                      const int* __restrict__ tstart, int njobs)
 {
     __shared__ double red[16 * 4];
-    __shared__ int s_last;
     const int job = find_job(tstart, njobs, blockIdx.x);
     const JobDesc<T>& J = jobs[job];
     if (job_inactive(J.st)) return;
     const int tile = blockIdx.x - tstart[job];
-    const int ntiles = tstart[job + 1] - tstart[job];
     const int n = J.N * 3;
     const SolverState* stc = J.st;
     const Cx<T> al = from_d<T>(stc->alpha[0], stc->alpha[1]);
@@ -879,31 +864,73 @@ __global__ void __launch_bounds__(kThreads) k_update(const JobDesc<T>* __restric
         }
     }
     block_sum<4>(acc, red);
-    double tot[4];
-    if (!publish_partials<4>(J.part, J.cnt, tile, ntiles, acc, tot, &s_last)) return;
+    store_partials(J.part, tile, acc);
+}
+
+// Totals of the partial sums of one reducing pass and the scalar step that
+// follows it.  One workgroup per job; the per-tile partials are summed in a
+// fixed order (thread-strided, then block_sum), so results do not depend on
+// which other jobs share the launches.
+enum { FIN_RES = 0, FIN_AP = 1, FIN_AT = 2, FIN_INIT = 3, FIN_UPDATE = 4 };
+
+template <typename T, int KIND>
+__global__ void __launch_bounds__(kThreads) k_finish(const JobDesc<T>* __restrict__ jobs, int njobs)
+{
+    __shared__ double red[16 * 4];
+    const JobDesc<T>& J = jobs[blockIdx.x];
+    if (job_inactive(J.st)) return;
+    const int ntiles = (KIND == FIN_INIT || KIND == FIN_UPDATE)
+                           ? (int)((3 * (size_t)J.N + kThreads * kVecPer - 1) / (kThreads * kVecPer))
+                           : (J.Nx * J.Ny + J.pz - 1) / J.pz;
+    double acc[4] = {0.0, 0.0, 0.0, 0.0};
+    for (int i = threadIdx.x; i < ntiles; i += kThreads) {
+#pragma unroll
+        for (int k = 0; k < 4; k++) acc[k] += J.part[i * 4 + k];
+    }
+    block_sum<4>(acc, red);
+    if (threadIdx.x != 0) return;
     SolverState* st = J.st;
-    st->rho_prev[0] = st->rho[0];
-    st->rho_prev[1] = st->rho[1];
-    st->rho[0] = tot[1];
-    st->rho[1] = tot[2];
-    st->rnorm = sqrt(tot[0]);
-    st->err = st->rnorm / st->enorm;
-    if (st->first) {
-        st->first = 0;
+    const double* tot = acc;
+    if (KIND == FIN_RES) {
+        st->rho[0] = tot[0];  // r0.r = |r|^2
+        st->rho[1] = 0.0;
+        st->enorm = sqrt(tot[1]);
+        st->rnorm = sqrt(tot[0]);
+    } else if (KIND == FIN_AP) {
+        // alpha = r0.r / r0.Ap   (DDAModel.cpp:168, 221)
+        cdiv(st->rho[0], st->rho[1], tot[0], tot[1], st->alpha[0], st->alpha[1]);
+    } else if (KIND == FIN_AT) {
+        // zeta = At.t / At.At   (DDAModel.cpp:172, 225)
+        cdiv(tot[0], tot[1], tot[2], 0.0, st->zeta[0], st->zeta[1]);
+    } else if (KIND == FIN_INIT) {
+        st->rho[0] = tot[0];
+        st->rho[1] = 0.0;
+        st->enorm = sqrt(tot[0]);
+        st->rnorm = sqrt(tot[0]);
     } else {
-        st->iter += 1;
-        if (st->err <= st->tol) {
-            st->status = ST_CONVERGED;
+        st->rho_prev[0] = st->rho[0];
+        st->rho_prev[1] = st->rho[1];
+        st->rho[0] = tot[1];
+        st->rho[1] = tot[2];
+        st->rnorm = sqrt(tot[0]);
+        st->err = st->rnorm / st->enorm;
+        if (st->first) {
+            st->first = 0;
+        } else {
+            st->iter += 1;
+            if (st->err <= st->tol) {
+                st->status = ST_CONVERGED;
+                return;
+            }
+        }
+        if (st->iter >= st->maxit) {
+            st->status = ST_MAXIT;
             return;
         }
-    }
-    if (st->iter >= st->maxit) {
-        st->status = ST_MAXIT;
-        return;
-    }
-    if (st->rho_prev[0] == 0.0 && st->rho_prev[1] == 0.0) {
-        st->status = ST_BREAKDOWN;
-        st->iter += 1;
+        if (st->rho_prev[0] == 0.0 && st->rho_prev[1] == 0.0) {
+            st->status = ST_BREAKDOWN;
+            st->iter += 1;
+        }
     }
 }
 

@@ -22,8 +22,10 @@ template <typename T>
 void launch_p5(int L, int mode, int diag, const PassLaunch& pl, const JobDesc<T>* jobs, const int* tstart, int nj);
 template <typename T>
 void launch_py(int L, bool inv, int solver, const PassLaunch& pl, const JobDesc<T>* jobs, const int* tstart, int nj);
+// P3: tstart / nj run over Green-sharing job groups, gjob[g] = first job of group g
 template <typename T>
-void launch_px(int L, int solver, const PassLaunch& pl, const JobDesc<T>* jobs, const int* tstart, int nj);
+void launch_px(int L, int solver, const PassLaunch& pl, const JobDesc<T>* jobs, const int* tstart, int nj,
+               const int* gjob);
 // true when a specialised kernel exists for this length and pass kind
 // (0: z pass, 1: y pass, 2: x pass)
 template <typename T>
