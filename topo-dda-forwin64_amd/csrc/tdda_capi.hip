@@ -176,10 +176,10 @@ template <typename T>
 struct PassGroup {
     int L = 0;                   // FFT length of the group's kernel (0: runtime plan)
     const JobDesc<T>* d = nullptr;  // descriptors of the group's jobs
-    const int* ts = nullptr;     // tile starts, nj + 1 (x pass: per Green-sharing job run)
-    int nj = 0;                  // jobs (x pass: Green-sharing job runs)
-    const int* gj = nullptr;     // x pass: first job of each run, nj + 1
-    int grid = 0;                // persistent grid
+    const int* ts = nullptr;     // tile starts of the group, nj + 1
+    int nj = 0;
+    int tiles = 0;               // tiles of the launch (upper bound of its persistent grid)
+    int ncu = 0;
     size_t lds = 0;
 };
 
@@ -196,7 +196,6 @@ struct JobTable {
     std::vector<PassGroup<T>> gz, gz1, gy, gx;  // P5, P1, P2/P4, P3
     SolverState* st = nullptr;
     double* part = nullptr;
-    unsigned* cnt = nullptr;
 };
 
 struct tdda_model_s {
@@ -410,12 +409,6 @@ int build_groups(const tdda_ctx_s* c, JobTable<T>& jt, Put&& put)
     jt.tv = const_cast<int*>(put(hv.data(), (size_t)nj + 1));
     if (!jt.d || !jt.tv) return fail(TDDA_ERR_OOM, "job table upload");
     jt.nv = hv[nj];
-    // persistent grids: as many workgroups as stay resident (LDS-limited,
-    // at most 8 per CU)
-    auto grid = [&](int ntiles, size_t lds) {
-        const int per_cu = (int)std::max<size_t>(1, std::min<size_t>(8, c->lds_cu / std::max<size_t>(lds, 1)));
-        return std::max(1, std::min(ntiles, c->ncu * per_cu));
-    };
     // per pass kind: FFT length, pass kind of pass_length_compiled, tiles, LDS bytes
     auto build = [&](std::vector<PassGroup<T>>& groups, int kind, auto len, auto ntiles, auto lds) -> int {
         groups.clear();
@@ -437,27 +430,10 @@ int build_groups(const tdda_ctx_s* c, JobTable<T>& jt, Put&& put)
             }
             if (g.lds > c->lds_budget) return fail(TDDA_ERR_UNSUPPORTED, "pass tile exceeds the LDS budget");
             std::vector<int> ts(1, 0);
-            if (kind == 2) {
-                // runs of consecutive jobs that share a Green spectrum (same
-                // structure); a run of R > 1 jobs takes R * ceil(T / 8) * 8
-                // tile slots (x_tile interleaves them by 8)
-                std::vector<int> gj(1, 0);
-                for (size_t a = 0; a < sub.size();) {
-                    size_t b = a + 1;
-                    while (b < sub.size() && sub[b].green == sub[a].green && subt[b] == subt[a]) b++;
-                    const int R = (int)(b - a), Tt = subt[a];
-                    ts.push_back(ts.back() + (R > 1 ? R * ((Tt + 7) / 8) * 8 : Tt));
-                    gj.push_back((int)b);
-                    a = b;
-                }
-                g.nj = (int)gj.size() - 1;
-                g.gj = put(gj.data(), gj.size());
-                if (!g.gj) return fail(TDDA_ERR_OOM, "job table upload");
-            } else {
-                for (int t : subt) ts.push_back(ts.back() + t);
-                g.nj = (int)sub.size();
-            }
-            g.grid = grid(ts.back(), g.lds);
+            for (int t : subt) ts.push_back(ts.back() + t);
+            g.nj = (int)sub.size();
+            g.tiles = ts.back();
+            g.ncu = c->ncu;
             g.d = (sub.size() == (size_t)nj) ? jt.d : put(sub.data(), sub.size());
             g.ts = put(ts.data(), ts.size());
             if (!g.d || !g.ts) return fail(TDDA_ERR_OOM, "job table upload");
@@ -497,13 +473,10 @@ int finalize_table(std::vector<void*>& arena, const tdda_ctx_s* c, JobTable<T>& 
     for (int j = 0; j < nj; j++) poff[j + 1] = poff[j] + job_part_size(jt.h[j]);
     CHK(dev_alloc(arena, &jt.st, nj));
     CHK(dev_alloc(arena, &jt.part, poff[nj]));
-    CHK(dev_alloc(arena, &jt.cnt, nj));
-    HIPCHK(hipMemsetAsync(jt.cnt, 0, sizeof(unsigned) * nj, c->stream));
     HIPCHK(hipMemsetAsync(jt.st, 0, sizeof(SolverState) * nj, c->stream));
     for (int j = 0; j < nj; j++) {
         jt.h[j].st = jt.st + j;
         jt.h[j].part = jt.part + poff[j];
-        jt.h[j].cnt = jt.cnt + j;
     }
     auto put = [&](const auto* host, size_t n) {
         using U = std::remove_const_t<std::remove_pointer_t<decltype(host)>>;
@@ -518,28 +491,27 @@ int finalize_table(std::vector<void*>& arena, const tdda_ctx_s* c, JobTable<T>& 
 template <typename T>
 void run_p1(const JobTable<T>& jt, int mode, hipStream_t s)
 {
-    for (const PassGroup<T>& g : jt.gz1) launch_p1<T>(g.L, mode, PassLaunch{dim3(g.grid), g.lds, s}, g.d, g.ts, g.nj);
+    for (const PassGroup<T>& g : jt.gz1) launch_p1<T>(g.L, mode, PassLaunch{dim3(g.tiles), g.lds, s, g.ncu}, g.d, g.ts, g.nj);
 }
 
 template <typename T>
 void run_p5(const JobTable<T>& jt, int mode, int diag, hipStream_t s)
 {
     for (const PassGroup<T>& g : jt.gz)
-        launch_p5<T>(g.L, mode, diag, PassLaunch{dim3(g.grid), g.lds, s}, g.d, g.ts, g.nj);
+        launch_p5<T>(g.L, mode, diag, PassLaunch{dim3(g.tiles), g.lds, s, g.ncu}, g.d, g.ts, g.nj);
 }
 
 template <typename T>
 void run_py(const JobTable<T>& jt, bool inv, int solver, hipStream_t s)
 {
     for (const PassGroup<T>& g : jt.gy)
-        launch_py<T>(g.L, inv, solver, PassLaunch{dim3(g.grid), g.lds, s}, g.d, g.ts, g.nj);
+        launch_py<T>(g.L, inv, solver, PassLaunch{dim3(g.tiles), g.lds, s, g.ncu}, g.d, g.ts, g.nj);
 }
 
 template <typename T>
 void run_px(const JobTable<T>& jt, int solver, hipStream_t s)
 {
-    for (const PassGroup<T>& g : jt.gx)
-        launch_px<T>(g.L, solver, PassLaunch{dim3(g.grid), g.lds, s}, g.d, g.ts, g.nj, g.gj);
+    for (const PassGroup<T>& g : jt.gx) launch_px<T>(g.L, solver, PassLaunch{dim3(g.tiles), g.lds, s, g.ncu}, g.d, g.ts, g.nj);
 }
 
 // one operator application: P1 .. P5

@@ -105,8 +105,7 @@ struct JobDesc {
     Cx<T>* at;
     const Cx<T>* e;
     SolverState* st;
-    double* part;   // [tiles][6] partial sums
-    unsigned* cnt;  // last-tile counter
+    double* part;   // [tiles][4] partial sums of the reducing passes
 };
 
 // P1 input modes
@@ -305,41 +304,6 @@ __device__ __forceinline__ STile s_tile(const JobDesc<T>* __restrict__ jobs, con
     t.outer = r / nfb;
     t.f0 = (r - t.outer * nfb) * tf;
     t.nf = min(tf, J->Mz * 3 - t.f0);
-    t.live = !(check_state && job_inactive(J->st));
-    return t;
-}
-
-// Geometry of a P3 tile.  tstart runs over groups of consecutive jobs that
-// share one Green spectrum (the right-hand sides of one structure); gjob
-// holds each group's first job.  Inside a group of R > 1 jobs, tiles come
-// in blocks of 8R: slot w of a block is tile 8 * block + w % 8 of job w / 8,
-// so the R tiles that read the same Green entries are 8 apart in the
-// launch order -- on the same XCD (workgroups go round-robin over the 8
-// XCDs and persistent grids are multiples of 8) and in the same round --
-// and share its L2.
-template <typename T>
-__device__ __forceinline__ STile x_tile(const JobDesc<T>* __restrict__ jobs, const int* __restrict__ tstart,
-                                        const int* __restrict__ gjob, int ngroups, int tl, bool check_state)
-{
-    STile t;
-    const int g = find_job(tstart, ngroups, tl);
-    const int j0 = gjob[g], R = gjob[g + 1] - j0;
-    const int u = tl - tstart[g];
-    int r = u;
-    t.job = j0;
-    if (R > 1) {
-        const int blk = u / (8 * R), w = u - blk * 8 * R;
-        t.job = j0 + (w >> 3);
-        r = blk * 8 + (w & 7);
-    }
-    const JobDesc<T>* J = jobs + t.job;
-    if (r >= J->My * J->nfx) {
-        t.live = false;
-        return t;
-    }
-    t.outer = r / J->nfx;
-    t.f0 = (r - t.outer * J->nfx) * J->tfx;
-    t.nf = min(J->tfx, J->Mz * 3 - t.f0);
     t.live = !(check_state && job_inactive(J->st));
     return t;
 }
@@ -578,11 +542,11 @@ __global__ void TDDA_PASS_BOUNDS k_py(const JobDesc<T>* __restrict__ jobs, const
 // Tile = (ky, tfx consecutive flattened (kz, c) columns, tfx % 3 == 0).
 template <typename T, int LC>
 __global__ void TDDA_PASS_BOUNDS k_px(const JobDesc<T>* __restrict__ jobs, const int* __restrict__ tstart,
-                                      int ngroups, const int* __restrict__ gjob, int solver)
+                                      int njobs, int solver)
 {
     extern __shared__ __align__(16) char smem[];
     constexpr int EP = ept_len<T>(LC);
-    const int ntot = tstart[ngroups];
+    const int ntot = tstart[njobs];
     Pre<T, EP> pr;
     STile cur;
     cur.live = false;
@@ -612,7 +576,7 @@ __global__ void TDDA_PASS_BOUNDS k_px(const JobDesc<T>* __restrict__ jobs, const
         STile nxt;
         nxt.live = false;
         if (tl < ntot) {
-            nxt = x_tile(jobs, tstart, gjob, ngroups, tl, solver != 0);
+            nxt = s_tile(jobs, tstart, njobs, tl, true, solver != 0);
             if (nxt.live) {
                 const JobDesc<T>* J = jobs + nxt.job;
                 const int F = J->Mz * 3, nf = nxt.nf;

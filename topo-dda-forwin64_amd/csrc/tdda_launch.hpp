@@ -11,9 +11,10 @@
 namespace tdda {
 
 struct PassLaunch {
-    dim3 grid;
+    dim3 grid;  // tiles: the launcher caps it at the resident workgroups
     size_t lds;
     hipStream_t stream;
+    int ncu;
 };
 
 template <typename T>
@@ -22,10 +23,8 @@ template <typename T>
 void launch_p5(int L, int mode, int diag, const PassLaunch& pl, const JobDesc<T>* jobs, const int* tstart, int nj);
 template <typename T>
 void launch_py(int L, bool inv, int solver, const PassLaunch& pl, const JobDesc<T>* jobs, const int* tstart, int nj);
-// P3: tstart / nj run over Green-sharing job groups, gjob[g] = first job of group g
 template <typename T>
-void launch_px(int L, int solver, const PassLaunch& pl, const JobDesc<T>* jobs, const int* tstart, int nj,
-               const int* gjob);
+void launch_px(int L, int solver, const PassLaunch& pl, const JobDesc<T>* jobs, const int* tstart, int nj);
 // true when a specialised kernel exists for this length and pass kind
 // (0: z pass, 1: y pass, 2: x pass)
 template <typename T>
