@@ -10,6 +10,7 @@
 // long double (forward sign, exp(-2 pi i q / L)); the inverse uses conjugates.
 #pragma once
 #include <hip/hip_runtime.h>
+#include <type_traits>
 
 namespace tdda {
 
@@ -318,7 +319,12 @@ __device__ __forceinline__ void load_twiddles(Cx<T>* __restrict__ dst, const Cx<
                                               const FftPlan& pl)
 {
     const int n = tw2_size(pl.L, pl.tb);
-    for (int i = threadIdx.x; i < n; i += blockDim.x) dst[i] = src[i];
+    for (int i = threadIdx.x; i < n; i += blockDim.x) {
+        using V = typename std::conditional<sizeof(T) == 4, float __attribute__((ext_vector_type(2))),
+                                            double __attribute__((ext_vector_type(2)))>::type;
+        const V v = *(const __attribute__((address_space(1))) V*)(src + i);
+        dst[i] = Cx<T>{v.x, v.y};
+    }
 }
 
 }  // namespace tdda
@@ -381,6 +387,12 @@ constexpr int ct_tb(int L)
     return t;
 }
 
+// Butterfly groups per lane that a stage interleaves (unroll factor of
+// the per-lane butterfly loop); full unrolling spills in the x pass.
+#ifndef TDDA_STAGE_UNROLL
+#define TDDA_STAGE_UNROLL 1
+#endif
+
 // Wave-local stage: pencil p belongs to wavefront p % 4, so a stage only
 // reads pencils its own wavefront wrote.  LDS instructions of one wavefront
 // complete in order, so consecutive stages need no workgroup barrier, only
@@ -400,7 +412,7 @@ __device__ __forceinline__ void ct_stage(const Cx<T>* __restrict__ src, Cx<T>* _
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int npw = (np - wave + kWaves - 1) / kWaves;  // this wavefront's pencils
     const int nb = npw * LR;
-#pragma unroll 1
+#pragma unroll TDDA_STAGE_UNROLL
     for (int it = 0; it < (NBMAX + 63) / 64; it++) {
         const int b = lane + it * 64;
         if (b < nb) {

@@ -124,6 +124,58 @@ __device__ __forceinline__ int find_job(const int* __restrict__ tstart, int njob
     return lo;
 }
 
+// Global memory through the global address space.  The vectors of a job
+// are generic pointers in its descriptor, for which the compiler emits flat
+// instructions; a flat load counts in both the vector-memory and the LDS
+// wait counters, so each LDS wait of a transform would also wait for the
+// prefetch of the next tile, and loads issued back to back get serialised.
+#define TDDA_GLOBAL __attribute__((address_space(1)))
+typedef float tdda_f2 __attribute__((ext_vector_type(2)));
+typedef double tdda_d2 __attribute__((ext_vector_type(2)));
+template <typename T>
+struct Vec2;
+template <>
+struct Vec2<float> {
+    using type = tdda_f2;
+};
+template <>
+struct Vec2<double> {
+    using type = tdda_d2;
+};
+
+template <typename T>
+__device__ __forceinline__ Cx<T> gld(const Cx<T>* p)
+{
+    using V = typename Vec2<T>::type;
+    const V v = *(const TDDA_GLOBAL V*)p;
+    return Cx<T>{v.x, v.y};
+}
+
+template <typename T>
+__device__ __forceinline__ void gst(Cx<T>* p, Cx<T> c)
+{
+    using V = typename Vec2<T>::type;
+    V v;
+    v.x = c.x;
+    v.y = c.y;
+    *(TDDA_GLOBAL V*)p = v;
+}
+
+__device__ __forceinline__ int gldi(const int* p) { return *(const TDDA_GLOBAL int*)p; }
+
+// Wait for every outstanding vector-memory load (s_waitcnt vmcnt(0); the
+// other counters untouched: expcnt 7, lgkmcnt 15).  Issued at the top of
+// each persistent-loop iteration, where the prefetched registers are
+// consumed on every path, so that the compiler knows no load is in flight
+// when it reuses those registers for the next prefetch's addresses --
+// otherwise it waits for all loads before each load of the prefetch.
+__device__ __forceinline__ void prefetch_landed() { __builtin_amdgcn_s_waitcnt(0x0F70); }
+
+__device__ __forceinline__ const TDDA_GLOBAL SolverState* gstate(const SolverState* st)
+{
+    return (const TDDA_GLOBAL SolverState*)st;
+}
+
 template <typename T>
 __device__ __forceinline__ Cx<T> ld(const Cx<T>* p) { return *p; }
 
@@ -190,13 +242,13 @@ __device__ __forceinline__ void store_partials(double* part, int tile, const dou
 {
     if (threadIdx.x == 0) {
 #pragma unroll
-        for (int k = 0; k < 4; k++) part[tile * 4 + k] = v[k];
+        for (int k = 0; k < 4; k++) *(TDDA_GLOBAL double*)(part + tile * 4 + k) = v[k];
     }
 }
 
 __device__ __forceinline__ bool job_inactive(const SolverState* st)
 {
-    return st != nullptr && st->status != ST_RUNNING;
+    return st != nullptr && gstate(st)->status != ST_RUNNING;
 }
 
 // ---------------------------------------------------------------------------
@@ -322,6 +374,7 @@ __global__ void TDDA_PASS_BOUNDS k_p1(const JobDesc<T>* __restrict__ jobs, const
     ZTile cur;
     cur.live = false;
     for (int tl = blockIdx.x;; tl += gridDim.x) {
+        prefetch_landed();
         // 1. inputs of the current tile -> LDS (fused BiCGSTAB vector updates)
         if (cur.live) {
             const JobDesc<T>* J = jobs + cur.job;
@@ -332,7 +385,7 @@ __global__ void TDDA_PASS_BOUNDS k_p1(const JobDesc<T>* __restrict__ jobs, const
             Cx<T> sa{T(0), T(0)}, sb{T(0), T(0)};
             bool first = false;
             if (MODE == IN_P) {
-                const SolverState* st = J->st;
+                const TDDA_GLOBAL SolverState* st = gstate(J->st);
                 first = st->first != 0;
                 if (!first) {
                     // beta = (alpha / zeta) * rho / rho_prev   (DDAModel.cpp:218)
@@ -343,7 +396,7 @@ __global__ void TDDA_PASS_BOUNDS k_p1(const JobDesc<T>* __restrict__ jobs, const
                     sb = from_d<T>(st->zeta[0], st->zeta[1]);
                 }
             } else if (MODE == IN_T) {
-                sa = from_d<T>(J->st->alpha[0], J->st->alpha[1]);
+                sa = from_d<T>(gstate(J->st)->alpha[0], gstate(J->st)->alpha[1]);
             }
             const int nin = cur.np * Nz * 3;
             const int base = cur.q0 * Nz;
@@ -359,7 +412,7 @@ __global__ void TDDA_PASS_BOUNDS k_p1(const JobDesc<T>* __restrict__ jobs, const
                 if (MODE == IN_P || MODE == IN_T) {
                     int idx = -1;
                     if (J->cellmap) {
-                        const int site = J->cellmap[base + pl * Nz + z];
+                        const int site = gldi(J->cellmap + base + pl * Nz + z);
                         if (site >= 0) idx = site * 3 + c;
                     } else {
                         idx = base * 3 + e;
@@ -367,11 +420,11 @@ __global__ void TDDA_PASS_BOUNDS k_p1(const JobDesc<T>* __restrict__ jobs, const
                     if (MODE == IN_P) {
                         // p = r + beta (p - u), u = zeta Ap   (DDAModel.cpp:219)
                         if (!first) v = cadd(pr[0].get(k), cmul(sa, csub(pr[1].get(k), cmul(sb, pr[2].get(k)))));
-                        if (idx >= 0) J->p[idx] = v;
+                        if (idx >= 0) gst(J->p + idx, v);
                     } else {
                         // t = r - alpha Ap   (DDAModel.cpp:222)
                         v = csub(pr[0].get(k), cmul(sa, pr[1].get(k)));
-                        if (idx >= 0) J->t[idx] = v;
+                        if (idx >= 0) gst(J->t + idx, v);
                     }
                 }
                 A[(pl * 3 + c) * ps + lpad(z)] = v;
@@ -406,7 +459,7 @@ __global__ void TDDA_PASS_BOUNDS k_p1(const JobDesc<T>* __restrict__ jobs, const
                         int pl, rem, z, c;
                         fdivmod(e, Nz * 3, rNz3, pl, rem);
                         fdivmod(rem, 3, r3, z, c);
-                        const int site = J->cellmap[base + pl * Nz + z];
+                        const int site = gldi(J->cellmap + base + pl * Nz + z);
                         idx = site >= 0 ? site * 3 + c : -1;
                     } else {
                         idx = base * 3 + e;  // bulk order: the tile is one contiguous run
@@ -416,15 +469,15 @@ __global__ void TDDA_PASS_BOUNDS k_p1(const JobDesc<T>* __restrict__ jobs, const
                         for (int v = 0; v < NV; v++) pr[v].set(k, Cx<T>{T(0), T(0)});
                         continue;
                     }
-                    if (MODE == IN_PLAIN) pr[0].set(k, J->in[idx]);
-                    else if (MODE == IN_X) pr[0].set(k, J->x[idx]);
+                    if (MODE == IN_PLAIN) pr[0].set(k, gld(J->in + idx));
+                    else if (MODE == IN_X) pr[0].set(k, gld(J->x + idx));
                     else if (MODE == IN_P) {
-                        pr[0].set(k, J->r[idx]);
-                        pr[1].set(k, J->p[idx]);
-                        pr[2].set(k, J->ap[idx]);
+                        pr[0].set(k, gld(J->r + idx));
+                        pr[1].set(k, gld(J->p + idx));
+                        pr[2].set(k, gld(J->ap + idx));
                     } else {
-                        pr[0].set(k, J->r[idx]);
-                        pr[1].set(k, J->ap[idx]);
+                        pr[0].set(k, gld(J->r + idx));
+                        pr[1].set(k, gld(J->ap + idx));
                     }
                 }
             }
@@ -445,7 +498,7 @@ __global__ void TDDA_PASS_BOUNDS k_p1(const JobDesc<T>* __restrict__ jobs, const
                 int pl, rem, kz, c;
                 fdivmod(e, Mz * 3, rMz3, pl, rem);
                 fdivmod(rem, 3, r3, kz, c);
-                w1[e] = res[(pl * 3 + c) * ps + lpad(kz)];
+                gst(w1 + e, res[(pl * 3 + c) * ps + lpad(kz)]);
             }
             __syncthreads();
         }
@@ -467,6 +520,7 @@ __global__ void TDDA_PASS_BOUNDS k_py(const JobDesc<T>* __restrict__ jobs, const
     STile cur;
     cur.live = false;
     for (int tl = blockIdx.x;; tl += gridDim.x) {
+        prefetch_landed();
         if (cur.live) {
             const JobDesc<T>* J = jobs + cur.job;
             const int Ny = J->Ny, My = J->My, ps = J->ps_y, nf = cur.nf;
@@ -509,7 +563,7 @@ __global__ void TDDA_PASS_BOUNDS k_py(const JobDesc<T>* __restrict__ jobs, const
                     if (e < nin * nf) {
                         int y, f;
                         fdivmod(e, nf, rnf, y, f);
-                        pr.set(k, src[(size_t)y * F + f]);
+                        pr.set(k, gld(src + (size_t)y * F + f));
                     }
                 }
             }
@@ -529,7 +583,7 @@ __global__ void TDDA_PASS_BOUNDS k_py(const JobDesc<T>* __restrict__ jobs, const
             for (int e = threadIdx.x; e < nout * nf; e += kThreads) {
                 int y, f;
                 fdivmod(e, nf, rnf, y, f);
-                dst[(size_t)y * F + f] = res[f * ps + lpad(y)];
+                gst(dst + (size_t)y * F + f, res[f * ps + lpad(y)]);
             }
             __syncthreads();
         }
@@ -551,6 +605,7 @@ __global__ void TDDA_PASS_BOUNDS k_px(const JobDesc<T>* __restrict__ jobs, const
     STile cur;
     cur.live = false;
     for (int tl = blockIdx.x;; tl += gridDim.x) {
+        prefetch_landed();
         if (cur.live) {
             const JobDesc<T>* J = jobs + cur.job;
             const int Nx = J->Nx, Mx = J->Mx, ps = J->ps_x, nf = cur.nf;
@@ -589,7 +644,7 @@ __global__ void TDDA_PASS_BOUNDS k_px(const JobDesc<T>* __restrict__ jobs, const
                     if (e < J->Nx * nf) {
                         int x, f;
                         fdivmod(e, nf, rnf, x, f);
-                        pr.set(k, base[x * sx + f]);
+                        pr.set(k, gld(base + x * sx + f));
                     }
                 }
             }
@@ -620,10 +675,10 @@ __global__ void TDDA_PASS_BOUNDS k_px(const JobDesc<T>* __restrict__ jobs, const
                 const int kzf = (kz > Mz / 2) ? Mz - kz : kz;
                 const T szv = (kz > Mz / 2) ? T(-1) : T(1);
                 const Cx<T>* gp = J->green + ((size_t)(kxf * Hy + kyf) * Hz + kzf) * 6;
-                const Cx<T> gxx = gp[0], gyy = gp[3], gzz = gp[5];
-                const Cx<T> gxy = cscale(gp[1], sxv * sy);
-                const Cx<T> gxz = cscale(gp[2], sxv * szv);
-                const Cx<T> gyz = cscale(gp[4], sy * szv);
+                const Cx<T> gxx = gld(gp), gyy = gld(gp + 3), gzz = gld(gp + 5);
+                const Cx<T> gxy = cscale(gld(gp + 1), sxv * sy);
+                const Cx<T> gxz = cscale(gld(gp + 2), sxv * szv);
+                const Cx<T> gyz = cscale(gld(gp + 4), sy * szv);
                 Cx<T>* v = res + (kl * 3) * ps + lpad(kx);
                 const Cx<T> bx = v[0], by = v[ps], bz = v[2 * ps];
                 v[0] = cadd(cadd(cmul(gxx, bx), cmul(gxy, by)), cmul(gxz, bz));
@@ -638,7 +693,7 @@ __global__ void TDDA_PASS_BOUNDS k_px(const JobDesc<T>* __restrict__ jobs, const
             for (int e = threadIdx.x; e < Nx * nf; e += kThreads) {
                 int x, f;
                 fdivmod(e, nf, rnf, x, f);
-                base[x * sx + f] = res2[f * ps + lpad(x)];
+                gst(base + x * sx + f, res2[f * ps + lpad(x)]);
             }
             __syncthreads();
         }
@@ -662,6 +717,7 @@ __global__ void TDDA_PASS_BOUNDS k_p5(const JobDesc<T>* __restrict__ jobs, const
     ZTile cur;
     cur.live = false;
     for (int tl = blockIdx.x;; tl += gridDim.x) {
+        prefetch_landed();
         if (cur.live) {
             const JobDesc<T>* J = jobs + cur.job;
             const int Mz = J->Mz, ps = J->ps_z;
@@ -691,7 +747,7 @@ __global__ void TDDA_PASS_BOUNDS k_p5(const JobDesc<T>* __restrict__ jobs, const
 #pragma unroll
                 for (int k = 0; k < EP; k++) {
                     const int e = threadIdx.x + k * kThreads;
-                    if (e < nxt.np * Mz * 3) pr.set(k, w1[e]);
+                    if (e < nxt.np * Mz * 3) pr.set(k, gld(w1 + e));
                 }
             }
         }
@@ -713,7 +769,7 @@ __global__ void TDDA_PASS_BOUNDS k_p5(const JobDesc<T>* __restrict__ jobs, const
                 fdivmod(rem, 3, r3, z, c);
                 int i;
                 if (J->cellmap) {
-                    const int site = J->cellmap[base + pl * Nz + z];
+                    const int site = gldi(J->cellmap + base + pl * Nz + z);
                     if (site < 0) continue;
                     i = site * 3 + c;
                 } else {
@@ -722,28 +778,29 @@ __global__ void TDDA_PASS_BOUNDS k_p5(const JobDesc<T>* __restrict__ jobs, const
                 const Cx<T> conv = res[(pl * 3 + c) * ps + lpad(z)];
                 if (MODE == OUT_PLAIN) {
                     Cx<T> y = conv;
-                    if (diag) y = cadd(y, cmul(J->al[i], J->in[i]));
-                    J->out[i] = y;
+                    if (diag) y = cadd(y, cmul(gld(J->al + i), gld(J->in + i)));
+                    gst(J->out + i, y);
                 } else if (MODE == OUT_RES) {
                     // r = E - A x ; r0 = r   (DDAModel.cpp:163-166)
-                    const Cx<T> ax = cadd(conv, cmul(J->al[i], J->x[i]));
-                    const Cx<T> ev = J->e[i];
+                    const Cx<T> ax = cadd(conv, cmul(gld(J->al + i), gld(J->x + i)));
+                    const Cx<T> ev = gld(J->e + i);
                     const Cx<T> r = csub(ev, ax);
-                    J->r[i] = r;
-                    J->r0[i] = r;
+                    gst(J->r + i, r);
+                    gst(J->r0 + i, r);
                     const double2 rd = to_d(r), ed = to_d(ev);
                     acc[0] += rd.x * rd.x + rd.y * rd.y;
                     acc[1] += ed.x * ed.x + ed.y * ed.y;
                 } else if (MODE == OUT_AP) {
-                    const Cx<T> ap = cadd(conv, cmul(J->al[i], J->p[i]));
-                    J->ap[i] = ap;
-                    const double2 a = to_d(J->r0[i]), b = to_d(ap);
+                    const Cx<T> ap = cadd(conv, cmul(gld(J->al + i), gld(J->p + i)));
+                    gst(J->ap + i, ap);
+                    const double2 a = to_d(gld(J->r0 + i)), b = to_d(ap);
                     acc[0] += a.x * b.x + a.y * b.y;  // conj(r0) Ap
                     acc[1] += a.x * b.y - a.y * b.x;
                 } else {  // OUT_AT
-                    const Cx<T> at = cadd(conv, cmul(J->al[i], J->t[i]));
-                    J->at[i] = at;
-                    const double2 a = to_d(at), b = to_d(J->t[i]);
+                    const Cx<T> tv = gld(J->t + i);
+                    const Cx<T> at = cadd(conv, cmul(gld(J->al + i), tv));
+                    gst(J->at + i, at);
+                    const double2 a = to_d(at), b = to_d(tv);
                     acc[0] += a.x * b.x + a.y * b.y;  // conj(At) t
                     acc[1] += a.x * b.y - a.y * b.x;
                     acc[2] += a.x * a.x + a.y * a.y;  // conj(At) At
@@ -782,10 +839,10 @@ __global__ void __launch_bounds__(kThreads) k_init_zero(const JobDesc<T>* __rest
     for (int k = 0; k < kVecPer; k++) {
         const int i = i0 + k * kThreads + threadIdx.x;
         if (i < n) {
-            const Cx<T> ev = J.e[i];
-            J.r[i] = ev;
-            J.r0[i] = ev;
-            J.x[i] = Cx<T>{T(0), T(0)};
+            const Cx<T> ev = gld(J.e + i);
+            gst(J.r + i, ev);
+            gst(J.r0 + i, ev);
+            gst(J.x + i, Cx<T>{T(0), T(0)});
             const double2 d = to_d(ev);
             acc[0] += d.x * d.x + d.y * d.y;
         }
@@ -807,7 +864,7 @@ __global__ void __launch_bounds__(kThreads) k_update(const JobDesc<T>* __restric
     if (job_inactive(J.st)) return;
     const int tile = blockIdx.x - tstart[job];
     const int n = J.N * 3;
-    const SolverState* stc = J.st;
+    const TDDA_GLOBAL SolverState* stc = gstate(J.st);
     const Cx<T> al = from_d<T>(stc->alpha[0], stc->alpha[1]);
     const Cx<T> ze = from_d<T>(stc->zeta[0], stc->zeta[1]);
     double acc[4] = {0.0, 0.0, 0.0, 0.0};
@@ -815,13 +872,13 @@ __global__ void __launch_bounds__(kThreads) k_update(const JobDesc<T>* __restric
     for (int k = 0; k < kVecPer; k++) {
         const int i = i0 + k * kThreads + threadIdx.x;
         if (i < n) {
-            const Cx<T> r = J.r[i];
-            const Cx<T> u = cmul(ze, J.ap[i]);
+            const Cx<T> r = gld(J.r + i);
+            const Cx<T> u = cmul(ze, gld(J.ap + i));
             const Cx<T> z = csub(cmul(ze, r), cmul(al, u));
-            J.x[i] = cadd(cadd(J.x[i], cmul(al, J.p[i])), z);
-            const Cx<T> rn = csub(J.t[i], cmul(ze, J.at[i]));
-            J.r[i] = rn;
-            const double2 a = to_d(J.r0[i]), b = to_d(rn);
+            gst(J.x + i, cadd(cadd(gld(J.x + i), cmul(al, gld(J.p + i))), z));
+            const Cx<T> rn = csub(gld(J.t + i), cmul(ze, gld(J.at + i)));
+            gst(J.r + i, rn);
+            const double2 a = to_d(gld(J.r0 + i)), b = to_d(rn);
             acc[0] += b.x * b.x + b.y * b.y;
             acc[1] += a.x * b.x + a.y * b.y;
             acc[2] += a.x * b.y - a.y * b.x;
@@ -849,7 +906,7 @@ __global__ void __launch_bounds__(kThreads) k_finish(const JobDesc<T>* __restric
     double acc[4] = {0.0, 0.0, 0.0, 0.0};
     for (int i = threadIdx.x; i < ntiles; i += kThreads) {
 #pragma unroll
-        for (int k = 0; k < 4; k++) acc[k] += J.part[i * 4 + k];
+        for (int k = 0; k < 4; k++) acc[k] += *(const TDDA_GLOBAL double*)(J.part + i * 4 + k);
     }
     block_sum<4>(acc, red);
     if (threadIdx.x != 0) return;
