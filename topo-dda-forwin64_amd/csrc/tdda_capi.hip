@@ -251,9 +251,9 @@ int plan_tiles(const tdda_op_s* op, TilePlan& t)
     const int capz = std::min(fz, env), capy = std::min(fy, env), capx = std::min(fx, env);
     const int cap1 = fz / 3;  // P1: up to three input vectors
     const int unit = (int)(128 / s);
-    t.ps_z = lds_pencil_stride(op->Mz);
-    t.ps_y = lds_pencil_stride(op->My);
-    t.ps_x = lds_pencil_stride(op->Mx);
+    t.ps_z = lds_pencil_stride<T>(op->Mz);
+    t.ps_y = lds_pencil_stride<T>(op->My);
+    t.ps_x = lds_pencil_stride<T>(op->Mx);
     // one tile must hold: 3 z pencils (P5 reads all kz), one y column of My
     // rows (P4), and the Nx loaded rows of 3 x columns (P3; the Mx - Nx
     // padding rows are zero-filled in LDS, bounded by the LDS check below)

@@ -218,29 +218,48 @@ namespace tdda {
 // ---------------------------------------------------------------------------
 // LDS transform used by the operator passes.
 //
-// Pencil p, element i lives at p * ps + pad(i) with pad(i) = i + (i >> 5):
-// one spare slot per 32 elements spreads the strided Stockham writes of the
-// early stages over the LDS banks.  Twiddles come from an LDS copy of a
+// Pencil p, element i lives at p * ps + lpad<T>(i): spare slots spread the
+// strided Stockham writes of the early stages over the LDS banks.  Twiddles come from an LDS copy of a
 // two-level table of the L-th roots of unity (tb + ceil(L / tb) entries,
 // tb ~ sqrt(L)): one table product per butterfly, then powers by recurrence.
 // Index arithmetic uses float reciprocals instead of integer division.
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ int lpad(int i) { return i + (i >> 5); }
-__host__ __device__ __forceinline__ int lds_pencil_stride(int L) { return L + (L >> 5) + 1; }
+// One spare slot every 2^shift elements: 16 for complex128 (16-byte
+// elements, whose strided first-stage writes conflict most), 32 for
+// complex64 (measured: denser padding costs complex64 more in occupancy than
+// it saves in conflicts).
+#ifndef TDDA_LDS_PAD_SHIFT_C128
+#define TDDA_LDS_PAD_SHIFT_C128 4
+#endif
+#ifndef TDDA_LDS_PAD_SHIFT_C64
+#define TDDA_LDS_PAD_SHIFT_C64 5
+#endif
+template <typename T>
+__host__ __device__ constexpr int pad_shift()
+{
+    return sizeof(T) == 8 ? TDDA_LDS_PAD_SHIFT_C128 : TDDA_LDS_PAD_SHIFT_C64;
+}
+template <typename T>
+__device__ __forceinline__ int lpad(int i)
+{
+    return i + (i >> pad_shift<T>());
+}
+template <typename T>
+__host__ __device__ __forceinline__ int lds_pencil_stride(int L)
+{
+    return L + (L >> pad_shift<T>()) + 1;
+}
 
 // q = n / d, r = n % d for 0 <= n < 2^24 via a float reciprocal and one
-// correction step (exact in that range).
+// correction step (exact in that range).  Branch-free, with a 24-bit
+// multiply (full rate, unlike the 32-bit one).
 __device__ __forceinline__ void fdivmod(int n, int d, float rd, int& q, int& r)
 {
     q = (int)((float)n * rd);
-    r = n - q * d;
-    if (r < 0) {
-        q -= 1;
-        r += d;
-    } else if (r >= d) {
-        q += 1;
-        r -= d;
-    }
+    r = n - (int)__umul24((unsigned)q, (unsigned)d);
+    const bool lo = r < 0, hi = r >= d;
+    q = lo ? q - 1 : (hi ? q + 1 : q);
+    r = lo ? r + d : (hi ? r - d : r);
 }
 
 // One Stockham stage of the operator passes: NP pencils, src -> dst (LDS,
@@ -262,7 +281,7 @@ __device__ __forceinline__ void pp_stage(const Cx<T>* __restrict__ src, Cx<T>* _
         const Cx<T>* sp = src + p * ps;
         Cx<T> v[R];
 #pragma unroll
-        for (int m = 0; m < R; m++) v[m] = sp[lpad(j + m * LR)];
+        for (int m = 0; m < R; m++) v[m] = sp[lpad<T>(j + m * LR)];
         if (Ns > 1) {
             // w^m by recurrence from w = exp(-+2 pi i kk twstep / L)
             int q1, q0;
@@ -279,7 +298,7 @@ __device__ __forceinline__ void pp_stage(const Cx<T>* __restrict__ src, Cx<T>* _
         Cx<T>* dp = dst + p * ps;
         const int base = (j - kk) * R + kk;
 #pragma unroll
-        for (int m = 0; m < R; m++) dp[lpad(base + m * Ns)] = v[m];
+        for (int m = 0; m < R; m++) dp[lpad<T>(base + m * Ns)] = v[m];
     }
 }
 
@@ -404,7 +423,7 @@ __device__ __forceinline__ void ct_stage(const Cx<T>* __restrict__ src, Cx<T>* _
                                          const Cx<T>* __restrict__ tw)
 {
     constexpr int LR = L / R;
-    constexpr int PS = L + (L >> 5) + 1;
+    constexpr int PS = L + (L >> pad_shift<T>()) + 1;
     constexpr int TWSTEP = L / (NS * R);
     constexpr int TB = ct_tb(L);
     constexpr int NPW = (NPMAX + kWaves - 1) / kWaves;  // pencils per wavefront, at most
@@ -423,7 +442,7 @@ __device__ __forceinline__ void ct_stage(const Cx<T>* __restrict__ src, Cx<T>* _
             const Cx<T>* sp = src + p * PS;
             Cx<T> v[R];
 #pragma unroll
-            for (int m = 0; m < R; m++) v[m] = sp[lpad(j + m * LR)];
+            for (int m = 0; m < R; m++) v[m] = sp[lpad<T>(j + m * LR)];
             if constexpr (NS > 1) {
                 const int q = kk * TWSTEP;
                 const Cx<T> w1 = cmul(tw[q % TB], tw[TB + q / TB]);
@@ -438,7 +457,7 @@ __device__ __forceinline__ void ct_stage(const Cx<T>* __restrict__ src, Cx<T>* _
             Cx<T>* dp = dst + p * PS;
             const int base = (j - kk) * R + kk;
 #pragma unroll
-            for (int m = 0; m < R; m++) dp[lpad(base + m * NS)] = v[m];
+            for (int m = 0; m < R; m++) dp[lpad<T>(base + m * NS)] = v[m];
         }
     }
 }

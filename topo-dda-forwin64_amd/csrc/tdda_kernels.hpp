@@ -427,7 +427,7 @@ __global__ void TDDA_PASS_BOUNDS k_p1(const JobDesc<T>* __restrict__ jobs, const
                         if (idx >= 0) gst(J->t + idx, v);
                     }
                 }
-                A[(pl * 3 + c) * ps + lpad(z)] = v;
+                A[(pl * 3 + c) * ps + lpad<T>(z)] = v;
             }
             const int npad = Mz - Nz;
             if (npad > 0) {
@@ -435,7 +435,7 @@ __global__ void TDDA_PASS_BOUNDS k_p1(const JobDesc<T>* __restrict__ jobs, const
                 for (int e = threadIdx.x; e < cur.np * 3 * npad; e += kThreads) {
                     int pc, zz;
                     fdivmod(e, npad, rpad, pc, zz);
-                    A[pc * ps + lpad(Nz + zz)] = Cx<T>{T(0), T(0)};
+                    A[pc * ps + lpad<T>(Nz + zz)] = Cx<T>{T(0), T(0)};
                 }
             }
         }
@@ -498,7 +498,7 @@ __global__ void TDDA_PASS_BOUNDS k_p1(const JobDesc<T>* __restrict__ jobs, const
                 int pl, rem, kz, c;
                 fdivmod(e, Mz * 3, rMz3, pl, rem);
                 fdivmod(rem, 3, r3, kz, c);
-                gst(w1 + e, res[(pl * 3 + c) * ps + lpad(kz)]);
+                gst(w1 + e, res[(pl * 3 + c) * ps + lpad<T>(kz)]);
             }
             __syncthreads();
         }
@@ -535,14 +535,14 @@ __global__ void TDDA_PASS_BOUNDS k_py(const JobDesc<T>* __restrict__ jobs, const
                 if (e < nin * nf) {
                     int y, f;
                     fdivmod(e, nf, rnf, y, f);
-                    A[f * ps + lpad(y)] = pr.get(k);
+                    A[f * ps + lpad<T>(y)] = pr.get(k);
                 }
             }
             if (!INV) {
                 for (int e = threadIdx.x; e < (My - Ny) * nf; e += kThreads) {
                     int y, f;
                     fdivmod(e, nf, rnf, y, f);
-                    A[f * ps + lpad(Ny + y)] = Cx<T>{T(0), T(0)};
+                    A[f * ps + lpad<T>(Ny + y)] = Cx<T>{T(0), T(0)};
                 }
             }
         }
@@ -563,7 +563,7 @@ __global__ void TDDA_PASS_BOUNDS k_py(const JobDesc<T>* __restrict__ jobs, const
                     if (e < nin * nf) {
                         int y, f;
                         fdivmod(e, nf, rnf, y, f);
-                        pr.set(k, gld(src + (size_t)y * F + f));
+                        pr.set(k, gld(src + (y * F + f)));
                     }
                 }
             }
@@ -583,7 +583,7 @@ __global__ void TDDA_PASS_BOUNDS k_py(const JobDesc<T>* __restrict__ jobs, const
             for (int e = threadIdx.x; e < nout * nf; e += kThreads) {
                 int y, f;
                 fdivmod(e, nf, rnf, y, f);
-                gst(dst + (size_t)y * F + f, res[f * ps + lpad(y)]);
+                gst(dst + (y * F + f), res[f * ps + lpad<T>(y)]);
             }
             __syncthreads();
         }
@@ -619,13 +619,13 @@ __global__ void TDDA_PASS_BOUNDS k_px(const JobDesc<T>* __restrict__ jobs, const
                 if (e < Nx * nf) {
                     int x, f;
                     fdivmod(e, nf, rnf, x, f);
-                    A[f * ps + lpad(x)] = pr.get(k);
+                    A[f * ps + lpad<T>(x)] = pr.get(k);
                 }
             }
             for (int e = threadIdx.x; e < (Mx - Nx) * nf; e += kThreads) {
                 int x, f;
                 fdivmod(e, nf, rnf, x, f);
-                A[f * ps + lpad(Nx + x)] = Cx<T>{T(0), T(0)};
+                A[f * ps + lpad<T>(Nx + x)] = Cx<T>{T(0), T(0)};
             }
         }
         STile nxt;
@@ -635,7 +635,7 @@ __global__ void TDDA_PASS_BOUNDS k_px(const JobDesc<T>* __restrict__ jobs, const
             if (nxt.live) {
                 const JobDesc<T>* J = jobs + nxt.job;
                 const int F = J->Mz * 3, nf = nxt.nf;
-                const size_t sx = (size_t)J->My * F;
+                const int sx = J->My * F;  // one slab row: < 2^31 elements
                 const Cx<T>* base = J->w2 + (size_t)nxt.outer * F + nxt.f0;
                 const float rnf = 1.0f / (float)nf;
 #pragma unroll
@@ -644,7 +644,7 @@ __global__ void TDDA_PASS_BOUNDS k_px(const JobDesc<T>* __restrict__ jobs, const
                     if (e < J->Nx * nf) {
                         int x, f;
                         fdivmod(e, nf, rnf, x, f);
-                        pr.set(k, gld(base + x * sx + f));
+                        pr.set(k, gld(base + (x * sx + f)));
                     }
                 }
             }
@@ -679,7 +679,7 @@ __global__ void TDDA_PASS_BOUNDS k_px(const JobDesc<T>* __restrict__ jobs, const
                 const Cx<T> gxy = cscale(gld(gp + 1), sxv * sy);
                 const Cx<T> gxz = cscale(gld(gp + 2), sxv * szv);
                 const Cx<T> gyz = cscale(gld(gp + 4), sy * szv);
-                Cx<T>* v = res + (kl * 3) * ps + lpad(kx);
+                Cx<T>* v = res + (kl * 3) * ps + lpad<T>(kx);
                 const Cx<T> bx = v[0], by = v[ps], bz = v[2 * ps];
                 v[0] = cadd(cadd(cmul(gxx, bx), cmul(gxy, by)), cmul(gxz, bz));
                 v[ps] = cadd(cadd(cmul(gxy, bx), cmul(gyy, by)), cmul(gyz, bz));
@@ -687,13 +687,13 @@ __global__ void TDDA_PASS_BOUNDS k_px(const JobDesc<T>* __restrict__ jobs, const
             }
             __syncthreads();
             const Cx<T>* res2 = pass_fft<T, LC, true>(res, other, nf, ps, J->plx, twl);
-            const size_t sx = (size_t)My * F;
+            const int sx = My * F;
             Cx<T>* base = J->w2 + (size_t)ky * F + f0;
             const float rnf = 1.0f / (float)nf;
             for (int e = threadIdx.x; e < Nx * nf; e += kThreads) {
                 int x, f;
                 fdivmod(e, nf, rnf, x, f);
-                gst(base + x * sx + f, res2[f * ps + lpad(x)]);
+                gst(base + (x * sx + f), res2[f * ps + lpad<T>(x)]);
             }
             __syncthreads();
         }
@@ -732,7 +732,7 @@ __global__ void TDDA_PASS_BOUNDS k_p5(const JobDesc<T>* __restrict__ jobs, const
                     int pl, rem, kz, c;
                     fdivmod(e, Mz * 3, rMz3, pl, rem);
                     fdivmod(rem, 3, r3, kz, c);
-                    A[(pl * 3 + c) * ps + lpad(kz)] = pr.get(k);
+                    A[(pl * 3 + c) * ps + lpad<T>(kz)] = pr.get(k);
                 }
             }
         }
@@ -775,7 +775,7 @@ __global__ void TDDA_PASS_BOUNDS k_p5(const JobDesc<T>* __restrict__ jobs, const
                 } else {
                     i = base * 3 + e;
                 }
-                const Cx<T> conv = res[(pl * 3 + c) * ps + lpad(z)];
+                const Cx<T> conv = res[(pl * 3 + c) * ps + lpad<T>(z)];
                 if (MODE == OUT_PLAIN) {
                     Cx<T> y = conv;
                     if (diag) y = cadd(y, cmul(gld(J->al + i), gld(J->in + i)));
