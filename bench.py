@@ -244,7 +244,9 @@ def bench_serve(T, seconds, rate=200.0, seed=5):
             k += 1
             if tk is None:
                 return
-            srv.wait(tk, -1.0)
+            while srv.wait(tk, 10.0) == 0:
+                # progress line for long runs
+                print(f"serve: {len(done)} of {len(reqs)} requests done", file=sys.stderr, flush=True)
             try:
                 _, infos, tm = srv.result(tk, want_P=False)
                 done.append((at, tm, infos))

@@ -406,6 +406,13 @@ constexpr int ct_tb(int L)
     return t;
 }
 
+constexpr int kPruneIn = 1, kPruneOut = 2;
+// Entries a pruned first stage still reads: [0, ceil(R0/2) L/R0).
+constexpr int ct_zin_limit(int L)
+{
+    return L > 1 ? ((ct_rad(L, 0) + 1) / 2) * (L / ct_rad(L, 0)) : L;
+}
+
 // Butterfly groups per lane that a stage interleaves (unroll factor of
 // the per-lane butterfly loop); full unrolling spills in the x pass.
 #ifndef TDDA_STAGE_UNROLL
@@ -418,10 +425,17 @@ constexpr int ct_tb(int L)
 // a compiler-ordering barrier.
 constexpr int kWaves = kThreadsFft / 64;
 
-template <typename T, int L, int R, int NS, bool INV, int NPMAX>
+// Pruning of the zero-padded transforms (the passes keep L >= 2N - 1):
+//   ZIN  -- input entries at or above N are zero: the first stage's inputs
+//           j + m L/R with m >= ceil(R/2) lie at or above ceil(R/2) L/R >= N
+//           and are taken as zero instead of read;
+//   HOUT -- only outputs below N are needed: the last stage's outputs
+//           base + m L/R with m >= ceil(R/2) are neither computed nor stored.
+template <typename T, int L, int R, int NS, bool INV, int NPMAX, bool ZIN = false, bool HOUT = false>
 __device__ __forceinline__ void ct_stage(const Cx<T>* __restrict__ src, Cx<T>* __restrict__ dst, int np,
                                          const Cx<T>* __restrict__ tw)
 {
+    constexpr int RH = (R + 1) / 2;
     constexpr int LR = L / R;
     constexpr int PS = L + (L >> pad_shift<T>()) + 1;
     constexpr int TWSTEP = L / (NS * R);
@@ -442,7 +456,7 @@ __device__ __forceinline__ void ct_stage(const Cx<T>* __restrict__ src, Cx<T>* _
             const Cx<T>* sp = src + p * PS;
             Cx<T> v[R];
 #pragma unroll
-            for (int m = 0; m < R; m++) v[m] = sp[lpad<T>(j + m * LR)];
+            for (int m = 0; m < R; m++) v[m] = (ZIN && m >= RH) ? Cx<T>{T(0), T(0)} : sp[lpad<T>(j + m * LR)];
             if constexpr (NS > 1) {
                 const int q = kk * TWSTEP;
                 const Cx<T> w1 = cmul(tw[q % TB], tw[TB + q / TB]);
@@ -457,33 +471,38 @@ __device__ __forceinline__ void ct_stage(const Cx<T>* __restrict__ src, Cx<T>* _
             Cx<T>* dp = dst + p * PS;
             const int base = (j - kk) * R + kk;
 #pragma unroll
-            for (int m = 0; m < R; m++) dp[lpad<T>(base + m * NS)] = v[m];
+            for (int m = 0; m < R; m++)
+                if (!(HOUT && m >= RH)) dp[lpad<T>(base + m * NS)] = v[m];
         }
     }
 }
 
-template <typename T, int L, int S, int NS, bool INV, int NPMAX>
+template <typename T, int L, int S, int NS, bool INV, int NPMAX, int PRUNE>
 __device__ __forceinline__ Cx<T>* ct_stages(Cx<T>* a, Cx<T>* b, int np, const Cx<T>* tw)
 {
     if constexpr (S == ct_nst(L)) {
         return a;
     } else {
         constexpr int R = ct_rad(L, S);
-        ct_stage<T, L, R, NS, INV, NPMAX>(a, b, np, tw);
+        constexpr bool zin = (PRUNE & kPruneIn) && S == 0;
+        constexpr bool hout = (PRUNE & kPruneOut) && S + 1 == ct_nst(L);
+        ct_stage<T, L, R, NS, INV, NPMAX, zin, hout>(a, b, np, tw);
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-        return ct_stages<T, L, S + 1, NS * R, INV, NPMAX>(b, a, np, tw);
+        return ct_stages<T, L, S + 1, NS * R, INV, NPMAX, PRUNE>(b, a, np, tw);
     }
 }
 
 // Transform of np <= NPMAX pencils of the compile-time length L; same
 // contract as pp_fft (input visible to all threads, ends with a workgroup
-// barrier).
-template <typename T, int L, bool INV, int NPMAX>
+// barrier).  PRUNE: kPruneIn (input zero at and above N; entries of
+// [N, ct_zin_limit(L)) must still hold zeros) and/or kPruneOut (only
+// outputs below N are valid afterwards).
+template <typename T, int L, bool INV, int NPMAX, int PRUNE = 0>
 __device__ __forceinline__ Cx<T>* ct_fft(Cx<T>* a, Cx<T>* b, int np, const Cx<T>* tw)
 {
-    Cx<T>* r = ct_stages<T, L, 0, 1, INV, NPMAX>(a, b, np, tw);
+    Cx<T>* r = ct_stages<T, L, 0, 1, INV, NPMAX, PRUNE>(a, b, np, tw);
     __syncthreads();
     return r;
 }

@@ -306,14 +306,24 @@ __host__ __device__ constexpr int np_max(int L)
 }
 
 // Transform of a pass tile: compile-time length LC > 0, or the runtime plan.
-template <typename T, int LC, bool INV>
+template <typename T, int LC, bool INV, int PRUNE = 0>
 __device__ __forceinline__ Cx<T>* pass_fft(Cx<T>* a, Cx<T>* b, int np, int ps, const FftPlan& pl, const Cx<T>* tw)
 {
     if constexpr (LC > 0) {
-        return ct_fft<T, LC, INV, np_max<T>(LC)>(a, b, np, tw);
+        return ct_fft<T, LC, INV, np_max<T>(LC), PRUNE>(a, b, np, tw);
     } else {
         return pp_fft<T, INV>(a, b, np, ps, pl, tw);
     }
+}
+
+// Upper end of the zero padding a forward pass must write in LDS: the
+// pruned first stage of the compile-time transform reads only below
+// ct_zin_limit(L); the runtime-plan transform reads everything.
+template <int LC>
+__device__ __forceinline__ int zero_fill_end(int M)
+{
+    if constexpr (LC > 0) return min(M, ct_zin_limit(LC));
+    else return M;
 }
 
 // Geometry of a z-pass tile (P1 / P5): pencils [q0, q0 + np) of job `job`.
@@ -429,7 +439,7 @@ __global__ void TDDA_PASS_BOUNDS k_p1(const JobDesc<T>* __restrict__ jobs, const
                 }
                 A[(pl * 3 + c) * ps + lpad<T>(z)] = v;
             }
-            const int npad = Mz - Nz;
+            const int npad = zero_fill_end<LC>(Mz) - Nz;
             if (npad > 0) {
                 const float rpad = 1.0f / (float)npad;
                 for (int e = threadIdx.x; e < cur.np * 3 * npad; e += kThreads) {
@@ -490,7 +500,7 @@ __global__ void TDDA_PASS_BOUNDS k_p1(const JobDesc<T>* __restrict__ jobs, const
             Cx<T>* A = twl + tw2_slots(Mz, J->plz.tb);
             Cx<T>* B = A + 3 * J->pz1 * ps;
             __syncthreads();
-            const Cx<T>* res = pass_fft<T, LC, false>(A, B, cur.np * 3, ps, J->plz, twl);
+            const Cx<T>* res = pass_fft<T, LC, false, kPruneIn>(A, B, cur.np * 3, ps, J->plz, twl);
             const int nout = cur.np * Mz * 3;
             Cx<T>* w1 = J->w1 + (size_t)cur.q0 * Mz * 3;
             const float rMz3 = 1.0f / (float)(Mz * 3);
@@ -539,7 +549,7 @@ __global__ void TDDA_PASS_BOUNDS k_py(const JobDesc<T>* __restrict__ jobs, const
                 }
             }
             if (!INV) {
-                for (int e = threadIdx.x; e < (My - Ny) * nf; e += kThreads) {
+                for (int e = threadIdx.x; e < (zero_fill_end<LC>(My) - Ny) * nf; e += kThreads) {
                     int y, f;
                     fdivmod(e, nf, rnf, y, f);
                     A[f * ps + lpad<T>(Ny + y)] = Cx<T>{T(0), T(0)};
@@ -575,7 +585,7 @@ __global__ void TDDA_PASS_BOUNDS k_py(const JobDesc<T>* __restrict__ jobs, const
             Cx<T>* A = twl + tw2_slots(My, J->ply.tb);
             Cx<T>* B = A + J->tfy * ps;
             __syncthreads();
-            const Cx<T>* res = pass_fft<T, LC, INV>(A, B, nf, ps, J->ply, twl);
+            const Cx<T>* res = pass_fft<T, LC, INV, INV ? kPruneOut : kPruneIn>(A, B, nf, ps, J->ply, twl);
             const int nout = INV ? Ny : My;
             Cx<T>* dst =
                 (INV ? J->w1 + (size_t)cur.outer * Ny * F : J->w2 + (size_t)cur.outer * My * F) + cur.f0;
@@ -622,7 +632,7 @@ __global__ void TDDA_PASS_BOUNDS k_px(const JobDesc<T>* __restrict__ jobs, const
                     A[f * ps + lpad<T>(x)] = pr.get(k);
                 }
             }
-            for (int e = threadIdx.x; e < (Mx - Nx) * nf; e += kThreads) {
+            for (int e = threadIdx.x; e < (zero_fill_end<LC>(Mx) - Nx) * nf; e += kThreads) {
                 int x, f;
                 fdivmod(e, nf, rnf, x, f);
                 A[f * ps + lpad<T>(Nx + x)] = Cx<T>{T(0), T(0)};
@@ -657,7 +667,7 @@ __global__ void TDDA_PASS_BOUNDS k_px(const JobDesc<T>* __restrict__ jobs, const
             Cx<T>* A = twl + tw2_slots(Mx, J->plx.tb);
             Cx<T>* B = A + J->tfx * ps;
             __syncthreads();
-            Cx<T>* res = pass_fft<T, LC, false>(A, B, nf, ps, J->plx, twl);
+            Cx<T>* res = pass_fft<T, LC, false, kPruneIn>(A, B, nf, ps, J->plx, twl);
             Cx<T>* other = (res == A) ? B : A;
             // Green product.  Octant fold per axis: k' = min(k, M - k); odd
             // components flip sign when k was folded.
@@ -686,7 +696,7 @@ __global__ void TDDA_PASS_BOUNDS k_px(const JobDesc<T>* __restrict__ jobs, const
                 v[2 * ps] = cadd(cadd(cmul(gxz, bx), cmul(gyz, by)), cmul(gzz, bz));
             }
             __syncthreads();
-            const Cx<T>* res2 = pass_fft<T, LC, true>(res, other, nf, ps, J->plx, twl);
+            const Cx<T>* res2 = pass_fft<T, LC, true, kPruneOut>(res, other, nf, ps, J->plx, twl);
             const int sx = My * F;
             Cx<T>* base = J->w2 + (size_t)ky * F + f0;
             const float rnf = 1.0f / (float)nf;
@@ -758,7 +768,7 @@ __global__ void TDDA_PASS_BOUNDS k_p5(const JobDesc<T>* __restrict__ jobs, const
             Cx<T>* A = twl + tw2_slots(Mz, J->plz.tb);
             Cx<T>* B = A + 3 * J->pz * ps;
             __syncthreads();
-            const Cx<T>* res = pass_fft<T, LC, true>(A, B, cur.np * 3, ps, J->plz, twl);
+            const Cx<T>* res = pass_fft<T, LC, true, kPruneOut>(A, B, cur.np * 3, ps, J->plz, twl);
             double acc[4] = {0.0, 0.0, 0.0, 0.0};
             const int nout = cur.np * Nz * 3;
             const int base = cur.q0 * Nz;
