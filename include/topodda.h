@@ -12,7 +12,13 @@
  *   DDAModel state (P, E, al)   DDAModel.h:14-30          tdda_model_create/_set/_get
  *   DDAModel::Aproductwithalb   DDAModel.cpp:577-586      tdda_model_apply
  *   DDAModel::bicgstab          DDAModel.cpp:131-260      tdda_model_solve
+ *   DDAModel ctor (E, al)       DDAModel.cpp:12-69        tdda_incident_field, tdda_polarizability
+ *   Get_Alpha                   tools.cpp:617-630         tdda_get_alpha
+ *   update_E_in_structure       DDAModel.cpp:567-575      tdda_local_field
  *   (kernel.cu A2As/B2Bs/Conv/Conv2B and the cuFFT calls are internal.)
+ *   No counterpart (one solve at a time in the reference, T/main.cpp:584-598):
+ *   tdda_batch_* (many structures per launch), tdda_server_* (request loop),
+ *   the measurement hooks tdda_*_time_pass(es).
  *
  * Conventions
  *   - Every function returns TDDA_OK (0) or a negative status; the message of
@@ -22,7 +28,8 @@
  *     (AProductCore.cpp:705-714).  Batched calls take nrhs such vectors back
  *     to back.
  *   - Handles own their device memory; callers own host buffers.  A handle is
- *     not thread-safe: one host thread at a time per context.
+ *     not thread-safe: one host thread at a time per context.  A server runs
+ *     its own worker thread and context; its calls are thread-safe.
  *   - No exceptions cross this boundary.
  */
 #ifndef TOPODDA_H_

@@ -141,7 +141,32 @@ struct tdda_ctx_s {
     size_t lds_cu;      // per CU
     int ncu;
     int refs;
+    // stream-ordered allocation (request loop): hipFreeAsync does not
+    // synchronise the device the way hipFree does
+    bool async_alloc = false;
 };
+
+namespace {
+
+int ctx_malloc(const tdda_ctx_s* c, void** p, size_t bytes)
+{
+    if (c->async_alloc) {
+        if (hipMallocAsync(p, bytes, c->stream) != hipSuccess) return TDDA_ERR_OOM;
+        // usable from any stream (and by blocking copies) once the stream has reached it
+        if (hipStreamSynchronize(c->stream) != hipSuccess) return TDDA_ERR_HIP;
+        return TDDA_OK;
+    }
+    return hipMalloc(p, bytes) == hipSuccess ? TDDA_OK : TDDA_ERR_OOM;
+}
+
+void ctx_free(const tdda_ctx_s* c, void* p)
+{
+    if (!p) return;
+    if (c->async_alloc) (void)hipFreeAsync(p, c->stream);
+    else (void)hipFree(p);
+}
+
+}  // namespace
 
 struct TilePlan {
     int pz, pz1, tfy, tfx, ps_z, ps_y, ps_x;
@@ -280,12 +305,12 @@ int plan_tiles(const tdda_op_s* op, TilePlan& t)
 }
 
 template <typename T>
-int dev_alloc(std::vector<void*>& arena, T** p, size_t count)
+int dev_alloc(const tdda_ctx_s* c, std::vector<void*>& arena, T** p, size_t count)
 {
     void* q = nullptr;
     if (count == 0) count = 1;
-    hipError_t e = hipMalloc(&q, count * sizeof(T));
-    if (e != hipSuccess) return fail(TDDA_ERR_OOM, std::string("hipMalloc failed: ") + hipGetErrorString(e));
+    if (ctx_malloc(c, &q, count * sizeof(T)) != TDDA_OK)
+        return fail(TDDA_ERR_OOM, "device allocation of " + std::to_string(count * sizeof(T)) + " bytes failed");
     arena.push_back(q);
     *p = reinterpret_cast<T*>(q);
     return TDDA_OK;
@@ -297,17 +322,17 @@ int alloc_vecs(tdda_model_s* m, VecSet<T>& v)
     const size_t n = m->n3 * m->nrhs;
     const tdda_op_s* op = m->op;
     std::vector<void*>& a = m->allocs;
-    CHK(dev_alloc(a, &v.al, n));
-    CHK(dev_alloc(a, &v.e, n));
-    CHK(dev_alloc(a, &v.x, n));
-    CHK(dev_alloc(a, &v.r, n));
-    CHK(dev_alloc(a, &v.r0, n));
-    CHK(dev_alloc(a, &v.p, n));
-    CHK(dev_alloc(a, &v.ap, n));
-    CHK(dev_alloc(a, &v.t, n));
-    CHK(dev_alloc(a, &v.at, n));
-    CHK(dev_alloc(a, &v.w1, (size_t)op->Nx * op->Ny * op->Mz * 3 * m->nrhs));
-    CHK(dev_alloc(a, &v.w2, (size_t)op->Nx * op->My * op->Mz * 3 * m->nrhs));
+    CHK(dev_alloc(op->ctx, a, &v.al, n));
+    CHK(dev_alloc(op->ctx, a, &v.e, n));
+    CHK(dev_alloc(op->ctx, a, &v.x, n));
+    CHK(dev_alloc(op->ctx, a, &v.r, n));
+    CHK(dev_alloc(op->ctx, a, &v.r0, n));
+    CHK(dev_alloc(op->ctx, a, &v.p, n));
+    CHK(dev_alloc(op->ctx, a, &v.ap, n));
+    CHK(dev_alloc(op->ctx, a, &v.t, n));
+    CHK(dev_alloc(op->ctx, a, &v.at, n));
+    CHK(dev_alloc(op->ctx, a, &v.w1, (size_t)op->Nx * op->Ny * op->Mz * 3 * m->nrhs));
+    CHK(dev_alloc(op->ctx, a, &v.w2, (size_t)op->Nx * op->My * op->Mz * 3 * m->nrhs));
     HIPCHK(hipMemsetAsync(v.x, 0, n * sizeof(Cx<T>), op->ctx->stream));
     HIPCHK(hipMemsetAsync(v.al, 0, n * sizeof(Cx<T>), op->ctx->stream));
     HIPCHK(hipMemsetAsync(v.e, 0, n * sizeof(Cx<T>), op->ctx->stream));
@@ -471,8 +496,8 @@ int finalize_table(std::vector<void*>& arena, const tdda_ctx_s* c, JobTable<T>& 
     const int nj = (int)jt.h.size();
     std::vector<size_t> poff(nj + 1, 0);
     for (int j = 0; j < nj; j++) poff[j + 1] = poff[j] + job_part_size(jt.h[j]);
-    CHK(dev_alloc(arena, &jt.st, nj));
-    CHK(dev_alloc(arena, &jt.part, poff[nj]));
+    CHK(dev_alloc(c, arena, &jt.st, nj));
+    CHK(dev_alloc(c, arena, &jt.part, poff[nj]));
     HIPCHK(hipMemsetAsync(jt.st, 0, sizeof(SolverState) * nj, c->stream));
     for (int j = 0; j < nj; j++) {
         jt.h[j].st = jt.st + j;
@@ -481,7 +506,7 @@ int finalize_table(std::vector<void*>& arena, const tdda_ctx_s* c, JobTable<T>& 
     auto put = [&](const auto* host, size_t n) {
         using U = std::remove_const_t<std::remove_pointer_t<decltype(host)>>;
         U* d = nullptr;
-        if (dev_alloc(arena, &d, n) != TDDA_OK) return (const U*)nullptr;
+        if (dev_alloc(c, arena, &d, n) != TDDA_OK) return (const U*)nullptr;
         if (hipMemcpy(d, host, sizeof(U) * n, hipMemcpyHostToDevice) != hipSuccess) return (const U*)nullptr;
         return (const U*)d;
     };
@@ -1063,7 +1088,7 @@ int tdda_op_create(tdda_ctx ctx, const tdda_op_desc* desc, tdda_op* out)
         return rc;
     };
     if (!bulk) {
-        if (hipMalloc(&op->cellmap, sizeof(int) * box) != hipSuccess) return cleanup(fail(TDDA_ERR_OOM, "cell map"));
+        if (ctx_malloc(ctx, (void**)&op->cellmap, sizeof(int) * box) != TDDA_OK) return cleanup(fail(TDDA_ERR_OOM, "cell map"));
         if (hipMemcpy(op->cellmap, cmap.data(), sizeof(int) * box, hipMemcpyHostToDevice) != hipSuccess)
             return cleanup(fail(TDDA_ERR_HIP, "cell map upload"));
     }
@@ -1072,8 +1097,8 @@ int tdda_op_create(tdda_ctx ctx, const tdda_op_desc* desc, tdda_op* out)
         std::vector<Cx<double>> t = twiddles(M[a]);
         std::vector<Cx<float>> tf(M[a]);
         for (int q = 0; q < M[a]; q++) tf[q] = Cx<float>{(float)t[q].x, (float)t[q].y};
-        if (hipMalloc(&op->tw64[a], sizeof(Cx<double>) * M[a]) != hipSuccess ||
-            hipMalloc(&op->tw32[a], sizeof(Cx<float>) * M[a]) != hipSuccess)
+        if (ctx_malloc(ctx, (void**)&op->tw64[a], sizeof(Cx<double>) * M[a]) != TDDA_OK ||
+            ctx_malloc(ctx, (void**)&op->tw32[a], sizeof(Cx<float>) * M[a]) != TDDA_OK)
             return cleanup(fail(TDDA_ERR_OOM, "twiddles"));
         if (hipMemcpy(op->tw64[a], t.data(), sizeof(Cx<double>) * M[a], hipMemcpyHostToDevice) != hipSuccess ||
             hipMemcpy(op->tw32[a], tf.data(), sizeof(Cx<float>) * M[a], hipMemcpyHostToDevice) != hipSuccess)
@@ -1081,8 +1106,8 @@ int tdda_op_create(tdda_ctx ctx, const tdda_op_desc* desc, tdda_op* out)
         std::vector<Cx<double>> t2 = twiddles2(M[a], op->pl[a].tb);
         std::vector<Cx<float>> t2f(t2.size());
         for (size_t q = 0; q < t2.size(); q++) t2f[q] = Cx<float>{(float)t2[q].x, (float)t2[q].y};
-        if (hipMalloc(&op->tws64[a], sizeof(Cx<double>) * t2.size()) != hipSuccess ||
-            hipMalloc(&op->tws32[a], sizeof(Cx<float>) * t2.size()) != hipSuccess)
+        if (ctx_malloc(ctx, (void**)&op->tws64[a], sizeof(Cx<double>) * t2.size()) != TDDA_OK ||
+            ctx_malloc(ctx, (void**)&op->tws32[a], sizeof(Cx<float>) * t2.size()) != TDDA_OK)
             return cleanup(fail(TDDA_ERR_OOM, "twiddles"));
         if (hipMemcpy(op->tws64[a], t2.data(), sizeof(Cx<double>) * t2.size(), hipMemcpyHostToDevice) != hipSuccess ||
             hipMemcpy(op->tws32[a], t2f.data(), sizeof(Cx<float>) * t2.size(), hipMemcpyHostToDevice) != hipSuccess)
@@ -1092,7 +1117,7 @@ int tdda_op_create(tdda_ctx ctx, const tdda_op_desc* desc, tdda_op* out)
     // (AProductCore.cpp:54-229).
     const size_t Mt = (size_t)M[0] * M[1] * M[2];
     Cx<double>* full = nullptr;
-    if (hipMalloc(&full, sizeof(Cx<double>) * Mt * 6) != hipSuccess)
+    if (ctx_malloc(ctx, (void**)&full, sizeof(Cx<double>) * Mt * 6) != TDDA_OK)
         return cleanup(fail(TDDA_ERR_OOM, "Green tensor grid"));
     hipStream_t s = ctx->stream;
     k_green_fill<<<(unsigned)((Mt + kThreads - 1) / kThreads), kThreads, 0, s>>>(full, D.Nx, D.Ny, D.Nz, M[0], M[1],
@@ -1104,7 +1129,7 @@ int tdda_op_create(tdda_ctx ctx, const tdda_op_desc* desc, tdda_op* out)
         int P = (int)(budget / (2 * 6 * ps * sizeof(Cx<double>)));
         P = std::max(1, std::min(P, 32));
         if (2 * 6 * (size_t)ps * sizeof(Cx<double>) > 65536) {
-            (void)hipFree(full);
+            ctx_free(ctx, full);
             return cleanup(fail(TDDA_ERR_UNSUPPORTED, "z FFT length too large for the Green set-up pass"));
         }
         const int Q = M[0] * M[1];
@@ -1129,16 +1154,16 @@ int tdda_op_create(tdda_ctx ctx, const tdda_op_desc* desc, tdda_op* out)
     const double scale = 1.0 / (double)Mt;
     const unsigned gb = (unsigned)((Ht + kThreads - 1) / kThreads);
     if (prec & TDDA_C128) {
-        if (hipMalloc(&op->green64, sizeof(Cx<double>) * Ht) != hipSuccess) {
-            (void)hipFree(full);
+        if (ctx_malloc(ctx, (void**)&op->green64, sizeof(Cx<double>) * Ht) != TDDA_OK) {
+            ctx_free(ctx, full);
             return cleanup(fail(TDDA_ERR_OOM, "Green spectrum"));
         }
         k_green_octant<double><<<gb, kThreads, 0, s>>>(full, op->green64, M[0], M[1], M[2], op->Hx, op->Hy, op->Hz,
                                                        scale);
     }
     if (prec & TDDA_C64) {
-        if (hipMalloc(&op->green32, sizeof(Cx<float>) * Ht) != hipSuccess) {
-            (void)hipFree(full);
+        if (ctx_malloc(ctx, (void**)&op->green32, sizeof(Cx<float>) * Ht) != TDDA_OK) {
+            ctx_free(ctx, full);
             return cleanup(fail(TDDA_ERR_OOM, "Green spectrum"));
         }
         k_green_octant<float><<<gb, kThreads, 0, s>>>(full, op->green32, M[0], M[1], M[2], op->Hx, op->Hy, op->Hz,
@@ -1146,7 +1171,7 @@ int tdda_op_create(tdda_ctx ctx, const tdda_op_desc* desc, tdda_op* out)
     }
     hipError_t e = hipGetLastError();
     hipError_t e2 = hipStreamSynchronize(s);
-    (void)hipFree(full);
+    ctx_free(ctx, full);
     if (e != hipSuccess || e2 != hipSuccess)
         return cleanup(fail(TDDA_ERR_HIP, std::string("Green set-up: ") + hipGetErrorString(e != hipSuccess ? e : e2)));
     {
@@ -1163,14 +1188,14 @@ static void op_release(tdda_op op)
     if (--op->refs > 0) return;
     (void)hipSetDevice(op->ctx->device);
     (void)hipStreamSynchronize(op->ctx->stream);
-    (void)hipFree(op->cellmap);
-    (void)hipFree(op->green64);
-    (void)hipFree(op->green32);
+    ctx_free(op->ctx, op->cellmap);
+    ctx_free(op->ctx, op->green64);
+    ctx_free(op->ctx, op->green32);
     for (int a = 0; a < 3; a++) {
-        (void)hipFree(op->tw64[a]);
-        (void)hipFree(op->tw32[a]);
-        (void)hipFree(op->tws64[a]);
-        (void)hipFree(op->tws32[a]);
+        ctx_free(op->ctx, op->tw64[a]);
+        ctx_free(op->ctx, op->tw32[a]);
+        ctx_free(op->ctx, op->tws64[a]);
+        ctx_free(op->ctx, op->tws32[a]);
     }
     tdda_ctx ctx = op->ctx;
     delete op;
@@ -1231,12 +1256,6 @@ int tdda_model_create(tdda_op op, int nrhs, tdda_model* out)
         fill_jobs<float>(m, m->j32.h, m->v32, op->green32, tw, op->tp32, m->v32.r, m->v32.r0);
         if ((rc = finalize_table<float>(m->allocs, op->ctx, m->j32)) != TDDA_OK) return bail(rc);
     }
-    if (hipHostMalloc((void**)&m->h_st, sizeof(SolverState) * 3 * nrhs, hipHostMallocDefault) != hipSuccess ||
-        hipHostMalloc((void**)&m->h_poll, sizeof(SolverState) * 2 * nrhs, hipHostMallocDefault) != hipSuccess)
-        return bail(fail(TDDA_ERR_OOM, "pinned host state"));
-    for (int k = 0; k < 2; k++)
-        if (hipEventCreateWithFlags(&m->ev[k], hipEventDisableTiming) != hipSuccess)
-            return bail(fail(TDDA_ERR_HIP, "event creation"));
     if (hipStreamSynchronize(op->ctx->stream) != hipSuccess) return bail(fail(TDDA_ERR_HIP, "model set-up"));
     *out = m;
     return TDDA_OK;
@@ -1256,13 +1275,13 @@ static void model_release(tdda_model m)
     if (--m->refs > 0) return;
     (void)hipSetDevice(m->op->ctx->device);
     (void)hipStreamSynchronize(m->op->ctx->stream);
-    for (void* p : m->allocs) (void)hipFree(p);
+    for (void* p : m->allocs) ctx_free(m->op->ctx, p);
     if (m->h_st) (void)hipHostFree(m->h_st);
     if (m->h_poll) (void)hipHostFree(m->h_poll);
     for (int k = 0; k < 2; k++)
         if (m->ev[k]) (void)hipEventDestroy(m->ev[k]);
-    if (m->d_idx) (void)hipFree(m->d_idx);
-    if (m->d_vals) (void)hipFree(m->d_vals);
+    ctx_free(m->op->ctx, m->d_idx);
+    ctx_free(m->op->ctx, m->d_vals);
     if (m->h_idx) (void)hipHostFree(m->h_idx);
     if (m->h_vals) (void)hipHostFree(m->h_vals);
     tdda_op op = m->op;
@@ -1324,8 +1343,8 @@ static int ensure_entries(tdda_model m, int n)
 {
     if (n <= m->cap_entries) return TDDA_OK;
     int cap = std::max(n, 2 * m->cap_entries);
-    if (m->d_idx) (void)hipFree(m->d_idx);
-    if (m->d_vals) (void)hipFree(m->d_vals);
+    ctx_free(m->op->ctx, m->d_idx);
+    ctx_free(m->op->ctx, m->d_vals);
     if (m->h_idx) (void)hipHostFree(m->h_idx);
     if (m->h_vals) (void)hipHostFree(m->h_vals);
     m->d_idx = nullptr;
@@ -1333,8 +1352,9 @@ static int ensure_entries(tdda_model m, int n)
     m->h_idx = nullptr;
     m->h_vals = nullptr;
     m->cap_entries = 0;
-    HIPCHK(hipMalloc(&m->d_idx, sizeof(int) * cap));
-    HIPCHK(hipMalloc(&m->d_vals, sizeof(Cx<double>) * cap));
+    if (ctx_malloc(m->op->ctx, (void**)&m->d_idx, sizeof(int) * cap) != TDDA_OK ||
+        ctx_malloc(m->op->ctx, (void**)&m->d_vals, sizeof(Cx<double>) * cap) != TDDA_OK)
+        return fail(TDDA_ERR_OOM, "entry staging");
     HIPCHK(hipHostMalloc((void**)&m->h_idx, sizeof(int) * cap, hipHostMallocDefault));
     HIPCHK(hipHostMalloc((void**)&m->h_vals, sizeof(Cx<double>) * cap, hipHostMallocDefault));
     m->cap_entries = cap;
@@ -1504,6 +1524,22 @@ int tdda_model_solve(tdda_model m, const tdda_solve_opts* opts, tdda_solve_info*
     if (!m || !opts || !info) return fail(TDDA_ERR_ARG, "null argument");
     HIPCHK(hipSetDevice(m->op->ctx->device));
     if (!m->v64.x) return fail(TDDA_ERR_STATE, "solves need an operator built with TDDA_C128");
+    // pinned status mirrors and events on first use (models that only the
+    // request loop drives never need them)
+    if (!m->h_st && hipHostMalloc((void**)&m->h_st, sizeof(SolverState) * 3 * m->nrhs, hipHostMallocDefault) != hipSuccess) {
+        m->h_st = nullptr;
+        return fail(TDDA_ERR_OOM, "pinned host state");
+    }
+    if (!m->h_poll &&
+        hipHostMalloc((void**)&m->h_poll, sizeof(SolverState) * 2 * m->nrhs, hipHostMallocDefault) != hipSuccess) {
+        m->h_poll = nullptr;
+        return fail(TDDA_ERR_OOM, "pinned host state");
+    }
+    for (int k = 0; k < 2; k++)
+        if (!m->ev[k] && hipEventCreateWithFlags(&m->ev[k], hipEventDisableTiming) != hipSuccess) {
+            m->ev[k] = nullptr;
+            return fail(TDDA_ERR_HIP, "event creation");
+        }
     SolveSet S{m->op->ctx->stream, m->nrhs, &m->j64, &m->jo, m->v32.x ? &m->j32 : nullptr, m->h_st, m->h_poll, m->ev};
     return solve_set(S, opts, info);
 }
@@ -1565,7 +1601,7 @@ int tdda_batch_destroy(tdda_batch b)
     if (!b) return TDDA_OK;
     (void)hipSetDevice(b->ctx->device);
     (void)hipStreamSynchronize(b->ctx->stream);
-    for (void* p : b->allocs) (void)hipFree(p);
+    for (void* p : b->allocs) ctx_free(b->ctx, p);
     if (b->h_st) (void)hipHostFree(b->h_st);
     if (b->h_poll) (void)hipHostFree(b->h_poll);
     for (int k = 0; k < 2; k++)
