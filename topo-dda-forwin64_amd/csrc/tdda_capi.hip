@@ -182,9 +182,11 @@ struct tdda_op_s {
     int* cellmap = nullptr;
     Cx<double>* green64 = nullptr;
     Cx<float>* green32 = nullptr;
-    Cx<double>*tw64[3] = {nullptr, nullptr, nullptr};   // full tables (Green set-up)
+    // full tables (Green set-up, compile-time pass kernels)
+    Cx<double>*tw64[3] = {nullptr, nullptr, nullptr};
     Cx<float>*tw32[3] = {nullptr, nullptr, nullptr};
-    Cx<double>*tws64[3] = {nullptr, nullptr, nullptr};  // two-level tables (operator passes)
+    // two-level tables (runtime-plan pass kernels)
+    Cx<double>*tws64[3] = {nullptr, nullptr, nullptr};
     Cx<float>*tws32[3] = {nullptr, nullptr, nullptr};
     FftPlan pl[3];
     TilePlan tp64, tp32;
@@ -295,10 +297,11 @@ int plan_tiles(const tdda_op_s* op, TilePlan& t)
     if (tx >= 3 * unit && tx < 3 * op->Mz) tx = (tx / (3 * unit)) * (3 * unit);
     t.tfx = std::max(3, tx);
     if ((t.tfx / 3) * op->Mx > 8 * kThreads) return fail(TDDA_ERR_UNSUPPORTED, "x pass tile too wide");
-    t.lds_z = (size_t)(tw2_slots(op->Mz, op->pl[2].tb) + 2 * 3 * t.pz * t.ps_z) * s;
-    t.lds_z1 = (size_t)(tw2_slots(op->Mz, op->pl[2].tb) + 2 * 3 * t.pz1 * t.ps_z) * s;
-    t.lds_y = (size_t)(tw2_slots(op->My, op->pl[1].tb) + 2 * t.tfy * t.ps_y) * s;
-    t.lds_x = (size_t)(tw2_slots(op->Mx, op->pl[0].tb) + 2 * t.tfx * t.ps_x) * s;
+    auto tws = [&](int kind, int L, int tb) { return pass_tw_slots(pass_length_compiled<T>(kind, L) && pass_full_twiddles<T>(kind, L), L, tb); };
+    t.lds_z = (size_t)(tws(0, op->Mz, op->pl[2].tb) + 2 * 3 * t.pz * t.ps_z) * s;
+    t.lds_z1 = (size_t)(tws(0, op->Mz, op->pl[2].tb) + 2 * 3 * t.pz1 * t.ps_z) * s;
+    t.lds_y = (size_t)(tws(1, op->My, op->pl[1].tb) + 2 * t.tfy * t.ps_y) * s;
+    t.lds_x = (size_t)(tws(2, op->Mx, op->pl[0].tb) + 2 * t.tfx * t.ps_x) * s;
     if (std::max({t.lds_z, t.lds_z1, t.lds_y, t.lds_x}) > op->ctx->lds_budget)
         return fail(TDDA_ERR_UNSUPPORTED, "pass tile exceeds the LDS budget");
     return TDDA_OK;
@@ -377,6 +380,9 @@ void fill_jobs(const tdda_model_s* m, std::vector<JobDesc<T>>& out, const VecSet
         J.twx = tw[0];
         J.twy = tw[1];
         J.twz = tw[2];
+        J.twfx = tw[3];
+        J.twfy = tw[4];
+        J.twfz = tw[5];
         J.plx = op->pl[0];
         J.ply = op->pl[1];
         J.plz = op->pl[2];
@@ -467,14 +473,15 @@ int build_groups(const tdda_ctx_s* c, JobTable<T>& jt, Put&& put)
         return TDDA_OK;
     };
     auto Lz = [](const JobDesc<T>& J) { return J.Mz; };
+    auto tws = [](int kind, int L, int tb) { return pass_tw_slots(pass_length_compiled<T>(kind, L) && pass_full_twiddles<T>(kind, L), L, tb); };
     CHK(build(jt.gz, 0, Lz, [](const JobTiles& t) { return t.z; },
-              [&](const JobDesc<T>& J) { return (size_t)(tw2_slots(J.Mz, J.plz.tb) + 2 * 3 * J.pz * J.ps_z) * s; }));
+              [&](const JobDesc<T>& J) { return (size_t)(tws(0, J.Mz, J.plz.tb) + 2 * 3 * J.pz * J.ps_z) * s; }));
     CHK(build(jt.gz1, 0, Lz, [](const JobTiles& t) { return t.z1; },
-              [&](const JobDesc<T>& J) { return (size_t)(tw2_slots(J.Mz, J.plz.tb) + 2 * 3 * J.pz1 * J.ps_z) * s; }));
+              [&](const JobDesc<T>& J) { return (size_t)(tws(0, J.Mz, J.plz.tb) + 2 * 3 * J.pz1 * J.ps_z) * s; }));
     CHK(build(jt.gy, 1, [](const JobDesc<T>& J) { return J.My; }, [](const JobTiles& t) { return t.y; },
-              [&](const JobDesc<T>& J) { return (size_t)(tw2_slots(J.My, J.ply.tb) + 2 * J.tfy * J.ps_y) * s; }));
+              [&](const JobDesc<T>& J) { return (size_t)(tws(1, J.My, J.ply.tb) + 2 * J.tfy * J.ps_y) * s; }));
     CHK(build(jt.gx, 2, [](const JobDesc<T>& J) { return J.Mx; }, [](const JobTiles& t) { return t.x; },
-              [&](const JobDesc<T>& J) { return (size_t)(tw2_slots(J.Mx, J.plx.tb) + 2 * J.tfx * J.ps_x) * s; }));
+              [&](const JobDesc<T>& J) { return (size_t)(tws(2, J.Mx, J.plx.tb) + 2 * J.tfx * J.ps_x) * s; }));
     return TDDA_OK;
 }
 
@@ -1243,7 +1250,7 @@ int tdda_model_create(tdda_op op, int nrhs, tdda_model* out)
     int rc;
     if (op->green64) {
         if ((rc = alloc_vecs<double>(m, m->v64)) != TDDA_OK) return bail(rc);
-        Cx<double>* tw[3] = {op->tws64[0], op->tws64[1], op->tws64[2]};
+        Cx<double>* tw[6] = {op->tws64[0], op->tws64[1], op->tws64[2], op->tw64[0], op->tw64[1], op->tw64[2]};
         fill_jobs<double>(m, m->j64.h, m->v64, op->green64, tw, op->tp64, m->v64.r, m->v64.r0);
         // outer residual of the mixed solver: r -> p, its r0 copy -> t (scratch)
         fill_jobs<double>(m, m->jo.h, m->v64, op->green64, tw, op->tp64, m->v64.p, m->v64.t);
@@ -1252,7 +1259,7 @@ int tdda_model_create(tdda_op op, int nrhs, tdda_model* out)
     }
     if (op->green32) {
         if ((rc = alloc_vecs<float>(m, m->v32)) != TDDA_OK) return bail(rc);
-        Cx<float>* tw[3] = {op->tws32[0], op->tws32[1], op->tws32[2]};
+        Cx<float>* tw[6] = {op->tws32[0], op->tws32[1], op->tws32[2], op->tw32[0], op->tw32[1], op->tw32[2]};
         fill_jobs<float>(m, m->j32.h, m->v32, op->green32, tw, op->tp32, m->v32.r, m->v32.r0);
         if ((rc = finalize_table<float>(m->allocs, op->ctx, m->j32)) != TDDA_OK) return bail(rc);
     }

@@ -219,10 +219,12 @@ namespace tdda {
 // LDS transform used by the operator passes.
 //
 // Pencil p, element i lives at p * ps + lpad<T>(i): spare slots spread the
-// strided Stockham writes of the early stages over the LDS banks.  Twiddles come from an LDS copy of a
+// strided Stockham writes of the early stages over the LDS banks.  The
+// runtime-plan transform (pp_fft) takes its twiddles from an LDS copy of a
 // two-level table of the L-th roots of unity (tb + ceil(L / tb) entries,
-// tb ~ sqrt(L)): one table product per butterfly, then powers by recurrence.
-// Index arithmetic uses float reciprocals instead of integer division.
+// tb ~ sqrt(L)): one table product per butterfly, then powers by
+// recurrence; its index arithmetic uses float reciprocals instead of integer
+// division.  The compile-time transforms (ct_fft) read the full table.
 // ---------------------------------------------------------------------------
 // One spare slot every 2^shift elements: 16 for complex128 (16-byte
 // elements, whose strided first-stage writes conflict most), 32 for
@@ -328,9 +330,9 @@ __device__ __forceinline__ Cx<T>* pp_fft(Cx<T>* a, Cx<T>* b, int NP, int ps, con
 }
 
 // Entries of the two-level twiddle table of a plan.
-__host__ __device__ __forceinline__ int tw2_size(int L, int tb) { return tb + (L + tb - 1) / tb; }
+__host__ __device__ constexpr int tw2_size(int L, int tb) { return tb + (L + tb - 1) / tb; }
 // LDS slots reserved for it (even, so the data after it stays 16-byte aligned).
-__host__ __device__ __forceinline__ int tw2_slots(int L, int tb) { return (tw2_size(L, tb) + 1) & ~1; }
+__host__ __device__ constexpr int tw2_slots(int L, int tb) { return (tw2_size(L, tb) + 1) & ~1; }
 
 // Cooperative copy of the two-level twiddle table into LDS.
 template <typename T>
@@ -354,7 +356,8 @@ namespace tdda {
 // Compile-time specialisation of the pass transform for a fixed length L:
 // radix sequence, strides, twiddle steps and pencil stride are constants, so
 // every index division becomes a multiply-shift and the stage loops unroll.
-// Same data layout and twiddle table as pp_fft.
+// Same data layout as pp_fft; twiddles from the full table of L roots, or
+// from the two-level table where the full one would cost occupancy.
 // ---------------------------------------------------------------------------
 constexpr int ct_nst(int L)
 {
@@ -399,7 +402,8 @@ constexpr int ct_rad(int L, int s)
     return 1;
 }
 
-constexpr int ct_tb(int L)
+// base of the two-level twiddle table (make_plan: smallest tb, tb^2 >= L)
+__host__ __device__ constexpr int ct_tb(int L)
 {
     int t = 1;
     while (t * t < L) t++;
@@ -419,6 +423,73 @@ constexpr int ct_zin_limit(int L)
 #define TDDA_STAGE_UNROLL 1
 #endif
 
+// Padded LDS offset of element i + m * STEP.  When STEP is a multiple of the
+// padding group, lpad(i + m STEP) = lpad(i) + m (STEP + STEP / group), so
+// the R accesses of a butterfly share one address and differ by constants.
+template <typename T, unsigned STEP>
+__device__ __forceinline__ unsigned lpad_step(unsigned i, int m)
+{
+    constexpr int SH = pad_shift<T>();
+    if constexpr (STEP % (1u << SH) == 0) return (i + (i >> SH)) + (unsigned)m * (STEP + (STEP >> SH));
+    else {
+        const unsigned x = i + (unsigned)m * STEP;
+        return x + (x >> SH);
+    }
+}
+
+// Twiddles of a butterfly of a compile-time stage from the full LDS table
+// tw[q] = exp(-2 pi i q / L): v[m] *= w^m with w = tw[q], read directly for
+// m = 1, 2 (and 4), the other powers as one product of two table entries.
+template <typename T, int R, bool INV>
+__device__ __forceinline__ void twiddle_direct(Cx<T>* v, const Cx<T>* __restrict__ tw, unsigned q)
+{
+    auto app = [&](int m, Cx<T> w) { v[m] = INV ? cmulc(v[m], w) : cmul(v[m], w); };
+    const Cx<T> w1 = tw[q];
+    app(1, w1);
+    if constexpr (R >= 3) {
+        const Cx<T> w2 = tw[2 * q];
+        app(2, w2);
+        if constexpr (R == 4) {
+            app(3, cmul(w1, w2));
+        } else if constexpr (R == 5) {
+            app(3, cmul(w1, w2));
+            app(4, cmul(w2, w2));
+        } else if constexpr (R == 8) {
+            const Cx<T> w4 = tw[4 * q];
+            const Cx<T> w3 = cmul(w1, w2);
+            app(3, w3);
+            app(4, w4);
+            app(5, cmul(w1, w4));
+            app(6, cmul(w2, w4));
+            app(7, cmul(w3, w4));
+        }
+    }
+}
+
+// LDS slots of a pass's twiddle table: the full table of L roots, or the
+// two-level table (even, so the data after it stays 16-byte aligned).
+__host__ __device__ constexpr int pass_tw_slots(bool full, int L, int tb)
+{
+    return full ? ((L + 1) & ~1) : tw2_slots(L, tb);
+}
+
+// Cooperative copy of a pass's twiddle table into LDS (see pass_tw_slots).
+template <typename T, bool FULL>
+__device__ __forceinline__ void load_pass_twiddles(Cx<T>* __restrict__ dst, const Cx<T>* __restrict__ two_level,
+                                                   const Cx<T>* __restrict__ full, const FftPlan& pl)
+{
+    if constexpr (FULL) {
+        using V = typename std::conditional<sizeof(T) == 4, float __attribute__((ext_vector_type(2))),
+                                            double __attribute__((ext_vector_type(2)))>::type;
+        for (int i = threadIdx.x; i < pl.L; i += blockDim.x) {
+            const V w = *(const __attribute__((address_space(1))) V*)(full + i);
+            dst[i] = Cx<T>{w.x, w.y};
+        }
+    } else {
+        load_twiddles(dst, two_level, pl);
+    }
+}
+
 // Wave-local stage: pencil p belongs to wavefront p % 4, so a stage only
 // reads pencils its own wavefront wrote.  LDS instructions of one wavefront
 // complete in order, so consecutive stages need no workgroup barrier, only
@@ -431,53 +502,59 @@ constexpr int kWaves = kThreadsFft / 64;
 //           and are taken as zero instead of read;
 //   HOUT -- only outputs below N are needed: the last stage's outputs
 //           base + m L/R with m >= ceil(R/2) are neither computed nor stored.
-template <typename T, int L, int R, int NS, bool INV, int NPMAX, bool ZIN = false, bool HOUT = false>
+template <typename T, int L, int R, int NS, bool INV, int NPMAX, bool FULLTW, bool ZIN = false, bool HOUT = false>
 __device__ __forceinline__ void ct_stage(const Cx<T>* __restrict__ src, Cx<T>* __restrict__ dst, int np,
                                          const Cx<T>* __restrict__ tw)
 {
     constexpr int RH = (R + 1) / 2;
-    constexpr int LR = L / R;
-    constexpr int PS = L + (L >> pad_shift<T>()) + 1;
-    constexpr int TWSTEP = L / (NS * R);
-    constexpr int TB = ct_tb(L);
+    constexpr unsigned LR = L / R;
+    constexpr unsigned PS = L + (L >> pad_shift<T>()) + 1;
+    constexpr unsigned TWSTEP = L / (NS * R);
     constexpr int NPW = (NPMAX + kWaves - 1) / kWaves;  // pencils per wavefront, at most
     constexpr int NBMAX = NPW * LR;
-    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int npw = (np - wave + kWaves - 1) / kWaves;  // this wavefront's pencils
-    const int nb = npw * LR;
+    const unsigned wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int npw = (np - (int)wave + kWaves - 1) / kWaves;  // this wavefront's pencils (>= 0)
+    const int nb = npw * (int)LR;
 #pragma unroll TDDA_STAGE_UNROLL
     for (int it = 0; it < (NBMAX + 63) / 64; it++) {
-        const int b = lane + it * 64;
+        const int b = (int)lane + it * 64;
         if (b < nb) {
-            const int pi = b / LR;
-            const int j = b - pi * LR;
-            const int kk = j % NS;
-            const int p = wave + pi * kWaves;
+            const unsigned pi = (unsigned)b / LR;
+            const unsigned j = (unsigned)b - pi * LR;
+            const unsigned kk = j % NS;
+            const unsigned p = wave + pi * kWaves;
             const Cx<T>* sp = src + p * PS;
             Cx<T> v[R];
 #pragma unroll
-            for (int m = 0; m < R; m++) v[m] = (ZIN && m >= RH) ? Cx<T>{T(0), T(0)} : sp[lpad<T>(j + m * LR)];
+            for (int m = 0; m < R; m++)
+                v[m] = (ZIN && m >= RH) ? Cx<T>{T(0), T(0)} : sp[lpad_step<T, LR>(j, m)];
             if constexpr (NS > 1) {
-                const int q = kk * TWSTEP;
-                const Cx<T> w1 = cmul(tw[q % TB], tw[TB + q / TB]);
-                Cx<T> w = w1;
+                if constexpr (FULLTW) {
+                    twiddle_direct<T, R, INV>(v, tw, kk * TWSTEP);
+                } else {
+                    // two-level table, then powers by recurrence
+                    constexpr unsigned TB = ct_tb(L);
+                    const unsigned q = kk * TWSTEP;
+                    const Cx<T> w1 = cmul(tw[q % TB], tw[TB + q / TB]);
+                    Cx<T> w = w1;
 #pragma unroll
-                for (int m = 1; m < R; m++) {
-                    v[m] = INV ? cmulc(v[m], w) : cmul(v[m], w);
-                    if (m + 1 < R) w = cmul(w, w1);
+                    for (int m = 1; m < R; m++) {
+                        v[m] = INV ? cmulc(v[m], w) : cmul(v[m], w);
+                        if (m + 1 < R) w = cmul(w, w1);
+                    }
                 }
             }
             dft<T, R, INV>(v);
             Cx<T>* dp = dst + p * PS;
-            const int base = (j - kk) * R + kk;
+            const unsigned base = (j - kk) * R + kk;
 #pragma unroll
             for (int m = 0; m < R; m++)
-                if (!(HOUT && m >= RH)) dp[lpad<T>(base + m * NS)] = v[m];
+                if (!(HOUT && m >= RH)) dp[lpad_step<T, NS>(base, m)] = v[m];
         }
     }
 }
 
-template <typename T, int L, int S, int NS, bool INV, int NPMAX, int PRUNE>
+template <typename T, int L, int S, int NS, bool INV, int NPMAX, int PRUNE, bool FULLTW>
 __device__ __forceinline__ Cx<T>* ct_stages(Cx<T>* a, Cx<T>* b, int np, const Cx<T>* tw)
 {
     if constexpr (S == ct_nst(L)) {
@@ -486,11 +563,11 @@ __device__ __forceinline__ Cx<T>* ct_stages(Cx<T>* a, Cx<T>* b, int np, const Cx
         constexpr int R = ct_rad(L, S);
         constexpr bool zin = (PRUNE & kPruneIn) && S == 0;
         constexpr bool hout = (PRUNE & kPruneOut) && S + 1 == ct_nst(L);
-        ct_stage<T, L, R, NS, INV, NPMAX, zin, hout>(a, b, np, tw);
+        ct_stage<T, L, R, NS, INV, NPMAX, FULLTW, zin, hout>(a, b, np, tw);
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-        return ct_stages<T, L, S + 1, NS * R, INV, NPMAX, PRUNE>(b, a, np, tw);
+        return ct_stages<T, L, S + 1, NS * R, INV, NPMAX, PRUNE, FULLTW>(b, a, np, tw);
     }
 }
 
@@ -498,11 +575,12 @@ __device__ __forceinline__ Cx<T>* ct_stages(Cx<T>* a, Cx<T>* b, int np, const Cx
 // contract as pp_fft (input visible to all threads, ends with a workgroup
 // barrier).  PRUNE: kPruneIn (input zero at and above N; entries of
 // [N, ct_zin_limit(L)) must still hold zeros) and/or kPruneOut (only
-// outputs below N are valid afterwards).
-template <typename T, int L, bool INV, int NPMAX, int PRUNE = 0>
+// outputs below N are valid afterwards).  FULLTW: tw is the full table of
+// the L-th roots (else the two-level table).
+template <typename T, int L, bool INV, int NPMAX, int PRUNE = 0, bool FULLTW = false>
 __device__ __forceinline__ Cx<T>* ct_fft(Cx<T>* a, Cx<T>* b, int np, const Cx<T>* tw)
 {
-    Cx<T>* r = ct_stages<T, L, 0, 1, INV, NPMAX, PRUNE>(a, b, np, tw);
+    Cx<T>* r = ct_stages<T, L, 0, 1, INV, NPMAX, PRUNE, FULLTW>(a, b, np, tw);
     __syncthreads();
     return r;
 }

@@ -85,9 +85,12 @@ struct JobDesc {
     int ps_z, ps_y, ps_x;  // LDS pencil strides
     const int* cellmap;    // [Nx][Ny][Nz] -> site or -1; nullptr for the full box in bulk order
     const Cx<T>* green;
-    const Cx<T>* twx;
+    const Cx<T>* twx;   // two-level twiddle tables (runtime-plan kernels)
     const Cx<T>* twy;
     const Cx<T>* twz;
+    const Cx<T>* twfx;  // full tables of the L-th roots (compile-time kernels)
+    const Cx<T>* twfy;
+    const Cx<T>* twfz;
     FftPlan plx, ply, plz;
     Cx<T>* w1;
     Cx<T>* w2;
@@ -305,12 +308,61 @@ __host__ __device__ constexpr int np_max(int L)
     return L > 0 ? (kThreads * ept_len<T>(L)) / L : 0;
 }
 
+// LDS slots of the largest data tile (two buffers) the host plans for a
+// compile-time pass kernel of length L (kind 0: z, 1: y, 2: x), as
+// plan_tiles in tdda_capi.hip does with an unclipped third axis.
+template <typename T>
+__host__ __device__ constexpr int max_tile_slots(int kind, int L)
+{
+    const int cap = kThreads * ept_len<T>(L);
+    const int ps = L + (L >> pad_shift<T>()) + 1;
+    const int unit = 128 / (int)sizeof(Cx<T>);
+    if (kind == 0) {
+        const int pz = cap / (3 * L) < 64 ? cap / (3 * L) : 64;
+        return 2 * 3 * pz * ps;
+    }
+    if (kind == 1) {
+        int ty = cap / L;
+        if (ty >= unit) ty = (ty / unit) * unit;
+        return 2 * ty * ps;
+    }
+    int tx = cap / L < 3 * (8 * kThreads / L) ? cap / L : 3 * (8 * kThreads / L);
+    tx = (tx / 3) * 3;
+    if (tx >= 3 * unit) tx = (tx / (3 * unit)) * (3 * unit);
+    return 2 * (tx > 3 ? tx : 3) * ps;
+}
+
+// LDS of one CU on gfx950, and the granule in which a workgroup's LDS is
+// allocated.
+__host__ __device__ constexpr int lds_per_cu() { return 160 * 1024; }
+__host__ __device__ constexpr int lds_granule() { return 512; }
+// Workgroups that fit in one CU's LDS when each takes `bytes`.
+__host__ __device__ constexpr int lds_resident_workgroups(size_t bytes)
+{
+    const size_t g = ((bytes + lds_granule() - 1) / lds_granule()) * lds_granule();
+    return g == 0 ? 1 << 20 : (int)(lds_per_cu() / g);
+}
+
+// Whether a compile-time pass kernel keeps the full table of the L-th roots
+// in LDS (direct twiddle reads, fewer instructions per butterfly) or the
+// two-level table: the full table is used unless it would lower the
+// workgroups resident per CU at the largest tile (512 B allowed for a
+// kernel's static reduction buffer).
+template <typename T>
+__host__ __device__ constexpr bool pass_full_twiddles(int kind, int L)
+{
+    const int data = max_tile_slots<T>(kind, L) * (int)sizeof(Cx<T>) + 512;
+    const int full = pass_tw_slots(true, L, ct_tb(L)) * (int)sizeof(Cx<T>);
+    const int two = pass_tw_slots(false, L, ct_tb(L)) * (int)sizeof(Cx<T>);
+    return lds_resident_workgroups(data + full) >= lds_resident_workgroups(data + two);
+}
+
 // Transform of a pass tile: compile-time length LC > 0, or the runtime plan.
-template <typename T, int LC, bool INV, int PRUNE = 0>
+template <typename T, int LC, bool INV, int PRUNE = 0, bool FULLTW = false>
 __device__ __forceinline__ Cx<T>* pass_fft(Cx<T>* a, Cx<T>* b, int np, int ps, const FftPlan& pl, const Cx<T>* tw)
 {
     if constexpr (LC > 0) {
-        return ct_fft<T, LC, INV, np_max<T>(LC), PRUNE>(a, b, np, tw);
+        return ct_fft<T, LC, INV, np_max<T>(LC), PRUNE, FULLTW>(a, b, np, tw);
     } else {
         return pp_fft<T, INV>(a, b, np, ps, pl, tw);
     }
@@ -376,6 +428,7 @@ __global__ void TDDA_PASS_BOUNDS k_p1(const JobDesc<T>* __restrict__ jobs, const
                                       int njobs)
 {
     extern __shared__ __align__(16) char smem[];
+    constexpr bool FT = LC > 0 && pass_full_twiddles<T>(0, LC);
     constexpr int NV = (MODE == IN_P) ? 3 : ((MODE == IN_T) ? 2 : 1);
     constexpr int EP = ept_len<T>(LC) / 3;
     const int ntot = tstart[njobs];
@@ -390,8 +443,8 @@ __global__ void TDDA_PASS_BOUNDS k_p1(const JobDesc<T>* __restrict__ jobs, const
             const JobDesc<T>* J = jobs + cur.job;
             const int Nz = J->Nz, Mz = J->Mz, ps = J->ps_z;
             Cx<T>* twl = reinterpret_cast<Cx<T>*>(smem);
-            Cx<T>* A = twl + tw2_slots(Mz, J->plz.tb);
-            load_twiddles(twl, J->twz, J->plz);
+            Cx<T>* A = twl + pass_tw_slots(FT, Mz, J->plz.tb);
+            load_pass_twiddles<T, FT>(twl, J->twz, J->twfz, J->plz);
             Cx<T> sa{T(0), T(0)}, sb{T(0), T(0)};
             bool first = false;
             if (MODE == IN_P) {
@@ -497,10 +550,10 @@ __global__ void TDDA_PASS_BOUNDS k_p1(const JobDesc<T>* __restrict__ jobs, const
             const JobDesc<T>* J = jobs + cur.job;
             const int Mz = J->Mz, ps = J->ps_z;
             Cx<T>* twl = reinterpret_cast<Cx<T>*>(smem);
-            Cx<T>* A = twl + tw2_slots(Mz, J->plz.tb);
+            Cx<T>* A = twl + pass_tw_slots(FT, Mz, J->plz.tb);
             Cx<T>* B = A + 3 * J->pz1 * ps;
             __syncthreads();
-            const Cx<T>* res = pass_fft<T, LC, false, kPruneIn>(A, B, cur.np * 3, ps, J->plz, twl);
+            const Cx<T>* res = pass_fft<T, LC, false, kPruneIn, FT>(A, B, cur.np * 3, ps, J->plz, twl);
             const int nout = cur.np * Mz * 3;
             Cx<T>* w1 = J->w1 + (size_t)cur.q0 * Mz * 3;
             const float rMz3 = 1.0f / (float)(Mz * 3);
@@ -524,6 +577,7 @@ __global__ void TDDA_PASS_BOUNDS k_py(const JobDesc<T>* __restrict__ jobs, const
                                       int njobs, int solver)
 {
     extern __shared__ __align__(16) char smem[];
+    constexpr bool FT = LC > 0 && pass_full_twiddles<T>(1, LC);
     constexpr int EP = ept_len<T>(LC);
     const int ntot = tstart[njobs];
     Pre<T, EP> pr;
@@ -535,8 +589,8 @@ __global__ void TDDA_PASS_BOUNDS k_py(const JobDesc<T>* __restrict__ jobs, const
             const JobDesc<T>* J = jobs + cur.job;
             const int Ny = J->Ny, My = J->My, ps = J->ps_y, nf = cur.nf;
             Cx<T>* twl = reinterpret_cast<Cx<T>*>(smem);
-            Cx<T>* A = twl + tw2_slots(My, J->ply.tb);
-            load_twiddles(twl, J->twy, J->ply);
+            Cx<T>* A = twl + pass_tw_slots(FT, My, J->ply.tb);
+            load_pass_twiddles<T, FT>(twl, J->twy, J->twfy, J->ply);
             const int nin = INV ? My : Ny;
             const float rnf = 1.0f / (float)nf;
 #pragma unroll
@@ -582,10 +636,10 @@ __global__ void TDDA_PASS_BOUNDS k_py(const JobDesc<T>* __restrict__ jobs, const
             const JobDesc<T>* J = jobs + cur.job;
             const int Ny = J->Ny, My = J->My, ps = J->ps_y, nf = cur.nf, F = J->Mz * 3;
             Cx<T>* twl = reinterpret_cast<Cx<T>*>(smem);
-            Cx<T>* A = twl + tw2_slots(My, J->ply.tb);
+            Cx<T>* A = twl + pass_tw_slots(FT, My, J->ply.tb);
             Cx<T>* B = A + J->tfy * ps;
             __syncthreads();
-            const Cx<T>* res = pass_fft<T, LC, INV, INV ? kPruneOut : kPruneIn>(A, B, nf, ps, J->ply, twl);
+            const Cx<T>* res = pass_fft<T, LC, INV, INV ? kPruneOut : kPruneIn, FT>(A, B, nf, ps, J->ply, twl);
             const int nout = INV ? Ny : My;
             Cx<T>* dst =
                 (INV ? J->w1 + (size_t)cur.outer * Ny * F : J->w2 + (size_t)cur.outer * My * F) + cur.f0;
@@ -609,6 +663,7 @@ __global__ void TDDA_PASS_BOUNDS k_px(const JobDesc<T>* __restrict__ jobs, const
                                       int njobs, int solver)
 {
     extern __shared__ __align__(16) char smem[];
+    constexpr bool FT = LC > 0 && pass_full_twiddles<T>(2, LC);
     constexpr int EP = ept_len<T>(LC);
     const int ntot = tstart[njobs];
     Pre<T, EP> pr;
@@ -620,8 +675,8 @@ __global__ void TDDA_PASS_BOUNDS k_px(const JobDesc<T>* __restrict__ jobs, const
             const JobDesc<T>* J = jobs + cur.job;
             const int Nx = J->Nx, Mx = J->Mx, ps = J->ps_x, nf = cur.nf;
             Cx<T>* twl = reinterpret_cast<Cx<T>*>(smem);
-            Cx<T>* A = twl + tw2_slots(Mx, J->plx.tb);
-            load_twiddles(twl, J->twx, J->plx);
+            Cx<T>* A = twl + pass_tw_slots(FT, Mx, J->plx.tb);
+            load_pass_twiddles<T, FT>(twl, J->twx, J->twfx, J->plx);
             const float rnf = 1.0f / (float)nf;
 #pragma unroll
             for (int k = 0; k < EP; k++) {
@@ -664,10 +719,10 @@ __global__ void TDDA_PASS_BOUNDS k_px(const JobDesc<T>* __restrict__ jobs, const
             const int Nx = J->Nx, Mx = J->Mx, My = J->My, Mz = J->Mz, F = Mz * 3;
             const int ps = J->ps_x, nf = cur.nf, ky = cur.outer, f0 = cur.f0;
             Cx<T>* twl = reinterpret_cast<Cx<T>*>(smem);
-            Cx<T>* A = twl + tw2_slots(Mx, J->plx.tb);
+            Cx<T>* A = twl + pass_tw_slots(FT, Mx, J->plx.tb);
             Cx<T>* B = A + J->tfx * ps;
             __syncthreads();
-            Cx<T>* res = pass_fft<T, LC, false, kPruneIn>(A, B, nf, ps, J->plx, twl);
+            Cx<T>* res = pass_fft<T, LC, false, kPruneIn, FT>(A, B, nf, ps, J->plx, twl);
             Cx<T>* other = (res == A) ? B : A;
             // Green product.  Octant fold per axis: k' = min(k, M - k); odd
             // components flip sign when k was folded.
@@ -696,7 +751,7 @@ __global__ void TDDA_PASS_BOUNDS k_px(const JobDesc<T>* __restrict__ jobs, const
                 v[2 * ps] = cadd(cadd(cmul(gxz, bx), cmul(gyz, by)), cmul(gzz, bz));
             }
             __syncthreads();
-            const Cx<T>* res2 = pass_fft<T, LC, true, kPruneOut>(res, other, nf, ps, J->plx, twl);
+            const Cx<T>* res2 = pass_fft<T, LC, true, kPruneOut, FT>(res, other, nf, ps, J->plx, twl);
             const int sx = My * F;
             Cx<T>* base = J->w2 + (size_t)ky * F + f0;
             const float rnf = 1.0f / (float)nf;
@@ -719,6 +774,7 @@ __global__ void TDDA_PASS_BOUNDS k_p5(const JobDesc<T>* __restrict__ jobs, const
                                       int njobs, int diag)
 {
     extern __shared__ __align__(16) char smem[];
+    constexpr bool FT = LC > 0 && pass_full_twiddles<T>(0, LC);
     __shared__ double red[16 * 4];
     constexpr int EP = ept_len<T>(LC);
     const int ntot = tstart[njobs];
@@ -732,8 +788,8 @@ __global__ void TDDA_PASS_BOUNDS k_p5(const JobDesc<T>* __restrict__ jobs, const
             const JobDesc<T>* J = jobs + cur.job;
             const int Mz = J->Mz, ps = J->ps_z;
             Cx<T>* twl = reinterpret_cast<Cx<T>*>(smem);
-            Cx<T>* A = twl + tw2_slots(Mz, J->plz.tb);
-            load_twiddles(twl, J->twz, J->plz);
+            Cx<T>* A = twl + pass_tw_slots(FT, Mz, J->plz.tb);
+            load_pass_twiddles<T, FT>(twl, J->twz, J->twfz, J->plz);
             const float rMz3 = 1.0f / (float)(Mz * 3);
 #pragma unroll
             for (int k = 0; k < EP; k++) {
@@ -765,10 +821,10 @@ __global__ void TDDA_PASS_BOUNDS k_p5(const JobDesc<T>* __restrict__ jobs, const
             const JobDesc<T>* J = jobs + cur.job;
             const int Nz = J->Nz, Mz = J->Mz, ps = J->ps_z;
             Cx<T>* twl = reinterpret_cast<Cx<T>*>(smem);
-            Cx<T>* A = twl + tw2_slots(Mz, J->plz.tb);
+            Cx<T>* A = twl + pass_tw_slots(FT, Mz, J->plz.tb);
             Cx<T>* B = A + 3 * J->pz * ps;
             __syncthreads();
-            const Cx<T>* res = pass_fft<T, LC, true, kPruneOut>(A, B, cur.np * 3, ps, J->plz, twl);
+            const Cx<T>* res = pass_fft<T, LC, true, kPruneOut, FT>(A, B, cur.np * 3, ps, J->plz, twl);
             double acc[4] = {0.0, 0.0, 0.0, 0.0};
             const int nout = cur.np * Nz * 3;
             const int base = cur.q0 * Nz;
