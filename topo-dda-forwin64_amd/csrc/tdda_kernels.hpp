@@ -301,6 +301,34 @@ struct Pre {
     }
 };
 
+// Walk of one thread over the elements e = threadIdx.x + k kThreads of a
+// tile viewed as rows of n columns (row = e / n, col = e % n): one division
+// per tile, then an add with carry per step instead of a division per
+// element.
+struct RowWalk {
+    int row, col, drow, dcol, n;
+    __device__ __forceinline__ RowWalk(int n_, float rn) : n(n_)
+    {
+        fdivmod((int)threadIdx.x, n_, rn, row, col);
+        fdivmod(kThreads, n_, rn, drow, dcol);
+    }
+    __device__ __forceinline__ void next()
+    {
+        col += dcol;
+        row += drow;
+        const bool wrap = col >= n;
+        col = wrap ? col - n : col;
+        row = wrap ? row + 1 : row;
+    }
+};
+
+// (z, c) of a flattened (z, component) column index
+__device__ __forceinline__ void split3(int zc, int& z, int& c)
+{
+    z = (int)((unsigned)zc / 3u);
+    c = zc - 3 * z;
+}
+
 // Largest pencil count of a tile of length L (compile-time lengths).
 template <typename T>
 __host__ __device__ constexpr int np_max(int L)
@@ -432,7 +460,6 @@ __global__ void TDDA_PASS_BOUNDS k_p1(const JobDesc<T>* __restrict__ jobs, const
     constexpr int NV = (MODE == IN_P) ? 3 : ((MODE == IN_T) ? 2 : 1);
     constexpr int EP = ept_len<T>(LC) / 3;
     const int ntot = tstart[njobs];
-    const float r3 = 1.0f / 3.0f;
     Pre<T, EP> pr[NV];
     ZTile cur;
     cur.live = false;
@@ -463,14 +490,14 @@ __global__ void TDDA_PASS_BOUNDS k_p1(const JobDesc<T>* __restrict__ jobs, const
             }
             const int nin = cur.np * Nz * 3;
             const int base = cur.q0 * Nz;
-            const float rNz3 = 1.0f / (float)(Nz * 3);
+            RowWalk w(Nz * 3, 1.0f / (float)(Nz * 3));
 #pragma unroll
-            for (int k = 0; k < EP; k++) {
+            for (int k = 0; k < EP; k++, w.next()) {
                 const int e = threadIdx.x + k * kThreads;
                 if (e >= nin) continue;
-                int pl, rem, z, c;
-                fdivmod(e, Nz * 3, rNz3, pl, rem);
-                fdivmod(rem, 3, r3, z, c);
+                const int pl = w.row;
+                int z, c;
+                split3(w.col, z, c);
                 Cx<T> v = pr[0].get(k);
                 if (MODE == IN_P || MODE == IN_T) {
                     int idx = -1;
@@ -494,12 +521,8 @@ __global__ void TDDA_PASS_BOUNDS k_p1(const JobDesc<T>* __restrict__ jobs, const
             }
             const int npad = zero_fill_end<LC>(Mz) - Nz;
             if (npad > 0) {
-                const float rpad = 1.0f / (float)npad;
-                for (int e = threadIdx.x; e < cur.np * 3 * npad; e += kThreads) {
-                    int pc, zz;
-                    fdivmod(e, npad, rpad, pc, zz);
-                    A[pc * ps + lpad<T>(Nz + zz)] = Cx<T>{T(0), T(0)};
-                }
+                for (RowWalk w(npad, 1.0f / (float)npad); w.row < cur.np * 3; w.next())
+                    A[w.row * ps + lpad<T>(Nz + w.col)] = Cx<T>{T(0), T(0)};
             }
         }
         // 2. prefetch of the next tile
@@ -521,7 +544,7 @@ __global__ void TDDA_PASS_BOUNDS k_p1(const JobDesc<T>* __restrict__ jobs, const
                     if (J->cellmap) {
                         int pl, rem, z, c;
                         fdivmod(e, Nz * 3, rNz3, pl, rem);
-                        fdivmod(rem, 3, r3, z, c);
+                        split3(rem, z, c);
                         const int site = gldi(J->cellmap + base + pl * Nz + z);
                         idx = site >= 0 ? site * 3 + c : -1;
                     } else {
@@ -556,12 +579,11 @@ __global__ void TDDA_PASS_BOUNDS k_p1(const JobDesc<T>* __restrict__ jobs, const
             const Cx<T>* res = pass_fft<T, LC, false, kPruneIn, FT>(A, B, cur.np * 3, ps, J->plz, twl);
             const int nout = cur.np * Mz * 3;
             Cx<T>* w1 = J->w1 + (size_t)cur.q0 * Mz * 3;
-            const float rMz3 = 1.0f / (float)(Mz * 3);
-            for (int e = threadIdx.x; e < nout; e += kThreads) {
-                int pl, rem, kz, c;
-                fdivmod(e, Mz * 3, rMz3, pl, rem);
-                fdivmod(rem, 3, r3, kz, c);
-                gst(w1 + e, res[(pl * 3 + c) * ps + lpad<T>(kz)]);
+            RowWalk w(Mz * 3, 1.0f / (float)(Mz * 3));
+            for (int e = threadIdx.x; e < nout; e += kThreads, w.next()) {
+                int kz, c;
+                split3(w.col, kz, c);
+                gst(w1 + e, res[(w.row * 3 + c) * ps + lpad<T>(kz)]);
             }
             __syncthreads();
         }
@@ -593,21 +615,13 @@ __global__ void TDDA_PASS_BOUNDS k_py(const JobDesc<T>* __restrict__ jobs, const
             load_pass_twiddles<T, FT>(twl, J->twy, J->twfy, J->ply);
             const int nin = INV ? My : Ny;
             const float rnf = 1.0f / (float)nf;
+            RowWalk w(nf, rnf);
 #pragma unroll
-            for (int k = 0; k < EP; k++) {
-                const int e = threadIdx.x + k * kThreads;
-                if (e < nin * nf) {
-                    int y, f;
-                    fdivmod(e, nf, rnf, y, f);
-                    A[f * ps + lpad<T>(y)] = pr.get(k);
-                }
-            }
+            for (int k = 0; k < EP; k++, w.next())
+                if (w.row < nin) A[w.col * ps + lpad<T>(w.row)] = pr.get(k);
             if (!INV) {
-                for (int e = threadIdx.x; e < (zero_fill_end<LC>(My) - Ny) * nf; e += kThreads) {
-                    int y, f;
-                    fdivmod(e, nf, rnf, y, f);
-                    A[f * ps + lpad<T>(Ny + y)] = Cx<T>{T(0), T(0)};
-                }
+                for (RowWalk z(nf, rnf); z.row < zero_fill_end<LC>(My) - Ny; z.next())
+                    A[z.col * ps + lpad<T>(Ny + z.row)] = Cx<T>{T(0), T(0)};
             }
         }
         STile nxt;
@@ -620,16 +634,10 @@ __global__ void TDDA_PASS_BOUNDS k_py(const JobDesc<T>* __restrict__ jobs, const
                 const int nin = INV ? J->My : J->Ny;
                 const Cx<T>* src =
                     (INV ? J->w2 + (size_t)nxt.outer * J->My * F : J->w1 + (size_t)nxt.outer * J->Ny * F) + nxt.f0;
-                const float rnf = 1.0f / (float)nf;
+                RowWalk w(nf, 1.0f / (float)nf);
 #pragma unroll
-                for (int k = 0; k < EP; k++) {
-                    const int e = threadIdx.x + k * kThreads;
-                    if (e < nin * nf) {
-                        int y, f;
-                        fdivmod(e, nf, rnf, y, f);
-                        pr.set(k, gld(src + (y * F + f)));
-                    }
-                }
+                for (int k = 0; k < EP; k++, w.next())
+                    if (w.row < nin) pr.set(k, gld(src + (w.row * F + w.col)));
             }
         }
         if (cur.live) {
@@ -643,12 +651,8 @@ __global__ void TDDA_PASS_BOUNDS k_py(const JobDesc<T>* __restrict__ jobs, const
             const int nout = INV ? Ny : My;
             Cx<T>* dst =
                 (INV ? J->w1 + (size_t)cur.outer * Ny * F : J->w2 + (size_t)cur.outer * My * F) + cur.f0;
-            const float rnf = 1.0f / (float)nf;
-            for (int e = threadIdx.x; e < nout * nf; e += kThreads) {
-                int y, f;
-                fdivmod(e, nf, rnf, y, f);
-                gst(dst + (y * F + f), res[f * ps + lpad<T>(y)]);
-            }
+            for (RowWalk w(nf, 1.0f / (float)nf); w.row < nout; w.next())
+                gst(dst + (w.row * F + w.col), res[w.col * ps + lpad<T>(w.row)]);
             __syncthreads();
         }
         if (tl >= ntot) break;
@@ -678,20 +682,12 @@ __global__ void TDDA_PASS_BOUNDS k_px(const JobDesc<T>* __restrict__ jobs, const
             Cx<T>* A = twl + pass_tw_slots(FT, Mx, J->plx.tb);
             load_pass_twiddles<T, FT>(twl, J->twx, J->twfx, J->plx);
             const float rnf = 1.0f / (float)nf;
+            RowWalk w(nf, rnf);
 #pragma unroll
-            for (int k = 0; k < EP; k++) {
-                const int e = threadIdx.x + k * kThreads;
-                if (e < Nx * nf) {
-                    int x, f;
-                    fdivmod(e, nf, rnf, x, f);
-                    A[f * ps + lpad<T>(x)] = pr.get(k);
-                }
-            }
-            for (int e = threadIdx.x; e < (zero_fill_end<LC>(Mx) - Nx) * nf; e += kThreads) {
-                int x, f;
-                fdivmod(e, nf, rnf, x, f);
-                A[f * ps + lpad<T>(Nx + x)] = Cx<T>{T(0), T(0)};
-            }
+            for (int k = 0; k < EP; k++, w.next())
+                if (w.row < Nx) A[w.col * ps + lpad<T>(w.row)] = pr.get(k);
+            for (RowWalk z(nf, rnf); z.row < zero_fill_end<LC>(Mx) - Nx; z.next())
+                A[z.col * ps + lpad<T>(Nx + z.row)] = Cx<T>{T(0), T(0)};
         }
         STile nxt;
         nxt.live = false;
@@ -702,16 +698,11 @@ __global__ void TDDA_PASS_BOUNDS k_px(const JobDesc<T>* __restrict__ jobs, const
                 const int F = J->Mz * 3, nf = nxt.nf;
                 const int sx = J->My * F;  // one slab row: < 2^31 elements
                 const Cx<T>* base = J->w2 + (size_t)nxt.outer * F + nxt.f0;
-                const float rnf = 1.0f / (float)nf;
+                const int Nx = J->Nx;
+                RowWalk w(nf, 1.0f / (float)nf);
 #pragma unroll
-                for (int k = 0; k < EP; k++) {
-                    const int e = threadIdx.x + k * kThreads;
-                    if (e < J->Nx * nf) {
-                        int x, f;
-                        fdivmod(e, nf, rnf, x, f);
-                        pr.set(k, gld(base + (x * sx + f)));
-                    }
-                }
+                for (int k = 0; k < EP; k++, w.next())
+                    if (w.row < Nx) pr.set(k, gld(base + (w.row * sx + w.col)));
             }
         }
         if (cur.live) {
@@ -730,10 +721,8 @@ __global__ void TDDA_PASS_BOUNDS k_px(const JobDesc<T>* __restrict__ jobs, const
             const T sy = (ky > My / 2) ? T(-1) : T(1);
             const int nkz = nf / 3;
             const int Hy = J->Hy, Hz = J->Hz;
-            const float rnkz = 1.0f / (float)nkz;
-            for (int e = threadIdx.x; e < Mx * nkz; e += kThreads) {
-                int kx, kl;
-                fdivmod(e, nkz, rnkz, kx, kl);
+            for (RowWalk w(nkz, 1.0f / (float)nkz); w.row < Mx; w.next()) {
+                const int kx = w.row, kl = w.col;
                 const int kz = f0 / 3 + kl;
                 const int kxf = (kx > Mx / 2) ? Mx - kx : kx;
                 const T sxv = (kx > Mx / 2) ? T(-1) : T(1);
@@ -754,12 +743,8 @@ __global__ void TDDA_PASS_BOUNDS k_px(const JobDesc<T>* __restrict__ jobs, const
             const Cx<T>* res2 = pass_fft<T, LC, true, kPruneOut, FT>(res, other, nf, ps, J->plx, twl);
             const int sx = My * F;
             Cx<T>* base = J->w2 + (size_t)ky * F + f0;
-            const float rnf = 1.0f / (float)nf;
-            for (int e = threadIdx.x; e < Nx * nf; e += kThreads) {
-                int x, f;
-                fdivmod(e, nf, rnf, x, f);
-                gst(base + (x * sx + f), res2[f * ps + lpad<T>(x)]);
-            }
+            for (RowWalk w(nf, 1.0f / (float)nf); w.row < Nx; w.next())
+                gst(base + (w.row * sx + w.col), res2[w.col * ps + lpad<T>(w.row)]);
             __syncthreads();
         }
         if (tl >= ntot) break;
@@ -778,7 +763,6 @@ __global__ void TDDA_PASS_BOUNDS k_p5(const JobDesc<T>* __restrict__ jobs, const
     __shared__ double red[16 * 4];
     constexpr int EP = ept_len<T>(LC);
     const int ntot = tstart[njobs];
-    const float r3 = 1.0f / 3.0f;
     Pre<T, EP> pr;
     ZTile cur;
     cur.live = false;
@@ -790,15 +774,13 @@ __global__ void TDDA_PASS_BOUNDS k_p5(const JobDesc<T>* __restrict__ jobs, const
             Cx<T>* twl = reinterpret_cast<Cx<T>*>(smem);
             Cx<T>* A = twl + pass_tw_slots(FT, Mz, J->plz.tb);
             load_pass_twiddles<T, FT>(twl, J->twz, J->twfz, J->plz);
-            const float rMz3 = 1.0f / (float)(Mz * 3);
+            RowWalk w(Mz * 3, 1.0f / (float)(Mz * 3));
 #pragma unroll
-            for (int k = 0; k < EP; k++) {
-                const int e = threadIdx.x + k * kThreads;
-                if (e < cur.np * Mz * 3) {
-                    int pl, rem, kz, c;
-                    fdivmod(e, Mz * 3, rMz3, pl, rem);
-                    fdivmod(rem, 3, r3, kz, c);
-                    A[(pl * 3 + c) * ps + lpad<T>(kz)] = pr.get(k);
+            for (int k = 0; k < EP; k++, w.next()) {
+                if (w.row < cur.np) {
+                    int kz, c;
+                    split3(w.col, kz, c);
+                    A[(w.row * 3 + c) * ps + lpad<T>(kz)] = pr.get(k);
                 }
             }
         }
@@ -828,11 +810,11 @@ __global__ void TDDA_PASS_BOUNDS k_p5(const JobDesc<T>* __restrict__ jobs, const
             double acc[4] = {0.0, 0.0, 0.0, 0.0};
             const int nout = cur.np * Nz * 3;
             const int base = cur.q0 * Nz;
-            const float rNz3 = 1.0f / (float)(Nz * 3);
-            for (int e = threadIdx.x; e < nout; e += kThreads) {
-                int pl, rem, z, c;
-                fdivmod(e, Nz * 3, rNz3, pl, rem);
-                fdivmod(rem, 3, r3, z, c);
+            RowWalk w(Nz * 3, 1.0f / (float)(Nz * 3));
+            for (int e = threadIdx.x; e < nout; e += kThreads, w.next()) {
+                const int pl = w.row;
+                int z, c;
+                split3(w.col, z, c);
                 int i;
                 if (J->cellmap) {
                     const int site = gldi(J->cellmap + base + pl * Nz + z);
