@@ -262,6 +262,14 @@ struct tdda_batch_s {
 
 namespace {
 
+// LDS buffers of an x-pass tile of length L: one for the in-place
+// compile-time kernels, two otherwise.
+template <typename T>
+int px_buffers(int L)
+{
+    return pass_length_compiled<T>(2, L) && px_inplace<T>(L) ? 1 : 2;
+}
+
 // Tile widths of the passes for element type T: a tile holds at most
 // `cap` elements (pencils x length, two LDS buffers of it), rows of the
 // strided passes are kept to whole 128-byte lines where the budget allows,
@@ -273,7 +281,9 @@ int plan_tiles(const tdda_op_s* op, TilePlan& t)
     // prefetch capacity of a workgroup in each pass (elements), as compiled
     // into the pass kernel that will run: ept_len of its FFT length
     auto full = [&](int kind, int L) { return kThreads * ept_len<T>(pass_length_compiled<T>(kind, L) ? L : 0); };
-    const int fz = full(0, op->Mz), fy = full(1, op->My), fx = full(2, op->Mx);
+    const bool ipx = px_buffers<T>(op->Mx) == 1;  // in-place x pass: loaded rows only
+    const int fz = full(0, op->Mz), fy = full(1, op->My);
+    const int fx = ipx ? kThreads * px_ept<T>(op->Mx) : full(2, op->Mx);
     const int env = env_int(sizeof(T) == 8 ? "TDDA_TILE_C128" : "TDDA_TILE_C64", 1 << 30);
     const int capz = std::min(fz, env), capy = std::min(fy, env), capx = std::min(fx, env);
     const int cap1 = fz / 3;  // P1: up to three input vectors
@@ -292,16 +302,22 @@ int plan_tiles(const tdda_op_s* op, TilePlan& t)
     int ty = std::min(capy / op->My, 3 * op->Mz);
     if (ty >= unit && ty < 3 * op->Mz) ty = (ty / unit) * unit;
     t.tfy = std::max(1, ty);
-    int tx = std::min({capx / op->Mx, 3 * op->Mz, 3 * (8 * kThreads / op->Mx)});
-    tx = (tx / 3) * 3;
-    if (tx >= 3 * unit && tx < 3 * op->Mz) tx = (tx / (3 * unit)) * (3 * unit);
+    int tx;
+    if (ipx) {
+        tx = std::min({kPxInplacePencils, 3 * op->Mz, capx / op->Nx, 3 * (8 * kThreads / op->Mx)});
+        tx = (tx / 3) * 3;
+    } else {
+        tx = std::min({capx / op->Mx, 3 * op->Mz, 3 * (8 * kThreads / op->Mx)});
+        tx = (tx / 3) * 3;
+        if (tx >= 3 * unit && tx < 3 * op->Mz) tx = (tx / (3 * unit)) * (3 * unit);
+    }
     t.tfx = std::max(3, tx);
     if ((t.tfx / 3) * op->Mx > 8 * kThreads) return fail(TDDA_ERR_UNSUPPORTED, "x pass tile too wide");
     auto tws = [&](int kind, int L, int tb) { return pass_tw_slots(pass_length_compiled<T>(kind, L) && pass_full_twiddles<T>(kind, L), L, tb); };
     t.lds_z = (size_t)(tws(0, op->Mz, op->pl[2].tb) + 2 * 3 * t.pz * t.ps_z) * s;
     t.lds_z1 = (size_t)(tws(0, op->Mz, op->pl[2].tb) + 2 * 3 * t.pz1 * t.ps_z) * s;
     t.lds_y = (size_t)(tws(1, op->My, op->pl[1].tb) + 2 * t.tfy * t.ps_y) * s;
-    t.lds_x = (size_t)(tws(2, op->Mx, op->pl[0].tb) + 2 * t.tfx * t.ps_x) * s;
+    t.lds_x = (size_t)(tws(2, op->Mx, op->pl[0].tb) + px_buffers<T>(op->Mx) * t.tfx * t.ps_x) * s;
     if (std::max({t.lds_z, t.lds_z1, t.lds_y, t.lds_x}) > op->ctx->lds_budget)
         return fail(TDDA_ERR_UNSUPPORTED, "pass tile exceeds the LDS budget");
     return TDDA_OK;
@@ -481,7 +497,7 @@ int build_groups(const tdda_ctx_s* c, JobTable<T>& jt, Put&& put)
     CHK(build(jt.gy, 1, [](const JobDesc<T>& J) { return J.My; }, [](const JobTiles& t) { return t.y; },
               [&](const JobDesc<T>& J) { return (size_t)(tws(1, J.My, J.ply.tb) + 2 * J.tfy * J.ps_y) * s; }));
     CHK(build(jt.gx, 2, [](const JobDesc<T>& J) { return J.Mx; }, [](const JobTiles& t) { return t.x; },
-              [&](const JobDesc<T>& J) { return (size_t)(tws(2, J.Mx, J.plx.tb) + 2 * J.tfx * J.ps_x) * s; }));
+              [&](const JobDesc<T>& J) { return (size_t)(tws(2, J.Mx, J.plx.tb) + px_buffers<T>(J.Mx) * J.tfx * J.ps_x) * s; }));
     return TDDA_OK;
 }
 

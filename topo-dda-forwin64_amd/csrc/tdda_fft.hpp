@@ -571,6 +571,121 @@ __device__ __forceinline__ Cx<T>* ct_stages(Cx<T>* a, Cx<T>* b, int np, const Cx
     }
 }
 
+// In-place variant of ct_stage: a wavefront first reads every input of its
+// butterflies into registers, then writes every output back into the same
+// buffer.  LDS instructions of one wavefront complete in order, so no write
+// can overtake a read of the same stage; the pencils of a wavefront are
+// touched by no other wavefront.  Registers: NIT x R complex values per lane.
+template <typename T, int L, int R, int NS, bool INV, int NPMAX, bool FULLTW, bool ZIN = false, bool HOUT = false>
+__device__ __forceinline__ void ct_stage_ip(Cx<T>* buf, int np, const Cx<T>* __restrict__ tw)
+{
+    constexpr int RH = (R + 1) / 2;
+    constexpr unsigned LR = L / R;
+    constexpr unsigned PS = L + (L >> pad_shift<T>()) + 1;
+    constexpr unsigned TWSTEP = L / (NS * R);
+    constexpr int NPW = (NPMAX + kWaves - 1) / kWaves;
+    // The last stage (NS R = L) writes each butterfly's outputs to the slots
+    // it read, so its butterflies can run one at a time.
+    constexpr bool LAST = NS * R == L;
+    constexpr int NIT = LAST ? 1 : (NPW * (int)LR + 63) / 64;
+    constexpr int NB = LAST ? (NPW * (int)LR + 63) / 64 : 1;
+    const unsigned wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int npw = (np - (int)wave + kWaves - 1) / kWaves;
+    const int nb = npw * (int)LR;
+#pragma unroll 1
+    for (int ob = 0; ob < NB; ob++) {
+        Cx<T> v[NIT][R];
+#pragma unroll
+        for (int it = 0; it < NIT; it++) {
+            const int b = (int)lane + (it + ob) * 64;
+            if (b < nb) {
+                const unsigned pi = (unsigned)b / LR;
+                const unsigned j = (unsigned)b - pi * LR;
+                const Cx<T>* sp = buf + (wave + pi * kWaves) * PS;
+#pragma unroll
+                for (int m = 0; m < R; m++)
+                    v[it][m] = (ZIN && m >= RH) ? Cx<T>{T(0), T(0)} : sp[lpad_step<T, LR>(j, m)];
+            }
+        }
+        if constexpr (!LAST) {
+            // keep every read of this stage ahead of its writes
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        }
+#pragma unroll
+        for (int it = 0; it < NIT; it++) {
+            const int b = (int)lane + (it + ob) * 64;
+            if (b < nb) {
+                const unsigned pi = (unsigned)b / LR;
+                const unsigned j = (unsigned)b - pi * LR;
+                const unsigned kk = j % NS;
+                if constexpr (NS > 1) {
+                    if constexpr (FULLTW) {
+                        twiddle_direct<T, R, INV>(v[it], tw, kk * TWSTEP);
+                    } else {
+                        constexpr unsigned TB = ct_tb(L);
+                        const unsigned q = kk * TWSTEP;
+                        const Cx<T> w1 = cmul(tw[q % TB], tw[TB + q / TB]);
+                        Cx<T> w = w1;
+#pragma unroll
+                        for (int m = 1; m < R; m++) {
+                            v[it][m] = INV ? cmulc(v[it][m], w) : cmul(v[it][m], w);
+                            if (m + 1 < R) w = cmul(w, w1);
+                        }
+                    }
+                }
+                dft<T, R, INV>(v[it]);
+                Cx<T>* dp = buf + (wave + pi * kWaves) * PS;
+                const unsigned base = (j - kk) * R + kk;
+#pragma unroll
+                for (int m = 0; m < R; m++)
+                    if (!(HOUT && m >= RH)) dp[lpad_step<T, NS>(base, m)] = v[it][m];
+            }
+        }
+    }
+}
+
+template <typename T, int L, int S, int NS, bool INV, int NPMAX, int PRUNE, bool FULLTW>
+__device__ __forceinline__ void ct_stages_ip(Cx<T>* a, int np, const Cx<T>* tw)
+{
+    if constexpr (S < ct_nst(L)) {
+        constexpr int R = ct_rad(L, S);
+        constexpr bool zin = (PRUNE & kPruneIn) && S == 0;
+        constexpr bool hout = (PRUNE & kPruneOut) && S + 1 == ct_nst(L);
+        ct_stage_ip<T, L, R, NS, INV, NPMAX, FULLTW, zin, hout>(a, np, tw);
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        ct_stages_ip<T, L, S + 1, NS * R, INV, NPMAX, PRUNE, FULLTW>(a, np, tw);
+    }
+}
+
+// Complex values per lane an in-place transform of up to NPMAX pencils of
+// length L holds in registers (largest stage; the last stage runs one
+// butterfly at a time).
+constexpr int ct_ip_values(int L, int npmax)
+{
+    const int npw = (npmax + kWaves - 1) / kWaves;
+    int most = 0;
+    for (int s = 0; s < ct_nst(L); s++) {
+        const int R = ct_rad(L, s);
+        const int nit = s + 1 == ct_nst(L) ? 1 : (npw * (L / R) + 63) / 64;
+        most = nit * R > most ? nit * R : most;
+    }
+    return most;
+}
+
+// In-place transform of np <= NPMAX pencils of the compile-time length L in
+// a; same contract as ct_fft (returns a).
+template <typename T, int L, bool INV, int NPMAX, int PRUNE = 0, bool FULLTW = false>
+__device__ __forceinline__ Cx<T>* ct_fft_ip(Cx<T>* a, int np, const Cx<T>* tw)
+{
+    ct_stages_ip<T, L, 0, 1, INV, NPMAX, PRUNE, FULLTW>(a, np, tw);
+    __syncthreads();
+    return a;
+}
+
 // Transform of np <= NPMAX pencils of the compile-time length L; same
 // contract as pp_fft (input visible to all threads, ends with a workgroup
 // barrier).  PRUNE: kPruneIn (input zero at and above N; entries of

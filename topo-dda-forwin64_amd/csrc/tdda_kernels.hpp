@@ -336,9 +336,45 @@ __host__ __device__ constexpr int np_max(int L)
     return L > 0 ? (kThreads * ept_len<T>(L)) / L : 0;
 }
 
-// LDS slots of the largest data tile (two buffers) the host plans for a
-// compile-time pass kernel of length L (kind 0: z, 1: y, 2: x), as
-// plan_tiles in tdda_capi.hip does with an unclipped third axis.
+// x pass (P3) tile geometry.  For pencils of 120 points or more, where the
+// in-place transform's registers allow it, a tile holds kPxInplacePencils
+// pencils (three per wavefront, a multiple of 3 for the Green product) in one
+// LDS buffer, and the prefetch holds only the Nx <= (L + 1) / 2 loaded rows.
+// Otherwise (and for shorter pencils, whose butterflies would fill too few
+// lanes of three pencils) the tile is sized like the other passes (np_max
+// pencils, two LDS buffers).
+constexpr int kPxInplacePencils = 12;
+template <typename T>
+__host__ __device__ constexpr bool px_inplace(int L)
+{
+    // up to 8 complex values per lane; complex64 power-of-two lengths up to
+    // 16 (other lengths that need more spill registers to scratch)
+    if (L < 120) return false;
+    const int v = ct_ip_values(L, kPxInplacePencils);
+    return v <= 8 || (sizeof(T) == 4 && v <= 16 && (L & (L - 1)) == 0);
+}
+// Waves per SIMD the x-pass kernel is compiled for: the in-place complex128
+// kernels get the registers of three (with those of four they spill), all
+// others those of TDDA_PASS_WAVES.
+template <typename T>
+__host__ __device__ constexpr int px_waves(int L)
+{
+    return (px_inplace<T>(L) && sizeof(T) == 8) ? 3 : TDDA_PASS_WAVES;
+}
+template <typename T>
+__host__ __device__ constexpr int px_np(int L)
+{
+    return px_inplace<T>(L) ? kPxInplacePencils : np_max<T>(L);
+}
+template <typename T>
+__host__ __device__ constexpr int px_ept(int L)
+{
+    return px_inplace<T>(L) ? (((L + 1) / 2) * kPxInplacePencils + kThreads - 1) / kThreads : ept_len<T>(L);
+}
+
+// LDS slots of the largest data tile the host plans for a compile-time pass
+// kernel of length L (kind 0: z, 1: y, 2: x), as plan_tiles in
+// tdda_capi.hip does with an unclipped third axis.
 template <typename T>
 __host__ __device__ constexpr int max_tile_slots(int kind, int L)
 {
@@ -354,6 +390,7 @@ __host__ __device__ constexpr int max_tile_slots(int kind, int L)
         if (ty >= unit) ty = (ty / unit) * unit;
         return 2 * ty * ps;
     }
+    if (px_inplace<T>(L)) return kPxInplacePencils * ps;
     int tx = cap / L < 3 * (8 * kThreads / L) ? cap / L : 3 * (8 * kThreads / L);
     tx = (tx / 3) * 3;
     if (tx >= 3 * unit) tx = (tx / (3 * unit)) * (3 * unit);
@@ -663,12 +700,13 @@ __global__ void TDDA_PASS_BOUNDS k_py(const JobDesc<T>* __restrict__ jobs, const
 // P3: x pass fused with the Green product (kernel.cu:74-95).
 // Tile = (ky, tfx consecutive flattened (kz, c) columns, tfx % 3 == 0).
 template <typename T, int LC>
-__global__ void TDDA_PASS_BOUNDS k_px(const JobDesc<T>* __restrict__ jobs, const int* __restrict__ tstart,
+__global__ void __launch_bounds__(kThreads, px_waves<T>(LC)) k_px(const JobDesc<T>* __restrict__ jobs, const int* __restrict__ tstart,
                                       int njobs, int solver)
 {
     extern __shared__ __align__(16) char smem[];
     constexpr bool FT = LC > 0 && pass_full_twiddles<T>(2, LC);
-    constexpr int EP = ept_len<T>(LC);
+    constexpr bool IP = LC > 0 && px_inplace<T>(LC);  // one LDS buffer, in-place transforms
+    constexpr int EP = px_ept<T>(LC);
     const int ntot = tstart[njobs];
     Pre<T, EP> pr;
     STile cur;
@@ -711,9 +749,11 @@ __global__ void TDDA_PASS_BOUNDS k_px(const JobDesc<T>* __restrict__ jobs, const
             const int ps = J->ps_x, nf = cur.nf, ky = cur.outer, f0 = cur.f0;
             Cx<T>* twl = reinterpret_cast<Cx<T>*>(smem);
             Cx<T>* A = twl + pass_tw_slots(FT, Mx, J->plx.tb);
-            Cx<T>* B = A + J->tfx * ps;
+            Cx<T>* B = IP ? A : A + J->tfx * ps;
             __syncthreads();
-            Cx<T>* res = pass_fft<T, LC, false, kPruneIn, FT>(A, B, nf, ps, J->plx, twl);
+            Cx<T>* res;
+            if constexpr (IP) res = ct_fft_ip<T, LC, false, kPxInplacePencils, kPruneIn, FT>(A, nf, twl);
+            else res = pass_fft<T, LC, false, kPruneIn, FT>(A, B, nf, ps, J->plx, twl);
             Cx<T>* other = (res == A) ? B : A;
             // Green product.  Octant fold per axis: k' = min(k, M - k); odd
             // components flip sign when k was folded.
@@ -740,7 +780,9 @@ __global__ void TDDA_PASS_BOUNDS k_px(const JobDesc<T>* __restrict__ jobs, const
                 v[2 * ps] = cadd(cadd(cmul(gxz, bx), cmul(gyz, by)), cmul(gzz, bz));
             }
             __syncthreads();
-            const Cx<T>* res2 = pass_fft<T, LC, true, kPruneOut, FT>(res, other, nf, ps, J->plx, twl);
+            const Cx<T>* res2;
+            if constexpr (IP) res2 = ct_fft_ip<T, LC, true, kPxInplacePencils, kPruneOut, FT>(res, nf, twl);
+            else res2 = pass_fft<T, LC, true, kPruneOut, FT>(res, other, nf, ps, J->plx, twl);
             const int sx = My * F;
             Cx<T>* base = J->w2 + (size_t)ky * F + f0;
             for (RowWalk w(nf, 1.0f / (float)nf); w.row < Nx; w.next())
