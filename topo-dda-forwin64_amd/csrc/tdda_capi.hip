@@ -262,8 +262,13 @@ struct tdda_batch_s {
 
 namespace {
 
-// LDS buffers of an x-pass tile of length L: one for the in-place
+// LDS buffers of a y-pass / x-pass tile of length L: one for the in-place
 // compile-time kernels, two otherwise.
+template <typename T>
+int py_buffers(int L)
+{
+    return pass_length_compiled<T>(1, L) && py_inplace<T>(L) ? 1 : 2;
+}
 template <typename T>
 int px_buffers(int L)
 {
@@ -282,7 +287,9 @@ int plan_tiles(const tdda_op_s* op, TilePlan& t)
     // into the pass kernel that will run: ept_len of its FFT length
     auto full = [&](int kind, int L) { return kThreads * ept_len<T>(pass_length_compiled<T>(kind, L) ? L : 0); };
     const bool ipx = px_buffers<T>(op->Mx) == 1;  // in-place x pass: loaded rows only
-    const int fz = full(0, op->Mz), fy = full(1, op->My);
+    const bool ipy = py_buffers<T>(op->My) == 1;  // in-place y pass: rows read only
+    const int fz = full(0, op->Mz);
+    const int fy = ipy ? kThreads * py_ept<T, true>(op->My) : full(1, op->My);
     const int fx = ipx ? kThreads * px_ept<T>(op->Mx) : full(2, op->Mx);
     const int env = env_int(sizeof(T) == 8 ? "TDDA_TILE_C128" : "TDDA_TILE_C64", 1 << 30);
     const int capz = std::min(fz, env), capy = std::min(fy, env), capx = std::min(fx, env);
@@ -300,7 +307,8 @@ int plan_tiles(const tdda_op_s* op, TilePlan& t)
     if (3 * op->Nz > cap1) return fail(TDDA_ERR_UNSUPPORTED, "Nz too large for the z-pass tiles of this precision");
     t.pz1 = std::max(1, std::min({cap1 / (3 * op->Nz), capz / (3 * op->Mz), 64, op->Nx * op->Ny}));
     int ty = std::min(capy / op->My, 3 * op->Mz);
-    if (ty >= unit && ty < 3 * op->Mz) ty = (ty / unit) * unit;
+    if (ipy) ty = std::min(ty, py_inplace_pencils<T>());
+    else if (ty >= unit && ty < 3 * op->Mz) ty = (ty / unit) * unit;
     t.tfy = std::max(1, ty);
     int tx;
     if (ipx) {
@@ -316,7 +324,7 @@ int plan_tiles(const tdda_op_s* op, TilePlan& t)
     auto tws = [&](int kind, int L, int tb) { return pass_tw_slots(pass_length_compiled<T>(kind, L) && pass_full_twiddles<T>(kind, L), L, tb); };
     t.lds_z = (size_t)(tws(0, op->Mz, op->pl[2].tb) + 2 * 3 * t.pz * t.ps_z) * s;
     t.lds_z1 = (size_t)(tws(0, op->Mz, op->pl[2].tb) + 2 * 3 * t.pz1 * t.ps_z) * s;
-    t.lds_y = (size_t)(tws(1, op->My, op->pl[1].tb) + 2 * t.tfy * t.ps_y) * s;
+    t.lds_y = (size_t)(tws(1, op->My, op->pl[1].tb) + py_buffers<T>(op->My) * t.tfy * t.ps_y) * s;
     t.lds_x = (size_t)(tws(2, op->Mx, op->pl[0].tb) + px_buffers<T>(op->Mx) * t.tfx * t.ps_x) * s;
     if (std::max({t.lds_z, t.lds_z1, t.lds_y, t.lds_x}) > op->ctx->lds_budget)
         return fail(TDDA_ERR_UNSUPPORTED, "pass tile exceeds the LDS budget");
@@ -495,7 +503,7 @@ int build_groups(const tdda_ctx_s* c, JobTable<T>& jt, Put&& put)
     CHK(build(jt.gz1, 0, Lz, [](const JobTiles& t) { return t.z1; },
               [&](const JobDesc<T>& J) { return (size_t)(tws(0, J.Mz, J.plz.tb) + 2 * 3 * J.pz1 * J.ps_z) * s; }));
     CHK(build(jt.gy, 1, [](const JobDesc<T>& J) { return J.My; }, [](const JobTiles& t) { return t.y; },
-              [&](const JobDesc<T>& J) { return (size_t)(tws(1, J.My, J.ply.tb) + 2 * J.tfy * J.ps_y) * s; }));
+              [&](const JobDesc<T>& J) { return (size_t)(tws(1, J.My, J.ply.tb) + py_buffers<T>(J.My) * J.tfy * J.ps_y) * s; }));
     CHK(build(jt.gx, 2, [](const JobDesc<T>& J) { return J.Mx; }, [](const JobTiles& t) { return t.x; },
               [&](const JobDesc<T>& J) { return (size_t)(tws(2, J.Mx, J.plx.tb) + px_buffers<T>(J.Mx) * J.tfx * J.ps_x) * s; }));
     return TDDA_OK;

@@ -372,6 +372,35 @@ __host__ __device__ constexpr int px_ept(int L)
     return px_inplace<T>(L) ? (((L + 1) / 2) * kPxInplacePencils + kThreads - 1) / kThreads : ept_len<T>(L);
 }
 
+// y pass (P2/P4) tile geometry, in the same way: where registers allow, a
+// tile holds kPyInplacePencils[precision] pencils (128-byte rows of the
+// strided intermediate) in one LDS buffer, transformed in place, and the
+// prefetch holds only the rows the pass reads (Ny <= (L + 1) / 2 forward, all
+// L inverse).
+template <typename T>
+__host__ __device__ constexpr int py_inplace_pencils()
+{
+    return sizeof(T) == 8 ? 8 : 16;
+}
+template <typename T>
+__host__ __device__ constexpr bool py_inplace(int L)
+{
+    if (L < 120) return false;
+    const int v = ct_ip_values(L, py_inplace_pencils<T>());
+    return v <= 8 || (sizeof(T) == 4 && v <= 16 && (L & (L - 1)) == 0);
+}
+template <typename T>
+__host__ __device__ constexpr int py_np(int L)
+{
+    return py_inplace<T>(L) ? py_inplace_pencils<T>() : np_max<T>(L);
+}
+template <typename T, bool INV>
+__host__ __device__ constexpr int py_ept(int L)
+{
+    return py_inplace<T>(L) ? ((INV ? L : (L + 1) / 2) * py_inplace_pencils<T>() + kThreads - 1) / kThreads
+                            : ept_len<T>(L);
+}
+
 // LDS slots of the largest data tile the host plans for a compile-time pass
 // kernel of length L (kind 0: z, 1: y, 2: x), as plan_tiles in
 // tdda_capi.hip does with an unclipped third axis.
@@ -386,6 +415,7 @@ __host__ __device__ constexpr int max_tile_slots(int kind, int L)
         return 2 * 3 * pz * ps;
     }
     if (kind == 1) {
+        if (py_inplace<T>(L)) return py_inplace_pencils<T>() * ps;
         int ty = cap / L;
         if (ty >= unit) ty = (ty / unit) * unit;
         return 2 * ty * ps;
@@ -637,7 +667,8 @@ __global__ void TDDA_PASS_BOUNDS k_py(const JobDesc<T>* __restrict__ jobs, const
 {
     extern __shared__ __align__(16) char smem[];
     constexpr bool FT = LC > 0 && pass_full_twiddles<T>(1, LC);
-    constexpr int EP = ept_len<T>(LC);
+    constexpr bool IP = LC > 0 && py_inplace<T>(LC);  // one LDS buffer, in-place transforms
+    constexpr int EP = py_ept<T, INV>(LC);
     const int ntot = tstart[njobs];
     Pre<T, EP> pr;
     STile cur;
@@ -682,9 +713,11 @@ __global__ void TDDA_PASS_BOUNDS k_py(const JobDesc<T>* __restrict__ jobs, const
             const int Ny = J->Ny, My = J->My, ps = J->ps_y, nf = cur.nf, F = J->Mz * 3;
             Cx<T>* twl = reinterpret_cast<Cx<T>*>(smem);
             Cx<T>* A = twl + pass_tw_slots(FT, My, J->ply.tb);
-            Cx<T>* B = A + J->tfy * ps;
+            Cx<T>* B = IP ? A : A + J->tfy * ps;
             __syncthreads();
-            const Cx<T>* res = pass_fft<T, LC, INV, INV ? kPruneOut : kPruneIn, FT>(A, B, nf, ps, J->ply, twl);
+            const Cx<T>* res;
+            if constexpr (IP) res = ct_fft_ip<T, LC, INV, py_inplace_pencils<T>(), INV ? kPruneOut : kPruneIn, FT>(A, nf, twl);
+            else res = pass_fft<T, LC, INV, INV ? kPruneOut : kPruneIn, FT>(A, B, nf, ps, J->ply, twl);
             const int nout = INV ? Ny : My;
             Cx<T>* dst =
                 (INV ? J->w1 + (size_t)cur.outer * Ny * F : J->w2 + (size_t)cur.outer * My * F) + cur.f0;
