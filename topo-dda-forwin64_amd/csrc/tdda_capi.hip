@@ -312,7 +312,7 @@ int plan_tiles(const tdda_op_s* op, TilePlan& t)
     t.tfy = std::max(1, ty);
     int tx;
     if (ipx) {
-        tx = std::min({kPxInplacePencils, 3 * op->Mz, capx / op->Nx, 3 * (8 * kThreads / op->Mx)});
+        tx = std::min({px_np<T>(op->Mx), 3 * op->Mz, capx / op->Nx, 3 * (8 * kThreads / op->Mx)});
         tx = (tx / 3) * 3;
     } else {
         tx = std::min({capx / op->Mx, 3 * op->Mz, 3 * (8 * kThreads / op->Mx)});

@@ -336,22 +336,36 @@ __host__ __device__ constexpr int np_max(int L)
     return L > 0 ? (kThreads * ept_len<T>(L)) / L : 0;
 }
 
+// Whether an in-place transform of np pencils of length L keeps few enough
+// values in registers: up to 8 complex values per lane; complex64
+// power-of-two lengths up to 16 (other lengths that need more spill
+// registers to scratch).
+template <typename T>
+__host__ __device__ constexpr bool ip_fits(int L, int np)
+{
+    const int v = ct_ip_values(L, np);
+    return v <= 8 || (sizeof(T) == 4 && v <= 16 && (L & (L - 1)) == 0);
+}
+
 // x pass (P3) tile geometry.  For pencils of 120 points or more, where the
-// in-place transform's registers allow it, a tile holds kPxInplacePencils
-// pencils (three per wavefront, a multiple of 3 for the Green product) in one
+// in-place transform's registers allow it, a tile holds 12 pencils (three
+// per wavefront), else 6 (a multiple of 3 for the Green product), in one
 // LDS buffer, and the prefetch holds only the Nx <= (L + 1) / 2 loaded rows.
 // Otherwise (and for shorter pencils, whose butterflies would fill too few
-// lanes of three pencils) the tile is sized like the other passes (np_max
-// pencils, two LDS buffers).
-constexpr int kPxInplacePencils = 12;
+// lanes) the tile is sized like the other passes (np_max pencils, two LDS
+// buffers).
+template <typename T>
+__host__ __device__ constexpr int px_inplace_pencils(int L)
+{
+    if (L < 120) return 0;
+    if (ip_fits<T>(L, 12)) return 12;
+    if (ip_fits<T>(L, 6)) return 6;
+    return 0;
+}
 template <typename T>
 __host__ __device__ constexpr bool px_inplace(int L)
 {
-    // up to 8 complex values per lane; complex64 power-of-two lengths up to
-    // 16 (other lengths that need more spill registers to scratch)
-    if (L < 120) return false;
-    const int v = ct_ip_values(L, kPxInplacePencils);
-    return v <= 8 || (sizeof(T) == 4 && v <= 16 && (L & (L - 1)) == 0);
+    return px_inplace_pencils<T>(L) > 0;
 }
 // Waves per SIMD the x-pass kernel is compiled for: the in-place complex128
 // kernels get the registers of three (with those of four they spill), all
@@ -364,12 +378,12 @@ __host__ __device__ constexpr int px_waves(int L)
 template <typename T>
 __host__ __device__ constexpr int px_np(int L)
 {
-    return px_inplace<T>(L) ? kPxInplacePencils : np_max<T>(L);
+    return px_inplace<T>(L) ? px_inplace_pencils<T>(L) : np_max<T>(L);
 }
 template <typename T>
 __host__ __device__ constexpr int px_ept(int L)
 {
-    return px_inplace<T>(L) ? (((L + 1) / 2) * kPxInplacePencils + kThreads - 1) / kThreads : ept_len<T>(L);
+    return px_inplace<T>(L) ? (((L + 1) / 2) * px_inplace_pencils<T>(L) + kThreads - 1) / kThreads : ept_len<T>(L);
 }
 
 // y pass (P2/P4) tile geometry, in the same way: where registers allow, a
@@ -385,9 +399,7 @@ __host__ __device__ constexpr int py_inplace_pencils()
 template <typename T>
 __host__ __device__ constexpr bool py_inplace(int L)
 {
-    if (L < 120) return false;
-    const int v = ct_ip_values(L, py_inplace_pencils<T>());
-    return v <= 8 || (sizeof(T) == 4 && v <= 16 && (L & (L - 1)) == 0);
+    return L >= 120 && ip_fits<T>(L, py_inplace_pencils<T>());
 }
 template <typename T>
 __host__ __device__ constexpr int py_np(int L)
@@ -420,7 +432,7 @@ __host__ __device__ constexpr int max_tile_slots(int kind, int L)
         if (ty >= unit) ty = (ty / unit) * unit;
         return 2 * ty * ps;
     }
-    if (px_inplace<T>(L)) return kPxInplacePencils * ps;
+    if (px_inplace<T>(L)) return px_inplace_pencils<T>(L) * ps;
     int tx = cap / L < 3 * (8 * kThreads / L) ? cap / L : 3 * (8 * kThreads / L);
     tx = (tx / 3) * 3;
     if (tx >= 3 * unit) tx = (tx / (3 * unit)) * (3 * unit);
@@ -785,7 +797,7 @@ __global__ void __launch_bounds__(kThreads, px_waves<T>(LC)) k_px(const JobDesc<
             Cx<T>* B = IP ? A : A + J->tfx * ps;
             __syncthreads();
             Cx<T>* res;
-            if constexpr (IP) res = ct_fft_ip<T, LC, false, kPxInplacePencils, kPruneIn, FT>(A, nf, twl);
+            if constexpr (IP) res = ct_fft_ip<T, LC, false, px_np<T>(LC), kPruneIn, FT>(A, nf, twl);
             else res = pass_fft<T, LC, false, kPruneIn, FT>(A, B, nf, ps, J->plx, twl);
             Cx<T>* other = (res == A) ? B : A;
             // Green product.  Octant fold per axis: k' = min(k, M - k); odd
@@ -814,7 +826,7 @@ __global__ void __launch_bounds__(kThreads, px_waves<T>(LC)) k_px(const JobDesc<
             }
             __syncthreads();
             const Cx<T>* res2;
-            if constexpr (IP) res2 = ct_fft_ip<T, LC, true, kPxInplacePencils, kPruneOut, FT>(res, nf, twl);
+            if constexpr (IP) res2 = ct_fft_ip<T, LC, true, px_np<T>(LC), kPruneOut, FT>(res, nf, twl);
             else res2 = pass_fft<T, LC, true, kPruneOut, FT>(res, other, nf, ps, J->plx, twl);
             const int sx = My * F;
             Cx<T>* base = J->w2 + (size_t)ky * F + f0;
