@@ -262,8 +262,13 @@ struct tdda_batch_s {
 
 namespace {
 
-// LDS buffers of a y-pass / x-pass tile of length L: one for the in-place
-// compile-time kernels, two otherwise.
+// LDS buffers of a z-pass / y-pass / x-pass tile of length L: one for the
+// in-place compile-time kernels, two otherwise.
+template <typename T>
+int pz_buffers(int L)
+{
+    return pass_length_compiled<T>(0, L) && pz_inplace<T>(L) ? 1 : 2;
+}
 template <typename T>
 int py_buffers(int L)
 {
@@ -322,8 +327,8 @@ int plan_tiles(const tdda_op_s* op, TilePlan& t)
     t.tfx = std::max(3, tx);
     if ((t.tfx / 3) * op->Mx > 8 * kThreads) return fail(TDDA_ERR_UNSUPPORTED, "x pass tile too wide");
     auto tws = [&](int kind, int L, int tb) { return pass_tw_slots(pass_length_compiled<T>(kind, L) && pass_full_twiddles<T>(kind, L), L, tb); };
-    t.lds_z = (size_t)(tws(0, op->Mz, op->pl[2].tb) + 2 * 3 * t.pz * t.ps_z) * s;
-    t.lds_z1 = (size_t)(tws(0, op->Mz, op->pl[2].tb) + 2 * 3 * t.pz1 * t.ps_z) * s;
+    t.lds_z = (size_t)(tws(0, op->Mz, op->pl[2].tb) + pz_buffers<T>(op->Mz) * 3 * t.pz * t.ps_z) * s;
+    t.lds_z1 = (size_t)(tws(0, op->Mz, op->pl[2].tb) + pz_buffers<T>(op->Mz) * 3 * t.pz1 * t.ps_z) * s;
     t.lds_y = (size_t)(tws(1, op->My, op->pl[1].tb) + py_buffers<T>(op->My) * t.tfy * t.ps_y) * s;
     t.lds_x = (size_t)(tws(2, op->Mx, op->pl[0].tb) + px_buffers<T>(op->Mx) * t.tfx * t.ps_x) * s;
     if (std::max({t.lds_z, t.lds_z1, t.lds_y, t.lds_x}) > op->ctx->lds_budget)
@@ -499,9 +504,9 @@ int build_groups(const tdda_ctx_s* c, JobTable<T>& jt, Put&& put)
     auto Lz = [](const JobDesc<T>& J) { return J.Mz; };
     auto tws = [](int kind, int L, int tb) { return pass_tw_slots(pass_length_compiled<T>(kind, L) && pass_full_twiddles<T>(kind, L), L, tb); };
     CHK(build(jt.gz, 0, Lz, [](const JobTiles& t) { return t.z; },
-              [&](const JobDesc<T>& J) { return (size_t)(tws(0, J.Mz, J.plz.tb) + 2 * 3 * J.pz * J.ps_z) * s; }));
+              [&](const JobDesc<T>& J) { return (size_t)(tws(0, J.Mz, J.plz.tb) + pz_buffers<T>(J.Mz) * 3 * J.pz * J.ps_z) * s; }));
     CHK(build(jt.gz1, 0, Lz, [](const JobTiles& t) { return t.z1; },
-              [&](const JobDesc<T>& J) { return (size_t)(tws(0, J.Mz, J.plz.tb) + 2 * 3 * J.pz1 * J.ps_z) * s; }));
+              [&](const JobDesc<T>& J) { return (size_t)(tws(0, J.Mz, J.plz.tb) + pz_buffers<T>(J.Mz) * 3 * J.pz1 * J.ps_z) * s; }));
     CHK(build(jt.gy, 1, [](const JobDesc<T>& J) { return J.My; }, [](const JobTiles& t) { return t.y; },
               [&](const JobDesc<T>& J) { return (size_t)(tws(1, J.My, J.ply.tb) + py_buffers<T>(J.My) * J.tfy * J.ps_y) * s; }));
     CHK(build(jt.gx, 2, [](const JobDesc<T>& J) { return J.Mx; }, [](const JobTiles& t) { return t.x; },

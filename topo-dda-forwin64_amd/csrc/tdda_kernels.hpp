@@ -386,6 +386,14 @@ __host__ __device__ constexpr int px_ept(int L)
     return px_inplace<T>(L) ? (((L + 1) / 2) * px_inplace_pencils<T>(L) + kThreads - 1) / kThreads : ept_len<T>(L);
 }
 
+// z passes (P1/P5): tiles keep their size (np_max pencils) and are
+// transformed in place in one LDS buffer where the registers allow it.
+template <typename T>
+__host__ __device__ constexpr bool pz_inplace(int L)
+{
+    return L > 0 && np_max<T>(L) >= 3 && ip_fits<T>(L, np_max<T>(L));
+}
+
 // y pass (P2/P4) tile geometry, in the same way: where registers allow, a
 // tile holds kPyInplacePencils[precision] pencils (128-byte rows of the
 // strided intermediate) in one LDS buffer, transformed in place, and the
@@ -424,7 +432,7 @@ __host__ __device__ constexpr int max_tile_slots(int kind, int L)
     const int unit = 128 / (int)sizeof(Cx<T>);
     if (kind == 0) {
         const int pz = cap / (3 * L) < 64 ? cap / (3 * L) : 64;
-        return 2 * 3 * pz * ps;
+        return (pz_inplace<T>(L) ? 1 : 2) * 3 * pz * ps;
     }
     if (kind == 1) {
         if (py_inplace<T>(L)) return py_inplace_pencils<T>() * ps;
@@ -536,6 +544,7 @@ __global__ void TDDA_PASS_BOUNDS k_p1(const JobDesc<T>* __restrict__ jobs, const
 {
     extern __shared__ __align__(16) char smem[];
     constexpr bool FT = LC > 0 && pass_full_twiddles<T>(0, LC);
+    constexpr bool IP = LC > 0 && pz_inplace<T>(LC);  // one LDS buffer, in-place transforms
     constexpr int NV = (MODE == IN_P) ? 3 : ((MODE == IN_T) ? 2 : 1);
     constexpr int EP = ept_len<T>(LC) / 3;
     const int ntot = tstart[njobs];
@@ -653,9 +662,11 @@ __global__ void TDDA_PASS_BOUNDS k_p1(const JobDesc<T>* __restrict__ jobs, const
             const int Mz = J->Mz, ps = J->ps_z;
             Cx<T>* twl = reinterpret_cast<Cx<T>*>(smem);
             Cx<T>* A = twl + pass_tw_slots(FT, Mz, J->plz.tb);
-            Cx<T>* B = A + 3 * J->pz1 * ps;
+            Cx<T>* B = IP ? A : A + 3 * J->pz1 * ps;
             __syncthreads();
-            const Cx<T>* res = pass_fft<T, LC, false, kPruneIn, FT>(A, B, cur.np * 3, ps, J->plz, twl);
+            const Cx<T>* res;
+            if constexpr (IP) res = ct_fft_ip<T, LC, false, np_max<T>(LC), kPruneIn, FT>(A, cur.np * 3, twl);
+            else res = pass_fft<T, LC, false, kPruneIn, FT>(A, B, cur.np * 3, ps, J->plz, twl);
             const int nout = cur.np * Mz * 3;
             Cx<T>* w1 = J->w1 + (size_t)cur.q0 * Mz * 3;
             RowWalk w(Mz * 3, 1.0f / (float)(Mz * 3));
@@ -847,6 +858,7 @@ __global__ void TDDA_PASS_BOUNDS k_p5(const JobDesc<T>* __restrict__ jobs, const
 {
     extern __shared__ __align__(16) char smem[];
     constexpr bool FT = LC > 0 && pass_full_twiddles<T>(0, LC);
+    constexpr bool IP = LC > 0 && pz_inplace<T>(LC);  // one LDS buffer, in-place transforms
     __shared__ double red[16 * 4];
     constexpr int EP = ept_len<T>(LC);
     const int ntot = tstart[njobs];
@@ -891,9 +903,11 @@ __global__ void TDDA_PASS_BOUNDS k_p5(const JobDesc<T>* __restrict__ jobs, const
             const int Nz = J->Nz, Mz = J->Mz, ps = J->ps_z;
             Cx<T>* twl = reinterpret_cast<Cx<T>*>(smem);
             Cx<T>* A = twl + pass_tw_slots(FT, Mz, J->plz.tb);
-            Cx<T>* B = A + 3 * J->pz * ps;
+            Cx<T>* B = IP ? A : A + 3 * J->pz * ps;
             __syncthreads();
-            const Cx<T>* res = pass_fft<T, LC, true, kPruneOut, FT>(A, B, cur.np * 3, ps, J->plz, twl);
+            const Cx<T>* res;
+            if constexpr (IP) res = ct_fft_ip<T, LC, true, np_max<T>(LC), kPruneOut, FT>(A, cur.np * 3, twl);
+            else res = pass_fft<T, LC, true, kPruneOut, FT>(A, B, cur.np * 3, ps, J->plz, twl);
             double acc[4] = {0.0, 0.0, 0.0, 0.0};
             const int nout = cur.np * Nz * 3;
             const int base = cur.q0 * Nz;
