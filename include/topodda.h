@@ -15,6 +15,9 @@
  *   DDAModel ctor (E, al)       DDAModel.cpp:12-69        tdda_incident_field, tdda_polarizability
  *   Get_Alpha                   tools.cpp:617-630         tdda_get_alpha
  *   update_E_in_structure       DDAModel.cpp:567-575      tdda_local_field
+ *   CoreStructure::UpdateStr    CoreStructure.cpp:31-104  tdda_design_step (with
+ *     + DDAModel::UpdateAlpha   DDAModel.cpp:470-484       tdda_design_create/_get)
+ *   FilterOption::update_beta   FilterOption.cpp:24-46    tdda_beta_update
  *   (kernel.cu A2As/B2Bs/Conv/Conv2B and the cuFFT calls are internal.)
  *   No counterpart (one solve at a time in the reference, T/main.cpp:584-598):
  *   tdda_batch_* (many structures per launch), tdda_server_* (request loop),
@@ -72,6 +75,7 @@ typedef struct tdda_op_s* tdda_op;
 typedef struct tdda_model_s* tdda_model;
 typedef struct tdda_batch_s* tdda_batch;
 typedef struct tdda_server_s* tdda_server;
+typedef struct tdda_design_s* tdda_design;
 
 /* Last error message of the calling thread ("" if none). */
 const char* tdda_last_error(void);
@@ -280,6 +284,40 @@ int tdda_server_wait(tdda_server s, long long ticket, double timeout_s);
  * information of a finished request and releases its ticket. */
 int tdda_server_result(tdda_server s, long long ticket, double* P, tdda_solve_info* info, tdda_request_times* t);
 int tdda_server_get_stats(tdda_server s, tdda_server_stats* out);
+
+/* ------------------------------------------------------------------------
+ * Device-resident design update of an extruded structure: one free
+ * parameter per (x, y) column of a full box (2D extrusion, no symmetry, not
+ * periodic), kept in HBM.  Replaces, per optimisation step,
+ * CoreStructure::UpdateStr(step, it, max_it) (CoreStructure.cpp:31-104) with
+ * the FreeWeight cone filter of SpacePara (SpacePara.cpp:37-48, 1241-1270)
+ * and FilterOption::SmoothDensity (FilterOption.cpp:49-60), followed by
+ * DDAModel::UpdateAlpha (DDAModel.cpp:470-484) of every right-hand side.
+ * ---------------------------------------------------------------------- */
+#define TDDA_BETA_EXP 0       /* exp_update, tools.cpp:776-778 */
+#define TDDA_BETA_PIECEWISE 1 /* piecewise_update, tools.cpp:780-794 */
+#define TDDA_BETA_LINEAR 2    /* linear_update, tools.cpp:796-798 */
+
+/* Design state of model m (its operator must be a full box in build_a_bulk
+ * order).  para_origin: Nx * Ny values in [0, 1], column index x * Ny + y
+ * (SpacePara's Para order for an extruded box); rfilter: cone-filter radius
+ * in lattice units (> 0.1); material: the operator's material table (re, im
+ * pairs, after the AProductCore.cpp:19-24 rule), nmat >= 2; inc: nK[3],
+ * nE[3] per right-hand side of m.  The design holds a reference to m. */
+int tdda_design_create(tdda_model m, const double* para_origin, double rfilter, const double* material, int nmat,
+                       const double* inc, tdda_design* out);
+/* Para_origin += step (Nx * Ny host values; NULL: no step), clamped to
+ * [0, 1]; Para_filtered = cone filter; Para = SmoothDensity(beta, eta); then
+ * al of every right-hand side of the model (complex128 and, when present,
+ * complex64 copies) from diel_old = Para of each site's column. */
+int tdda_design_step(tdda_design dz, const double* step, double beta, double eta);
+/* Copies Para_origin, Para_filtered and Para (each Nx * Ny; any may be
+ * NULL). */
+int tdda_design_get(tdda_design dz, double* para_origin, double* para_filtered, double* para);
+void tdda_design_destroy(tdda_design dz);
+/* FilterOption::update_beta (FilterOption.cpp:24-46): beta at iteration it
+ * of max_it for the TDDA_BETA_* schedule. */
+double tdda_beta_update(int kind, double it, double max_it, double beta_min, double beta_max);
 
 #ifdef __cplusplus
 }

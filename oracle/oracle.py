@@ -17,8 +17,14 @@ Two layers:
   convolution.  They restate the same formulas independently (different
   algebraic form for the tensor: -G with the textbook dyadic Green function).
 
-``oracle/_ref/libref_tools.so`` is the reference's own tools.cpp compiled from
-its sources (oracle/Makefile target ``ref``); ``ref_lib()`` loads it when present.
+* numpy restatement of the design update of an extruded structure: the beta
+  schedules (tools.cpp:776-798), the cone filter (SpacePara.cpp:37-48,
+  1241-1270; CoreStructure.cpp:31-104) and SmoothDensity
+  (FilterOption.cpp:49-60).
+
+``oracle/_ref/libref_tools.so`` is the reference's own tools.cpp, Structure,
+Space, SpacePara, FilterOption and CoreStructure compiled from its sources
+(oracle/Makefile target ``ref``); ``ref_lib()`` loads it when present.
 """
 from __future__ import annotations
 
@@ -92,6 +98,10 @@ def ref_lib():
         L.ref_get_alpha.argtypes = [ctypes.c_double] * 5 + [_dp, _dp, _dp]
         L.ref_nE_perp_in_xz.argtypes = [ctypes.c_double, ctypes.c_double, _dp]
         L.ref_build_a_bulk.argtypes = [ctypes.c_int] * 3 + [_ip]
+        L.ref_beta_update.argtypes = [ctypes.c_int] + [ctypes.c_double] * 4
+        L.ref_beta_update.restype = ctypes.c_double
+        L.ref_design_update.argtypes = [ctypes.c_int] * 3 + [_dp, _dp] + [ctypes.c_double] * 3 + [_dp] * 3
+        L.ref_design_update.restype = ctypes.c_int
         _REF = L
     return _REF
 
@@ -330,3 +340,77 @@ def rel_l2(a, b) -> float:
     b = np.asarray(b)
     nb = np.linalg.norm(b)
     return float(np.linalg.norm(a - b) / (nb if nb > 0 else 1.0))
+
+
+# ---------------------------------------------------------------------------
+# Design update of an extruded structure (numpy restatement)
+# ---------------------------------------------------------------------------
+def beta_update(kind: int, x: float, x_max: float, y_min: float, y_max: float) -> float:
+    """exp_update / piecewise_update / linear_update (kind 0 / 1 / 2),
+    tools.cpp:776-798, as FilterOption::update_beta picks them
+    (FilterOption.cpp:24-46)."""
+    if kind == 0:
+        return y_min + (y_max - y_min) * math.exp(x / x_max - 1.0)
+    if kind == 1:
+        if x <= 0.5 * x_max:
+            return y_min
+        if 0.5 * x_max < x <= 0.7 * x_max:
+            return y_min + (y_max - y_min) / 20
+        if 0.7 * x_max < x <= 0.8 * x_max:
+            return y_min + (y_max - y_min) / 10
+        return y_max
+    return y_min + (y_max - y_min) * x / x_max
+
+
+def smooth_density(x: float, beta: float, eta: float) -> float:
+    """FilterOption::SmoothDensity (FilterOption.cpp:49-60); raises outside [0, 1]."""
+    if 0.0 <= x <= eta:
+        return eta * (math.exp(-beta * (1 - x / eta)) - (1 - x / eta) * math.exp(-beta))
+    if eta < x <= 1.0:
+        return (1 - eta) * (1 - math.exp(-beta * (x - eta) / (1 - eta)) + (x - eta) / (1 - eta) * math.exp(-beta)) + eta
+    raise ValueError("SmoothDensity input out of range")
+
+
+def cone_weights(Nx: int, Ny: int, r: float):
+    """FreeWeight of an extruded Nx x Ny box (SpacePara.cpp:1241-1270): for
+    every column (index x * Ny + y) the columns j with
+    dx^2 + dy^2 <= r^2 - 0.01 (circlerange, SpacePara.cpp:41-48), in column
+    order, with weight r - dist (calweight, SpacePara.cpp:37-39)."""
+    out = []
+    for xo in range(Nx):
+        for yo in range(Ny):
+            row = []
+            for x in range(Nx):
+                for y in range(Ny):
+                    d2 = float(x - xo) * float(x - xo) + float(y - yo) * float(y - yo)
+                    if d2 <= r * r - 0.01:
+                        row.append((x * Ny + y, r - math.sqrt(d2)))
+            out.append(row)
+    return out
+
+
+def design_update(para_origin, step, Nx: int, Ny: int, r: float, beta: float, eta: float):
+    """CoreStructure::UpdateStr with a filter (CoreStructure.cpp:31-80):
+    origin += step clamped to [0, 1]; filtered = sum w origin / sum w (in
+    FreeWeight order); para = SmoothDensity(filtered).  Returns
+    (origin, filtered, para)."""
+    o = np.asarray(para_origin, np.float64).copy()
+    if step is not None:
+        o = o + np.asarray(step, np.float64)
+        o = np.where(o >= 1, 1.0, o)
+        o = np.where(o <= 0, 0.0, o)
+    f = np.zeros_like(o)
+    for i, row in enumerate(cone_weights(Nx, Ny, r)):
+        num = den = 0.0
+        for j, w in row:
+            num += w * o[j]
+            den += w
+        f[i] = num / den
+    p = np.array([smooth_density(v, beta, eta) for v in f])
+    return o, f, p
+
+
+def design_diel_old(para, Nx: int, Ny: int, Nz: int) -> np.ndarray:
+    """diel_old of a full box in build_a_bulk order: each site takes the
+    value of its column, three components each (CoreStructure.cpp:98-104)."""
+    return np.repeat(np.repeat(np.asarray(para, np.float64).reshape(Nx * Ny), Nz), 3)

@@ -3,9 +3,15 @@
 //
 // TEST INFRASTRUCTURE ONLY: loaded by tests/ and tests/golden/make_golden.py to
 // pin the oracle's restatement of Get_Alpha (tools.cpp:617-630),
-// nEPerpinXZ (tools.cpp:686-709) and build_a_bulk (tools.cpp:497-510).
+// nEPerpinXZ (tools.cpp:686-709), build_a_bulk (tools.cpp:497-510), the beta
+// schedules (tools.cpp:776-798) and the filtered design update
+// (CoreStructure::UpdateStr, CoreStructure.cpp:31-104, over SpacePara's
+// FreeWeight cone filter, SpacePara.cpp:37-48,1087-1270, and
+// FilterOption::SmoothDensity, FilterOption.cpp:49-60).
 #include <complex>
+#include <vector>
 #include "Tools.h"
+#include "CoreStructure.h"
 
 extern "C" {
 
@@ -29,6 +35,56 @@ void ref_build_a_bulk(int Nx, int Ny, int Nz, int* out)
 {
     VectorXi r = build_a_bulk(Nx, Ny, Nz);
     for (int i = 0; i < r.size(); i++) out[i] = r(i);
+}
+
+double ref_beta_update(int kind, double x, double x_max, double y_min, double y_max)
+{
+    if (kind == 0) return exp_update(x, x_max, y_min, y_max);
+    if (kind == 1) return piecewise_update(x, x_max, y_min, y_max);
+    return linear_update(x, x_max, y_min, y_max);
+}
+
+// One filtered design step of an extruded Nx x Ny x Nz box through the
+// reference's own classes: one free parameter per (x, y) column (bind
+// (1, 1, Nz), no symmetry, not periodic), Para_origin = para0, then
+// UpdateStr(step) with the filter radius, beta and eta given.  Outputs the
+// per-site diel_old (3N), Para_filtered and Para (Nx Ny each).
+int ref_design_update(int Nx, int Ny, int Nz, const double* para0, const double* step, double rfilter, double beta,
+                      double eta, double* diel_out, double* filtered_out, double* para_out)
+{
+    try {
+        const int N = Nx * Ny * Nz;
+        VectorXi total_space = build_a_bulk(Nx, Ny, Nz);
+        VectorXi geometry = build_a_bulk(Nx, Ny, Nz);
+        vector<Structure> ln;
+        ln.push_back(Structure(&total_space, &geometry, true));
+        Space space(&total_space, Nx, Ny, Nz, N, &ln);
+        VectorXd diel0 = VectorXd::Zero(3 * N);
+        for (int i = 0; i < N; i++) {
+            const int x = geometry(3 * i), y = geometry(3 * i + 1);
+            diel0(3 * i) = diel0(3 * i + 1) = diel0(3 * i + 2) = para0[x * Ny + y];
+        }
+        vector<filterinfo> fl{filterinfo{0, rfilter}};
+        FilterOption fo(beta, beta, eta, "linear", fl, false, 0);
+        SpacePara sp(&space, Vector3i(1, 1, Nz), &geometry, &diel0, true, &fo, "None", vector<double>{}, false, 0, 0);
+        CoreStructure cs(&sp, 1.0);
+        const int np = (int)sp.get_Free()->size();
+        if (np != Nx * Ny) return -1;
+        VectorXd st(np);
+        for (int i = 0; i < np; i++) st(i) = step[i];
+        cs.UpdateStr(st, 0, 1);
+        const VectorXd& dl = *cs.get_diel_old();
+        for (int i = 0; i < 3 * N; i++) diel_out[i] = dl(i);
+        const VectorXd& pf = *sp.get_Para_filtered();
+        const VectorXd& pp = *sp.get_Para();
+        for (int i = 0; i < np; i++) {
+            filtered_out[i] = pf(i);
+            para_out[i] = pp(i);
+        }
+        return 0;
+    } catch (...) {
+        return -2;
+    }
 }
 
 }

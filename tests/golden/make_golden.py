@@ -15,6 +15,13 @@ outputs for this path, so the fixtures come from:
   (what AProductCore::Aproduct, AProductCore.cpp:697-758, computes).
 * solve_tiny.npz -- dense numpy.linalg.solve of the assembled 3N x 3N system
   A P = E of DDAModel (DDAModel.cpp:12-69, 577-586) for small structures.
+* design_ref.npz -- the reference's own filtered design update of an extruded
+  box (CoreStructure::UpdateStr, CoreStructure.cpp:31-104, through SpacePara's
+  FreeWeight and FilterOption::SmoothDensity) and its beta schedules
+  (tools.cpp:776-798), compiled from its sources into oracle/_ref.
+
+``python tests/golden/make_golden.py`` writes every fixture;
+``python tests/golden/make_golden.py design`` only design_ref.npz.
 """
 import os
 import sys
@@ -123,11 +130,46 @@ def solve_fixture():
     np.savez(os.path.join(OUT, "solve_tiny.npz"), **cases)
 
 
+# (Nx, Ny, Nz, rfilter, beta, eta, step scale): radii below one lattice step
+# (self only), 2 and 3.3 (9 and 37 columns), beta 0 (identity projection)
+# and steep projections, steps that cross the clamps
+DESIGN_CASES = [(5, 4, 2, 2.0, 0.0, 0.5, 0.1), (7, 6, 1, 3.3, 20.0, 0.3, 0.2), (3, 9, 3, 0.9, 8.0, 0.5, 0.5),
+                (6, 6, 2, 2.0, 50.0, 0.7, 0.8)]
+
+
+def design_fixture():
+    ref = O.ref_lib()
+    if ref is None:
+        raise SystemExit("oracle/_ref/libref_tools.so missing: run make -C oracle ref")
+    rng = np.random.default_rng(21)
+    cases = {}
+    for k, (Nx, Ny, Nz, r, beta, eta, sc) in enumerate(DESIGN_CASES):
+        p0 = rng.random(Nx * Ny)
+        st = sc * rng.standard_normal(Nx * Ny)
+        diel = np.zeros(3 * Nx * Ny * Nz)
+        f = np.zeros(Nx * Ny)
+        p = np.zeros(Nx * Ny)
+        rc = ref.ref_design_update(Nx, Ny, Nz, O._ptr(p0), O._ptr(st), r, beta, eta, O._ptr(diel), O._ptr(f),
+                                   O._ptr(p))
+        if rc != 0:
+            raise SystemExit(f"reference design update failed: {rc}")
+        cases.update({f"c{k}_case": np.array([Nx, Ny, Nz, r, beta, eta]), f"c{k}_p0": p0, f"c{k}_step": st,
+                      f"c{k}_diel": diel, f"c{k}_filtered": f, f"c{k}_para": p})
+    xs = np.array([0.0, 1.0, 2.5, 5.0, 6.0, 7.5, 8.0, 9.9, 10.0])
+    cases["beta_x"] = xs
+    cases["beta"] = np.array([[ref.ref_beta_update(kind, x, 10.0, 1.0, 200.0) for x in xs] for kind in range(3)])
+    np.savez(os.path.join(OUT, "design_ref.npz"), **cases)
+
+
 if __name__ == "__main__":
+    if sys.argv[1:] == ["design"]:
+        design_fixture()
+        raise SystemExit(0)
     alpha_fixture()
     green_fixture()
     aproduct_fixture()
     solve_fixture()
+    design_fixture()
     for f in sorted(os.listdir(OUT)):
         if f.endswith(".npz"):
             print(f, os.path.getsize(os.path.join(OUT, f)))

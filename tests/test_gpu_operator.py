@@ -82,6 +82,26 @@ def test_aproduct_long_x_pencils(ctx, dims):
     assert O.rel_l2(core.Aproduct(b, precision=T.C64), ref) < TOL32
 
 
+@pytest.mark.parametrize("dims,grid", [((80, 80, 16), (160, 160, 32)),
+                                       ((128, 20, 6), (256, 40, 12)),
+                                       ((96, 96, 4), (192, 192, 8)),
+                                       ((10, 10, 64), (20, 20, 128)),
+                                       ((100, 100, 8), (320, 240, 16))])
+def test_aproduct_inplace_tiles(ctx, dims, grid):
+    # in-place LDS transforms: x tiles of 12 pencils (160; 256 in complex64)
+    # or 6 (192, 256, 320 in complex128), y tiles of 128-byte rows (160 -
+    # 256), z tiles (32, 128), also on a grid larger than the automatic one
+    Nx, Ny, Nz = dims
+    print(f"aproduct in-place tiles {dims} grid {grid}", flush=True)
+    R, N, b, al = problem(Nx, Ny, Nz, 10.0, 500.0, seed=Ny)
+    core = T.AProductCore(ctx, None, Nx, Ny, Nz, 10.0, 500.0, grid=grid)
+    assert core.grid == grid
+    ref = O.aproduct_numpy_fft(R, Nx, Ny, Nz, 10.0, O.wavenumber(500.0), b, grid=grid)
+    assert O.rel_l2(core.Aproduct(b), ref) < TOL64
+    assert O.rel_l2(core.Aproduct(b, precision=T.C64), ref) < TOL32
+    core.close()
+
+
 def test_grid_size_independence(ctx):
     # any FFT grid >= 2N-1 gives the same linear convolution (the reference uses 2N-1)
     R, N, b, _ = problem(6, 5, 4, 10.0, 500.0, seed=4)

@@ -1708,3 +1708,4 @@ int tdda_batch_apply(tdda_batch b, int precision)
 }  // extern "C"
 
 #include "tdda_serve.inc"
+#include "tdda_design.inc"

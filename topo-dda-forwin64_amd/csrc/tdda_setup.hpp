@@ -201,14 +201,40 @@ __device__ __forceinline__ Cx<double> cdiv(Cx<double> a, Cx<double> b)
     return Cx<double>{(a.x * b.x + a.y * b.y) / q, (a.y * b.x - a.x * b.y) / q};
 }
 
+// eps from a design value t by the material interpolation (DDAModel.cpp:54-61):
+// mat[floor t] + (t - floor t) (mat[next] - mat[floor t]), next = floor t + 1
+// for floor t = 0, else floor t.
+__device__ __forceinline__ Cx<double> interp_eps(double tv, const Cx<double>* __restrict__ mat, int nmat)
+{
+    const int lf = (int)floor(tv);
+    const int ln = lf >= 1 ? lf : lf + 1;
+    const Cx<double> mf = mat[lf], mn = mat[min(ln, nmat - 1)];
+    return cadd(mf, cscale(csub(mn, mf), tv - (double)lf));
+}
+
+// 1 / alpha_LDR (Get_Alpha, tools.cpp:617-630; DDAModel.cpp:62) for one
+// incidence q = {nK[3], nE[3], ...}.
+__device__ __forceinline__ Cx<double> ldr_inv_alpha(Cx<double> eps, const double* q, double K, double d)
+{
+    const double b1 = -1.891531, b2 = 0.1648469, b3 = -1.7700004;
+    const double d3 = d * d * d, kd = K * d;
+    const Cx<double> a1 =
+        cscale(cdiv(Cx<double>{eps.x - 1.0, eps.y}, Cx<double>{eps.x + 2.0, eps.y}), 3.0 * d3 / (4.0 * M_PI));
+    const double S = (q[3] * q[0]) * (q[3] * q[0]) + (q[4] * q[1]) * (q[4] * q[1]) + (q[5] * q[2]) * (q[5] * q[2]);
+    const Cx<double> poly = Cx<double>{b1 + eps.x * b2 + eps.x * b3 * S, eps.y * b2 + eps.y * b3 * S};
+    Cx<double> inner = cscale(poly, kd * kd);
+    inner.y -= (2.0 / 3.0) * kd * kd * kd;
+    const Cx<double> res = cadd(Cx<double>{1.0, 0.0}, cmul(cscale(a1, 1.0 / d3), inner));
+    return cdiv(res, a1);  // 1 / (a1 / res)
+}
+
 // The DDAModel constructor per right-hand side (DDAModel.cpp:12-69):
-// eps from the design value t by the material interpolation
-// (DDAModel.cpp:54-61), al = 1 / alpha_LDR (Get_Alpha, tools.cpp:617-630,
-// DDAModel.cpp:62) and, when `plane`, the incident plane wave
-// E = E0 nE exp(i K d nK . R) (DDAModel.cpp:47-51); otherwise E = 0.
-// inc holds 7 doubles per right-hand side: nK[3], nE[3], E0.  R == nullptr
-// stands for the full box in build_a_bulk order.  Design values must lie in
-// the material table (checked on the host).
+// eps from the design value t (interp_eps), al = 1 / alpha_LDR and, when
+// `plane`, the incident plane wave E = E0 nE exp(i K d nK . R)
+// (DDAModel.cpp:47-51); otherwise E = 0.  inc holds 7 doubles per
+// right-hand side: nK[3], nE[3], E0.  R == nullptr stands for the full box in
+// build_a_bulk order.  Design values must lie in the material table (checked
+// on the host).
 static __global__ void __launch_bounds__(kThreads) k_request_fields(
     int N, const int* __restrict__ R, int Ny, int Nz, const double* __restrict__ t, int nmat,
     const Cx<double>* __restrict__ mat, double K, double d, const double* __restrict__ inc, int nrhs, int plane,
@@ -221,21 +247,7 @@ static __global__ void __launch_bounds__(kThreads) k_request_fields(
         const size_t e = g - j * n3;
         const int i = (int)(e / 3), c = (int)(e - 3 * (size_t)i);
         const double* q = inc + 7 * j;
-        const double tv = t[e];
-        const int lf = (int)floor(tv);
-        const int ln = lf >= 1 ? lf : lf + 1;
-        const Cx<double> mf = mat[lf], mn = mat[min(ln, nmat - 1)];
-        const Cx<double> eps = cadd(mf, cscale(csub(mn, mf), tv - (double)lf));
-        const double b1 = -1.891531, b2 = 0.1648469, b3 = -1.7700004;
-        const double d3 = d * d * d, kd = K * d;
-        const Cx<double> a1 =
-            cscale(cdiv(Cx<double>{eps.x - 1.0, eps.y}, Cx<double>{eps.x + 2.0, eps.y}), 3.0 * d3 / (4.0 * M_PI));
-        const double S = (q[3] * q[0]) * (q[3] * q[0]) + (q[4] * q[1]) * (q[4] * q[1]) + (q[5] * q[2]) * (q[5] * q[2]);
-        const Cx<double> poly = Cx<double>{b1 + eps.x * b2 + eps.x * b3 * S, eps.y * b2 + eps.y * b3 * S};
-        Cx<double> inner = cscale(poly, kd * kd);
-        inner.y -= (2.0 / 3.0) * kd * kd * kd;
-        const Cx<double> res = cadd(Cx<double>{1.0, 0.0}, cmul(cscale(a1, 1.0 / d3), inner));
-        al[g] = cdiv(res, a1);  // 1 / (a1 / res)
+        al[g] = ldr_inv_alpha(interp_eps(t[e], mat, nmat), q, K, d);
         Cx<double> ev{0.0, 0.0};
         if (plane) {
             int x, y, z;
@@ -255,6 +267,75 @@ static __global__ void __launch_bounds__(kThreads) k_request_fields(
             ev = Cx<double>{q[6] * q[3 + c] * cs, q[6] * q[3 + c] * sn};
         }
         E[g] = ev;
+    }
+}
+
+// Design update of an extruded structure: one value per (x, y) column,
+// column index x Ny + y.  Para_origin += step, clamped to [0, 1]
+// (CoreStructure.cpp:48-56).
+static __global__ void __launch_bounds__(kThreads) k_design_step(double* __restrict__ origin,
+                                                              const double* __restrict__ step, int n)
+{
+    const int i = blockIdx.x * kThreads + threadIdx.x;
+    if (i < n) {
+        double v = origin[i] + step[i];
+        v = v >= 1.0 ? 1.0 : v;
+        v = v <= 0.0 ? 0.0 : v;
+        origin[i] = v;
+    }
+}
+
+// Cone filter of the column values: the columns within dist^2 <= r^2 - 0.01
+// of a column, weighted r - dist (circlerange / calweight, SpacePara.cpp:37-48;
+// FreeWeight, SpacePara.cpp:1241-1270), normalised by the weights' sum and
+// summed in column order (CoreStructure.cpp:60-80), then projected by
+// SmoothDensity (FilterOption.cpp:49-60).  Contraction is off so that the
+// sums round like the reference's.
+static __global__ void __launch_bounds__(kThreads) k_design_filter(const double* __restrict__ origin, int Nx, int Ny,
+                                                                double r, int reach, double beta, double eta,
+                                                                double* __restrict__ filtered,
+                                                                double* __restrict__ para)
+{
+#pragma clang fp contract(off)
+    const int i = blockIdx.x * kThreads + threadIdx.x;
+    if (i >= Nx * Ny) return;
+    const int xo = i / Ny, yo = i - xo * Ny;
+    double num = 0.0, den = 0.0;
+    for (int x = max(0, xo - reach); x <= min(Nx - 1, xo + reach); x++) {
+        for (int y = max(0, yo - reach); y <= min(Ny - 1, yo + reach); y++) {
+            const double dx = double(x - xo), dy = double(y - yo);
+            const double d2 = dx * dx + dy * dy;
+            if (d2 <= r * r - 0.01) {
+                const double w = r - sqrt(d2);
+                num += w * origin[x * Ny + y];
+                den += w;
+            }
+        }
+    }
+    const double f = num / den;
+    filtered[i] = f;
+    double p;
+    if (f <= eta) p = eta * (exp(-beta * (1 - f / eta)) - (1 - f / eta) * exp(-beta));
+    else p = (1 - eta) * (1 - exp(-beta * (f - eta) / (1 - eta)) + (f - eta) / (1 - eta) * exp(-beta)) + eta;
+    para[i] = p;
+}
+
+// al of every right-hand side of a full-box model from its column values
+// (DDAModel::UpdateAlpha, DDAModel.cpp:470-484; diel_old of site i is the
+// value of its column, CoreStructure.cpp:98-104).  inc: 6 doubles per
+// right-hand side (nK[3], nE[3]).
+static __global__ void __launch_bounds__(kThreads) k_design_alpha(const double* __restrict__ para, int N, int Nz,
+                                                               const Cx<double>* __restrict__ mat, int nmat, double K,
+                                                               double d, const double* __restrict__ inc, int nrhs,
+                                                               Cx<double>* __restrict__ al)
+{
+    const size_t n3 = 3 * (size_t)N;
+    const size_t total = n3 * (size_t)nrhs;
+    for (size_t g = (size_t)blockIdx.x * kThreads + threadIdx.x; g < total; g += (size_t)gridDim.x * kThreads) {
+        const size_t j = g / n3;
+        const size_t e = g - j * n3;
+        const int i = (int)(e / 3);
+        al[g] = ldr_inv_alpha(interp_eps(para[i / Nz], mat, nmat), inc + 6 * j, K, d);
     }
 }
 

@@ -33,7 +33,8 @@ EXPORTS = [
     "tdda_polarizability", "tdda_local_field", "tdda_batch_create", "tdda_batch_destroy", "tdda_batch_size",
     "tdda_batch_solve", "tdda_batch_apply", "tdda_batch_time_passes", "tdda_server_opts_default",
     "tdda_server_create", "tdda_server_destroy", "tdda_server_submit", "tdda_server_poll", "tdda_server_wait",
-    "tdda_server_result", "tdda_server_get_stats",
+    "tdda_server_result", "tdda_server_get_stats", "tdda_design_create", "tdda_design_step", "tdda_design_get",
+    "tdda_design_destroy", "tdda_beta_update",
 ]
 
 
@@ -138,6 +139,13 @@ def lib():
         L.tdda_server_result.argtypes = [vp, ctypes.c_longlong, dp, ctypes.POINTER(SolveInfo),
                                          ctypes.POINTER(RequestTimes)]
         L.tdda_server_get_stats.argtypes = [vp, ctypes.POINTER(ServerStats)]
+        L.tdda_design_create.argtypes = [vp, dp, ctypes.c_double, dp, ctypes.c_int, dp, ctypes.POINTER(vp)]
+        L.tdda_design_step.argtypes = [vp, dp, ctypes.c_double, ctypes.c_double]
+        L.tdda_design_get.argtypes = [vp, dp, dp, dp]
+        L.tdda_design_destroy.argtypes = [vp]
+        L.tdda_design_destroy.restype = None
+        L.tdda_beta_update.argtypes = [ctypes.c_int] + [ctypes.c_double] * 4
+        L.tdda_beta_update.restype = ctypes.c_double
         _lib = L
     return _lib
 
@@ -407,6 +415,59 @@ class Batch:
     def close(self):
         if self.h:
             lib().tdda_batch_destroy(self.h)
+            self.h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+BETA_EXP, BETA_PIECEWISE, BETA_LINEAR = 0, 1, 2
+
+
+def beta_update(kind, it, max_it, beta_min, beta_max):
+    """FilterOption::update_beta (FilterOption.cpp:24-46)."""
+    return lib().tdda_beta_update(kind, float(it), float(max_it), float(beta_min), float(beta_max))
+
+
+class Design:
+    """Device-resident design of an extruded full box: one parameter per
+    (x, y) column (index x * Ny + y) kept in HBM.  step() is
+    CoreStructure::UpdateStr (CoreStructure.cpp:31-104: add, clamp, cone
+    filter, SmoothDensity) followed by DDAModel::UpdateAlpha
+    (DDAModel.cpp:470-484) for every right-hand side of the model.
+    incidences: one (nK, nE) pair per right-hand side of the model."""
+
+    def __init__(self, model: "DeviceModel", para_origin, rfilter, material, incidences):
+        self.model = model
+        po = np.ascontiguousarray(para_origin, dtype=np.float64)
+        mat = np.ascontiguousarray(material, dtype=np.complex128)
+        inc = np.ascontiguousarray([np.concatenate([np.asarray(k, float), np.asarray(e, float)])
+                                    for k, e in incidences], dtype=np.float64).reshape(-1)
+        if len(inc) != 6 * model.nrhs:
+            raise TddaError("one (nK, nE) incidence per right-hand side of the model")
+        self.ncol = len(po)
+        self.h = ctypes.c_void_p()
+        _check(lib().tdda_design_create(model.h, _dptr(po), float(rfilter), _dptr(mat.view(np.float64)), len(mat),
+                                        _dptr(inc), ctypes.byref(self.h)))
+
+    def step(self, step=None, beta=0.0, eta=0.5):
+        st = None if step is None else np.ascontiguousarray(step, dtype=np.float64)
+        if st is not None and len(st) != self.ncol:
+            raise TddaError("step needs one value per column")
+        _check(lib().tdda_design_step(self.h, None if st is None else _dptr(st), float(beta), float(eta)))
+
+    def get(self):
+        """(Para_origin, Para_filtered, Para)."""
+        a, b, c = (np.zeros(self.ncol) for _ in range(3))
+        _check(lib().tdda_design_get(self.h, _dptr(a), _dptr(b), _dptr(c)))
+        return a, b, c
+
+    def close(self):
+        if self.h:
+            lib().tdda_design_destroy(self.h)
             self.h = ctypes.c_void_p()
 
     def __del__(self):
