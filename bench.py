@@ -192,6 +192,39 @@ def bench_batch64(T, W, ctx, reps):
     return out
 
 
+def bench_design(T, W, ctx, reps):
+    """Design step of the topology workload on the device (tdda_design_step,
+    the reference's CoreStructure::UpdateStr + DDAModel::UpdateAlpha): add a
+    step to the 80 x 80 column parameters, cone filter r = 2, SmoothDensity
+    (beta 8, eta 0.5), 1/alpha of 2 right-hand sides (complex128 and its
+    complex64 copy).  One call moves the step in and runs three kernels."""
+    prob, _ = W.topo80()
+    core = T.AProductCore(ctx, prob.R, prob.Nx, prob.Ny, prob.Nz, prob.d, prob.lam, prob.nback)
+    incs = [(np.array([0.0, 0.0, 1.0]), np.array([1.0, 0.0, 0.0])), (np.array([0.0, 0.0, 1.0]), np.array([0.0, 1.0, 0.0]))]
+    m = T.DeviceModel(core, len(incs))
+    ncol = prob.Nx * prob.Ny
+    para0 = prob.t[::3][::prob.Nz].copy()
+    dz = T.Design(m, para0, 2.0, prob.material, incs)
+    rng = np.random.default_rng(7)
+    steps = [0.01 * rng.standard_normal(ncol) for _ in range(max(reps, 1) * 20)]
+    dz.step(steps[0], beta=8.0, eta=0.5)
+    t0 = time.perf_counter()
+    for st in steps:
+        dz.step(st, beta=8.0, eta=0.5)
+    dt = (time.perf_counter() - t0) / len(steps)
+    n3 = 3 * prob.N
+    # step in, origin read + write, filtered + para written, per column;
+    # 1/alpha written per entry and right-hand side (16 B + the 8 B copy)
+    algo = ncol * 8 * 5 + len(incs) * n3 * 24
+    out = {"workload": "design: topo80 columns 80x80 (extruded over 16), cone filter r=2, beta=8, eta=0.5, 2 RHS",
+           "columns": ncol, "rhs": len(incs), "step_us": round(dt * 1e6, 1),
+           "algorithmic_bytes": int(algo), "effective_gbs": round(algo / dt / 1e9, 1)}
+    dz.close()
+    m.close()
+    core.close()
+    return out
+
+
 def bench_serve(T, seconds, rate=200.0, seed=5):
     """Request stream: Poisson arrivals at `rate`/s for `seconds`; dipole
     count log-uniform in [8k, 2M], Nz in {8, 16, 32}, Nx = Ny =
@@ -321,7 +354,7 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--workloads", default="mixed128,batch64,serve",
+    ap.add_argument("--workloads", default="mixed128,batch64,design,serve",
                     help="extra workloads reported beside the headline (comma list, or none)")
     ap.add_argument("--extra-reps", type=int, default=3)
     ap.add_argument("--serve-seconds", type=float, default=1.0, help="arrival window of the request stream")
@@ -397,7 +430,7 @@ def main():
         if w == "serve":
             extras[w] = bench_serve(T, args.serve_seconds)
             continue
-        fn = {"mixed128": bench_mixed128, "batch64": bench_batch64}[w]
+        fn = {"mixed128": bench_mixed128, "batch64": bench_batch64, "design": bench_design}[w]
         extras[w] = fn(T, W, ctx, args.extra_reps)
 
     cpu = None
