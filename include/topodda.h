@@ -6,6 +6,7 @@
  *
  *   reference                                          this ABI
  *   AProductCore::AProductCore  AProductCore.cpp:12-249   tdda_op_create
+ *     (periodic overload)       AProductCore.cpp:251-536  tdda_op_create_periodic
  *   AProductCore::~AProductCore AProductCore.cpp:538-558  tdda_op_destroy
  *   AProductCore::Aproduct      AProductCore.cpp:697-758  tdda_aproduct
  *   AProductCore::LDR_inter     AProductCore.cpp:588-623  tdda_green_tensor
@@ -108,6 +109,13 @@ typedef struct {
 /* AProductCore::AProductCore (AProductCore.cpp:12-249): builds the Green
  * tensor on the device, transforms it and keeps one octant of its spectrum. */
 int tdda_op_create(tdda_ctx ctx, const tdda_op_desc* desc, tdda_op* out);
+/* Periodic AProductCore (AProductCore.cpp:251-536): the tensor of each grid
+ * offset is summed over the images m = -max_m..max_m, n = -max_n..max_n
+ * shifted by (m Lm, n Ln, 0) (lengths in the units of d), each damped by
+ * exp(-(0.01 K r)^4) (A_dic_generator, AProductCore.cpp:573-586).  At most
+ * 1e6 images. */
+int tdda_op_create_periodic(tdda_ctx ctx, const tdda_op_desc* desc, int max_m, int max_n, double Lm, double Ln,
+                            tdda_op* out);
 /* AProductCore::~AProductCore (AProductCore.cpp:538-558). */
 int tdda_op_destroy(tdda_op op);
 /* grid3 <- (Mx, My, Mz); K <- wavenumber.  Either may be NULL. */

@@ -17,6 +17,9 @@ Two layers:
   convolution.  They restate the same formulas independently (different
   algebraic form for the tensor: -G with the textbook dyadic Green function).
 
+* numpy restatement of the periodic operator (AProductCore.cpp:251-536,
+  573-586): an O(N^2 x images) direct sum over the image lattice with the
+  textbook tensor above.
 * numpy restatement of the design update of an extruded structure: the beta
   schedules (tools.cpp:776-798), the cone filter (SpacePara.cpp:37-48,
   1241-1270; CoreStructure.cpp:31-104) and SmoothDensity
@@ -414,3 +417,45 @@ def design_diel_old(para, Nx: int, Ny: int, Nz: int) -> np.ndarray:
     """diel_old of a full box in build_a_bulk order: each site takes the
     value of its column, three components each (CoreStructure.cpp:98-104)."""
     return np.repeat(np.repeat(np.asarray(para, np.float64).reshape(Nx * Ny), Nz), 3)
+
+
+# ---------------------------------------------------------------------------
+# Periodic operator (numpy restatement)
+# ---------------------------------------------------------------------------
+PERIODIC_GAMMA = 0.01  # A_dic_generator, AProductCore.cpp:574
+
+
+def green_periodic(K, x, y, z, pm: int, pn: int, Lm: float, Ln: float):
+    """Tensor of a structure periodic in x and y at offsets (x, y, z)
+    (arrays): sum over the images m = -pm..pm, n = -pn..pn of the tensor at
+    (x + m Lm, y + n Ln, z), each damped by exp(-(0.01 K r)^4) with r the
+    image's distance (A_dic_generator, AProductCore.cpp:573-586; the image
+    sums of AProductCore.cpp:301-312).  The tensor is green_textbook (zero at
+    r = 0).  Returns [..., 6]."""
+    x = np.asarray(x, np.float64)
+    y = np.asarray(y, np.float64)
+    z = np.asarray(z, np.float64)
+    acc = np.zeros(np.broadcast(x, y, z).shape + (6,), np.complex128)
+    for m in range(-pm, pm + 1):
+        for n in range(-pn, pn + 1):
+            xi, yi = x + m * Lm, y + n * Ln
+            r = np.sqrt(xi * xi + yi * yi + z * z)
+            damp = np.exp(-(PERIODIC_GAMMA * K * r) ** 4)
+            acc += green_textbook(K, xi, yi, z) * damp[..., None]
+    return acc
+
+
+def aproduct_periodic_direct(R, d, K, b, pm: int, pn: int, Lm: float, Ln: float):
+    """y_i = sum_j G_per(r_i - r_j) b_j over all site pairs, the self pair
+    included (its (0, 0) image is zero, the other images are not)."""
+    R = np.asarray(R, np.float64).reshape(-1, 3) * d
+    b3 = np.asarray(b, np.complex128).reshape(-1, 3)
+    D = R[:, None, :] - R[None, :, :]
+    G = green_periodic(K, D[..., 0], D[..., 1], D[..., 2], pm, pn, Lm, Ln)
+    idx = [[0, 1, 2], [1, 3, 4], [2, 4, 5]]
+    y = np.zeros_like(b3)
+    for a in range(3):
+        for c in range(3):
+            y[:, a] += G[..., idx[a][c]] @ b3[:, c]
+    return y.reshape(-1)
+

@@ -1074,7 +1074,43 @@ int tdda_local_field(int n3, const double* al, const double* P, const double* di
     return TDDA_OK;
 }
 
+namespace {
+
+// Images of a structure periodic in x and y (pm < 0: not periodic).
+struct PeriodicImages {
+    int pm = -1, pn = -1;
+    double Lm = 0.0, Ln = 0.0;
+};
+
+int op_create(tdda_ctx ctx, const tdda_op_desc* desc, const PeriodicImages& per, tdda_op* out);
+
+}  // namespace
+
 int tdda_op_create(tdda_ctx ctx, const tdda_op_desc* desc, tdda_op* out)
+{
+    return op_create(ctx, desc, PeriodicImages(), out);
+}
+
+int tdda_op_create_periodic(tdda_ctx ctx, const tdda_op_desc* desc, int max_m, int max_n, double Lm, double Ln,
+                            tdda_op* out)
+{
+    if (max_m < 0 || max_n < 0) return fail(TDDA_ERR_ARG, "image counts must be >= 0");
+    if (!(Lm > 0) || !(Ln > 0) || !std::isfinite(Lm) || !std::isfinite(Ln))
+        return fail(TDDA_ERR_ARG, "periods must be positive");
+    if ((double)(2 * max_m + 1) * (2 * max_n + 1) > 1e6) return fail(TDDA_ERR_UNSUPPORTED, "more than 1e6 images");
+    PeriodicImages per;
+    per.pm = max_m;
+    per.pn = max_n;
+    per.Lm = Lm;
+    per.Ln = Ln;
+    return op_create(ctx, desc, per, out);
+}
+
+}  // extern "C"
+
+namespace {
+
+int op_create(tdda_ctx ctx, const tdda_op_desc* desc, const PeriodicImages& per, tdda_op* out)
 {
     if (!ctx || !desc || !out) return fail(TDDA_ERR_ARG, "null argument");
     const tdda_op_desc& D = *desc;
@@ -1156,8 +1192,8 @@ int tdda_op_create(tdda_ctx ctx, const tdda_op_desc* desc, tdda_op* out)
     if (ctx_malloc(ctx, (void**)&full, sizeof(Cx<double>) * Mt * 6) != TDDA_OK)
         return cleanup(fail(TDDA_ERR_OOM, "Green tensor grid"));
     hipStream_t s = ctx->stream;
-    k_green_fill<<<(unsigned)((Mt + kThreads - 1) / kThreads), kThreads, 0, s>>>(full, D.Nx, D.Ny, D.Nz, M[0], M[1],
-                                                                                M[2], D.d, op->K);
+    k_green_fill<<<(unsigned)((Mt + kThreads - 1) / kThreads), kThreads, 0, s>>>(
+        full, D.Nx, D.Ny, D.Nz, M[0], M[1], M[2], D.d, op->K, per.pm, per.pn, per.Lm, per.Ln);
     {
         // z: contiguous pencils [x][y][z][6]
         const size_t budget = ctx->lds_budget;
@@ -1218,6 +1254,10 @@ int tdda_op_create(tdda_ctx ctx, const tdda_op_desc* desc, tdda_op* out)
     *out = op;
     return TDDA_OK;
 }
+
+}  // namespace
+
+extern "C" {
 
 static void op_release(tdda_op op)
 {

@@ -81,9 +81,15 @@ __device__ __forceinline__ void ldr_inter(double K, double x, double y, double z
 
 // Fills the 6-component tensor on the whole padded grid [Mx][My][Mz][6]:
 // index i < N -> offset i d; i >= M - (N - 1) -> (i - M) d; otherwise zero
-// (AProductCore.cpp:54-194 with 2N-1 replaced by M).
+// (AProductCore.cpp:54-194 with 2N-1 replaced by M).  With pm >= 0 the
+// tensor of a structure periodic in x and y (AProductCore.cpp:251-536):
+// the sum over the images m = -pm..pm, n = -pn..pn of the tensor at
+// (x + m Lm, y + n Ln, z), each damped by exp(-(gamma K r)^4), gamma = 0.01,
+// r the image's distance (A_dic_generator, AProductCore.cpp:573-586); the
+// (0, 0) image of offset 0 is zero like the tensor itself.
 static __global__ void __launch_bounds__(kThreads) k_green_fill(Cx<double>* __restrict__ g, int Nx, int Ny, int Nz, int Mx,
-                                                         int My, int Mz, double d, double K)
+                                                         int My, int Mz, double d, double K, int pm, int pn,
+                                                         double Lm, double Ln)
 {
     const size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     const size_t M = (size_t)Mx * My * Mz;
@@ -99,7 +105,21 @@ static __global__ void __launch_bounds__(kThreads) k_green_fill(Cx<double>* __re
         const int oi = (i <= Nx - 1) ? i : i - Mx;
         const int oj = (j <= Ny - 1) ? j : j - My;
         const int ok = (k <= Nz - 1) ? k : k - Mz;
-        ldr_inter(K, d * oi, d * oj, d * ok, a);
+        if (pm < 0) {
+            ldr_inter(K, d * oi, d * oj, d * ok, a);
+        } else {
+            const double gamma = 0.01;
+            for (int l = 0; l < 6; l++) a[l] = Cx<double>{0.0, 0.0};
+            for (int m = -pm; m <= pm; m++) {
+                for (int n = -pn; n <= pn; n++) {
+                    const double x = d * oi + m * Lm, y = d * oj + n * Ln, z = d * ok;
+                    Cx<double> t[6];
+                    ldr_inter(K, x, y, z, t);
+                    const double damp = exp(-pow(gamma * K * sqrt(x * x + y * y + z * z), 4));
+                    for (int l = 0; l < 6; l++) a[l] = cadd(a[l], cscale(t[l], damp));
+                }
+            }
+        }
     } else {
         for (int l = 0; l < 6; l++) a[l] = Cx<double>{0.0, 0.0};
     }

@@ -34,7 +34,7 @@ EXPORTS = [
     "tdda_batch_solve", "tdda_batch_apply", "tdda_batch_time_passes", "tdda_server_opts_default",
     "tdda_server_create", "tdda_server_destroy", "tdda_server_submit", "tdda_server_poll", "tdda_server_wait",
     "tdda_server_result", "tdda_server_get_stats", "tdda_design_create", "tdda_design_step", "tdda_design_get",
-    "tdda_design_destroy", "tdda_beta_update",
+    "tdda_design_destroy", "tdda_beta_update", "tdda_op_create_periodic",
 ]
 
 
@@ -101,6 +101,8 @@ def lib():
         L.tdda_ctx_destroy.argtypes = [vp]
         L.tdda_ctx_sync.argtypes = [vp]
         L.tdda_op_create.argtypes = [vp, ctypes.POINTER(OpDesc), ctypes.POINTER(vp)]
+        L.tdda_op_create_periodic.argtypes = [vp, ctypes.POINTER(OpDesc), ctypes.c_int, ctypes.c_int,
+                                              ctypes.c_double, ctypes.c_double, ctypes.POINTER(vp)]
         L.tdda_op_destroy.argtypes = [vp]
         L.tdda_op_info.argtypes = [vp, ctypes.POINTER(ctypes.c_int), dp]
         L.tdda_aproduct.argtypes = [vp, dp, dp, ctypes.c_int, dp, ctypes.c_int, ctypes.c_int]
@@ -257,9 +259,13 @@ def ne_perp_in_xz(theta):
 
 
 class AProductCore:
-    """Device operator: AProductCore (AProductCore.cpp:12-249, 697-758)."""
+    """Device operator: AProductCore (AProductCore.cpp:12-249, 697-758).
+    periodic=(max_m, max_n, Lm, Ln) builds the periodic operator
+    (AProductCore.cpp:251-536): images m = -max_m..max_m, n = -max_n..max_n
+    shifted by (m Lm, n Ln, 0), each damped by exp(-(0.01 K r)^4)."""
 
-    def __init__(self, ctx: Context, R, Nx, Ny, Nz, d, lam, nback=1.0, grid=None, precisions=C128 | C64):
+    def __init__(self, ctx: Context, R, Nx, Ny, Nz, d, lam, nback=1.0, grid=None, precisions=C128 | C64,
+                 periodic=None):
         self.ctx = ctx
         # R = None: the full box in build_a_bulk order
         self.R = None if R is None else np.ascontiguousarray(R, dtype=np.int32)
@@ -270,7 +276,12 @@ class AProductCore:
         Rp = None if R is None else self.R.ctypes.data_as(ctypes.POINTER(ctypes.c_int))
         desc = OpDesc(self.N, Nx, Ny, Nz, Rp, d, lam, nback, g[0], g[1], g[2], precisions)
         self.h = ctypes.c_void_p()
-        _check(lib().tdda_op_create(ctx.h, ctypes.byref(desc), ctypes.byref(self.h)))
+        if periodic is None:
+            _check(lib().tdda_op_create(ctx.h, ctypes.byref(desc), ctypes.byref(self.h)))
+        else:
+            pm, pn, Lm, Ln = periodic
+            _check(lib().tdda_op_create_periodic(ctx.h, ctypes.byref(desc), int(pm), int(pn), float(Lm), float(Ln),
+                                                 ctypes.byref(self.h)))
         g3 = (ctypes.c_int * 3)()
         K = ctypes.c_double()
         _check(lib().tdda_op_info(self.h, g3, ctypes.byref(K)))
