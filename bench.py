@@ -225,6 +225,27 @@ def bench_design(T, W, ctx, reps):
     return out
 
 
+def bench_periodic(T, W, ctx, reps):
+    """Periodic operator set-up on the device (tdda_op_create_periodic): the
+    image-lattice tensor of an 80 x 80 x 16 unit cell (grid 160 x 160 x 32)
+    with 21 x 21 images, then its spectrum -- the reference sums the images
+    on one host thread (AProductCore.cpp:301-536)."""
+    pm = pn = 10
+    Nx, Ny, Nz, d, lam = 80, 80, 16, 10.0, 542.0
+    T.AProductCore(ctx, None, 8, 8, 4, d, lam, periodic=(1, 1, 80.0, 80.0)).close()  # warm
+    ts = []
+    for _ in range(max(reps, 1)):
+        t0 = time.perf_counter()
+        core = T.AProductCore(ctx, None, Nx, Ny, Nz, d, lam, periodic=(pm, pn, Nx * d, Ny * d))
+        ts.append(time.perf_counter() - t0)
+        grid = core.grid
+        core.close()
+    evals = (2 * Nx - 1) * (2 * Ny - 1) * (2 * Nz - 1) * (2 * pm + 1) * (2 * pn + 1)
+    return {"workload": "periodic: 80x80x16 cell, grid 160x160x32, 21x21 images, Lm = Ln = 800 nm",
+            "grid": list(grid), "images": (2 * pm + 1) * (2 * pn + 1), "setup_ms": round(float(np.median(ts)) * 1e3, 1),
+            "tensor_evals_per_s": round(evals / float(np.median(ts)), 0)}
+
+
 def bench_serve(T, seconds, rate=200.0, seed=5):
     """Request stream: Poisson arrivals at `rate`/s for `seconds`; dipole
     count log-uniform in [8k, 2M], Nz in {8, 16, 32}, Nx = Ny =
@@ -354,7 +375,7 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--workloads", default="mixed128,batch64,design,serve",
+    ap.add_argument("--workloads", default="mixed128,batch64,design,periodic,serve",
                     help="extra workloads reported beside the headline (comma list, or none)")
     ap.add_argument("--extra-reps", type=int, default=3)
     ap.add_argument("--serve-seconds", type=float, default=1.0, help="arrival window of the request stream")
@@ -430,7 +451,8 @@ def main():
         if w == "serve":
             extras[w] = bench_serve(T, args.serve_seconds)
             continue
-        fn = {"mixed128": bench_mixed128, "batch64": bench_batch64, "design": bench_design}[w]
+        fn = {"mixed128": bench_mixed128, "batch64": bench_batch64, "design": bench_design,
+              "periodic": bench_periodic}[w]
         extras[w] = fn(T, W, ctx, args.extra_reps)
 
     cpu = None
