@@ -34,6 +34,20 @@ AProductCore::AProductCore(CoreStructure* CStr_, double lam_, VectorXcd material
                            std::string AMatrixMethod_, int precisions, const int* grid)
     : CStr(CStr_), lam(lam_), nback(nback_), material(material_), AMatrixMethod(AMatrixMethod_)
 {
+    build(nullptr, nullptr, precisions, grid);
+}
+
+AProductCore::AProductCore(CoreStructure* CStr_, double lam_, VectorXcd material_, double nback_, int MAXm, int MAXn,
+                           double Lm, double Ln, std::string AMatrixMethod_, int precisions, const int* grid)
+    : CStr(CStr_), lam(lam_), nback(nback_), material(material_), AMatrixMethod(AMatrixMethod_)
+{
+    const int mn[2] = {MAXm, MAXn};
+    const double L[2] = {Lm, Ln};
+    build(mn, L, precisions, grid);
+}
+
+void AProductCore::build(const int* periodic_mn, const double* periodic_L, int precisions, const int* grid)
+{
     if (AMatrixMethod != "LDR")
         throw std::invalid_argument("\"" + AMatrixMethod + "\" is not a supported polarizabiliy calculation method.");
     K = 2 * M_PI / (lam / nback);  // AProductCore.cpp:17
@@ -54,7 +68,11 @@ AProductCore::AProductCore(CoreStructure* CStr_, double lam_, VectorXcd material
         D.Mz = grid[2];
     }
     D.precisions = precisions;
-    check(tdda_op_create(default_context(), &D, &op));
+    if (periodic_mn)
+        check(tdda_op_create_periodic(default_context(), &D, periodic_mn[0], periodic_mn[1], periodic_L[0],
+                                      periodic_L[1], &op));
+    else
+        check(tdda_op_create(default_context(), &D, &op));
 }
 
 AProductCore::~AProductCore()

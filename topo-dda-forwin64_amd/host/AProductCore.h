@@ -28,6 +28,12 @@ public:
     // (AProductCore.cpp:567-570).  grid = {0,0,0} picks the FFT grid.
     AProductCore(CoreStructure* CStr, double lam, VectorXcd material, double nback, std::string AMatrixMethod,
                  int precisions = TDDA_C128 | TDDA_C64, const int* grid = nullptr);
+    // Periodic overload (AProductCore.cpp:251-536): images m = -MAXm..MAXm,
+    // n = -MAXn..MAXn shifted by (m Lm, n Ln, 0), damped per image
+    // (AProductCore.cpp:573-586).
+    AProductCore(CoreStructure* CStr, double lam, VectorXcd material, double nback, int MAXm, int MAXn, double Lm,
+                 double Ln, std::string AMatrixMethod, int precisions = TDDA_C128 | TDDA_C64,
+                 const int* grid = nullptr);
     ~AProductCore();
     AProductCore(const AProductCore&) = delete;
     AProductCore& operator=(const AProductCore&) = delete;
@@ -55,6 +61,7 @@ public:
     tdda_op handle() { return op; }
 
 private:
+    void build(const int* periodic_mn, const double* periodic_L, int precisions, const int* grid);
     CoreStructure* CStr;
     double K, lam, nback;
     VectorXcd material;
