@@ -19,6 +19,9 @@
  *   CoreStructure::UpdateStr    CoreStructure.cpp:31-104  tdda_design_step (with
  *     + DDAModel::UpdateAlpha   DDAModel.cpp:470-484       tdda_design_create/_get)
  *   FilterOption::update_beta   FilterOption.cpp:24-46    tdda_beta_update
+ *   ObjPointEDDAModel           ObjDDAModel.cpp:20-97     tdda_objective_point_e (value and
+ *   ObjIntegratedEDDAModel      ObjDDAModel.cpp:100-177   tdda_objective_integrated_e  the
+ *     + EvoDDAModel::devp       EvoDDAModel.cpp:353-391   adjoint source in closed form)
  *   (kernel.cu A2As/B2Bs/Conv/Conv2B and the cuFFT calls are internal.)
  *   No counterpart (one solve at a time in the reference, T/main.cpp:584-598):
  *   tdda_batch_* (many structures per launch), tdda_server_* (request loop),
@@ -326,6 +329,29 @@ void tdda_design_destroy(tdda_design dz);
 /* FilterOption::update_beta (FilterOption.cpp:24-46): beta at iteration it
  * of max_it for the TDDA_BETA_* schedule. */
 double tdda_beta_update(int kind, double it, double max_it, double beta_min, double beta_max);
+
+/* ------------------------------------------------------------------------
+ * Objectives and their adjoint sources on the device.  The value F of an
+ * objective for right-hand side `rhs` of model m (its current P), and, when
+ * dst is not NULL, the adjoint right-hand side 2 dF/dP in closed form --
+ * the quantity EvoDDAModel::devp (EvoDDAModel.cpp:353-391) estimates by
+ * one-sided finite differences -- written into the E vector of right-hand
+ * side dst_rhs of dst (a model on the same operator, e.g. m itself or the
+ * model of the adjoint solve).
+ * ---------------------------------------------------------------------- */
+/* ObjPointEDDAModel (ObjDDAModel.cpp:20-97): F = |E_ext - sum_i
+ * A(r0 - d R_i) P_i| with E_ext = E0 nE exp(i K nK . r0); r0 in the units of
+ * d (not scaled by d).  Source: -(1/F) sum_a A_ac conj(E_a). */
+int tdda_objective_point_e(tdda_model m, int rhs, const double* r0, const double* nK, const double* nE, double E0,
+                           double* value, tdda_model dst, int dst_rhs);
+/* ObjIntegratedEDDAModel (ObjDDAModel.cpp:100-177): F = log sum over the
+ * region's sites of SmoothDensity(diel_old_3i; eta, beta) sum_c
+ * |al_c P_c|^pow_num.  box6 = xMin, xMax, yMin, yMax, zMin, zMax tested as
+ * the reference writes it, (lo <= v <= hi) with C semantics
+ * (ObjDDAModel.cpp:138).  diel_old: 3N host values in [0, 1].  Source:
+ * pow_num f |al|^pow_num |P|^(pow_num - 2) conj(P) / E_int (0 where P = 0). */
+int tdda_objective_integrated_e(tdda_model m, int rhs, int pow_num, const int* box6, double eta, double beta,
+                                const double* diel_old, double* value, tdda_model dst, int dst_rhs);
 
 #ifdef __cplusplus
 }

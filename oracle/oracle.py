@@ -20,6 +20,9 @@ Two layers:
 * numpy restatement of the periodic operator (AProductCore.cpp:251-536,
   573-586): an O(N^2 x images) direct sum over the image lattice with the
   textbook tensor above.
+* numpy restatement of the objectives PointE and IntegratedE
+  (ObjDDAModel.cpp:20-177), of the reference's finite-difference adjoint
+  source (EvoDDAModel.cpp:353-391) and of the closed-form source.
 * numpy restatement of the design update of an extruded structure: the beta
   schedules (tools.cpp:776-798), the cone filter (SpacePara.cpp:37-48,
   1241-1270; CoreStructure.cpp:31-104) and SmoothDensity
@@ -458,4 +461,114 @@ def aproduct_periodic_direct(R, d, K, b, pm: int, pn: int, Lm: float, Ln: float)
         for c in range(3):
             y[:, a] += G[..., idx[a][c]] @ b3[:, c]
     return y.reshape(-1)
+
+
+# ---------------------------------------------------------------------------
+# Objectives and adjoint sources (numpy restatement)
+# ---------------------------------------------------------------------------
+def _tensor3(a6):
+    a = np.asarray(a6)
+    return np.array([[a[0], a[1], a[2]], [a[1], a[3], a[4]], [a[2], a[4], a[5]]])
+
+
+def point_e_field(R, d, K, P, r0, nK, nE, E0=1.0):
+    """ObjPointEDDAModel (ObjDDAModel.cpp:20-97): the field at r0,
+    E_ext - sum_i A(r0 - d R_i) P_i, E_ext = E0 nE exp(i K nK . r0), with the
+    tensor of the C restatement.  Returns (E, [A_i])."""
+    R = np.asarray(R).reshape(-1, 3)
+    P3 = np.asarray(P, np.complex128).reshape(-1, 3)
+    r0 = np.asarray(r0, np.float64)
+    ph = K * float(np.dot(nK, r0))
+    E = E0 * np.asarray(nE, np.float64) * complex(math.cos(ph), math.sin(ph))
+    As = []
+    for i in range(len(R)):
+        A = _tensor3(ldr_inter(K, *(r0 - d * R[i])))
+        As.append(A)
+        E = E - A @ P3[i]
+    return E, As
+
+
+def obj_point_e(R, d, K, P, r0, nK, nE, E0=1.0) -> float:
+    """ObjPointEDDAModel::GetVal: |E(r0)| (Eigen norm)."""
+    return float(np.sqrt(np.sum(np.abs(point_e_field(R, d, K, P, r0, nK, nE, E0)[0]) ** 2)))
+
+
+def _le_chain(lo, v, hi):
+    # (lo <= v <= hi) as C++ evaluates it: (lo <= v) is 0 or 1
+    return (1 if lo <= v else 0) <= hi
+
+
+def integrated_e_terms(R, al, P, diel_old, pw, box, eta, beta):
+    """Per entry f_i |al P|^pow of ObjIntegratedEDDAModel::SingleResponse
+    (ObjDDAModel.cpp:134-158), zero outside the region test (ObjDDAModel.cpp:138)."""
+    R = np.asarray(R).reshape(-1, 3)
+    t = np.zeros(3 * len(R))
+    fvec = np.zeros(3 * len(R))
+    for i in range(len(R)):
+        x, y, z = (int(v) for v in R[i])
+        if _le_chain(box[0], x, box[1]) and _le_chain(box[2], y, box[3]) and _le_chain(box[4], z, box[5]):
+            f = smooth_density(float(diel_old[3 * i]), beta, eta)
+            fvec[3 * i:3 * i + 3] = f
+            for c in range(3):
+                t[3 * i + c] = f * abs(al[3 * i + c] * P[3 * i + c]) ** pw
+    return t, fvec
+
+
+def obj_integrated_e(R, al, P, diel_old, pw, box, eta, beta) -> float:
+    """ObjIntegratedEDDAModel::GetVal: log of the sum (ObjDDAModel.cpp:160-177)."""
+    return math.log(float(np.sum(integrated_e_terms(R, al, P, diel_old, pw, box, eta, beta)[0])))
+
+
+def devp_fd_point_e(R, d, K, P, r0, nK, nE, E0, eps, central=False):
+    """EvoDDAModel::devp (EvoDDAModel.cpp:353-391) for PointE:
+    (F(P + eps e_i) - F) / eps + (F(P + i eps e_i) - F) / (i eps), each
+    entry updating only its site's term (SingleResponse); central=True
+    takes centred differences instead (for pinning the closed form)."""
+    E, As = point_e_field(R, d, K, P, r0, nK, nE, E0)
+    F = float(np.linalg.norm(E))
+    out = np.zeros(len(P), np.complex128)
+    for k in range(len(P)):
+        i, c = divmod(k, 3)
+        col = As[i][:, c]
+
+        def Fd(delta):
+            return float(np.linalg.norm(E - col * delta))
+        if central:
+            out[k] = (Fd(eps) - Fd(-eps)) / (2 * eps) + (Fd(1j * eps) - Fd(-1j * eps)) / (2j * eps)
+        else:
+            out[k] = (Fd(eps) - F) / eps + (Fd(1j * eps) - F) / (1j * eps)
+    return out
+
+
+def devp_point_e(R, d, K, P, r0, nK, nE, E0=1.0):
+    """Closed form of the PointE source: -(1/F) sum_a A_ac conj(E_a)."""
+    E, As = point_e_field(R, d, K, P, r0, nK, nE, E0)
+    F = float(np.linalg.norm(E))
+    return np.concatenate([-(A.T @ np.conj(E)) / F for A in As])
+
+
+def devp_fd_integrated_e(R, al, P, diel_old, pw, box, eta, beta, eps, central=False):
+    """EvoDDAModel::devp for IntegratedE (F = log E_int)."""
+    t, fvec = integrated_e_terms(R, al, P, diel_old, pw, box, eta, beta)
+    S = float(np.sum(t))
+    out = np.zeros(len(P), np.complex128)
+    for k in range(len(P)):
+        def Fd(delta):
+            return math.log(S - t[k] + fvec[k] * abs(al[k] * (P[k] + delta)) ** pw)
+        if central:
+            out[k] = (Fd(eps) - Fd(-eps)) / (2 * eps) + (Fd(1j * eps) - Fd(-1j * eps)) / (2j * eps)
+        else:
+            F = math.log(S)
+            out[k] = (Fd(eps) - F) / eps + (Fd(1j * eps) - F) / (1j * eps)
+    return out
+
+
+def devp_integrated_e(R, al, P, diel_old, pw, box, eta, beta):
+    """Closed form of the IntegratedE source: pow f |al|^pow |P|^(pow-2) conj(P) / E_int."""
+    t, fvec = integrated_e_terms(R, al, P, diel_old, pw, box, eta, beta)
+    S = float(np.sum(t))
+    aP = np.abs(P)
+    with np.errstate(divide="ignore", invalid="ignore"):
+        g = np.where(aP > 0, pw * fvec * np.abs(al) ** pw * aP ** (pw - 2.0) * np.conj(P), 0)
+    return g / S
 

@@ -180,6 +180,7 @@ struct tdda_op_s {
     double d, lam, nback, K;
     int prec;
     int* cellmap = nullptr;
+    int* site_xyz = nullptr;  // 3N site coordinates for the objectives, built on first use
     Cx<double>* green64 = nullptr;
     Cx<float>* green32 = nullptr;
     // full tables (Green set-up, compile-time pass kernels)
@@ -1265,6 +1266,7 @@ static void op_release(tdda_op op)
     (void)hipSetDevice(op->ctx->device);
     (void)hipStreamSynchronize(op->ctx->stream);
     ctx_free(op->ctx, op->cellmap);
+    ctx_free(op->ctx, op->site_xyz);
     ctx_free(op->ctx, op->green64);
     ctx_free(op->ctx, op->green32);
     for (int a = 0; a < 3; a++) {
@@ -1749,3 +1751,4 @@ int tdda_batch_apply(tdda_batch b, int precision)
 
 #include "tdda_serve.inc"
 #include "tdda_design.inc"
+#include "tdda_objective.inc"

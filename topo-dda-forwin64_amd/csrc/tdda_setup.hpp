@@ -359,6 +359,156 @@ static __global__ void __launch_bounds__(kThreads) k_design_alpha(const double* 
     }
 }
 
+// ---------------------------------------------------------------------------
+// Objectives of the reference (ObjDDAModel.cpp) and their adjoint sources.
+// The reference estimates the adjoint source 2 dF/dP of an objective F by a
+// one-sided finite difference per entry of P, real and imaginary
+// (EvoDDAModel::devp, EvoDDAModel.cpp:353-391):
+//   devp_i = (F(P + e_i eps) - F) / eps + (F(P + i e_i eps) - F) / (i eps)
+//          ~ dF/dRe P_i - i dF/dIm P_i.
+// Here it is the same quantity in closed form.  Site coordinates come from R
+// or, for R == nullptr, the full box in build_a_bulk order.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void site_xyz(const int* __restrict__ R, int i, int Ny, int Nz, int& x, int& y, int& z)
+{
+    if (R) {
+        x = R[3 * i];
+        y = R[3 * i + 1];
+        z = R[3 * i + 2];
+    } else {
+        x = i / (Ny * Nz);
+        const int r = i - x * Ny * Nz;
+        y = r / Nz;
+        z = r - y * Nz;
+    }
+}
+
+// ObjPointEDDAModel (ObjDDAModel.cpp:20-97): the field at r0 is
+// E = E_ext - sum_i A(r0 - r_i d) P_i; per workgroup the partial sum of
+// -A(r0 - r_i d) P_i (3 complex values) in part[6 * blockIdx.x ...].
+static __global__ void __launch_bounds__(kThreads) k_obj_point_e(const Cx<double>* __restrict__ P, const int* __restrict__ R,
+                                                              int N, int Ny, int Nz, double d, double K, double x0,
+                                                              double y0, double z0, double* __restrict__ part)
+{
+    __shared__ double red[16 * 6];
+    double acc[6] = {0, 0, 0, 0, 0, 0};
+    for (int i = blockIdx.x * kThreads + threadIdx.x; i < N; i += gridDim.x * kThreads) {
+        int x, y, z;
+        site_xyz(R, i, Ny, Nz, x, y, z);
+        Cx<double> a[6];
+        ldr_inter(K, x0 - d * x, y0 - d * y, z0 - d * z, a);
+        const Cx<double> p0 = P[3 * i], p1 = P[3 * i + 1], p2 = P[3 * i + 2];
+        const Cx<double> e0 = cadd(cadd(cmul(a[0], p0), cmul(a[1], p1)), cmul(a[2], p2));
+        const Cx<double> e1 = cadd(cadd(cmul(a[1], p0), cmul(a[3], p1)), cmul(a[4], p2));
+        const Cx<double> e2 = cadd(cadd(cmul(a[2], p0), cmul(a[4], p1)), cmul(a[5], p2));
+        acc[0] -= e0.x;
+        acc[1] -= e0.y;
+        acc[2] -= e1.x;
+        acc[3] -= e1.y;
+        acc[4] -= e2.x;
+        acc[5] -= e2.y;
+    }
+#pragma unroll
+    for (int k = 0; k < 6; k++) {
+        double v = acc[k];
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+        acc[k] = v;
+    }
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    if (lane == 0)
+        for (int k = 0; k < 6; k++) red[wid * 6 + k] = acc[k];
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int k = 0; k < 6; k++) {
+            double v = 0.0;
+            for (int w = 0; w < kThreads / 64; w++) v += red[w * 6 + k];
+            part[blockIdx.x * 6 + k] = v;
+        }
+    }
+}
+
+// Adjoint source of PointE, F = |E|: 2 dF/dP_ic = -(1/F) sum_a A_ac conj(E_a).
+static __global__ void __launch_bounds__(kThreads) k_obj_point_e_source(const int* __restrict__ R, int N, int Ny,
+                                                                     int Nz, double d, double K, double x0, double y0,
+                                                                     double z0, Cx<double> E0, Cx<double> E1,
+                                                                     Cx<double> E2, double invF,
+                                                                     Cx<double>* __restrict__ dst)
+{
+    for (int i = blockIdx.x * kThreads + threadIdx.x; i < N; i += gridDim.x * kThreads) {
+        int x, y, z;
+        site_xyz(R, i, Ny, Nz, x, y, z);
+        Cx<double> a[6];
+        ldr_inter(K, x0 - d * x, y0 - d * y, z0 - d * z, a);
+        const int col[3][3] = {{0, 1, 2}, {1, 3, 4}, {2, 4, 5}};
+        for (int c = 0; c < 3; c++) {
+            // sum_a A_ac conj(E_a)
+            Cx<double> s = cmulc(a[col[0][c]], E0);
+            s = cadd(s, cmulc(a[col[1][c]], E1));
+            s = cadd(s, cmulc(a[col[2][c]], E2));
+            dst[3 * i + c] = cscale(s, -invF);
+        }
+    }
+}
+
+// IntegratedE's region test as the reference writes it,
+// (lo <= v <= hi) with C++ semantics: (lo <= v) is 0 or 1, compared with hi
+// (ObjDDAModel.cpp:138).
+__device__ __forceinline__ bool chained_le(int lo, int v, int hi) { return (lo <= v ? 1 : 0) <= hi; }
+
+__device__ __forceinline__ double smooth_density_dev(double x, double eta, double beta)
+{
+    // ObjDDAModel.cpp:5-16 (same function as FilterOption::SmoothDensity)
+    if (x <= eta) return eta * (exp(-beta * (1 - x / eta)) - (1 - x / eta) * exp(-beta));
+    return (1 - eta) * (1 - exp(-beta * (x - eta) / (1 - eta)) + (x - eta) / (1 - eta) * exp(-beta)) + eta;
+}
+
+// ObjIntegratedEDDAModel (ObjDDAModel.cpp:100-177): E_int = sum over sites
+// in the region of SmoothDensity(diel_old_3i) sum_c |al_c P_c|^pow; per
+// workgroup partial sums in part[blockIdx.x].  With `src`, also the
+// adjoint source of F = log E_int up to the factor 1 / E_int:
+// pow f |al|^pow |P|^(pow - 2) conj(P) (zero where P = 0).
+static __global__ void __launch_bounds__(kThreads) k_obj_integrated_e(
+    const Cx<double>* __restrict__ P, const Cx<double>* __restrict__ al, const double* __restrict__ diel_old,
+    const int* __restrict__ R, int N, int Ny, int Nz, int pw, int xmin, int xmax, int ymin, int ymax, int zmin,
+    int zmax, double eta, double beta, double* __restrict__ part, Cx<double>* __restrict__ src)
+{
+    __shared__ double red[16];
+    double acc = 0.0;
+    for (int i = blockIdx.x * kThreads + threadIdx.x; i < N; i += gridDim.x * kThreads) {
+        int x, y, z;
+        site_xyz(R, i, Ny, Nz, x, y, z);
+        const bool in = chained_le(xmin, x, xmax) && chained_le(ymin, y, ymax) && chained_le(zmin, z, zmax);
+        const double f = in ? smooth_density_dev(diel_old[3 * i], eta, beta) : 0.0;
+        for (int c = 0; c < 3; c++) {
+            const Cx<double> a = al[3 * i + c], p = P[3 * i + c];
+            const double aa = sqrt(a.x * a.x + a.y * a.y), pp = sqrt(p.x * p.x + p.y * p.y);
+            if (in) acc += f * pow(aa * pp, (double)pw);
+            if (src) {
+                Cx<double> g{0.0, 0.0};
+                if (in && pp > 0.0) g = cscale(Cx<double>{p.x, -p.y}, pw * f * pow(aa, (double)pw) * pow(pp, (double)(pw - 2)));
+                src[3 * i + c] = g;
+            }
+        }
+    }
+    for (int off = 32; off > 0; off >>= 1) acc += __shfl_xor(acc, off, 64);
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    if (lane == 0) red[wid] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double v = 0.0;
+        for (int w = 0; w < kThreads / 64; w++) v += red[w];
+        part[blockIdx.x] = v;
+    }
+}
+
+// dst *= s (the 1 / E_int of the IntegratedE source)
+static __global__ void __launch_bounds__(kThreads) k_scale(Cx<double>* __restrict__ v, size_t n, double s)
+{
+    const size_t i = (size_t)blockIdx.x * kThreads + threadIdx.x;
+    if (i < n) v[i] = cscale(v[i], s);
+}
+
 // Sparse right-hand sides (adjoint sources): E[j][idx[j][k]] = val[j][k].
 static __global__ void __launch_bounds__(kThreads) k_sparse_rhs(Cx<double>* __restrict__ E, size_t n3, int nrhs,
                                                          int nsrc, const int* __restrict__ idx,

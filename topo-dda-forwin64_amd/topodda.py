@@ -34,7 +34,8 @@ EXPORTS = [
     "tdda_batch_solve", "tdda_batch_apply", "tdda_batch_time_passes", "tdda_server_opts_default",
     "tdda_server_create", "tdda_server_destroy", "tdda_server_submit", "tdda_server_poll", "tdda_server_wait",
     "tdda_server_result", "tdda_server_get_stats", "tdda_design_create", "tdda_design_step", "tdda_design_get",
-    "tdda_design_destroy", "tdda_beta_update", "tdda_op_create_periodic",
+    "tdda_design_destroy", "tdda_beta_update", "tdda_op_create_periodic", "tdda_objective_point_e",
+    "tdda_objective_integrated_e",
 ]
 
 
@@ -148,6 +149,9 @@ def lib():
         L.tdda_design_destroy.restype = None
         L.tdda_beta_update.argtypes = [ctypes.c_int] + [ctypes.c_double] * 4
         L.tdda_beta_update.restype = ctypes.c_double
+        L.tdda_objective_point_e.argtypes = [vp, ctypes.c_int, dp, dp, dp, ctypes.c_double, dp, vp, ctypes.c_int]
+        L.tdda_objective_integrated_e.argtypes = [vp, ctypes.c_int, ctypes.c_int, ip, ctypes.c_double,
+                                                  ctypes.c_double, dp, dp, vp, ctypes.c_int]
         _lib = L
     return _lib
 
@@ -433,6 +437,30 @@ class Batch:
             self.close()
         except Exception:
             pass
+
+
+def objective_point_e(model, rhs, r0, nK, nE, E0=1.0, source=None, source_rhs=0):
+    """ObjPointEDDAModel value (ObjDDAModel.cpp:20-97) of right-hand side rhs;
+    with source=(a DeviceModel on the same operator), its closed-form adjoint
+    source 2 dF/dP (EvoDDAModel::devp, EvoDDAModel.cpp:353-391) is written to
+    that model's E of source_rhs."""
+    v = ctypes.c_double()
+    a = [np.ascontiguousarray(x, dtype=np.float64) for x in (r0, nK, nE)]
+    _check(lib().tdda_objective_point_e(model.h, rhs, _dptr(a[0]), _dptr(a[1]), _dptr(a[2]), float(E0),
+                                        ctypes.byref(v), source.h if source is not None else None, source_rhs))
+    return v.value
+
+
+def objective_integrated_e(model, rhs, pow_num, box, eta, beta, diel_old, source=None, source_rhs=0):
+    """ObjIntegratedEDDAModel value (ObjDDAModel.cpp:100-177); box = (xMin,
+    xMax, yMin, yMax, zMin, zMax); source as for objective_point_e."""
+    v = ctypes.c_double()
+    b6 = np.ascontiguousarray(box, dtype=np.int32)
+    dl = np.ascontiguousarray(diel_old, dtype=np.float64)
+    _check(lib().tdda_objective_integrated_e(model.h, rhs, int(pow_num), b6.ctypes.data_as(ctypes.POINTER(ctypes.c_int)),
+                                             float(eta), float(beta), _dptr(dl), ctypes.byref(v),
+                                             source.h if source is not None else None, source_rhs))
+    return v.value
 
 
 BETA_EXP, BETA_PIECEWISE, BETA_LINEAR = 0, 1, 2
