@@ -219,6 +219,15 @@ def bench_design(T, W, ctx, reps):
     out = {"workload": "design: topo80 columns 80x80 (extruded over 16), cone filter r=2, beta=8, eta=0.5, 2 RHS",
            "columns": ncol, "rhs": len(incs), "step_us": round(dt * 1e6, 1),
            "algorithmic_bytes": int(algo), "effective_gbs": round(algo / dt / 1e9, 1)}
+    # objective value and its adjoint source on the device (PointE at a point
+    # above the slab; the source goes to the second right-hand side)
+    m.set(T.VEC_P, np.tile(prob.rhs(*incs[0])[1], len(incs)))
+    r0 = np.array([400.0, 400.0, 300.0])
+    T.objective_point_e(m, 0, r0, incs[0][0], incs[0][1], 1.0, source=m, source_rhs=1)
+    t0 = time.perf_counter()
+    for _ in range(20):
+        T.objective_point_e(m, 0, r0, incs[0][0], incs[0][1], 1.0, source=m, source_rhs=1)
+    out["point_e_objective_and_source_us"] = round((time.perf_counter() - t0) / 20 * 1e6, 1)
     dz.close()
     m.close()
     core.close()
