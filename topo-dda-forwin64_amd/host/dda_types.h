@@ -1,7 +1,7 @@
 // dda_types.h -- small dense vector types used by the host classes.
 //
 // The reference's classes take and return Eigen types (VectorXcd, VectorXi,
-// VectorXd, Vector3d, Matrix3cd).  Eigen is not a dependency of this build;
+// VectorXd, Vector3d, Matrix3cd).  Eigen is not a dependency of this library;
 // these types keep the same names and the operations the reference's callers
 // use: element access with (), size(), Zero(n), +, -, scalar *, and Eigen's
 // sesquilinear a.dot(b) = sum conj(a_i) b_i and norm()
