@@ -119,6 +119,15 @@ int tdda_op_create(tdda_ctx ctx, const tdda_op_desc* desc, tdda_op* out);
  * 1e6 images. */
 int tdda_op_create_periodic(tdda_ctx ctx, const tdda_op_desc* desc, int max_m, int max_n, double Lm, double Ln,
                             tdda_op* out);
+/* Tile autotuner (no counterpart in the reference, which leaves tiling to
+ * cuFFT): times the z passes with z tiles of 1, 1/2 and 1/4 of the planned
+ * width and the x pass with x tiles narrowed 3 columns at a time, each on a
+ * scratch model (reps launches per timing; <= 0: 20), and keeps the fastest
+ * per precision.  Affects models created afterwards. */
+int tdda_op_autotune(tdda_op op, int reps);
+/* Current tiles of one precision: out4 = z pencils per tile (P5), z pencils
+ * per tile (P1), y columns per tile, x columns per tile. */
+int tdda_op_tiles(tdda_op op, int precision, int* out4);
 /* AProductCore::~AProductCore (AProductCore.cpp:538-558). */
 int tdda_op_destroy(tdda_op op);
 /* grid3 <- (Mx, My, Mz); K <- wavenumber.  Either may be NULL. */

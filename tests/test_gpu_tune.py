@@ -1,0 +1,70 @@
+"""GPU tests of the tile autotuner (tdda_op_autotune): the tuned tiles stay
+within the planner's bounds, the tuned operator still matches the oracle
+(AProductCore.cpp:697-758) to the operator bar, and a tuned model gives the
+same bits alone and in a batch with an untuned one."""
+import numpy as np
+import pytest
+
+import topodda as T
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.parametrize("dims", [(24, 20, 8), (80, 30, 6)])
+def test_autotuned_operator_matches_oracle(ctx, dims):
+    Nx, Ny, Nz = dims
+    d, lam = 10.0, 500.0
+    print(f"autotune {dims}", flush=True)
+    core = T.AProductCore(ctx, None, Nx, Ny, Nz, d, lam)
+    before = {p: core.tiles(p) for p in (T.C128, T.C64)}
+    core.autotune(reps=10)
+    for p in (T.C128, T.C64):
+        pz, pz1, tfy, tfx = core.tiles(p)
+        bz, bz1, by, bx = before[p]
+        assert 1 <= pz <= bz and 1 <= pz1 <= bz1 and tfy == by and 3 <= tfx <= bx and tfx % 3 == 0
+    R = O.build_a_bulk(Nx, Ny, Nz)
+    rng = np.random.default_rng(3)
+    b = rng.standard_normal(len(R)) + 1j * rng.standard_normal(len(R))
+    ref = O.aproduct_numpy_fft(R, Nx, Ny, Nz, d, O.wavenumber(lam), b, grid=core.grid)
+    assert O.rel_l2(core.Aproduct(b), ref) < 1e-12
+    assert O.rel_l2(core.Aproduct(b, precision=T.C64), ref) < 5e-6
+    core.close()
+
+
+def test_tuned_model_alone_equals_batched(ctx):
+    specs = [(24, 20, 8, 10.0, 500.0), (12, 7, 5, 10.0, 542.0)]
+    cores, models, data = [], [], []
+    for i, (Nx, Ny, Nz, d, lam) in enumerate(specs):
+        core = T.AProductCore(ctx, None, Nx, Ny, Nz, d, lam)
+        if i == 0:
+            core.autotune(reps=10)
+        R = O.build_a_bulk(Nx, Ny, Nz)
+        K = O.wavenumber(lam)
+        t = np.random.default_rng(i).random(len(R) // 3).repeat(3)
+        nK, nE = np.array([0, 0, 1.0]), np.array([1.0, 0, 0])
+        _, al = O.diel_al(t, O.material_rule(np.array([1.0, 2.25 + 0.01j])), lam, K, d, nE, nK)
+        E = O.einc(R, K, d, 1.0, nK, nE)
+        m = T.DeviceModel(core, 1)
+        m.set(T.VEC_AL, al)
+        m.set(T.VEC_E, E)
+        m.set(T.VEC_P, np.zeros_like(E))
+        cores.append(core)
+        models.append(m)
+        data.append((al, E))
+    b = T.Batch(models)
+    infos = b.solve(mode=T.MODE_MIXED, tol=1e-8)
+    for core, m, (al, E), ib in zip(cores, models, data, infos):
+        one = T.DeviceModel(core, 1)
+        one.set(T.VEC_AL, al)
+        one.set(T.VEC_E, E)
+        one.set(T.VEC_P, np.zeros_like(E))
+        i1 = one.solve(mode=T.MODE_MIXED, tol=1e-8)[0]
+        assert i1["status"] == T.CONVERGED and i1["iterations"] == ib["iterations"]
+        assert np.array_equal(one.get(T.VEC_P)[0], m.get(T.VEC_P)[0])
+        one.close()
+    b.close()
+    for m in models:
+        m.close()
+    for c in cores:
+        c.close()

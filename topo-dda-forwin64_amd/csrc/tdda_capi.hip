@@ -173,6 +173,11 @@ struct TilePlan {
     size_t lds_z, lds_z1, lds_y, lds_x;
 };
 
+// Upper limits on the planner's tile widths (0: none), set by the autotuner.
+struct TileCaps {
+    int pz = 0, tfy = 0, tfx = 0;
+};
+
 struct tdda_op_s {
     tdda_ctx ctx;
     int refs = 1;
@@ -286,7 +291,7 @@ int px_buffers(int L)
 // strided passes are kept to whole 128-byte lines where the budget allows,
 // and the Green product of the x pass covers at most 8 points per thread.
 template <typename T>
-int plan_tiles(const tdda_op_s* op, TilePlan& t)
+int plan_tiles(const tdda_op_s* op, TilePlan& t, const TileCaps& caps = TileCaps())
 {
     const size_t s = sizeof(Cx<T>);
     // prefetch capacity of a workgroup in each pass (elements), as compiled
@@ -326,6 +331,12 @@ int plan_tiles(const tdda_op_s* op, TilePlan& t)
         if (tx >= 3 * unit && tx < 3 * op->Mz) tx = (tx / (3 * unit)) * (3 * unit);
     }
     t.tfx = std::max(3, tx);
+    if (caps.pz > 0) {
+        t.pz = std::min(t.pz, caps.pz);
+        t.pz1 = std::min(t.pz1, caps.pz);
+    }
+    if (caps.tfy > 0) t.tfy = std::min(t.tfy, caps.tfy);
+    if (caps.tfx > 0) t.tfx = std::max(3, std::min(t.tfx, (caps.tfx / 3) * 3));
     if ((t.tfx / 3) * op->Mx > 8 * kThreads) return fail(TDDA_ERR_UNSUPPORTED, "x pass tile too wide");
     auto tws = [&](int kind, int L, int tb) { return pass_tw_slots(pass_length_compiled<T>(kind, L) && pass_full_twiddles<T>(kind, L), L, tb); };
     t.lds_z = (size_t)(tws(0, op->Mz, op->pl[2].tb) + pz_buffers<T>(op->Mz) * 3 * t.pz * t.ps_z) * s;
@@ -1752,3 +1763,4 @@ int tdda_batch_apply(tdda_batch b, int precision)
 #include "tdda_serve.inc"
 #include "tdda_design.inc"
 #include "tdda_objective.inc"
+#include "tdda_tune.inc"

@@ -35,7 +35,7 @@ EXPORTS = [
     "tdda_server_create", "tdda_server_destroy", "tdda_server_submit", "tdda_server_poll", "tdda_server_wait",
     "tdda_server_result", "tdda_server_get_stats", "tdda_design_create", "tdda_design_step", "tdda_design_get",
     "tdda_design_destroy", "tdda_beta_update", "tdda_op_create_periodic", "tdda_objective_point_e",
-    "tdda_objective_integrated_e",
+    "tdda_objective_integrated_e", "tdda_op_autotune", "tdda_op_tiles",
 ]
 
 
@@ -149,6 +149,8 @@ def lib():
         L.tdda_design_destroy.restype = None
         L.tdda_beta_update.argtypes = [ctypes.c_int] + [ctypes.c_double] * 4
         L.tdda_beta_update.restype = ctypes.c_double
+        L.tdda_op_autotune.argtypes = [vp, ctypes.c_int]
+        L.tdda_op_tiles.argtypes = [vp, ctypes.c_int, ip]
         L.tdda_objective_point_e.argtypes = [vp, ctypes.c_int, dp, dp, dp, ctypes.c_double, dp, vp, ctypes.c_int]
         L.tdda_objective_integrated_e.argtypes = [vp, ctypes.c_int, ctypes.c_int, ip, ctypes.c_double,
                                                   ctypes.c_double, dp, dp, vp, ctypes.c_int]
@@ -291,6 +293,16 @@ class AProductCore:
         _check(lib().tdda_op_info(self.h, g3, ctypes.byref(K)))
         self.grid = tuple(g3)
         self.K = K.value
+
+    def autotune(self, reps=20):
+        """Tile autotuner (tdda_op_autotune): affects models created afterwards."""
+        _check(lib().tdda_op_autotune(self.h, int(reps)))
+
+    def tiles(self, precision=C128):
+        """(z pencils P5, z pencils P1, y columns, x columns) per tile."""
+        out = (ctypes.c_int * 4)()
+        _check(lib().tdda_op_tiles(self.h, precision, out))
+        return tuple(out)
 
     def Aproduct(self, b, al=None, precision=C128):
         """y = A_offdiag b (+ al o b); b is (3N,) or (nrhs, 3N) complex128."""
