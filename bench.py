@@ -32,6 +32,7 @@ sys.path.insert(0, PKG)
 
 HBM_PEAK_GBS = 8000.0
 TOL = 1e-8
+TUNE = True  # operators run the tile autotuner before their models are built (--no-tune: planner tiles)
 
 
 def pass_bytes(core, s):
@@ -98,6 +99,8 @@ def bench_mixed128(T, W, ctx, reps):
     prob = W.mixed128(seed=3)
     al, E, _ = prob.rhs(np.array([0.0, 0.0, 1.0]), np.array([1.0, 0.0, 0.0]))
     core = T.AProductCore(ctx, prob.R, prob.Nx, prob.Ny, prob.Nz, prob.d, prob.lam, prob.nback)
+    if TUNE:
+        core.autotune()
     m = T.DeviceModel(core, 1)
     m.set(T.VEC_AL, al)
     m.set(T.VEC_E, E)
@@ -138,6 +141,8 @@ def bench_batch64(T, W, ctx, reps):
     models = []
     for p in probs:
         core = T.AProductCore(ctx, p.R, p.Nx, p.Ny, p.Nz, p.d, p.lam, p.nback)
+        if TUNE:
+            core.autotune()
         rhs = [p.rhs(nK, nE)[:2] for nK, nE in incs]
         m = T.DeviceModel(core, len(incs))
         m.set(T.VEC_AL, np.concatenate([a for a, _ in rhs]))
@@ -388,7 +393,10 @@ def main():
                     help="extra workloads reported beside the headline (comma list, or none)")
     ap.add_argument("--extra-reps", type=int, default=3)
     ap.add_argument("--serve-seconds", type=float, default=1.0, help="arrival window of the request stream")
+    ap.add_argument("--no-tune", action="store_true", help="planner tiles instead of the autotuner's")
     args = ap.parse_args()
+    global TUNE
+    TUNE = not args.no_tune
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); see the sharding design notes.",
               file=sys.stderr)
@@ -403,6 +411,8 @@ def main():
     al, E, diel = prob.rhs(nK, nE)
     ctx = T.Context(0)
     core = T.AProductCore(ctx, prob.R, prob.Nx, prob.Ny, prob.Nz, prob.d, prob.lam, prob.nback)
+    if TUNE:
+        core.autotune()
     F = T.DeviceModel(core, 1)
     A = T.DeviceModel(core, 1)
     for m in (F, A):
@@ -484,7 +494,7 @@ def main():
         "data": "synthetic",
         "config": {"workload": "topo80: 80x80x16 slab, d=10nm, lambda=542nm, grid 160x160x32, "
                                "forward + adjoint (100 targets) per step", "grid": list(core.grid),
-                   "dipoles": prob.N},
+                   "dipoles": prob.N, "tiles_c128": list(core.tiles(T.C128)), "autotuned": TUNE},
         "latency_ms": {"p50": round(fp["p50_ms"], 3), "p99": round(fp["p99_ms"], 3)},
         "fp64": fp,
         "mixed": results["mixed"],

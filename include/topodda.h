@@ -25,7 +25,8 @@
  *   (kernel.cu A2As/B2Bs/Conv/Conv2B and the cuFFT calls are internal.)
  *   No counterpart (one solve at a time in the reference, T/main.cpp:584-598):
  *   tdda_batch_* (many structures per launch), tdda_server_* (request loop),
- *   the measurement hooks tdda_*_time_pass(es).
+ *   tdda_op_autotune / tdda_op_tiles (tile autotuner), the measurement hooks
+ *   tdda_*_time_pass(es).
  *
  * Conventions
  *   - Every function returns TDDA_OK (0) or a negative status; the message of
