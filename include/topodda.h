@@ -7,6 +7,7 @@
  *   reference                                          this ABI
  *   AProductCore::AProductCore  AProductCore.cpp:12-249   tdda_op_create
  *     (periodic overload)       AProductCore.cpp:251-536  tdda_op_create_periodic
+ *     (AMatrixMethod "FCD")     AProductCore.cpp:625-695  tdda_op_create_ex, tdda_green_tensor_fcd
  *   AProductCore::~AProductCore AProductCore.cpp:538-558  tdda_op_destroy
  *   AProductCore::Aproduct      AProductCore.cpp:697-758  tdda_aproduct
  *   AProductCore::LDR_inter     AProductCore.cpp:588-623  tdda_green_tensor
@@ -120,6 +121,18 @@ int tdda_op_create(tdda_ctx ctx, const tdda_op_desc* desc, tdda_op* out);
  * 1e6 images. */
 int tdda_op_create_periodic(tdda_ctx ctx, const tdda_op_desc* desc, int max_m, int max_n, double Lm, double Ln,
                             tdda_op* out);
+/* Interaction tensor of an operator (A_dic_generator, AProductCore.cpp:560-571). */
+#define TDDA_METHOD_LDR 0 /* LDR_inter, AProductCore.cpp:588-623 */
+#define TDDA_METHOD_FCD 1 /* FCD_inter, AProductCore.cpp:625-695 (needs K d < pi - 0.1) */
+typedef struct {
+    int method;         /* TDDA_METHOD_* */
+    int max_m, max_n;   /* images of a periodic structure; both < 0: not periodic */
+    double Lm, Ln;      /* periods (units of d) */
+} tdda_op_ext;
+/* AProductCore with the tensor method and, optionally, periodic images
+ * (the reference's constructors, AProductCore.cpp:12-249 and 251-536, with
+ * their AMatrixMethod argument). */
+int tdda_op_create_ex(tdda_ctx ctx, const tdda_op_desc* desc, const tdda_op_ext* ext, tdda_op* out);
 /* Tile autotuner (no counterpart in the reference, which leaves tiling to
  * cuFFT): times the z passes with z tiles of 1, 1/2 and 1/4 of the planned
  * width and the x pass with x tiles narrowed 3 columns at a time, each on a
@@ -147,6 +160,9 @@ int tdda_aproduct(tdda_op op, const double* b, double* y, int nrhs, const double
  * unique components xx xy xz yy yz zz of the interaction tensor at offset
  * (x, y, z) for wavenumber K, as 12 doubles.                                */
 int tdda_green_tensor(double K, double x, double y, double z, double* out12);
+/* AProductCore::FCD_inter (AProductCore.cpp:625-695) for lattice spacing d,
+ * with the reference's Si / Ci table interpolation (SiCi.cpp:27-46). */
+int tdda_green_tensor_fcd(double K, double d, double x, double y, double z, double* out12);
 
 /* Host-side DDAModel set-up math (pure host code, no device needed). */
 /* Material rule of AProductCore.cpp:19-24: eps /= nback^2, +0.01 if |eps| <= 1.01

@@ -20,6 +20,9 @@ Two layers:
 * numpy restatement of the periodic operator (AProductCore.cpp:251-536,
   573-586): an O(N^2 x images) direct sum over the image lattice with the
   textbook tensor above.
+* numpy restatement of the FCD tensor (AProductCore.cpp:625-695) with the
+  reference's Si / Ci tables (SiCi/SiCi.py: scipy.special.sici at
+  x = 0.1, 0.2, ...) and their linear interpolation (SiCi.cpp:27-46).
 * numpy restatement of the objectives PointE and IntegratedE
   (ObjDDAModel.cpp:20-177), of the reference's finite-difference adjoint
   source (EvoDDAModel.cpp:353-391) and of the closed-form source.
@@ -428,13 +431,14 @@ def design_diel_old(para, Nx: int, Ny: int, Nz: int) -> np.ndarray:
 PERIODIC_GAMMA = 0.01  # A_dic_generator, AProductCore.cpp:574
 
 
-def green_periodic(K, x, y, z, pm: int, pn: int, Lm: float, Ln: float):
+def green_periodic(K, x, y, z, pm: int, pn: int, Lm: float, Ln: float, fcd_d=None):
     """Tensor of a structure periodic in x and y at offsets (x, y, z)
     (arrays): sum over the images m = -pm..pm, n = -pn..pn of the tensor at
     (x + m Lm, y + n Ln, z), each damped by exp(-(0.01 K r)^4) with r the
     image's distance (A_dic_generator, AProductCore.cpp:573-586; the image
-    sums of AProductCore.cpp:301-312).  The tensor is green_textbook (zero at
-    r = 0).  Returns [..., 6]."""
+    sums of AProductCore.cpp:301-312).  The tensor is green_textbook, or the
+    FCD tensor of lattice spacing fcd_d when that is given (zero at r = 0).
+    Returns [..., 6]."""
     x = np.asarray(x, np.float64)
     y = np.asarray(y, np.float64)
     z = np.asarray(z, np.float64)
@@ -444,17 +448,19 @@ def green_periodic(K, x, y, z, pm: int, pn: int, Lm: float, Ln: float):
             xi, yi = x + m * Lm, y + n * Ln
             r = np.sqrt(xi * xi + yi * yi + z * z)
             damp = np.exp(-(PERIODIC_GAMMA * K * r) ** 4)
-            acc += green_textbook(K, xi, yi, z) * damp[..., None]
+            g = green_textbook(K, xi, yi, z) if fcd_d is None else fcd_inter(K, fcd_d, xi, yi, z)
+            acc += g * damp[..., None]
     return acc
 
 
-def aproduct_periodic_direct(R, d, K, b, pm: int, pn: int, Lm: float, Ln: float):
+def aproduct_periodic_direct(R, d, K, b, pm: int, pn: int, Lm: float, Ln: float, fcd=False):
     """y_i = sum_j G_per(r_i - r_j) b_j over all site pairs, the self pair
-    included (its (0, 0) image is zero, the other images are not)."""
+    included (its (0, 0) image is zero, the other images are not); fcd
+    selects the FCD tensor."""
     R = np.asarray(R, np.float64).reshape(-1, 3) * d
     b3 = np.asarray(b, np.complex128).reshape(-1, 3)
     D = R[:, None, :] - R[None, :, :]
-    G = green_periodic(K, D[..., 0], D[..., 1], D[..., 2], pm, pn, Lm, Ln)
+    G = green_periodic(K, D[..., 0], D[..., 1], D[..., 2], pm, pn, Lm, Ln, d if fcd else None)
     idx = [[0, 1, 2], [1, 3, 4], [2, 4, 5]]
     y = np.zeros_like(b3)
     for a in range(3):
@@ -571,4 +577,95 @@ def devp_integrated_e(R, al, P, diel_old, pw, box, eta, beta):
     with np.errstate(divide="ignore", invalid="ignore"):
         g = np.where(aP > 0, pw * fvec * np.abs(al) ** pw * aP ** (pw - 2.0) * np.conj(P), 0)
     return g / S
+
+
+# ---------------------------------------------------------------------------
+# FCD tensor (numpy restatement)
+# ---------------------------------------------------------------------------
+SICI_STEP = 0.1  # SiCi/SiCi.py:10
+
+
+def sici_table(x):
+    """SiCi::get_Si / get_Ci (SiCi.cpp:27-46): linear interpolation between
+    the nodes floor(x / 0.1) 0.1 and ceil(x / 0.1) 0.1 of tables holding
+    scipy.special.sici at the nodes (SiCi/SiCi.py); on a node, the node's
+    value.  Scalars or arrays."""
+    from scipy.special import sici
+    x = np.asarray(x, np.float64)
+    a1 = np.floor(x / SICI_STEP) * SICI_STEP
+    a2 = np.ceil(x / SICI_STEP) * SICI_STEP
+    s1, c1 = sici(a1)
+    s2, c2 = sici(a2)
+    on = a1 == a2
+    den = np.where(on, 1.0, a2 - a1)
+    si = np.where(on, s1, s1 + ((s2 - s1) / den) * (x - a1))
+    ci = np.where(on, c1, c1 + ((c2 - c1) / den) * (x - a1))
+    if si.ndim == 0:
+        return float(si), float(ci)
+    return si, ci
+
+
+def fcd_radial(K, d, r, sici=None):
+    """The radial functions of AProductCore::FCD_inter (AProductCore.cpp:
+    625-695): the filtered scalar Green function gf(r), its first and second
+    r-derivatives gf1, gf2, and the filter kernel hr (r > 0, scalars or
+    arrays).  `sici` (x -> (Si, Ci)) defaults to the reference's tables."""
+    sici = sici or sici_table
+    r = np.asarray(r, np.float64)
+    Kf = math.pi / d
+    up, dn = (Kf + K) * r, (Kf - K) * r
+    Sz, Cz = sici(up)
+    Sf, Cf = sici(dn)
+    Cz1, Cf1 = np.cos(up) / r, np.cos(dn) / r
+    Sz1, Sf1 = np.sin(up) / r, np.sin(dn) / r
+    Cz2 = (-up * np.sin(up) - np.cos(up)) / r ** 2
+    Cf2 = (-dn * np.sin(dn) - np.cos(dn)) / r ** 2
+    Sz2 = (up * np.cos(up) - np.sin(up)) / r ** 2
+    Sf2 = (dn * np.cos(dn) - np.sin(dn)) / r ** 2
+    kr = K * r
+    sk, ck = np.sin(kr), np.cos(kr)
+    CC = math.pi * 1j + Cf - Cz
+    C1C1, C2C2 = Cf1 - Cz1, Cf2 - Cz2
+    SS, S1S1, S2S2 = Sz + Sf, Sz1 + Sf1, Sz2 + Sf2
+    gf = (sk * CC + ck * SS) / (math.pi * r)
+    c1 = (kr * ck - sk) * CC - (kr * sk + ck) * SS + r * sk * C1C1 + r * ck * S1S1
+    c2 = (-K * kr * sk) * CC - (K * kr * ck) * SS + (2 * kr * ck) * C1C1 - (2 * kr * sk) * S1S1 + r * sk * C2C2 \
+        + r * ck * S2S2
+    gf1 = c1 / (math.pi * r ** 2)
+    gf2 = (r * c2 - 2.0 * c1) / (math.pi * r ** 3)
+    hr = (np.sin(Kf * r) - Kf * r * np.cos(Kf * r)) / (2.0 * math.pi ** 2 * r ** 3)
+    return gf, gf1, gf2, hr
+
+
+def fcd_inter(K, d, x, y, z, sici=None):
+    """AProductCore::FCD_inter (AProductCore.cpp:625-695): [..., 6] as
+    xx xy xz yy yz zz of -(K^2 gf I + grad grad gf + (4 pi / 3) hr I), zero
+    at r = 0."""
+    x = np.asarray(x, np.float64)
+    y = np.asarray(y, np.float64)
+    z = np.asarray(z, np.float64)
+    r = np.sqrt(x * x + y * y + z * z)
+    zero = r == 0.0
+    rs = np.where(zero, 1.0, r)
+    gf, gf1, gf2, hr = fcd_radial(K, d, rs, sici)
+    t1 = K * K * gf + gf1 / rs + (4.0 * math.pi / 3.0) * hr
+    t2 = (gf2 - gf1 / rs) / rs ** 2
+    out = -np.stack([t1 + x * x * t2, x * y * t2, z * x * t2, t1 + y * y * t2, y * z * t2, t1 + z * z * t2],
+                    axis=-1)
+    return np.where(zero[..., None], 0.0, out)
+
+
+def aproduct_fcd_direct(R, d, K, b):
+    """O(N^2) direct sum of the off-diagonal product with the FCD tensor."""
+    R = np.asarray(R, np.float64).reshape(-1, 3) * d
+    b3 = np.asarray(b, np.complex128).reshape(-1, 3)
+    D = R[:, None, :] - R[None, :, :]
+    G = fcd_inter(K, d, D[..., 0], D[..., 1], D[..., 2])
+    idx = [[0, 1, 2], [1, 3, 4], [2, 4, 5]]
+    y = np.zeros_like(b3)
+    for a in range(3):
+        for c in range(3):
+            y[:, a] += G[..., idx[a][c]] @ b3[:, c]
+    return y.reshape(-1)
+
 

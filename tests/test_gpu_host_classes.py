@@ -50,15 +50,38 @@ def test_host_classes_forward_adjoint(tmp_path):
     assert abs(lam_adj @ E - devp @ Pf) < 1e-7 * abs(devp @ Pf)
 
 
-def test_host_classes_periodic_overload(tmp_path):
+@pytest.mark.parametrize("method", ["LDR", "FCD"])
+def test_host_classes_periodic_overload(tmp_path, method):
     # the periodic AProductCore overload (AProductCore.cpp:251-536) through the
     # same driver: Aproductwithalb against the oracle's image sum, and the
     # forward solution's residual under that operator
     Nx, Ny, Nz, d, lam = 6, 5, 3, 10.0, 542.0
     images = (2, 1, 70.0, 60.0)
     out = tmp_path / "host_periodic.bin"
-    print("dda_host_check 6x5x3 periodic", flush=True)
-    r = subprocess.run([EXE, str(out), str(Nx), str(Ny), str(Nz), str(d), str(lam), "7"] + [str(v) for v in images],
+    print("dda_host_check 6x5x3 periodic", method, flush=True)
+    r = subprocess.run([EXE, str(out), str(Nx), str(Ny), str(Nz), str(d), str(lam), "7"] + [str(v) for v in images] +
+                       [method], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    raw = out.read_bytes()
+    N = int(np.frombuffer(raw[:4], np.int32)[0])
+    n3 = 3 * N
+    Pf, lam_adj, y, al, E, diel = np.frombuffer(raw[24:24 + 6 * n3 * 16], np.complex128).reshape(6, n3)
+    R = O.build_a_bulk(Nx, Ny, Nz)
+    K = O.wavenumber(lam)
+    fcd = method == "FCD"
+    assert O.rel_l2(y, O.aproduct_periodic_direct(R, d, K, E, *images, fcd=fcd) + al * E) < 1e-12
+    res = O.aproduct_periodic_direct(R, d, K, Pf, *images, fcd=fcd) + al * Pf
+    assert O.rel_l2(res, E) < 1e-9
+
+
+def test_host_classes_fcd(tmp_path):
+    # AProductCore(..., "FCD") (AProductCore.cpp:12-249 with FCD_inter,
+    # 625-695) through the driver: Aproductwithalb against the oracle's direct
+    # sum of the FCD tensor, and the forward solution's residual
+    Nx, Ny, Nz, d, lam = 7, 5, 4, 10.0, 542.0
+    out = tmp_path / "host_fcd.bin"
+    print("dda_host_check 7x5x4 FCD", flush=True)
+    r = subprocess.run([EXE, str(out), str(Nx), str(Ny), str(Nz), str(d), str(lam), "9", "FCD"],
                        capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stderr
     raw = out.read_bytes()
@@ -67,6 +90,5 @@ def test_host_classes_periodic_overload(tmp_path):
     Pf, lam_adj, y, al, E, diel = np.frombuffer(raw[24:24 + 6 * n3 * 16], np.complex128).reshape(6, n3)
     R = O.build_a_bulk(Nx, Ny, Nz)
     K = O.wavenumber(lam)
-    assert O.rel_l2(y, O.aproduct_periodic_direct(R, d, K, E, *images) + al * E) < 1e-12
-    res = O.aproduct_periodic_direct(R, d, K, Pf, *images) + al * Pf
-    assert O.rel_l2(res, E) < 1e-9
+    assert O.rel_l2(y, O.aproduct_fcd_direct(R, d, K, E) + al * E) < 1e-12
+    assert O.rel_l2(O.aproduct_fcd_direct(R, d, K, Pf) + al * Pf, E) < 1e-9

@@ -970,6 +970,14 @@ int tdda_ctx_sync(tdda_ctx ctx)
     return TDDA_OK;
 }
 
+int tdda_green_tensor_fcd(double K, double d, double x, double y, double z, double* out12)
+{
+    if (!out12) return fail(TDDA_ERR_ARG, "null output");
+    if (!(d > 0) || !(M_PI - K * d > kSiCiStep)) return fail(TDDA_ERR_ARG, "FCD needs d > 0 and K d < pi - 0.1");
+    fcd_inter(K, d, x, y, z, out12);
+    return TDDA_OK;
+}
+
 int tdda_green_tensor(double K, double x, double y, double z, double* out12)
 {
     if (!out12) return fail(TDDA_ERR_ARG, "null output");
@@ -1088,10 +1096,12 @@ int tdda_local_field(int n3, const double* al, const double* P, const double* di
 
 namespace {
 
-// Images of a structure periodic in x and y (pm < 0: not periodic).
+// Images of a structure periodic in x and y (pm < 0: not periodic) and the
+// interaction tensor (0: LDR, 1: FCD).
 struct PeriodicImages {
     int pm = -1, pn = -1;
     double Lm = 0.0, Ln = 0.0;
+    int method = TDDA_METHOD_LDR;
 };
 
 int op_create(tdda_ctx ctx, const tdda_op_desc* desc, const PeriodicImages& per, tdda_op* out);
@@ -1101,6 +1111,27 @@ int op_create(tdda_ctx ctx, const tdda_op_desc* desc, const PeriodicImages& per,
 int tdda_op_create(tdda_ctx ctx, const tdda_op_desc* desc, tdda_op* out)
 {
     return op_create(ctx, desc, PeriodicImages(), out);
+}
+
+int tdda_op_create_ex(tdda_ctx ctx, const tdda_op_desc* desc, const tdda_op_ext* ext, tdda_op* out)
+{
+    if (!ext) return fail(TDDA_ERR_ARG, "null argument");
+    if (ext->method != TDDA_METHOD_LDR && ext->method != TDDA_METHOD_FCD)
+        return fail(TDDA_ERR_ARG, "unknown interaction method");
+    PeriodicImages per;
+    if (ext->max_m >= 0 || ext->max_n >= 0) {
+        if (ext->max_m < 0 || ext->max_n < 0) return fail(TDDA_ERR_ARG, "image counts must both be >= 0");
+        if (!(ext->Lm > 0) || !(ext->Ln > 0) || !std::isfinite(ext->Lm) || !std::isfinite(ext->Ln))
+            return fail(TDDA_ERR_ARG, "periods must be positive");
+        if ((double)(2 * ext->max_m + 1) * (2 * ext->max_n + 1) > 1e6)
+            return fail(TDDA_ERR_UNSUPPORTED, "more than 1e6 images");
+        per.pm = ext->max_m;
+        per.pn = ext->max_n;
+        per.Lm = ext->Lm;
+        per.Ln = ext->Ln;
+    }
+    per.method = ext->method;
+    return op_create(ctx, desc, per, out);
 }
 
 int tdda_op_create_periodic(tdda_ctx ctx, const tdda_op_desc* desc, int max_m, int max_n, double Lm, double Ln,
@@ -1166,6 +1197,12 @@ int op_create(tdda_ctx ctx, const tdda_op_desc* desc, const PeriodicImages& per,
     op->Hx = M[0] / 2 + 1; op->Hy = M[1] / 2 + 1; op->Hz = M[2] / 2 + 1;
     op->d = D.d; op->lam = D.lam; op->nback = D.nback;
     op->K = 2.0 * M_PI / (D.lam / D.nback);  // AProductCore.cpp:17
+    if (per.method == TDDA_METHOD_FCD && !(M_PI - op->K * D.d > kSiCiStep)) {
+        // the smallest Si / Ci argument, (pi / d - K) d, must lie inside the
+        // reference's tables (x >= 0.1)
+        tdda_op_destroy(op);
+        return fail(TDDA_ERR_UNSUPPORTED, "FCD needs K d < pi - 0.1");
+    }
     op->prec = prec;
     auto cleanup = [&](int rc) {
         tdda_op_destroy(op);
@@ -1205,7 +1242,8 @@ int op_create(tdda_ctx ctx, const tdda_op_desc* desc, const PeriodicImages& per,
         return cleanup(fail(TDDA_ERR_OOM, "Green tensor grid"));
     hipStream_t s = ctx->stream;
     k_green_fill<<<(unsigned)((Mt + kThreads - 1) / kThreads), kThreads, 0, s>>>(
-        full, D.Nx, D.Ny, D.Nz, M[0], M[1], M[2], D.d, op->K, per.pm, per.pn, per.Lm, per.Ln);
+        full, D.Nx, D.Ny, D.Nz, M[0], M[1], M[2], D.d, op->K, per.pm, per.pn, per.Lm, per.Ln,
+        per.method == TDDA_METHOD_FCD ? 1 : 0);
     {
         // z: contiguous pencils [x][y][z][6]
         const size_t budget = ctx->lds_budget;

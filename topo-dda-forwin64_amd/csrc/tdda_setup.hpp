@@ -3,6 +3,7 @@
 // tensor on the padded grid, its 3D FFT, octant extraction).  Included by
 // tdda_capi.hip only.
 #pragma once
+#include "tdda_fcd.hpp"
 #include "tdda_kernels.hpp"
 
 namespace tdda {
@@ -86,10 +87,25 @@ __device__ __forceinline__ void ldr_inter(double K, double x, double y, double z
 // the sum over the images m = -pm..pm, n = -pn..pn of the tensor at
 // (x + m Lm, y + n Ln, z), each damped by exp(-(gamma K r)^4), gamma = 0.01,
 // r the image's distance (A_dic_generator, AProductCore.cpp:573-586); the
-// (0, 0) image of offset 0 is zero like the tensor itself.
+// (0, 0) image of offset 0 is zero like the tensor itself.  fcd != 0 selects
+// the FCD tensor instead of LDR.
+// The tensor of one offset: LDR (AProductCore.cpp:588-623) or, fcd != 0,
+// FCD (AProductCore.cpp:625-695); A_dic_generator, AProductCore.cpp:560-571.
+__device__ __forceinline__ void inter_tensor(int fcd, double K, double d, double x, double y, double z,
+                                             Cx<double>* out)
+{
+    if (fcd) {
+        double t[12];
+        fcd_inter(K, d, x, y, z, t);
+        for (int l = 0; l < 6; l++) out[l] = Cx<double>{t[2 * l], t[2 * l + 1]};
+    } else {
+        ldr_inter(K, x, y, z, out);
+    }
+}
+
 static __global__ void __launch_bounds__(kThreads) k_green_fill(Cx<double>* __restrict__ g, int Nx, int Ny, int Nz, int Mx,
                                                          int My, int Mz, double d, double K, int pm, int pn,
-                                                         double Lm, double Ln)
+                                                         double Lm, double Ln, int fcd)
 {
     const size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     const size_t M = (size_t)Mx * My * Mz;
@@ -106,7 +122,7 @@ static __global__ void __launch_bounds__(kThreads) k_green_fill(Cx<double>* __re
         const int oj = (j <= Ny - 1) ? j : j - My;
         const int ok = (k <= Nz - 1) ? k : k - Mz;
         if (pm < 0) {
-            ldr_inter(K, d * oi, d * oj, d * ok, a);
+            inter_tensor(fcd, K, d, d * oi, d * oj, d * ok, a);
         } else {
             const double gamma = 0.01;
             for (int l = 0; l < 6; l++) a[l] = Cx<double>{0.0, 0.0};
@@ -114,7 +130,7 @@ static __global__ void __launch_bounds__(kThreads) k_green_fill(Cx<double>* __re
                 for (int n = -pn; n <= pn; n++) {
                     const double x = d * oi + m * Lm, y = d * oj + n * Ln, z = d * ok;
                     Cx<double> t[6];
-                    ldr_inter(K, x, y, z, t);
+                    inter_tensor(fcd, K, d, x, y, z, t);
                     const double damp = exp(-pow(gamma * K * sqrt(x * x + y * y + z * z), 4));
                     for (int l = 0; l < 6; l++) a[l] = cadd(a[l], cscale(t[l], damp));
                 }

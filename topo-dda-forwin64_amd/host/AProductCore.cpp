@@ -48,8 +48,9 @@ AProductCore::AProductCore(CoreStructure* CStr_, double lam_, VectorXcd material
 
 void AProductCore::build(const int* periodic_mn, const double* periodic_L, int precisions, const int* grid)
 {
-    if (AMatrixMethod != "LDR")
-        throw std::invalid_argument("\"" + AMatrixMethod + "\" is not a supported polarizabiliy calculation method.");
+    // AProductCore.cpp:560-571: "FCD" or "LDR", anything else throws
+    if (AMatrixMethod != "LDR" && AMatrixMethod != "FCD")
+        throw std::invalid_argument("\"" + AMatrixMethod + "\" is not a valid polarizabiliy calculation method.");
     K = 2 * M_PI / (lam / nback);  // AProductCore.cpp:17
     // AProductCore.cpp:19-24
     check(tdda_material_rule((int)material.size(), reinterpret_cast<double*>(material.data()), nback));
@@ -68,11 +69,13 @@ void AProductCore::build(const int* periodic_mn, const double* periodic_L, int p
         D.Mz = grid[2];
     }
     D.precisions = precisions;
-    if (periodic_mn)
-        check(tdda_op_create_periodic(default_context(), &D, periodic_mn[0], periodic_mn[1], periodic_L[0],
-                                      periodic_L[1], &op));
-    else
-        check(tdda_op_create(default_context(), &D, &op));
+    tdda_op_ext E{};
+    E.method = AMatrixMethod == "FCD" ? TDDA_METHOD_FCD : TDDA_METHOD_LDR;
+    E.max_m = periodic_mn ? periodic_mn[0] : -1;
+    E.max_n = periodic_mn ? periodic_mn[1] : -1;
+    E.Lm = periodic_L ? periodic_L[0] : 0.0;
+    E.Ln = periodic_L ? periodic_L[1] : 0.0;
+    check(tdda_op_create_ex(default_context(), &D, &E, &op));
 }
 
 AProductCore::~AProductCore()
@@ -88,10 +91,9 @@ VectorXcd AProductCore::Aproduct(VectorXcd& b)
     return y;
 }
 
-Matrix3cd AProductCore::LDR_inter(double x, double y, double z)
+namespace {
+Matrix3cd tensor_matrix(const double* a)
 {
-    double a[12];
-    check(tdda_green_tensor(K, x, y, z, a));
     Matrix3cd m;
     const int first[6] = {0, 0, 0, 1, 1, 2};
     const int second[6] = {0, 1, 2, 1, 2, 2};
@@ -101,10 +103,25 @@ Matrix3cd AProductCore::LDR_inter(double x, double y, double z)
     }
     return m;
 }
+}  // namespace
+
+Matrix3cd AProductCore::LDR_inter(double x, double y, double z)
+{
+    double a[12];
+    check(tdda_green_tensor(K, x, y, z, a));
+    return tensor_matrix(a);
+}
+
+Matrix3cd AProductCore::FCD_inter(double x, double y, double z)
+{
+    double a[12];
+    check(tdda_green_tensor_fcd(K, CStr->get_d(), x, y, z, a));
+    return tensor_matrix(a);
+}
 
 Matrix3cd AProductCore::A_dic_generator(double x, double y, double z)
 {
-    return LDR_inter(x, y, z);
+    return AMatrixMethod == "FCD" ? FCD_inter(x, y, z) : LDR_inter(x, y, z);
 }
 
 }  // namespace topodda

@@ -22,10 +22,10 @@ void check(int rc);
 
 class AProductCore {
 public:
-    // AProductCore.cpp:12-249.  Only the "LDR" interaction is built on the
-    // device; any other method string throws std::invalid_argument, as the
-    // reference's A_dic_generator does for unknown methods
-    // (AProductCore.cpp:567-570).  grid = {0,0,0} picks the FFT grid.
+    // AProductCore.cpp:12-249.  AMatrixMethod "LDR" or "FCD" (the filtered
+    // tensor, which needs K d < pi - 0.1); any other string throws
+    // std::invalid_argument, as the reference's A_dic_generator does
+    // (AProductCore.cpp:560-571).  grid = {0,0,0} picks the FFT grid.
     AProductCore(CoreStructure* CStr, double lam, VectorXcd material, double nback, std::string AMatrixMethod,
                  int precisions = TDDA_C128 | TDDA_C64, const int* grid = nullptr);
     // Periodic overload (AProductCore.cpp:251-536): images m = -MAXm..MAXm,
@@ -44,6 +44,7 @@ public:
     // (AProductCore.cpp:560-623).
     Matrix3cd A_dic_generator(double x, double y, double z);
     Matrix3cd LDR_inter(double x, double y, double z);
+    Matrix3cd FCD_inter(double x, double y, double z);  // AProductCore.cpp:625-695
 
     CoreStructure* get_CStr() { return CStr; }
     int get_N() { return CStr->get_N(); }

@@ -2,15 +2,17 @@
 // DDAModel, EvoDDAModel) through one forward and one adjoint solve and one
 // Aproductwithalb, writing the results for comparison with the oracle.
 //
-// usage: dda_host_check OUT.bin Nx Ny Nz d lam seed [MAXm MAXn Lm Ln]
-// (the optional arguments select the periodic AProductCore overload,
-// AProductCore.cpp:251-536)
+// usage: dda_host_check OUT.bin Nx Ny Nz d lam seed [MAXm MAXn Lm Ln] [LDR|FCD]
+// (the optional numbers select the periodic AProductCore overload,
+// AProductCore.cpp:251-536; the last word the interaction tensor, LDR by
+// default)
 // OUT.bin layout (little endian): int32 N, int32 it_fwd, int32 it_adj, int32 grid[3],
 // then complex128 vectors P_fwd, lambda_adj, Aproductwithalb(E), al, E, diel (3N each),
 // then int32 adjoint target count and the targets.
 #include <cstdio>
 #include <cstdlib>
 #include <random>
+#include <string>
 
 #include "EvoDDAModel.h"
 
@@ -19,7 +21,7 @@ using namespace topodda;
 int main(int argc, char** argv)
 {
     if (argc < 8) {
-        std::fprintf(stderr, "usage: %s OUT.bin Nx Ny Nz d lam seed [MAXm MAXn Lm Ln]\n", argv[0]);
+        std::fprintf(stderr, "usage: %s OUT.bin Nx Ny Nz d lam seed [MAXm MAXn Lm Ln] [LDR|FCD]\n", argv[0]);
         return 2;
     }
     const int Nx = atoi(argv[2]), Ny = atoi(argv[3]), Nz = atoi(argv[4]);
@@ -37,10 +39,14 @@ int main(int argc, char** argv)
     VectorXcd material = VectorXcd::Zero(2);
     material(0) = cd(1.0, 0.0);
     material(1) = cd(2.25, 0.01);
-    const bool periodic = argc >= 12;
+    std::string method = "LDR";
+    int nargs = argc;
+    if (argc > 8 && (std::string(argv[argc - 1]) == "LDR" || std::string(argv[argc - 1]) == "FCD"))
+        method = argv[--nargs];
+    const bool periodic = nargs >= 12;
     AProductCore Core = periodic ? AProductCore(&CStr, lam, material, 1.0, atoi(argv[8]), atoi(argv[9]), atof(argv[10]),
-                                                atof(argv[11]), "LDR")
-                                 : AProductCore(&CStr, lam, material, 1.0, "LDR");
+                                                atof(argv[11]), method)
+                                 : AProductCore(&CStr, lam, material, 1.0, method);
     DDAModel model(&Core, Vector3d(0, 0, 1), 1.0, Vector3d(1, 0, 0), false);
     EvoDDAModel evo(&CStr, {&model});
     evo.ForwardSolve(1000, 1e-10);

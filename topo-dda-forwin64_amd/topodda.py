@@ -35,7 +35,8 @@ EXPORTS = [
     "tdda_server_create", "tdda_server_destroy", "tdda_server_submit", "tdda_server_poll", "tdda_server_wait",
     "tdda_server_result", "tdda_server_get_stats", "tdda_design_create", "tdda_design_step", "tdda_design_get",
     "tdda_design_destroy", "tdda_beta_update", "tdda_op_create_periodic", "tdda_objective_point_e",
-    "tdda_objective_integrated_e", "tdda_op_autotune", "tdda_op_tiles",
+    "tdda_objective_integrated_e", "tdda_op_autotune", "tdda_op_tiles", "tdda_op_create_ex",
+    "tdda_green_tensor_fcd",
 ]
 
 
@@ -149,6 +150,8 @@ def lib():
         L.tdda_design_destroy.restype = None
         L.tdda_beta_update.argtypes = [ctypes.c_int] + [ctypes.c_double] * 4
         L.tdda_beta_update.restype = ctypes.c_double
+        L.tdda_op_create_ex.argtypes = [vp, ctypes.POINTER(OpDesc), ctypes.POINTER(OpExt), ctypes.POINTER(vp)]
+        L.tdda_green_tensor_fcd.argtypes = [ctypes.c_double] * 5 + [dp]
         L.tdda_op_autotune.argtypes = [vp, ctypes.c_int]
         L.tdda_op_tiles.argtypes = [vp, ctypes.c_int, ip]
         L.tdda_objective_point_e.argtypes = [vp, ctypes.c_int, dp, dp, dp, ctypes.c_double, dp, vp, ctypes.c_int]
@@ -264,14 +267,30 @@ def ne_perp_in_xz(theta):
     return np.array([math.cos(theta), 0.0, -math.sin(theta)])
 
 
+class OpExt(ctypes.Structure):
+    _fields_ = [("method", ctypes.c_int), ("max_m", ctypes.c_int), ("max_n", ctypes.c_int),
+                ("Lm", ctypes.c_double), ("Ln", ctypes.c_double)]
+
+
+METHOD_LDR, METHOD_FCD = 0, 1
+
+
+def green_tensor_fcd(K, d, x, y, z):
+    """AProductCore::FCD_inter (AProductCore.cpp:625-695): [xx xy xz yy yz zz]."""
+    out = np.zeros(12)
+    _check(lib().tdda_green_tensor_fcd(K, d, x, y, z, _dptr(out)))
+    return out.view(np.complex128)
+
+
 class AProductCore:
     """Device operator: AProductCore (AProductCore.cpp:12-249, 697-758).
     periodic=(max_m, max_n, Lm, Ln) builds the periodic operator
     (AProductCore.cpp:251-536): images m = -max_m..max_m, n = -max_n..max_n
-    shifted by (m Lm, n Ln, 0), each damped by exp(-(0.01 K r)^4)."""
+    shifted by (m Lm, n Ln, 0), each damped by exp(-(0.01 K r)^4).
+    method: "LDR" (AProductCore.cpp:588-623) or "FCD" (625-695)."""
 
     def __init__(self, ctx: Context, R, Nx, Ny, Nz, d, lam, nback=1.0, grid=None, precisions=C128 | C64,
-                 periodic=None):
+                 periodic=None, method="LDR"):
         self.ctx = ctx
         # R = None: the full box in build_a_bulk order
         self.R = None if R is None else np.ascontiguousarray(R, dtype=np.int32)
@@ -282,12 +301,14 @@ class AProductCore:
         Rp = None if R is None else self.R.ctypes.data_as(ctypes.POINTER(ctypes.c_int))
         desc = OpDesc(self.N, Nx, Ny, Nz, Rp, d, lam, nback, g[0], g[1], g[2], precisions)
         self.h = ctypes.c_void_p()
-        if periodic is None:
+        if method not in ("LDR", "FCD"):
+            raise TddaError(f'"{method}" is not a valid polarizability calculation method')
+        if periodic is None and method == "LDR":
             _check(lib().tdda_op_create(ctx.h, ctypes.byref(desc), ctypes.byref(self.h)))
         else:
-            pm, pn, Lm, Ln = periodic
-            _check(lib().tdda_op_create_periodic(ctx.h, ctypes.byref(desc), int(pm), int(pn), float(Lm), float(Ln),
-                                                 ctypes.byref(self.h)))
+            pm, pn, Lm, Ln = periodic if periodic is not None else (-1, -1, 0.0, 0.0)
+            ext = OpExt(METHOD_FCD if method == "FCD" else METHOD_LDR, int(pm), int(pn), float(Lm), float(Ln))
+            _check(lib().tdda_op_create_ex(ctx.h, ctypes.byref(desc), ctypes.byref(ext), ctypes.byref(self.h)))
         g3 = (ctypes.c_int * 3)()
         K = ctypes.c_double()
         _check(lib().tdda_op_info(self.h, g3, ctypes.byref(K)))
