@@ -20,6 +20,9 @@
  *   CoreStructure::UpdateStr    CoreStructure.cpp:31-104  tdda_design_step (with
  *     + DDAModel::UpdateAlpha   DDAModel.cpp:470-484       tdda_design_create/_get)
  *   FilterOption::update_beta   FilterOption.cpp:24-46    tdda_beta_update
+ *   DDAModel::solve_E           DDAModel.cpp:530-565      tdda_field_at_points
+ *   DDAModel::output_to_file    DDAModel.cpp:589-697      tdda_write_model_results
+ *   CoreStructure::output_to_file CoreStructure.cpp:155-190 tdda_write_structure
  *   ObjPointEDDAModel           ObjDDAModel.cpp:20-97     tdda_objective_point_e (value and
  *   ObjIntegratedEDDAModel      ObjDDAModel.cpp:100-177   tdda_objective_integrated_e  the
  *     + EvoDDAModel::devp       EvoDDAModel.cpp:353-391   adjoint source in closed form)
@@ -56,6 +59,7 @@ extern "C" {
 #define TDDA_ERR_OOM (-3)
 #define TDDA_ERR_STATE (-4)
 #define TDDA_ERR_UNSUPPORTED (-5)
+#define TDDA_ERR_IO (-6)
 
 /* precision bits */
 #define TDDA_C128 1
@@ -378,6 +382,31 @@ int tdda_objective_point_e(tdda_model m, int rhs, const double* r0, const double
  * pow_num f |al|^pow_num |P|^(pow_num - 2) conj(P) / E_int (0 where P = 0). */
 int tdda_objective_integrated_e(tdda_model m, int rhs, int pow_num, const int* box6, double eta, double beta,
                                 const double* diel_old, double* value, tdda_model dst, int dst_rhs);
+
+/* ------------------------------------------------------------------------
+ * Fields at points and the reference's text result files.
+ * ---------------------------------------------------------------------- */
+/* DDAModel::solve_E (DDAModel.cpp:530-565): out (npts complex triples) =
+ * E_ext - sum_i A(d pt - d R_i) P_i at the lattice points pts (3 ints each,
+ * units of d) with the operator's tensor and without periodic images, as
+ * the reference evaluates it.  E_ext is the reference's incident field at
+ * the point, including its cosine argument of components y and z
+ * (K (nK_x y + nK_y y + nK_z z) and K (nK_x z + nK_y y + nK_z z)). */
+int tdda_field_at_points(tdda_model m, int rhs, const int* pts, int npts, const double* nK, const double* nE,
+                         double E0, double* out);
+/* CoreStructure::output_to_file (CoreStructure.cpp:155-190): full != 0
+ * writes Nx, Ny, Nz, N, R (3N), diel_old (3N) and d ("CoreStructure.txt",
+ * mode "normal"); full == 0 writes diel_old only (any other mode).  Same
+ * bytes as the reference's streams (6 significant digits, Eigen columns). */
+int tdda_write_structure(const char* path, int Nx, int Ny, int Nz, int N, const int* R, const double* diel_old,
+                         double d, int full);
+/* DDAModel::output_to_file (DDAModel.cpp:589-697); P and EResult are nP and
+ * nE complex numbers written "re+imj" / "re-imj", one per line. */
+#define TDDA_RESULTS_FULL 0 /* P, nK, nE0, nE, RResult (nR ints), EResult: DDAModel.cpp:589-616 */
+#define TDDA_RESULTS_E_P 1  /* EResult then P: DDAModel.cpp:618-656 */
+#define TDDA_RESULTS_E 2    /* EResult: DDAModel.cpp:658-697 */
+int tdda_write_model_results(const char* path, int layout, const double* P, int nP, const double* EResult, int nE,
+                             const int* RResult, int nR, const double* nK, const double* nE0);
 
 #ifdef __cplusplus
 }

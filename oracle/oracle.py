@@ -111,6 +111,12 @@ def ref_lib():
         L.ref_beta_update.restype = ctypes.c_double
         L.ref_design_update.argtypes = [ctypes.c_int] * 3 + [_dp, _dp] + [ctypes.c_double] * 3 + [_dp] * 3
         L.ref_design_update.restype = ctypes.c_int
+        L.ref_structure_output.argtypes = [ctypes.c_int] * 3 + [_dp, ctypes.c_double, ctypes.c_char_p] + \
+            [ctypes.c_int] * 2
+        L.ref_structure_output.restype = ctypes.c_int
+        L.ref_model_results_full.argtypes = [ctypes.c_char_p, _dp, ctypes.c_int, _dp, ctypes.c_int, _ip,
+                                             ctypes.c_int, _dp, _dp]
+        L.ref_model_results_full.restype = ctypes.c_int
         _REF = L
     return _REF
 
@@ -477,10 +483,11 @@ def _tensor3(a6):
     return np.array([[a[0], a[1], a[2]], [a[1], a[3], a[4]], [a[2], a[4], a[5]]])
 
 
-def point_e_field(R, d, K, P, r0, nK, nE, E0=1.0):
+def point_e_field(R, d, K, P, r0, nK, nE, E0=1.0, fcd=False):
     """ObjPointEDDAModel (ObjDDAModel.cpp:20-97): the field at r0,
     E_ext - sum_i A(r0 - d R_i) P_i, E_ext = E0 nE exp(i K nK . r0), with the
-    tensor of the C restatement.  Returns (E, [A_i])."""
+    tensor of the C restatement (or the FCD tensor, A_dic_generator,
+    AProductCore.cpp:560-571).  Returns (E, [A_i])."""
     R = np.asarray(R).reshape(-1, 3)
     P3 = np.asarray(P, np.complex128).reshape(-1, 3)
     r0 = np.asarray(r0, np.float64)
@@ -488,10 +495,34 @@ def point_e_field(R, d, K, P, r0, nK, nE, E0=1.0):
     E = E0 * np.asarray(nE, np.float64) * complex(math.cos(ph), math.sin(ph))
     As = []
     for i in range(len(R)):
-        A = _tensor3(ldr_inter(K, *(r0 - d * R[i])))
+        off = r0 - d * R[i]
+        A = _tensor3(fcd_inter(K, d, *off) if fcd else ldr_inter(K, *off))
         As.append(A)
         E = E - A @ P3[i]
     return E, As
+
+
+def field_at_points(R, d, K, P, pts, nK, nE, E0=1.0, fcd=False):
+    """DDAModel::solve_E (DDAModel.cpp:530-565): E_ext - sum_i A(d pt -
+    d R_i) P_i at lattice points pts (units of d), with the reference's
+    incident field: component c = (x, y, z)[c]'s cosine takes
+    K (nK_x q_c + nK_y y + nK_z z), its sine K nK . r (DDAModel.cpp:542-544).
+    Returns (npts, 3) complex."""
+    R = np.asarray(R, np.float64).reshape(-1, 3) * d
+    P3 = np.asarray(P, np.complex128).reshape(-1, 3)
+    pts = np.asarray(pts, np.float64).reshape(-1, 3) * d
+    out = np.zeros((len(pts), 3), np.complex128)
+    idx = [[0, 1, 2], [1, 3, 4], [2, 4, 5]]
+    for q, (x, y, z) in enumerate(pts):
+        D = np.array([x, y, z]) - R
+        G = fcd_inter(K, d, D[:, 0], D[:, 1], D[:, 2]) if fcd else green_textbook(K, D[:, 0], D[:, 1], D[:, 2])
+        s = np.array([sum(G[:, idx[a][c]] @ P3[:, c] for c in range(3)) for a in range(3)])
+        sn = math.sin(K * (nK[0] * x + nK[1] * y + nK[2] * z))
+        args = [nK[0] * x + nK[1] * y + nK[2] * z, nK[0] * y + nK[1] * y + nK[2] * z,
+                nK[0] * z + nK[1] * y + nK[2] * z]
+        Eext = np.array([E0 * nE[c] * complex(math.cos(K * args[c]), sn) for c in range(3)])
+        out[q] = Eext - s
+    return out
 
 
 def obj_point_e(R, d, K, P, r0, nK, nE, E0=1.0) -> float:

@@ -7,8 +7,13 @@
 // schedules (tools.cpp:776-798) and the filtered design update
 // (CoreStructure::UpdateStr, CoreStructure.cpp:31-104, over SpacePara's
 // FreeWeight cone filter, SpacePara.cpp:37-48,1087-1270, and
-// FilterOption::SmoothDensity, FilterOption.cpp:49-60).
+// FilterOption::SmoothDensity, FilterOption.cpp:49-60), and the text result
+// files (CoreStructure::output_to_file, CoreStructure.cpp:165-190, through
+// the reference's class; DDAModel's complex lines and Eigen prints,
+// DDAModel.cpp:589-616, restated with std::ofstream since DDAModel.cpp
+// includes CUDA headers).
 #include <complex>
+#include <fstream>
 #include <vector>
 #include "Tools.h"
 #include "CoreStructure.h"
@@ -85,6 +90,64 @@ int ref_design_update(int Nx, int Ny, int Nz, const double* para0, const double*
     } catch (...) {
         return -2;
     }
+}
+
+// CoreStructure::output_to_file(save_position, iteration, mode)
+// (CoreStructure.cpp:165-190) of the extruded Nx x Ny x Nz box built as in
+// ref_design_update, with the column design para0 and lattice spacing d;
+// normal != 0 selects mode "normal".  Writes save_position +
+// "CoreStructure" + iteration + ".txt".
+int ref_structure_output(int Nx, int Ny, int Nz, const double* para0, double d, const char* save_position,
+                         int iteration, int normal)
+{
+    try {
+        const int N = Nx * Ny * Nz;
+        VectorXi total_space = build_a_bulk(Nx, Ny, Nz);
+        VectorXi geometry = build_a_bulk(Nx, Ny, Nz);
+        vector<Structure> ln;
+        ln.push_back(Structure(&total_space, &geometry, true));
+        Space space(&total_space, Nx, Ny, Nz, N, &ln);
+        VectorXd diel0 = VectorXd::Zero(3 * N);
+        for (int i = 0; i < N; i++) {
+            const int x = geometry(3 * i), y = geometry(3 * i + 1);
+            diel0(3 * i) = diel0(3 * i + 1) = diel0(3 * i + 2) = para0[x * Ny + y];
+        }
+        vector<filterinfo> fl{filterinfo{0, 1.0}};
+        FilterOption fo(0.0, 0.0, 0.5, "linear", fl, false, 0);
+        SpacePara sp(&space, Vector3i(1, 1, Nz), &geometry, &diel0, false, &fo, "None", vector<double>{}, false, 0, 0);
+        CoreStructure cs(&sp, d);
+        cs.output_to_file(string(save_position), iteration, normal ? "normal" : "simple");
+        return 0;
+    } catch (...) {
+        return -2;
+    }
+}
+
+// DDAModel::output_to_file() (DDAModel.cpp:589-616) restated statement for
+// statement on std::ofstream and Eigen types: P lines, n_K, n_E0, the count
+// of EResult, RResult, EResult lines.
+int ref_model_results_full(const char* path, const double* P, int nP, const double* E, int nE, const int* RR, int nR,
+                           const double* nK, const double* nE0)
+{
+    ofstream fout(path);
+    auto lines = [&](const double* z, int n) {
+        for (int i = 0; i < n; i++) {
+            const complex<double> c(z[2 * i], z[2 * i + 1]);
+            if (c.imag() < 0) fout << c.real() << c.imag() << "j" << endl;
+            else fout << c.real() << "+" << c.imag() << "j" << endl;
+        }
+    };
+    lines(P, nP);
+    fout << Vector3d(nK[0], nK[1], nK[2]) << endl;
+    fout << Vector3d(nE0[0], nE0[1], nE0[2]) << endl;
+    VectorXcd ER = VectorXcd::Zero(nE);
+    fout << ER.size() << endl;
+    VectorXi R(nR);
+    for (int i = 0; i < nR; i++) R(i) = RR[i];
+    fout << R << endl;
+    lines(E, nE);
+    fout.close();
+    return fout.fail() ? -1 : 0;
 }
 
 }

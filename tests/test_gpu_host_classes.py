@@ -3,6 +3,7 @@ the C ABI): the dda_host_check driver runs a forward solve, an adjoint solve
 (EvoDDAModel.cpp:596-608 order) and Aproductwithalb; results are compared with
 the oracle's restatement of the same steps."""
 import os
+import re
 import subprocess
 
 import numpy as np
@@ -48,6 +49,42 @@ def test_host_classes_forward_adjoint(tmp_path):
     assert st == 0 and O.rel_l2(lam_adj, Lc) < 1e-7
     # reciprocity of the symmetric operator: lambda^T E = devp^T P
     assert abs(lam_adj @ E - devp @ Pf) < 1e-7 * abs(devp @ Pf)
+    # text files of the forward solve (CoreStructure.cpp:165-190,
+    # DDAModel.cpp:618-697; their bytes are pinned against the reference on
+    # the CPU): the structure's integers exactly and its design values, EResult
+    # (= Einternal) and P to the 6 significant digits the files hold
+    st = (tmp_path / "CoreStructure0.txt").read_text().split("\n")
+    assert [int(v) for v in st[:4]] == [Nx, Ny, Nz, N] and st[-1] == ""
+    assert np.array_equal(np.array([int(v) for v in st[4:4 + n3]]), R)
+    # diel = m0 + t (m1 - m0) with the material rule's m0 = 1 + 0.01, m1 = 2.25 + 0.01i
+    t = (diel.real - 1.01) / 1.24
+    assert np.all(np.abs(np.array([float(v) for v in st[4 + n3:4 + 2 * n3]]) - t) <= 5.01e-6 * np.abs(t) + 1e-12)
+    assert float(st[4 + 2 * n3]) == d and len(st) == 4 + 2 * n3 + 2
+    lines = (tmp_path / "Model_results0it0.txt").read_text().splitlines()
+    assert len(lines) == 2 * n3
+    _close6(_parse(lines[:n3]), O.local_field(al, Pf, diel))
+    _close6(_parse(lines[n3:]), Pf)
+    pts = np.array([-2, 0, 0, Nx + 1, 1, 0, 0, -3, 1, 1, Ny + 2, Nz - 1, 0, 0, -2, Nx // 2, Ny // 2, Nz + 3])
+    ER = O.field_at_points(R, d, K, Pf, pts, [0, 0, 1], [1, 0, 0], 1.0).reshape(-1)
+    _close6(_parse((tmp_path / "Model_resultsit1.txt").read_text().splitlines()), ER)
+
+
+_NUM = r"[+-]?(?:\d+(?:\.\d*)?(?:e[+-]?\d+)?|inf|nan)"
+
+
+def _parse(lines):
+    out = []
+    for ln in lines:
+        m = re.fullmatch(f"({_NUM})\\+?({_NUM})j", ln)
+        assert m, ln
+        out.append(complex(float(m.group(1)), float(m.group(2))))
+    return np.array(out)
+
+
+def _close6(got, want):
+    # "%g": 6 significant digits per real number
+    for a, b in ((got.real, want.real), (got.imag, want.imag)):
+        assert np.all(np.abs(a - b) <= 5.01e-6 * np.abs(b) + 1e-300)
 
 
 @pytest.mark.parametrize("method", ["LDR", "FCD"])

@@ -37,6 +37,15 @@ int fail(int code, const std::string& msg)
     return code;
 }
 
+}  // namespace
+
+// The library's error message for its other translation units.
+namespace tdda {
+int set_error(int code, const char* msg) { return fail(code, msg); }
+}  // namespace tdda
+
+namespace {
+
 #define HIPCHK(x)                                                                                  \
     do {                                                                                           \
         hipError_t e_ = (x);                                                                       \
@@ -183,6 +192,7 @@ struct tdda_op_s {
     int refs = 1;
     int N, Nx, Ny, Nz, Mx, My, Mz, Hx, Hy, Hz;
     double d, lam, nback, K;
+    int method = TDDA_METHOD_LDR;  // interaction tensor (A_dic_generator, AProductCore.cpp:560-571)
     int prec;
     int* cellmap = nullptr;
     int* site_xyz = nullptr;  // 3N site coordinates for the objectives, built on first use
@@ -1204,6 +1214,7 @@ int op_create(tdda_ctx ctx, const tdda_op_desc* desc, const PeriodicImages& per,
         return fail(TDDA_ERR_UNSUPPORTED, "FCD needs K d < pi - 0.1");
     }
     op->prec = prec;
+    op->method = per.method;
     auto cleanup = [&](int rc) {
         tdda_op_destroy(op);
         return rc;

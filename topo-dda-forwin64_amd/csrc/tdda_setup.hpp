@@ -403,8 +403,9 @@ __device__ __forceinline__ void site_xyz(const int* __restrict__ R, int i, int N
 // E = E_ext - sum_i A(r0 - r_i d) P_i; per workgroup the partial sum of
 // -A(r0 - r_i d) P_i (3 complex values) in part[6 * blockIdx.x ...].
 static __global__ void __launch_bounds__(kThreads) k_obj_point_e(const Cx<double>* __restrict__ P, const int* __restrict__ R,
-                                                              int N, int Ny, int Nz, double d, double K, double x0,
-                                                              double y0, double z0, double* __restrict__ part)
+                                                              int N, int Ny, int Nz, double d, double K, int fcd,
+                                                              double x0, double y0, double z0,
+                                                              double* __restrict__ part)
 {
     __shared__ double red[16 * 6];
     double acc[6] = {0, 0, 0, 0, 0, 0};
@@ -412,7 +413,7 @@ static __global__ void __launch_bounds__(kThreads) k_obj_point_e(const Cx<double
         int x, y, z;
         site_xyz(R, i, Ny, Nz, x, y, z);
         Cx<double> a[6];
-        ldr_inter(K, x0 - d * x, y0 - d * y, z0 - d * z, a);
+        inter_tensor(fcd, K, d, x0 - d * x, y0 - d * y, z0 - d * z, a);
         const Cx<double> p0 = P[3 * i], p1 = P[3 * i + 1], p2 = P[3 * i + 2];
         const Cx<double> e0 = cadd(cadd(cmul(a[0], p0), cmul(a[1], p1)), cmul(a[2], p2));
         const Cx<double> e1 = cadd(cadd(cmul(a[1], p0), cmul(a[3], p1)), cmul(a[4], p2));
@@ -446,8 +447,8 @@ static __global__ void __launch_bounds__(kThreads) k_obj_point_e(const Cx<double
 
 // Adjoint source of PointE, F = |E|: 2 dF/dP_ic = -(1/F) sum_a A_ac conj(E_a).
 static __global__ void __launch_bounds__(kThreads) k_obj_point_e_source(const int* __restrict__ R, int N, int Ny,
-                                                                     int Nz, double d, double K, double x0, double y0,
-                                                                     double z0, Cx<double> E0, Cx<double> E1,
+                                                                     int Nz, double d, double K, int fcd, double x0,
+                                                                     double y0, double z0, Cx<double> E0, Cx<double> E1,
                                                                      Cx<double> E2, double invF,
                                                                      Cx<double>* __restrict__ dst)
 {
@@ -455,7 +456,7 @@ static __global__ void __launch_bounds__(kThreads) k_obj_point_e_source(const in
         int x, y, z;
         site_xyz(R, i, Ny, Nz, x, y, z);
         Cx<double> a[6];
-        ldr_inter(K, x0 - d * x, y0 - d * y, z0 - d * z, a);
+        inter_tensor(fcd, K, d, x0 - d * x, y0 - d * y, z0 - d * z, a);
         const int col[3][3] = {{0, 1, 2}, {1, 3, 4}, {2, 4, 5}};
         for (int c = 0; c < 3; c++) {
             // sum_a A_ac conj(E_a)
@@ -464,6 +465,66 @@ static __global__ void __launch_bounds__(kThreads) k_obj_point_e_source(const in
             s = cadd(s, cmulc(a[col[2][c]], E2));
             dst[3 * i + c] = cscale(s, -invF);
         }
+    }
+}
+
+// DDAModel::solve_E (DDAModel.cpp:530-565): the field at lattice points
+// pts (units of d), one workgroup per point:
+// E = E_ext - sum_i A(d pt - d R_i) P_i, with the incident field as the
+// reference writes it -- component c's cosine takes K (nK_x q_c + nK_y y +
+// nK_z z) with q = (x, y, z), its sine K nK . r (DDAModel.cpp:542-544).
+static __global__ void __launch_bounds__(kThreads) k_field_points(const Cx<double>* __restrict__ P,
+                                                               const int* __restrict__ R, int N, int Ny, int Nz,
+                                                               double d, double K, int fcd,
+                                                               const int* __restrict__ pts, double E0, double nKx,
+                                                               double nKy, double nKz, double nEx, double nEy,
+                                                               double nEz, Cx<double>* __restrict__ out)
+{
+    __shared__ double red[(kThreads / 64) * 6];
+    const int q = blockIdx.x;
+    const double x0 = d * pts[3 * q], y0 = d * pts[3 * q + 1], z0 = d * pts[3 * q + 2];
+    double acc[6] = {0, 0, 0, 0, 0, 0};
+    for (int i = threadIdx.x; i < N; i += kThreads) {
+        int x, y, z;
+        site_xyz(R, i, Ny, Nz, x, y, z);
+        Cx<double> a[6];
+        inter_tensor(fcd, K, d, x0 - d * x, y0 - d * y, z0 - d * z, a);
+        const Cx<double> p0 = P[3 * i], p1 = P[3 * i + 1], p2 = P[3 * i + 2];
+        const Cx<double> e0 = cadd(cadd(cmul(a[0], p0), cmul(a[1], p1)), cmul(a[2], p2));
+        const Cx<double> e1 = cadd(cadd(cmul(a[1], p0), cmul(a[3], p1)), cmul(a[4], p2));
+        const Cx<double> e2 = cadd(cadd(cmul(a[2], p0), cmul(a[4], p1)), cmul(a[5], p2));
+        acc[0] += e0.x;
+        acc[1] += e0.y;
+        acc[2] += e1.x;
+        acc[3] += e1.y;
+        acc[4] += e2.x;
+        acc[5] += e2.y;
+    }
+#pragma unroll
+    for (int k = 0; k < 6; k++) {
+        double v = acc[k];
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+        acc[k] = v;
+    }
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    if (lane == 0)
+        for (int k = 0; k < 6; k++) red[wid * 6 + k] = acc[k];
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double sum[6];
+        for (int k = 0; k < 6; k++) {
+            double v = 0.0;
+            for (int w = 0; w < kThreads / 64; w++) v += red[w * 6 + k];
+            sum[k] = v;
+        }
+        const double ph = K * (nKx * x0 + nKy * y0 + nKz * z0);
+        const double sn = sin(ph);
+        const double cs[3] = {cos(ph), cos(K * (nKx * y0 + nKy * y0 + nKz * z0)),
+                              cos(K * (nKx * z0 + nKy * y0 + nKz * z0))};
+        const double ne[3] = {nEx, nEy, nEz};
+        for (int c = 0; c < 3; c++)
+            out[3 * q + c] = Cx<double>{E0 * ne[c] * cs[c] - sum[2 * c], E0 * ne[c] * sn - sum[2 * c + 1]};
     }
 }
 

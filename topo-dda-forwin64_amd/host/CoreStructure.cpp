@@ -1,5 +1,7 @@
 #include "CoreStructure.h"
 
+#include "AProductCore.h"
+
 namespace topodda {
 
 CoreStructure::CoreStructure(int Nx_, int Ny_, int Nz_, double d_, const VectorXi& R_, const VectorXd& diel_old_)
@@ -40,6 +42,17 @@ void CoreStructure::UpdateStrSingle(int idx, double value)
     diel_old(3 * idx) = value;
     diel_old(3 * idx + 1) = value;
     diel_old(3 * idx + 2) = value;
+}
+
+void CoreStructure::output_to_file()
+{
+    check(tdda_write_structure("CoreStructure.txt", Nx, Ny, Nz, N, R.data(), diel_old.data(), d, 1));
+}
+
+void CoreStructure::output_to_file(std::string save_position, int iteration, std::string mode)
+{
+    const std::string name = save_position + "CoreStructure" + std::to_string(iteration) + ".txt";
+    check(tdda_write_structure(name.c_str(), Nx, Ny, Nz, N, R.data(), diel_old.data(), d, mode == "normal" ? 1 : 0));
 }
 
 }  // namespace topodda

@@ -8,6 +8,7 @@
 // class is built directly from R and diel_old.
 #pragma once
 #include <stdexcept>
+#include <string>
 
 #include "dda_types.h"
 
@@ -23,6 +24,13 @@ public:
     // Replace the design (the effect of UpdateStr, CoreStructure.cpp:99-109).
     void UpdateStr(const VectorXd& diel_old_new);
     void UpdateStrSingle(int idx, double value);  // sets the 3 components of dipole idx
+
+    // The reference's text files (CoreStructure.cpp:155-190):
+    // "CoreStructure.txt" with Nx, Ny, Nz, N, R, diel_old, d; and save_position
+    // + "CoreStructure" + iteration + ".txt", the same for mode "normal",
+    // diel_old only for any other mode.
+    void output_to_file();
+    void output_to_file(std::string save_position, int iteration, std::string mode);
 
     int get_N() const { return N; }
     int get_Nx() const { return Nx; }

@@ -2,6 +2,7 @@
 
 #include <chrono>
 #include <iostream>
+#include <string>
 
 namespace topodda {
 
@@ -19,7 +20,18 @@ DDAModel::DDAModel(AProductCore* Core_, Vector3d n_K_, double E0_, Vector3d n_E0
     diel = VectorXcd::Zero(3 * (std::size_t)N);
     UpdateAlpha();
     al_max = al;
+    RResult = *Core->get_R();  // DDAModel.cpp:39-40
     check(tdda_model_create(Core->handle(), 1, &dev));
+}
+
+DDAModel::DDAModel(AProductCore* Core_, Vector3d n_K_, double E0_, Vector3d n_E0_, VectorXi* RResult_, bool verbose_)
+    : DDAModel(Core_, n_K_, E0_, n_E0_, verbose_)
+{
+    if (!RResult_) throw std::invalid_argument("RResult is null");
+    if (RResult_->size() % 3 != 0) throw std::invalid_argument("RResult does not have a size with 3*integer");
+    RResultSwitch = true;
+    RResult = *RResult_;
+    EResult = VectorXcd::Zero(RResult.size());
 }
 
 DDAModel::~DDAModel()
@@ -116,6 +128,42 @@ void DDAModel::update_E_in_structure()
     check(tdda_local_field((int)P.size(), reinterpret_cast<const double*>(al.data()),
                            reinterpret_cast<const double*>(P.data()), reinterpret_cast<const double*>(diel.data()),
                            reinterpret_cast<double*>(Einternal.data())));
+}
+
+void DDAModel::solve_E()
+{
+    if (!RResultSwitch) {
+        EResult = Einternal;
+        return;
+    }
+    check(tdda_model_set(dev, TDDA_VEC_P, 0, reinterpret_cast<const double*>(P.data())));
+    check(tdda_field_at_points(dev, 0, RResult.data(), (int)(RResult.size() / 3), n_K.v, n_E0.v, E0,
+                               reinterpret_cast<double*>(EResult.data())));
+}
+
+void DDAModel::output_to_file()
+{
+    Core->get_CStr()->output_to_file();
+    check(tdda_write_model_results("Model_results.txt", TDDA_RESULTS_FULL, reinterpret_cast<const double*>(P.data()),
+                                   (int)P.size(), reinterpret_cast<const double*>(EResult.data()),
+                                   (int)EResult.size(), RResult.data(), (int)RResult.size(), n_K.v, n_E0.v));
+}
+
+void DDAModel::output_to_file(std::string save_position, int iteration, int ModelLabel)
+{
+    const std::string name = save_position + "Model_results" + std::to_string(ModelLabel) + "it" +
+                             std::to_string(iteration) + ".txt";
+    check(tdda_write_model_results(name.c_str(), TDDA_RESULTS_E_P, reinterpret_cast<const double*>(P.data()),
+                                   (int)P.size(), reinterpret_cast<const double*>(EResult.data()),
+                                   (int)EResult.size(), nullptr, 0, nullptr, nullptr));
+}
+
+void DDAModel::output_to_file(std::string save_position, int iteration)
+{
+    const std::string name = save_position + "Model_results" + "it" + std::to_string(iteration) + ".txt";
+    check(tdda_write_model_results(name.c_str(), TDDA_RESULTS_E, nullptr, 0,
+                                   reinterpret_cast<const double*>(EResult.data()), (int)EResult.size(), nullptr, 0,
+                                   nullptr, nullptr));
 }
 
 VectorXcd DDAModel::Aproductwithalb(VectorXcd& b)

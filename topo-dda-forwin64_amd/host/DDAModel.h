@@ -15,6 +15,9 @@ class DDAModel {
 public:
     // DDAModel.cpp:12-69
     DDAModel(AProductCore* Core, Vector3d n_K, double E0, Vector3d n_E0, bool verbose = true);
+    // DDAModel.cpp:71-129: EResult is evaluated at the lattice points
+    // RResult (3 ints each) instead of the sites.
+    DDAModel(AProductCore* Core, Vector3d n_K, double E0, Vector3d n_E0, VectorXi* RResult, bool verbose = true);
     ~DDAModel();
     DDAModel(const DDAModel&) = delete;
     DDAModel& operator=(const DDAModel&) = delete;
@@ -26,6 +29,17 @@ public:
     void UpdateAlpha();                     // DDAModel.cpp:470-487
     void UpdateAlphaSingle(int idx);        // DDAModel.cpp:489-528
     void update_E_in_structure();           // DDAModel.cpp:567-575
+    // DDAModel.cpp:530-565: EResult = Einternal, or, with RResult, the field
+    // at those points by a direct sum over the sites on the device.
+    void solve_E();
+    // The reference's text result files (DDAModel.cpp:589-697):
+    // "Model_results.txt" (P, n_K, n_E0, EResult size, RResult, EResult; also
+    // writes "CoreStructure.txt"), save_position + "Model_results" +
+    // ModelLabel + "it" + iteration + ".txt" (EResult, P) and save_position +
+    // "Model_results" + "it" + iteration + ".txt" (EResult).
+    void output_to_file();
+    void output_to_file(std::string save_position, int iteration, int ModelLabel);
+    void output_to_file(std::string save_position, int iteration);
     VectorXcd Aproductwithalb(VectorXcd& b);  // DDAModel.cpp:577-586
     void InitializeP(VectorXcd& Initializer);  // DDAModel.cpp:731-733
 
@@ -34,6 +48,7 @@ public:
     Vector3d get_nK() { return n_K; }
     double get_E0() { return E0; }
     VectorXcd* get_Einternal() { return &Einternal; }
+    VectorXcd* get_EResult() { return &EResult; }
     AProductCore* get_Core() { return Core; }
     VectorXcd* get_al() { return &al; }
     VectorXcd* get_P_max() { return &P_max; }
@@ -67,7 +82,9 @@ private:
     AProductCore* Core;
     double E0;
     Vector3d n_E0, n_K;
-    VectorXcd P, E, Einternal, al, diel, P_max, al_max;
+    VectorXcd P, E, Einternal, EResult, al, diel, P_max, al_max;
+    bool RResultSwitch = false;
+    VectorXi RResult;
     VectorXcd al_dev, E_dev;  // last values uploaded
     bool verbose;
     int time = 0;

@@ -9,6 +9,11 @@
 // OUT.bin layout (little endian): int32 N, int32 it_fwd, int32 it_adj, int32 grid[3],
 // then complex128 vectors P_fwd, lambda_adj, Aproductwithalb(E), al, E, diel (3N each),
 // then int32 adjoint target count and the targets.
+// Beside OUT.bin it writes the reference's text files of the forward solve
+// (EvoDDAModel.cpp:428-447 order): CoreStructure0.txt (mode "normal"),
+// Model_results0it0.txt (EResult = Einternal, then P), and
+// Model_resultsit1.txt: EResult of a model with RResult = 6 points around
+// the box, from the forward P.
 #include <cstdio>
 #include <cstdlib>
 #include <random>
@@ -52,6 +57,21 @@ int main(int argc, char** argv)
     evo.ForwardSolve(1000, 1e-10);
     const int it_fwd = model.get_ITERATION();
     VectorXcd Pf = *model.get_P();
+    std::string dir = argv[1];
+    dir = dir.find('/') == std::string::npos ? std::string() : dir.substr(0, dir.rfind('/') + 1);
+    CStr.output_to_file(dir, 0, "normal");
+    model.update_E_in_structure();
+    model.solve_E();
+    model.output_to_file(dir, 0, 0);
+    {
+        VectorXi pts(18);
+        const int p[18] = {-2, 0, 0, Nx + 1, 1, 0, 0, -3, 1, 1, Ny + 2, Nz - 1, 0, 0, -2, Nx / 2, Ny / 2, Nz + 3};
+        for (int k = 0; k < 18; k++) pts(k) = p[k];
+        DDAModel probe(&Core, Vector3d(0, 0, 1), 1.0, Vector3d(1, 0, 0), &pts, false);
+        probe.InitializeP(Pf);
+        probe.solve_E();
+        probe.output_to_file(dir, 1);
+    }
     // adjoint source: conj(E_int) at 8 targets
     VectorXcd devp = VectorXcd::Zero(3 * (std::size_t)N);
     std::vector<int> targets;

@@ -36,7 +36,7 @@ EXPORTS = [
     "tdda_server_result", "tdda_server_get_stats", "tdda_design_create", "tdda_design_step", "tdda_design_get",
     "tdda_design_destroy", "tdda_beta_update", "tdda_op_create_periodic", "tdda_objective_point_e",
     "tdda_objective_integrated_e", "tdda_op_autotune", "tdda_op_tiles", "tdda_op_create_ex",
-    "tdda_green_tensor_fcd",
+    "tdda_green_tensor_fcd", "tdda_field_at_points", "tdda_write_structure", "tdda_write_model_results",
 ]
 
 
@@ -152,6 +152,11 @@ def lib():
         L.tdda_beta_update.restype = ctypes.c_double
         L.tdda_op_create_ex.argtypes = [vp, ctypes.POINTER(OpDesc), ctypes.POINTER(OpExt), ctypes.POINTER(vp)]
         L.tdda_green_tensor_fcd.argtypes = [ctypes.c_double] * 5 + [dp]
+        L.tdda_field_at_points.argtypes = [vp, ctypes.c_int, ip, ctypes.c_int, dp, dp, ctypes.c_double, dp]
+        L.tdda_write_structure.argtypes = [ctypes.c_char_p] + [ctypes.c_int] * 4 + [ip, dp, ctypes.c_double,
+                                                                                 ctypes.c_int]
+        L.tdda_write_model_results.argtypes = [ctypes.c_char_p, ctypes.c_int, dp, ctypes.c_int, dp, ctypes.c_int, ip,
+                                               ctypes.c_int, dp, dp]
         L.tdda_op_autotune.argtypes = [vp, ctypes.c_int]
         L.tdda_op_tiles.argtypes = [vp, ctypes.c_int, ip]
         L.tdda_objective_point_e.argtypes = [vp, ctypes.c_int, dp, dp, dp, ctypes.c_double, dp, vp, ctypes.c_int]
@@ -482,6 +487,43 @@ def objective_point_e(model, rhs, r0, nK, nE, E0=1.0, source=None, source_rhs=0)
     _check(lib().tdda_objective_point_e(model.h, rhs, _dptr(a[0]), _dptr(a[1]), _dptr(a[2]), float(E0),
                                         ctypes.byref(v), source.h if source is not None else None, source_rhs))
     return v.value
+
+
+def field_at_points(model, rhs, pts, nK, nE, E0=1.0):
+    """DDAModel::solve_E (DDAModel.cpp:530-565) on the device: the field at
+    lattice points pts ((npts, 3) ints, units of d) from right-hand side
+    rhs's P, with the reference's incident field.  Returns (npts, 3)."""
+    p = np.ascontiguousarray(np.asarray(pts).reshape(-1, 3), dtype=np.int32)
+    out = np.zeros((len(p), 6))
+    a = [np.ascontiguousarray(x, dtype=np.float64) for x in (nK, nE)]
+    _check(lib().tdda_field_at_points(model.h, rhs, p.ctypes.data_as(ctypes.POINTER(ctypes.c_int)), len(p),
+                                      _dptr(a[0]), _dptr(a[1]), float(E0), _dptr(out)))
+    return out.view(np.complex128)
+
+
+RESULTS_FULL, RESULTS_E_P, RESULTS_E = 0, 1, 2
+
+
+def write_structure(path, Nx, Ny, Nz, R, diel_old, d, full=True):
+    """CoreStructure::output_to_file (CoreStructure.cpp:155-190): the
+    reference's bytes; full=False writes diel_old only."""
+    r = np.ascontiguousarray(R, dtype=np.int32)
+    dl = np.ascontiguousarray(diel_old, dtype=np.float64)
+    _check(lib().tdda_write_structure(os.fsencode(path), Nx, Ny, Nz, len(r) // 3,
+                                      r.ctypes.data_as(ctypes.POINTER(ctypes.c_int)), _dptr(dl), float(d), int(full)))
+
+
+def write_model_results(path, layout, P=None, EResult=None, RResult=None, nK=(0, 0, 1), nE0=(1, 0, 0)):
+    """DDAModel::output_to_file (DDAModel.cpp:589-697) with layout
+    RESULTS_FULL, RESULTS_E_P or RESULTS_E."""
+    Pv = np.ascontiguousarray(P if P is not None else np.zeros(0), dtype=np.complex128)
+    Ev = np.ascontiguousarray(EResult if EResult is not None else np.zeros(0), dtype=np.complex128)
+    Rv = np.ascontiguousarray(RResult if RResult is not None else np.zeros(0), dtype=np.int32)
+    a = [np.ascontiguousarray(x, dtype=np.float64) for x in (nK, nE0)]
+    _check(lib().tdda_write_model_results(os.fsencode(path), int(layout), _dptr(Pv.view(np.float64)), len(Pv),
+                                          _dptr(Ev.view(np.float64)), len(Ev),
+                                          Rv.ctypes.data_as(ctypes.POINTER(ctypes.c_int)), len(Rv), _dptr(a[0]),
+                                          _dptr(a[1])))
 
 
 def objective_integrated_e(model, rhs, pow_num, box, eta, beta, diel_old, source=None, source_rhs=0):
