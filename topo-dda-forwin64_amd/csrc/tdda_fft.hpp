@@ -36,6 +36,8 @@ __device__ __forceinline__ Cx<T> cmulc(Cx<T> a, Cx<T> b)
 }
 template <typename T>
 __device__ __forceinline__ Cx<T> cscale(Cx<T> a, T s) { return {a.x * s, a.y * s}; }
+template <typename T>
+__device__ __forceinline__ Cx<T> cneg(Cx<T> a) { return {-a.x, -a.y}; }
 // multiply by -i (forward) or +i (inverse)
 template <typename T, bool INV>
 __device__ __forceinline__ Cx<T> rot90(Cx<T> a)

@@ -530,8 +530,13 @@ __device__ __forceinline__ STile s_tile(const JobDesc<T>* __restrict__ jobs, con
     const int nfb = xpass ? J->nfx : J->nfy;
     const int tf = xpass ? J->tfx : J->tfy;
     const int r = tl - tstart[t.job];
-    t.outer = r / nfb;
-    t.f0 = (r - t.outer * nfb) * tf;
+    const int o = r / nfb;
+    t.f0 = (r - o * nfb) * tf;
+    // x pass: ky in the order 0, 1, My-1, 2, My-2, ..., so the two tiles of
+    // a mirrored pair (which read the same Green octant rows) run at nearly
+    // the same time; nfb tiles apart, they share an XCD's L2 when nfb is a
+    // multiple of the XCD count.
+    t.outer = !xpass ? o : (o == 0 ? 0 : ((o & 1) ? (o + 1) >> 1 : J->My - (o >> 1)));
     t.nf = min(tf, J->Mz * 3 - t.f0);
     t.live = !(check_state && job_inactive(J->st));
     return t;
@@ -753,6 +758,17 @@ __global__ void TDDA_PASS_BOUNDS k_py(const JobDesc<T>* __restrict__ jobs, const
     }
 }
 
+// y = G b at one k point of the three component rows v[0], v[ps], v[2 ps].
+template <typename T>
+__device__ __forceinline__ void green_apply(Cx<T>* v, int ps, Cx<T> gxx, Cx<T> gxy, Cx<T> gxz, Cx<T> gyy, Cx<T> gyz,
+                                            Cx<T> gzz)
+{
+    const Cx<T> bx = v[0], by = v[ps], bz = v[2 * ps];
+    v[0] = cadd(cadd(cmul(gxx, bx), cmul(gxy, by)), cmul(gxz, bz));
+    v[ps] = cadd(cadd(cmul(gxy, bx), cmul(gyy, by)), cmul(gyz, bz));
+    v[2 * ps] = cadd(cadd(cmul(gxz, bx), cmul(gyz, by)), cmul(gzz, bz));
+}
+
 // P3: x pass fused with the Green product (kernel.cu:74-95).
 // Tile = (ky, tfx consecutive flattened (kz, c) columns, tfx % 3 == 0).
 template <typename T, int LC>
@@ -817,23 +833,22 @@ __global__ void __launch_bounds__(kThreads, px_waves<T>(LC)) k_px(const JobDesc<
             const T sy = (ky > My / 2) ? T(-1) : T(1);
             const int nkz = nf / 3;
             const int Hy = J->Hy, Hz = J->Hz;
-            for (RowWalk w(nkz, 1.0f / (float)nkz); w.row < Mx; w.next()) {
-                const int kx = w.row, kl = w.col;
+            // One Green entry serves kx = kxf and its mirror Mx - kxf (which
+            // flips the sign of the xy and xz components).
+            for (RowWalk w(nkz, 1.0f / (float)nkz); w.row <= Mx / 2; w.next()) {
+                const int kxf = w.row, kl = w.col;
                 const int kz = f0 / 3 + kl;
-                const int kxf = (kx > Mx / 2) ? Mx - kx : kx;
-                const T sxv = (kx > Mx / 2) ? T(-1) : T(1);
                 const int kzf = (kz > Mz / 2) ? Mz - kz : kz;
                 const T szv = (kz > Mz / 2) ? T(-1) : T(1);
                 const Cx<T>* gp = J->green + ((size_t)(kxf * Hy + kyf) * Hz + kzf) * 6;
                 const Cx<T> gxx = gld(gp), gyy = gld(gp + 3), gzz = gld(gp + 5);
-                const Cx<T> gxy = cscale(gld(gp + 1), sxv * sy);
-                const Cx<T> gxz = cscale(gld(gp + 2), sxv * szv);
+                const Cx<T> gxy = cscale(gld(gp + 1), sy);
+                const Cx<T> gxz = cscale(gld(gp + 2), szv);
                 const Cx<T> gyz = cscale(gld(gp + 4), sy * szv);
-                Cx<T>* v = res + (kl * 3) * ps + lpad<T>(kx);
-                const Cx<T> bx = v[0], by = v[ps], bz = v[2 * ps];
-                v[0] = cadd(cadd(cmul(gxx, bx), cmul(gxy, by)), cmul(gxz, bz));
-                v[ps] = cadd(cadd(cmul(gxy, bx), cmul(gyy, by)), cmul(gyz, bz));
-                v[2 * ps] = cadd(cadd(cmul(gxz, bx), cmul(gyz, by)), cmul(gzz, bz));
+                green_apply(res + (kl * 3) * ps + lpad<T>(kxf), ps, gxx, gxy, gxz, gyy, gyz, gzz);
+                if (kxf != 0 && 2 * kxf != Mx)
+                    green_apply(res + (kl * 3) * ps + lpad<T>(Mx - kxf), ps, gxx, cneg(gxy), cneg(gxz), gyy, gyz,
+                                gzz);
             }
             __syncthreads();
             const Cx<T>* res2;
