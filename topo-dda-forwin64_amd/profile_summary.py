@@ -2,10 +2,13 @@
 """Summarises rocprofv3 CSV output of the operator passes.
 
   python3 topo-dda-forwin64_amd/profile_summary.py stats  <kernel_stats.csv>  [--top 20]
+  python3 topo-dda-forwin64_amd/profile_summary.py last <kernel_trace.csv> --kernel 'k_px<double, 160>' [--last 200]
   python3 topo-dda-forwin64_amd/profile_summary.py traffic <fetch counter_collection.csv>
           <write counter_collection.csv> --precision c128|c64 --workload topo80 [--json out.json]
 
-`stats` prints the per-kernel duration table (markdown).  `traffic` prints the
+`stats` prints the per-kernel duration table (markdown).  `last` prints the
+mean, min and max duration of the last N dispatches of one kernel (the
+back-to-back timing launches that end a bench run).  `traffic` prints the
 average FETCH_SIZE / WRITE_SIZE per dispatch of each pass kernel, scaled by
 the calibration that the plain device copy in the same run gives (the copy
 of --copy-mib MiB run by profile_passes.py reads and writes exactly that
@@ -34,6 +37,14 @@ def stats(path, top):
         name = r["Name"].split("(")[0].replace("void ", "")
         print(f"| `{name}` | {r['Calls']} | {float(r['AverageNs']) / 1e3:.2f} | {float(r['MinNs']) / 1e3:.2f} | "
               f"{float(r['MaxNs']) / 1e3:.2f} | {float(r['Percentage']):.1f} |")
+
+
+def last(path, kernel, n):
+    d = [(int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) / 1e3 for r in csv.DictReader(open(path))
+         if r["Kernel_Name"].replace("void ", "").startswith("tdda::" + kernel)]
+    tail = d[-n:]
+    print(f"`{kernel}`: {len(d)} dispatches; last {len(tail)}: mean {sum(tail) / len(tail):.2f} us, "
+          f"min {min(tail):.2f} us, max {max(tail):.2f} us")
 
 
 def per_kernel(path, counter):
@@ -106,16 +117,20 @@ def next_smooth(n):
 
 def main():
     ap = argparse.ArgumentParser()
-    ap.add_argument("mode", choices=["stats", "traffic"])
+    ap.add_argument("mode", choices=["stats", "traffic", "last"])
     ap.add_argument("files", nargs="+")
     ap.add_argument("--top", type=int, default=20)
     ap.add_argument("--precision", default="c128")
     ap.add_argument("--workload", default="topo80")
     ap.add_argument("--copy-mib", type=int, default=256)
     ap.add_argument("--json", default=None)
+    ap.add_argument("--kernel", default="k_px<double, 160>")
+    ap.add_argument("--last", type=int, default=200)
     a = ap.parse_args()
     if a.mode == "stats":
         stats(a.files[0], a.top)
+    elif a.mode == "last":
+        last(a.files[0], a.kernel, a.last)
     else:
         traffic(a.files[0], a.files[1], a.precision, a.workload, a.copy_mib, a.json)
 
