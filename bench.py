@@ -422,6 +422,15 @@ def main():
         m.al_host = al
     F.set(T.VEC_E, E)
 
+    # field error of the forward solution against the FP64 device solver
+    # converged to 1e-11 (untimed)
+    Ref = T.DeviceModel(core, 1)
+    Ref.set(T.VEC_AL, al)
+    Ref.set(T.VEC_E, E)
+    ref_info = Ref.solve(mode=T.MODE_FP64, tol=1e-11, x0_zero=1, true_residual=1)[0]
+    Pref = Ref.get(T.VEC_P)[0]
+    Ref.close()
+
     results = {}
     for mode, name in ((T.MODE_FP64, "fp64"), (T.MODE_MIXED, "mixed")):
         lat = []
@@ -442,6 +451,7 @@ def main():
             "iterations_fwd": infos[-1][0]["iterations"], "iterations_adj": infos[-1][1]["iterations"],
             "true_residual_max": max(i["true_residual"] for pair in infos for i in pair),
             "all_converged": ok,
+            "field_error_fwd": float(np.linalg.norm(F.get(T.VEC_P)[0] - Pref) / np.linalg.norm(Pref)),
         }
 
     # per-pass timing of Y = A P + al o P on the device (HIP events)
@@ -500,6 +510,7 @@ def main():
         "latency_ms": {"p50": round(fp["p50_ms"], 3), "p99": round(fp["p99_ms"], 3)},
         "fp64": fp,
         "mixed": results["mixed"],
+        "reference_solve_fwd": {"iterations": ref_info["iterations"], "true_residual": ref_info["true_residual"]},
         "mixed_vs_plain_fp64": round(results["mixed"]["solves_per_s"] / fp["solves_per_s"], 3),
         "aproduct_passes": passes,
         "roofline": roof,
