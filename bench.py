@@ -108,6 +108,7 @@ def bench_mixed128(T, W, ctx, reps):
     Pref = m.get(T.VEC_P)[0]
     out = {"workload": "mixed128: 128x128x32, d=8nm, lambda=600nm, 50% occupancy, eps=(2+0.01i)^2",
            "grid": list(core.grid), "dipoles": prob.N,
+           "xpass": {"c128": core.xpass(T.C128), "c64": core.xpass(T.C64)},
            "reference_solve": {"iterations": ref["iterations"], "true_residual": ref["true_residual"]}}
     for mode, name in ((T.MODE_FP64, "fp64"), (T.MODE_MIXED, "mixed")):
         m.solve(mode=mode, tol=TOL, x0_zero=1, true_residual=1)
@@ -506,7 +507,8 @@ def main():
         "data": "synthetic",
         "config": {"workload": "topo80: 80x80x16 slab, d=10nm, lambda=542nm, grid 160x160x32, "
                                "forward + adjoint (100 targets) per step", "grid": list(core.grid),
-                   "dipoles": prob.N, "tiles_c128": list(core.tiles(T.C128)), "autotuned": TUNE},
+                   "dipoles": prob.N, "tiles_c128": list(core.tiles(T.C128)), "xpass_c128": core.xpass(T.C128),
+                   "xpass_c64": core.xpass(T.C64), "autotuned": TUNE},
         "latency_ms": {"p50": round(fp["p50_ms"], 3), "p99": round(fp["p99_ms"], 3)},
         "fp64": fp,
         "mixed": results["mixed"],

@@ -179,12 +179,14 @@ void ctx_free(const tdda_ctx_s* c, void* p)
 
 struct TilePlan {
     int pz, pz1, tfy, tfx, ps_z, ps_y, ps_x;
+    int xreg;  // P3 by the register-resident kernel k_pxr
     size_t lds_z, lds_z1, lds_y, lds_x;
 };
 
 // Upper limits on the planner's tile widths (0: none), set by the autotuner.
 struct TileCaps {
     int pz = 0, tfy = 0, tfx = 0;
+    int xlds = 0;  // 1: P3 by the LDS-staged kernel even where k_pxr exists
 };
 
 struct tdda_op_s {
@@ -296,6 +298,15 @@ int px_buffers(int L)
     return pass_length_compiled<T>(2, L) && px_inplace<T>(L) ? 1 : 2;
 }
 
+// LDS slots of an x-pass tile: twiddle table and data buffers
+template <typename T>
+size_t px_lds_slots(int xreg, int Mx, int tb, int tfx, int ps_x)
+{
+    if (xreg) return (size_t)pass_tw_slots(true, Mx, tb) + (size_t)tfx * ps_x;
+    return (size_t)pass_tw_slots(pass_length_compiled<T>(2, Mx) && pass_full_twiddles<T>(2, Mx), Mx, tb) +
+           (size_t)px_buffers<T>(Mx) * tfx * ps_x;
+}
+
 // Tile widths of the passes for element type T: a tile holds at most
 // `cap` elements (pencils x length, two LDS buffers of it), rows of the
 // strided passes are kept to whole 128-byte lines where the budget allows,
@@ -347,12 +358,21 @@ int plan_tiles(const tdda_op_s* op, TilePlan& t, const TileCaps& caps = TileCaps
     }
     if (caps.tfy > 0) t.tfy = std::min(t.tfy, caps.tfy);
     if (caps.tfx > 0) t.tfx = std::max(3, std::min(t.tfx, (caps.tfx / 3) * 3));
-    if ((t.tfx / 3) * op->Mx > 8 * kThreads) return fail(TDDA_ERR_UNSUPPORTED, "x pass tile too wide");
+    // register-resident x pass where its kernel exists (TDDA_XPASS=0 or the
+    // autotuner's xlds cap keep the LDS-staged kernel): tiles of at most
+    // pxr_nf columns, odd pencil stride, full twiddle table
+    t.xreg = pass_length_compiled<T>(3, op->Mx) && !caps.xlds && env_int("TDDA_XPASS", 1) != 0;
+    if (t.xreg) {
+        t.tfx = std::max(3, std::min(pxr_nf<T>(op->Mx), 3 * op->Mz));
+        if (caps.tfx > 0) t.tfx = std::max(3, std::min(t.tfx, (caps.tfx / 3) * 3));
+        t.ps_x = op->Mx | 1;
+    }
+    if (!t.xreg && (t.tfx / 3) * op->Mx > 8 * kThreads) return fail(TDDA_ERR_UNSUPPORTED, "x pass tile too wide");
     auto tws = [&](int kind, int L, int tb) { return pass_tw_slots(pass_length_compiled<T>(kind, L) && pass_full_twiddles<T>(kind, L), L, tb); };
     t.lds_z = (size_t)(tws(0, op->Mz, op->pl[2].tb) + pz_buffers<T>(op->Mz) * 3 * t.pz * t.ps_z) * s;
     t.lds_z1 = (size_t)(tws(0, op->Mz, op->pl[2].tb) + pz_buffers<T>(op->Mz) * 3 * t.pz1 * t.ps_z) * s;
     t.lds_y = (size_t)(tws(1, op->My, op->pl[1].tb) + py_buffers<T>(op->My) * t.tfy * t.ps_y) * s;
-    t.lds_x = (size_t)(tws(2, op->Mx, op->pl[0].tb) + px_buffers<T>(op->Mx) * t.tfx * t.ps_x) * s;
+    t.lds_x = px_lds_slots<T>(t.xreg, op->Mx, op->pl[0].tb, t.tfx, t.ps_x) * s;
     if (std::max({t.lds_z, t.lds_z1, t.lds_y, t.lds_x}) > op->ctx->lds_budget)
         return fail(TDDA_ERR_UNSUPPORTED, "pass tile exceeds the LDS budget");
     return TDDA_OK;
@@ -426,6 +446,7 @@ void fill_jobs(const tdda_model_s* m, std::vector<JobDesc<T>>& out, const VecSet
         J.ps_z = tp.ps_z;
         J.ps_y = tp.ps_y;
         J.ps_x = tp.ps_x;
+        J.xreg = tp.xreg;
         J.cellmap = op->cellmap;
         J.green = green;
         J.twx = tw[0];
@@ -495,7 +516,10 @@ int build_groups(const tdda_ctx_s* c, JobTable<T>& jt, Put&& put)
     auto build = [&](std::vector<PassGroup<T>>& groups, int kind, auto len, auto ntiles, auto lds) -> int {
         groups.clear();
         std::vector<int> key(nj);
-        for (int j = 0; j < nj; j++) key[j] = pass_length_compiled<T>(kind, len(jt.h[j])) ? len(jt.h[j]) : 0;
+        for (int j = 0; j < nj; j++) {
+            const int L = len(jt.h[j]);
+            key[j] = (kind == 2 && jt.h[j].xreg) ? (L | kPxrKey) : pass_length_compiled<T>(kind, L) ? L : 0;
+        }
         std::vector<int> keys;
         for (int j = 0; j < nj; j++)
             if (std::find(keys.begin(), keys.end(), key[j]) == keys.end()) keys.push_back(key[j]);
@@ -532,7 +556,7 @@ int build_groups(const tdda_ctx_s* c, JobTable<T>& jt, Put&& put)
     CHK(build(jt.gy, 1, [](const JobDesc<T>& J) { return J.My; }, [](const JobTiles& t) { return t.y; },
               [&](const JobDesc<T>& J) { return (size_t)(tws(1, J.My, J.ply.tb) + py_buffers<T>(J.My) * J.tfy * J.ps_y) * s; }));
     CHK(build(jt.gx, 2, [](const JobDesc<T>& J) { return J.Mx; }, [](const JobTiles& t) { return t.x; },
-              [&](const JobDesc<T>& J) { return (size_t)(tws(2, J.Mx, J.plx.tb) + px_buffers<T>(J.Mx) * J.tfx * J.ps_x) * s; }));
+              [&](const JobDesc<T>& J) { return px_lds_slots<T>(J.xreg, J.Mx, J.plx.tb, J.tfx, J.ps_x) * s; }));
     return TDDA_OK;
 }
 

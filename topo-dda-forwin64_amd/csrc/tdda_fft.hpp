@@ -156,6 +156,131 @@ __device__ __forceinline__ void dft(Cx<T>* v)
     else dft8<T, INV>(v);
 }
 
+// ---------------------------------------------------------------------------
+// Small DFTs of any length in {2, 3, 4, 5, 6, 8, 9, 10, 12, 15, 16}, held
+// entirely in registers, for the register-resident x pass.  Lengths other
+// than 2, 3, 4, 5, 8 are split as R = A * B (decimation in time:
+// n = B a + b, k = k1 + A k2):
+//   X[k1 + A k2] = sum_b w_B^(b k2) w_R^(b k1) sum_a v[B a + b] w_A^(a k1)
+// with the twiddles w_R^(b k1) as compile-time constants.
+// ---------------------------------------------------------------------------
+struct CosSin {
+    double c, s;
+};
+
+// cos and sin of x for |x| <= pi/4 (Taylor series, 13 terms: below one ulp)
+constexpr CosSin ct_cs_small(double x)
+{
+    const double x2 = x * x;
+    double c = 1.0, s = x, tc = 1.0, ts = x;
+    for (int k = 1; k <= 13; k++) {
+        tc *= -x2 / (double)((2 * k - 1) * (2 * k));
+        ts *= -x2 / (double)((2 * k) * (2 * k + 1));
+        c += tc;
+        s += ts;
+    }
+    return CosSin{c, s};
+}
+
+// (cos, sin) of 2 pi m / R at compile time: exact reduction to a quadrant
+// q and a remainder angle (pi/2) r / R, then to [0, pi/4] by the complement.
+constexpr CosSin ct_root(int m, int R)
+{
+    m %= R;
+    if (m < 0) m += R;
+    const int q = (4 * m) / R, r = 4 * m - q * R;
+    const double hp = 1.57079632679489661923132169163975144;
+    CosSin b{1.0, 0.0};
+    if (2 * r <= R) {
+        b = ct_cs_small(hp * (double)r / (double)R);
+    } else {
+        const CosSin t = ct_cs_small(hp * (double)(R - r) / (double)R);
+        b = CosSin{t.s, t.c};
+    }
+    switch (q & 3) {
+    case 0: return b;
+    case 1: return CosSin{-b.s, b.c};
+    case 2: return CosSin{-b.c, -b.s};
+    default: return CosSin{b.s, -b.c};
+    }
+}
+
+constexpr bool sdft_ok(int R)
+{
+    return R == 2 || R == 3 || R == 4 || R == 5 || R == 6 || R == 8 || R == 9 || R == 10 || R == 12 || R == 15 ||
+           R == 16;
+}
+// first factor A of the split R = A * B
+constexpr int sdft_split(int R) { return (R % 4 == 0) ? 4 : (R % 2 == 0) ? 2 : (R % 3 == 0) ? 3 : 5; }
+
+// Calls f(std::integral_constant<int, I>) for I = I0 .. N-1 (unrolled at
+// compile time, so every index is a constant).
+template <int I, int N, typename F>
+__device__ __forceinline__ void sfor(F&& f)
+{
+    if constexpr (I < N) {
+        f(std::integral_constant<int, I>{});
+        sfor<I + 1, N>(f);
+    }
+}
+
+template <typename T, int R, bool INV>
+__device__ __forceinline__ void sdft(Cx<T>* v)
+{
+    static_assert(R == 1 || sdft_ok(R), "unsupported register DFT length");
+    if constexpr (R == 1) {
+        return;
+    } else if constexpr (R == 2 || R == 3 || R == 4 || R == 5 || R == 8) {
+        dft<T, R, INV>(v);
+    } else {
+        constexpr int A = sdft_split(R), B = R / A;
+        Cx<T> u[B][A];
+        sfor<0, B>([&](auto bc) {
+            constexpr int b = decltype(bc)::value;
+            Cx<T> t[A];
+            sfor<0, A>([&](auto ac) {
+                constexpr int a = decltype(ac)::value;
+                t[a] = v[B * a + b];
+            });
+            sdft<T, A, INV>(t);
+            sfor<0, A>([&](auto kc) {
+                constexpr int k1 = decltype(kc)::value;
+                if constexpr (b * k1 == 0) {
+                    u[b][k1] = t[k1];
+                } else {
+                    // forward: times exp(-2 pi i b k1 / R); inverse: its conjugate
+                    constexpr CosSin w = ct_root(b * k1, R);
+                    const T c = (T)w.c, s = INV ? (T)w.s : (T)(-w.s);
+                    u[b][k1] = Cx<T>{t[k1].x * c - t[k1].y * s, t[k1].x * s + t[k1].y * c};
+                }
+            });
+        });
+        sfor<0, A>([&](auto kc) {
+            constexpr int k1 = decltype(kc)::value;
+            Cx<T> t[B];
+            sfor<0, B>([&](auto bc) {
+                constexpr int b = decltype(bc)::value;
+                t[b] = u[b][k1];
+            });
+            sdft<T, B, INV>(t);
+            sfor<0, B>([&](auto k2c) {
+                constexpr int k2 = decltype(k2c)::value;
+                v[k1 + A * k2] = t[k2];
+            });
+        });
+    }
+}
+
+// Factors L = L1 * L2 of the register-resident x pass: both in the small-DFT
+// set, L1 <= L2 and as close as possible (L1 = 0: no such split).
+constexpr int pxr_l1(int L)
+{
+    int best = 0;
+    for (int a = 2; a * a <= L; a++)
+        if (L % a == 0 && sdft_ok(a) && sdft_ok(L / a)) best = a;
+    return best;
+}
+
 // One Stockham stage over NP pencils of length L held in LDS (pencil stride
 // ps).  Ns is the length of the sub-transforms already completed.
 template <typename T, int R, bool INV>

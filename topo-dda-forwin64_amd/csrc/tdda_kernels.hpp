@@ -83,6 +83,7 @@ struct JobDesc {
     int nfy;     // tiles per x-slab in P2/P4: ceil(3 Mz / tfy)
     int nfx;     // tiles per ky row in P3: ceil(3 Mz / tfx)
     int ps_z, ps_y, ps_x;  // LDS pencil strides
+    int xreg;              // P3 runs the register-resident kernel (k_pxr)
     const int* cellmap;    // [Nx][Ny][Nz] -> site or -1; nullptr for the full box in bulk order
     const Cx<T>* green;
     const Cx<T>* twx;   // two-level twiddle tables (runtime-plan kernels)
@@ -862,6 +863,166 @@ __global__ void __launch_bounds__(kThreads, px_waves<T>(LC)) k_px(const JobDesc<
         }
         if (tl >= ntot) break;
         cur = nxt;
+    }
+}
+
+// Register-resident x pass (P3) for lengths L = L1 * L2 with both factors
+// in the small-DFT set (pxr_l1).  A tile is (ky, nf consecutive (kz, c)
+// columns), nf a multiple of 3 with nf * L2 <= kThreads.  With
+// x = n1 + L1 n2 and kx = k2 + L2 k1 (four-step split):
+//   A : thread (col, n1) loads x[n1 + L1 n2] from w2 (n2 < ceil(L2/2): the
+//       rest is zero padding), DFT over n2 in registers, times w_L^(n1 k2),
+//       to LDS slot n1 L2 + k2;
+//   B : thread (col, k2) reads the L1 slots of its k2, DFT over n1 in
+//       registers, back to the same slots, now kx = k2 + L2 k1 (natural
+//       order);
+//   G : the same thread forms component c of the symmetric Green product at
+//       its L1 frequencies (the other two components read from LDS), then the
+//       inverse DFT over k1, times conj(w_L^(n1 k2)), to LDS;
+//   A': thread (col, n1) reads its L2 slots, inverse DFT over k2, and stores
+//       the x < Nx outputs to w2.
+// Each element crosses LDS three times (the LDS-staged transforms cross it at
+// every radix stage).  No prefetch: the resident workgroups of a CU overlap
+// each other's loads.
+template <typename T>
+__host__ __device__ constexpr int pxr_nf(int L)
+{
+    return pxr_l1(L) > 0 ? ((kThreads / (L / pxr_l1(L))) / 3) * 3 : 0;
+}
+
+// Waves per SIMD k_pxr is compiled for: complex128 with a 12- or 16-point
+// step needs the registers of three (with those of four it spills), and
+// from 180 points those of two (170-195 VGPRs; at three it spills up to
+// 140 bytes per lane).  160 points keep three (20 bytes of spill, and
+// faster than two waves without).
+template <typename T>
+__host__ __device__ constexpr int pxr_waves(int L)
+{
+    return (sizeof(T) == 8 && L >= 180) ? 2 : (sizeof(T) == 8 && pxr_l1(L) > 0 && L / pxr_l1(L) >= 12) ? 3 : 4;
+}
+
+template <typename T, int LC>
+__global__ void __launch_bounds__(kThreads, pxr_waves<T>(LC)) k_pxr(const JobDesc<T>* __restrict__ jobs,
+                                                     const int* __restrict__ tstart, int njobs, int solver)
+{
+    constexpr int L1 = pxr_l1(LC), L2 = LC / L1, H2 = (L2 + 1) / 2;
+    static_assert(L1 > 0 && L1 <= L2, "length without a register split");
+    extern __shared__ __align__(16) char smem[];
+    Cx<T>* tw = reinterpret_cast<Cx<T>*>(smem);  // tw[q] = exp(-2 pi i q / L)
+    Cx<T>* A = tw + pass_tw_slots(true, LC, 0);
+    const int ntot = tstart[njobs];
+    // the full table of the L-th roots depends on L only: one per launch
+    for (int i = threadIdx.x; i < LC; i += kThreads) tw[i] = gld(jobs[0].twfx + i);
+    __syncthreads();
+    const int task = threadIdx.x;
+    for (int tl = blockIdx.x; tl < ntot; tl += gridDim.x) {
+        const STile t = s_tile(jobs, tstart, njobs, tl, true, solver != 0);
+        if (!t.live) continue;
+        const JobDesc<T>* J = jobs + t.job;
+        const int Nx = J->Nx, My = J->My, Mz = J->Mz, F = 3 * Mz, nf = t.nf, ky = t.outer, ps = J->ps_x;
+        const int sx = My * F;  // one slab row: < 2^31 elements
+        Cx<T>* base = J->w2 + (size_t)ky * F + t.f0;
+        const int qa = task / nf, ca = task - qa * nf;  // (n1, col) of steps A / A', (k2, col) of B / G
+        // A: forward DFT over n2
+        if (qa < L1) {
+            const int n1 = qa;
+            Cx<T> v[L2];
+            sfor<0, L2>([&](auto c2) {
+                constexpr int n2 = decltype(c2)::value;
+                if constexpr (n2 >= H2) {
+                    v[n2] = Cx<T>{T(0), T(0)};
+                } else {
+                    const int x = n1 + L1 * n2;
+                    v[n2] = x < Nx ? gld(base + (x * sx + ca)) : Cx<T>{T(0), T(0)};
+                }
+            });
+            sdft<T, L2, false>(v);
+            Cx<T>* row = A + ca * ps + n1 * L2;
+            row[0] = v[0];
+            sfor<1, L2>([&](auto c2) {
+                constexpr int k2 = decltype(c2)::value;
+                row[k2] = cmul(v[k2], tw[n1 * k2]);
+            });
+        }
+        __syncthreads();
+        const bool actb = qa < L2;
+        const int k2 = qa;
+        Cx<T> v[L1];
+        // B: forward DFT over n1; slot k2 + L2 k1 then holds kx = k2 + L2 k1
+        if (actb) {
+            Cx<T>* cp = A + ca * ps + k2;
+            sfor<0, L1>([&](auto c1) { v[decltype(c1)::value] = cp[decltype(c1)::value * L2]; });
+            sdft<T, L1, false>(v);
+            sfor<0, L1>([&](auto c1) { cp[decltype(c1)::value * L2] = v[decltype(c1)::value]; });
+        }
+        __syncthreads();
+        // G: component c of the Green product, then the inverse DFT over k1
+        if (actb) {
+            const int kzl = (int)((unsigned)ca / 3u), c = ca - 3 * kzl;
+            const int kz = t.f0 / 3 + kzl;
+            const int kyf = (ky > My / 2) ? My - ky : ky;
+            const int kzf = (kz > Mz / 2) ? Mz - kz : kz;
+            const T sy = (ky > My / 2) ? T(-1) : T(1);
+            const T sz = (kz > Mz / 2) ? T(-1) : T(1);
+            // row c of the tensor: octant indices (xx xy xz yy yz zz) and the
+            // fold signs, split into the part odd in x and the rest
+            const int i0 = c, i1 = c == 0 ? 1 : c + 2, i2 = c == 0 ? 2 : c + 3;
+            const T a0 = c == 0 ? T(1) : (c == 1 ? sy : sz);
+            const T a1 = c == 0 ? sy : (c == 1 ? T(1) : sy * sz);
+            const T a2 = c == 0 ? sz : (c == 1 ? sy * sz : T(1));
+            const bool ox0 = c != 0, ox1 = c == 0, ox2 = c == 0;
+            const Cx<T>* b0 = A + (ca - c) * ps;
+            const Cx<T>* grow = J->green + ((size_t)kyf * J->Hz + kzf) * 6;
+            const size_t gsx = (size_t)J->Hy * J->Hz * 6;
+            sfor<0, L1>([&](auto c1) {
+                constexpr int k1 = decltype(c1)::value;
+                const int kx = k2 + L2 * k1;
+                const bool fx = 2 * kx > LC;
+                const int kxf = fx ? LC - kx : kx;
+                const T sxv = fx ? T(-1) : T(1);
+                const Cx<T>* gp = grow + kxf * gsx;
+                const Cx<T> g0 = cscale(gld(gp + i0), ox0 ? a0 * sxv : a0);
+                const Cx<T> g1 = cscale(gld(gp + i1), ox1 ? a1 * sxv : a1);
+                const Cx<T> g2 = cscale(gld(gp + i2), ox2 ? a2 * sxv : a2);
+                const Cx<T> bx = b0[kx], by = b0[ps + kx], bz = b0[2 * ps + kx];
+                v[k1] = cadd(cadd(cmul(g0, bx), cmul(g1, by)), cmul(g2, bz));
+                // groups of 4 frequencies: the scheduler would otherwise
+                // hoist every load of the loop and spill
+                if constexpr (k1 % 4 == 3) __builtin_amdgcn_sched_barrier(0);
+            });
+            sdft<T, L1, true>(v);
+        }
+        __syncthreads();
+        if (actb) {
+            // an opaque copy of k2: step A read the same twiddles tw[qa m],
+            // and keeping them live from A to here would spill
+            int k2o = k2;
+            __asm__ volatile("" : "+v"(k2o));
+            Cx<T>* cp = A + ca * ps + k2o;
+            cp[0] = v[0];
+            sfor<1, L1>([&](auto c1) {
+                constexpr int n1 = decltype(c1)::value;
+                cp[n1 * L2] = cmulc(v[n1], tw[n1 * k2o]);
+            });
+        }
+        __syncthreads();
+        // A': inverse DFT over k2, store the x < Nx outputs
+        if (qa < L1) {
+            // opaque copies: the output addresses are recomputed rather than
+            // kept live from step A
+            int n1 = qa, cao = ca;
+            __asm__ volatile("" : "+v"(n1), "+v"(cao));
+            Cx<T> w[L2];
+            const Cx<T>* row = A + cao * ps + n1 * L2;
+            sfor<0, L2>([&](auto c2) { w[decltype(c2)::value] = row[decltype(c2)::value]; });
+            sdft<T, L2, true>(w);
+            sfor<0, H2>([&](auto c2) {
+                constexpr int n2 = decltype(c2)::value;
+                const int x = n1 + L1 * n2;
+                if (x < Nx) gst(base + (x * sx + cao), w[n2]);
+            });
+        }
+        __syncthreads();
     }
 }
 

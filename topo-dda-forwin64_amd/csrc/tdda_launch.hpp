@@ -25,8 +25,11 @@ template <typename T>
 void launch_py(int L, bool inv, int solver, const PassLaunch& pl, const JobDesc<T>* jobs, const int* tstart, int nj);
 template <typename T>
 void launch_px(int L, int solver, const PassLaunch& pl, const JobDesc<T>* jobs, const int* tstart, int nj);
+// Launch-group key bit of the register-resident x pass: launch_px runs
+// k_pxr<L> for L | kPxrKey.
+constexpr int kPxrKey = 1 << 20;
 // true when a specialised kernel exists for this length and pass kind
-// (0: z pass, 1: y pass, 2: x pass)
+// (0: z pass, 1: y pass, 2: x pass, 3: register-resident x pass)
 template <typename T>
 bool pass_length_compiled(int kind, int L);
 

@@ -35,7 +35,7 @@ EXPORTS = [
     "tdda_server_create", "tdda_server_destroy", "tdda_server_submit", "tdda_server_poll", "tdda_server_wait",
     "tdda_server_result", "tdda_server_get_stats", "tdda_design_create", "tdda_design_step", "tdda_design_get",
     "tdda_design_destroy", "tdda_beta_update", "tdda_op_create_periodic", "tdda_objective_point_e",
-    "tdda_objective_integrated_e", "tdda_op_autotune", "tdda_op_tiles", "tdda_op_create_ex",
+    "tdda_objective_integrated_e", "tdda_op_autotune", "tdda_op_tiles", "tdda_op_xpass", "tdda_op_create_ex",
     "tdda_green_tensor_fcd", "tdda_field_at_points", "tdda_write_structure", "tdda_write_model_results",
 ]
 
@@ -159,6 +159,7 @@ def lib():
                                                ctypes.c_int, dp, dp]
         L.tdda_op_autotune.argtypes = [vp, ctypes.c_int]
         L.tdda_op_tiles.argtypes = [vp, ctypes.c_int, ip]
+        L.tdda_op_xpass.argtypes = [vp, ctypes.c_int]
         L.tdda_objective_point_e.argtypes = [vp, ctypes.c_int, dp, dp, dp, ctypes.c_double, dp, vp, ctypes.c_int]
         L.tdda_objective_integrated_e.argtypes = [vp, ctypes.c_int, ctypes.c_int, ip, ctypes.c_double,
                                                   ctypes.c_double, dp, dp, vp, ctypes.c_int]
@@ -329,6 +330,12 @@ class AProductCore:
         out = (ctypes.c_int * 4)()
         _check(lib().tdda_op_tiles(self.h, precision, out))
         return tuple(out)
+
+    def xpass(self, precision=C128):
+        """x-pass kernel (tdda_op_xpass): "register" or "lds"."""
+        rc = lib().tdda_op_xpass(self.h, precision)
+        _check(min(rc, 0))
+        return "register" if rc == 1 else "lds"
 
     def Aproduct(self, b, al=None, precision=C128):
         """y = A_offdiag b (+ al o b); b is (3N,) or (nrhs, 3N) complex128."""
