@@ -473,7 +473,8 @@ def main():
     if os.path.exists(tfile):
         with open(tfile) as f:
             traffic = json.load(f).get(names[k])
-    roof = {"bound": "hbm", "kernel": names[k], "achieved": round(float(b128[k] / (t_k * 1e-3) / 1e9), 1),
+    kern = ["k_p1", "k_py<fwd>", "k_pxr" if core.xpass(T.C128) == "register" else "k_px", "k_py<inv>", "k_p5"][k]
+    roof = {"bound": "hbm", "kernel": names[k], "device_kernel": kern, "achieved": round(float(b128[k] / (t_k * 1e-3) / 1e9), 1),
             "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(float(b128[k] / (t_k * 1e-3) / 1e9 / HBM_PEAK_GBS), 4),
             "traffic": traffic, "algorithmic_bytes": int(b128[k]), "duration_us": round(t_k * 1e3, 2)}

@@ -272,12 +272,15 @@ __device__ __forceinline__ void sdft(Cx<T>* v)
 }
 
 // Factors L = L1 * L2 of the register-resident x pass: both in the small-DFT
-// set, L1 <= L2 and as close as possible (L1 = 0: no such split).
+// set, L1 >= L2 and as close as possible (L1 = 0: no such split).  The
+// larger factor goes to the step that holds the Green product: the steps
+// with the global loads and stores then hold fewer values (160 = 16 x 10
+// needs 146 VGPRs in complex128, 10 x 16 needs 171).
 constexpr int pxr_l1(int L)
 {
     int best = 0;
     for (int a = 2; a * a <= L; a++)
-        if (L % a == 0 && sdft_ok(a) && sdft_ok(L / a)) best = a;
+        if (L % a == 0 && sdft_ok(a) && sdft_ok(L / a)) best = L / a;
     return best;
 }
 

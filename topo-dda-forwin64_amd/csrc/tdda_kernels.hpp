@@ -531,13 +531,37 @@ __device__ __forceinline__ STile s_tile(const JobDesc<T>* __restrict__ jobs, con
     const int nfb = xpass ? J->nfx : J->nfy;
     const int tf = xpass ? J->tfx : J->tfy;
     const int r = tl - tstart[t.job];
-    const int o = r / nfb;
-    t.f0 = (r - o * nfb) * tf;
-    // x pass: ky in the order 0, 1, My-1, 2, My-2, ..., so the two tiles of
-    // a mirrored pair (which read the same Green octant rows) run at nearly
-    // the same time; nfb tiles apart, they share an XCD's L2 when nfb is a
-    // multiple of the XCD count.
-    t.outer = !xpass ? o : (o == 0 ? 0 : ((o & 1) ? (o + 1) >> 1 : J->My - (o >> 1)));
+    if (!xpass) {
+        const int o = r / nfb;
+        t.f0 = (r - o * nfb) * tf;
+        t.outer = o;
+    } else {
+        // x pass: the tiles (ky, f) and (My - ky, f) read the same Green
+        // octant rows.  Paired tiles come in blocks of 16: eight (pair, f)
+        // slots, then the same slots' mirrors, so the two tiles of a pair
+        // are 8 apart in the launch's tile list -- on the same XCD (workgroups
+        // go to the 8 XCDs round robin, and persistent grids are multiples
+        // of 8) at about the same time, sharing its L2.  The unpaired rows
+        // ky = 0 and ky = My / 2 (My even) follow.
+        const int My = J->My, npair = (My - 1) / 2, paired = 2 * npair * nfb;
+        int fb, ky;
+        if (r < paired) {
+            const int q = r >> 4, w = r & 15;
+            const int rem = paired - 16 * q;
+            const int n = rem >= 16 ? 8 : rem >> 1;  // slots in this block
+            const int h = w >= n ? 1 : 0, slot = 8 * q + (w - h * n);
+            const int pair = slot / nfb;
+            fb = slot - pair * nfb;
+            ky = h ? My - 1 - pair : pair + 1;
+        } else {
+            const int s = r - paired;
+            const int row = s >= nfb ? 1 : 0;
+            fb = s - row * nfb;
+            ky = row ? My / 2 : 0;
+        }
+        t.f0 = fb * tf;
+        t.outer = ky;
+    }
     t.nf = min(tf, J->Mz * 3 - t.f0);
     t.live = !(check_state && job_inactive(J->st));
     return t;
@@ -867,8 +891,8 @@ __global__ void __launch_bounds__(kThreads, px_waves<T>(LC)) k_px(const JobDesc<
 }
 
 // Register-resident x pass (P3) for lengths L = L1 * L2 with both factors
-// in the small-DFT set (pxr_l1).  A tile is (ky, nf consecutive (kz, c)
-// columns), nf a multiple of 3 with nf * L2 <= kThreads.  With
+// in the small-DFT set (pxr_l1, L1 >= L2).  A tile is (ky, nf consecutive
+// (kz, c) columns), nf a multiple of 3 with nf * L1 <= kThreads.  With
 // x = n1 + L1 n2 and kx = k2 + L2 k1 (four-step split):
 //   A : thread (col, n1) loads x[n1 + L1 n2] from w2 (n2 < ceil(L2/2): the
 //       rest is zero padding), DFT over n2 in registers, times w_L^(n1 k2),
@@ -876,29 +900,28 @@ __global__ void __launch_bounds__(kThreads, px_waves<T>(LC)) k_px(const JobDesc<
 //   B : thread (col, k2) reads the L1 slots of its k2, DFT over n1 in
 //       registers, back to the same slots, now kx = k2 + L2 k1 (natural
 //       order);
-//   G : the same thread forms component c of the symmetric Green product at
-//       its L1 frequencies (the other two components read from LDS), then the
-//       inverse DFT over k1, times conj(w_L^(n1 k2)), to LDS;
+//   G : the symmetric Green product in place in LDS, one thread per
+//       (kz, |kx|) pair of the tile (one octant load serves kx and L - kx);
+//   B': thread (col, k2) reads its L1 slots, inverse DFT over k1, times
+//       conj(w_L^(n1 k2)), back to the same slots;
 //   A': thread (col, n1) reads its L2 slots, inverse DFT over k2, and stores
 //       the x < Nx outputs to w2.
-// Each element crosses LDS three times (the LDS-staged transforms cross it at
+// Each element crosses LDS four times (the LDS-staged transforms cross it at
 // every radix stage).  No prefetch: the resident workgroups of a CU overlap
 // each other's loads.
 template <typename T>
 __host__ __device__ constexpr int pxr_nf(int L)
 {
-    return pxr_l1(L) > 0 ? ((kThreads / (L / pxr_l1(L))) / 3) * 3 : 0;
+    return pxr_l1(L) > 0 ? ((kThreads / pxr_l1(L)) / 3) * 3 : 0;
 }
 
-// Waves per SIMD k_pxr is compiled for: complex128 with a 12- or 16-point
-// step needs the registers of three (with those of four it spills), and
-// from 180 points those of two (170-195 VGPRs; at three it spills up to
-// 140 bytes per lane).  160 points keep three (20 bytes of spill, and
-// faster than two waves without).
+// Waves per SIMD k_pxr is compiled for, the most at which the compiler
+// keeps it out of scratch: four (complex64 50-64 VGPRs, complex128 73-122),
+// three for complex128 at 256 points (131 VGPRs; with four it spills).
 template <typename T>
 __host__ __device__ constexpr int pxr_waves(int L)
 {
-    return (sizeof(T) == 8 && L >= 180) ? 2 : (sizeof(T) == 8 && pxr_l1(L) > 0 && L / pxr_l1(L) >= 12) ? 3 : 4;
+    return (sizeof(T) == 8 && L >= 256) ? 3 : 4;
 }
 
 template <typename T, int LC>
@@ -906,7 +929,7 @@ __global__ void __launch_bounds__(kThreads, pxr_waves<T>(LC)) k_pxr(const JobDes
                                                      const int* __restrict__ tstart, int njobs, int solver)
 {
     constexpr int L1 = pxr_l1(LC), L2 = LC / L1, H2 = (L2 + 1) / 2;
-    static_assert(L1 > 0 && L1 <= L2, "length without a register split");
+    static_assert(L1 > 0 && L1 >= L2, "length without a register split");
     extern __shared__ __align__(16) char smem[];
     Cx<T>* tw = reinterpret_cast<Cx<T>*>(smem);  // tw[q] = exp(-2 pi i q / L)
     Cx<T>* A = tw + pass_tw_slots(true, LC, 0);
@@ -956,49 +979,41 @@ __global__ void __launch_bounds__(kThreads, pxr_waves<T>(LC)) k_pxr(const JobDes
             sfor<0, L1>([&](auto c1) { cp[decltype(c1)::value * L2] = v[decltype(c1)::value]; });
         }
         __syncthreads();
-        // G: component c of the Green product, then the inverse DFT over k1
-        if (actb) {
-            const int kzl = (int)((unsigned)ca / 3u), c = ca - 3 * kzl;
-            const int kz = t.f0 / 3 + kzl;
+        // G: the Green product in place in LDS, one thread per (kz, |kx|):
+        // the six octant components serve kx and its mirror L - kx (which
+        // flips the sign of xy and xz), the fold of ky and kz flips the
+        // components odd in y or z
+        {
+            const int nkz = nf / 3, nw = nkz * (LC / 2 + 1);
             const int kyf = (ky > My / 2) ? My - ky : ky;
-            const int kzf = (kz > Mz / 2) ? Mz - kz : kz;
             const T sy = (ky > My / 2) ? T(-1) : T(1);
-            const T sz = (kz > Mz / 2) ? T(-1) : T(1);
-            // row c of the tensor: octant indices (xx xy xz yy yz zz) and the
-            // fold signs, split into the part odd in x and the rest
-            const int i0 = c, i1 = c == 0 ? 1 : c + 2, i2 = c == 0 ? 2 : c + 3;
-            const T a0 = c == 0 ? T(1) : (c == 1 ? sy : sz);
-            const T a1 = c == 0 ? sy : (c == 1 ? T(1) : sy * sz);
-            const T a2 = c == 0 ? sz : (c == 1 ? sy * sz : T(1));
-            const bool ox0 = c != 0, ox1 = c == 0, ox2 = c == 0;
-            const Cx<T>* b0 = A + (ca - c) * ps;
-            const Cx<T>* grow = J->green + ((size_t)kyf * J->Hz + kzf) * 6;
+            const Cx<T>* grow = J->green + (size_t)kyf * J->Hz * 6;
             const size_t gsx = (size_t)J->Hy * J->Hz * 6;
-            sfor<0, L1>([&](auto c1) {
-                constexpr int k1 = decltype(c1)::value;
-                const int kx = k2 + L2 * k1;
-                const bool fx = 2 * kx > LC;
-                const int kxf = fx ? LC - kx : kx;
-                const T sxv = fx ? T(-1) : T(1);
-                const Cx<T>* gp = grow + kxf * gsx;
-                const Cx<T> g0 = cscale(gld(gp + i0), ox0 ? a0 * sxv : a0);
-                const Cx<T> g1 = cscale(gld(gp + i1), ox1 ? a1 * sxv : a1);
-                const Cx<T> g2 = cscale(gld(gp + i2), ox2 ? a2 * sxv : a2);
-                const Cx<T> bx = b0[kx], by = b0[ps + kx], bz = b0[2 * ps + kx];
-                v[k1] = cadd(cadd(cmul(g0, bx), cmul(g1, by)), cmul(g2, bz));
-                // groups of 4 frequencies: the scheduler would otherwise
-                // hoist every load of the loop and spill
-                if constexpr (k1 % 4 == 3) __builtin_amdgcn_sched_barrier(0);
-            });
-            sdft<T, L1, true>(v);
+            for (int e = task; e < nw; e += kThreads) {
+                const int kxf = (int)((unsigned)e / (unsigned)nkz), kzl = e - kxf * nkz;
+                const int kz = t.f0 / 3 + kzl;
+                const int kzf = (kz > Mz / 2) ? Mz - kz : kz;
+                const T sz = (kz > Mz / 2) ? T(-1) : T(1);
+                const Cx<T>* gp = grow + kxf * gsx + kzf * 6;
+                const Cx<T> gxx = gld(gp), gyy = gld(gp + 3), gzz = gld(gp + 5);
+                const Cx<T> gxy = cscale(gld(gp + 1), sy);
+                const Cx<T> gxz = cscale(gld(gp + 2), sz);
+                const Cx<T> gyz = cscale(gld(gp + 4), sy * sz);
+                Cx<T>* b0 = A + (3 * kzl) * ps;
+                green_apply(b0 + kxf, ps, gxx, gxy, gxz, gyy, gyz, gzz);
+                if (kxf != 0 && 2 * kxf != LC) green_apply(b0 + (LC - kxf), ps, gxx, cneg(gxy), cneg(gxz), gyy, gyz, gzz);
+            }
         }
         __syncthreads();
+        // B': inverse DFT over k1, times conj(w_L^(n1 k2)), to the same slots
         if (actb) {
             // an opaque copy of k2: step A read the same twiddles tw[qa m],
             // and keeping them live from A to here would spill
             int k2o = k2;
             __asm__ volatile("" : "+v"(k2o));
             Cx<T>* cp = A + ca * ps + k2o;
+            sfor<0, L1>([&](auto c1) { v[decltype(c1)::value] = cp[decltype(c1)::value * L2]; });
+            sdft<T, L1, true>(v);
             cp[0] = v[0];
             sfor<1, L1>([&](auto c1) {
                 constexpr int n1 = decltype(c1)::value;

@@ -6,7 +6,10 @@ calibration point for the counters (the copy moves exactly --copy-mib MiB
 in and out).
 
 usage: python3 topo-dda-forwin64_amd/profile_passes.py [--workload topo80|mixed128]
-       [--precision c128|c64] [--reps 20] [--copy-mib 256]
+       [--precision c128|c64] [--reps 20] [--copy-mib 256] [--no-tune]
+
+The operator runs the autotuner first (as in bench.py), so its tuner's trial
+launches precede the timed passes in a trace.
 """
 import argparse
 import ctypes
@@ -40,6 +43,7 @@ def main():
     ap.add_argument("--precision", default="c128")
     ap.add_argument("--reps", type=int, default=20)
     ap.add_argument("--copy-mib", type=int, default=256)
+    ap.add_argument("--no-tune", action="store_true", help="planner tiles instead of the autotuner's")
     a = ap.parse_args()
     if a.workload == "topo80":
         prob, _ = W.topo80()
@@ -50,6 +54,8 @@ def main():
     al, E, _ = prob.rhs(np.array([0.0, 0.0, 1.0]), np.array([1.0, 0.0, 0.0]))
     ctx = T.Context(0)
     core = T.AProductCore(ctx, prob.R, prob.Nx, prob.Ny, prob.Nz, prob.d, prob.lam)
+    if not a.no_tune:
+        core.autotune()  # the tiles and x-pass kernel bench.py measures
     m = T.DeviceModel(core, 1)
     m.set(T.VEC_AL, al)
     m.set(T.VEC_P, E)

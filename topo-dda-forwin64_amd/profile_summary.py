@@ -2,7 +2,7 @@
 """Summarises rocprofv3 CSV output of the operator passes.
 
   python3 topo-dda-forwin64_amd/profile_summary.py stats  <kernel_stats.csv>  [--top 20]
-  python3 topo-dda-forwin64_amd/profile_summary.py last <kernel_trace.csv> --kernel 'k_px<double, 160>' [--last 200]
+  python3 topo-dda-forwin64_amd/profile_summary.py last <kernel_trace.csv> --kernel 'k_pxr<double, 160>' [--last 200]
   python3 topo-dda-forwin64_amd/profile_summary.py traffic <fetch counter_collection.csv>
           <write counter_collection.csv> --precision c128|c64 --workload topo80 [--json out.json]
 
@@ -24,8 +24,10 @@ import sys
 
 sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
 
-PASS_NAMES = {"k_p1<": "k_p1_zfwd", "k_py<{t}, false": "k_py_yfwd", "k_px<": "k_px_xgreen",
-              "k_py<{t}, true": "k_py_yinv", "k_p5<": "k_p5_zinv"}
+# pass kernels: name patterns (the x pass has a register-resident and an
+# LDS-staged kernel) and the pass names of bench.py
+PASS_NAMES = {("k_p1<",): "k_p1_zfwd", ("k_py<{t}, false",): "k_py_yfwd", ("k_pxr<", "k_px<"): "k_px_xgreen",
+              ("k_py<{t}, true",): "k_py_yinv", ("k_p5<",): "k_p5_zinv"}
 
 
 def stats(path, top):
@@ -55,12 +57,20 @@ def per_kernel(path, counter):
     return agg
 
 
-def traffic(fetch_csv, write_csv, precision, workload, copy_mib, out_json):
+def traffic(fetch_csv, write_csv, precision, workload, copy_mib, out_json, last_n=0):
+    """Per-dispatch HBM bytes of each pass; last_n > 0 averages only the last
+    last_n dispatches of each pass kernel (those of profile_passes.py's
+    timing, after the autotuner's trial launches)."""
     import workloads as W
     t = "double" if precision == "c128" else "float"
     s = 16 if precision == "c128" else 8
     fetch = per_kernel(fetch_csv, "FETCH_SIZE")
     write = per_kernel(write_csv, "WRITE_SIZE")
+    if last_n > 0:
+        for d in (fetch, write):
+            for kn in d:
+                if not kn.startswith("__amd"):
+                    d[kn] = d[kn][-last_n:]
     copy_bytes = copy_mib * 1024.0 * 1024.0
     # the copy may be split into several dispatches: take the largest ones
     # that together move copy_bytes of writes
@@ -87,9 +97,8 @@ def traffic(fetch_csv, write_csv, precision, workload, copy_mib, out_json):
     print("| pass | kernel | dispatches | fetch MB | write MB | measured MB | algorithmic MB | ratio |")
     print("|---|---|---|---|---|---|---|---|")
     res = {}
-    for k, (pat, pname) in enumerate(PASS_NAMES.items()):
-        pat = pat.format(t=t)
-        names = [kn for kn in fetch if f"tdda::{pat}" in kn and f"<{t}" in kn]
+    for k, (pats, pname) in enumerate(PASS_NAMES.items()):
+        names = [kn for pat in pats for kn in fetch if f"tdda::{pat.format(t=t)}" in kn and f"<{t}" in kn]
         if not names:
             continue
         kn = names[0]
@@ -124,15 +133,16 @@ def main():
     ap.add_argument("--workload", default="topo80")
     ap.add_argument("--copy-mib", type=int, default=256)
     ap.add_argument("--json", default=None)
-    ap.add_argument("--kernel", default="k_px<double, 160>")
-    ap.add_argument("--last", type=int, default=200)
+    ap.add_argument("--kernel", default="k_pxr<double, 160>")
+    ap.add_argument("--last", type=int, default=200,
+                    help="last: dispatches averaged; traffic: last dispatches per pass kernel (0: all)")
     a = ap.parse_args()
     if a.mode == "stats":
         stats(a.files[0], a.top)
     elif a.mode == "last":
         last(a.files[0], a.kernel, a.last)
     else:
-        traffic(a.files[0], a.files[1], a.precision, a.workload, a.copy_mib, a.json)
+        traffic(a.files[0], a.files[1], a.precision, a.workload, a.copy_mib, a.json, a.last)
 
 
 if __name__ == "__main__":
