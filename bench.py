@@ -109,6 +109,7 @@ def bench_mixed128(T, W, ctx, reps):
     out = {"workload": "mixed128: 128x128x32, d=8nm, lambda=600nm, 50% occupancy, eps=(2+0.01i)^2",
            "grid": list(core.grid), "dipoles": prob.N,
            "xpass": {"c128": core.xpass(T.C128), "c64": core.xpass(T.C64)},
+           "ypass": {"c128": core.ypass(T.C128), "c64": core.ypass(T.C64)},
            "reference_solve": {"iterations": ref["iterations"], "true_residual": ref["true_residual"]}}
     for mode, name in ((T.MODE_FP64, "fp64"), (T.MODE_MIXED, "mixed")):
         m.solve(mode=mode, tol=TOL, x0_zero=1, true_residual=1)
@@ -509,7 +510,8 @@ def main():
         "config": {"workload": "topo80: 80x80x16 slab, d=10nm, lambda=542nm, grid 160x160x32, "
                                "forward + adjoint (100 targets) per step", "grid": list(core.grid),
                    "dipoles": prob.N, "tiles_c128": list(core.tiles(T.C128)), "xpass_c128": core.xpass(T.C128),
-                   "xpass_c64": core.xpass(T.C64), "autotuned": TUNE},
+                   "xpass_c64": core.xpass(T.C64), "ypass_c128": core.ypass(T.C128),
+                   "ypass_c64": core.ypass(T.C64), "autotuned": TUNE},
         "latency_ms": {"p50": round(fp["p50_ms"], 3), "p99": round(fp["p99_ms"], 3)},
         "fp64": fp,
         "mixed": results["mixed"],

@@ -139,18 +139,19 @@ typedef struct {
 int tdda_op_create_ex(tdda_ctx ctx, const tdda_op_desc* desc, const tdda_op_ext* ext, tdda_op* out);
 /* Tile autotuner (no counterpart in the reference, which leaves tiling to
  * cuFFT): times the z passes with z tiles of 1, 1/2 and 1/4 of the planned
- * width and the x pass with each of its kernels (register-resident where
- * one exists for the length, LDS-staged) at x tiles narrowed 3 columns at a
- * time, each on a scratch model (reps launches per timing; <= 0: 20), and
+ * width, the y passes with each of their kernels (register-resident where
+ * one exists for the length, LDS-staged) and the x pass with each of its
+ * kernels at x tiles narrowed 3 columns at a time, each on a scratch model (reps launches per timing; <= 0: 20), and
  * keeps the fastest per precision.  Affects models created afterwards. */
 int tdda_op_autotune(tdda_op op, int reps);
 /* Current tiles of one precision: out4 = z pencils per tile (P5), z pencils
  * per tile (P1), y columns per tile, x columns per tile. */
 int tdda_op_tiles(tdda_op op, int precision, int* out4);
-/* x-pass kernel of one precision (no counterpart in the reference): 1 for
- * the register-resident kernel, 0 for the LDS-staged one, < 0 on error.
- * The environment variable TDDA_XPASS=0 makes the planner keep the
- * LDS-staged kernel for operators created afterwards. */
+/* Pass kernels of one precision (no counterpart in the reference): bit 0
+ * set when the x pass runs the register-resident kernel (else the
+ * LDS-staged one), bit 1 likewise for the y passes; < 0 on error.  The
+ * environment variables TDDA_XPASS=0 / TDDA_YPASS=0 make the planner keep
+ * the LDS-staged kernels for operators created afterwards. */
 int tdda_op_xpass(tdda_op op, int precision);
 /* AProductCore::~AProductCore (AProductCore.cpp:538-558). */
 int tdda_op_destroy(tdda_op op);

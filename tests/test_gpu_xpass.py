@@ -1,4 +1,4 @@
-"""GPU parity of the register-resident x pass (k_pxr: four-step split
+"""GPU parity of the register-resident x and y passes (k_pxr: four-step split
 L = L1 * L2 with both sub-transforms in registers, the Green product between
 them) against the numpy FFT convolution of the oracle and against the
 LDS-staged x pass, on the lengths where the split takes each of its forms:
@@ -96,3 +96,38 @@ def test_register_x_pass_solve(ctx, monkeypatch):
         m.close()
         core.close()
     assert O.rel_l2(P["1"], P["0"]) < 1e-7
+
+
+YCASES = [  # (Nx, Ny, Nz), grid, occupied fraction
+    ((4, 16, 3), (8, 32, 8), 1.0),
+    ((3, 23, 2), (8, 45, 8), 0.7),
+    ((3, 38, 2), (8, 75, 8), 1.0),
+    ((4, 40, 3), (8, 96, 8), 1.0),
+    ((5, 80, 3), (10, 160, 8), 0.6),
+    ((3, 90, 2), (8, 180, 8), 1.0),
+    ((3, 113, 2), (8, 225, 8), 1.0),
+    ((4, 128, 2), (8, 256, 8), 0.5),
+]
+
+
+@pytest.mark.parametrize("dims,grid,frac", YCASES)
+def test_register_y_pass(ctx, monkeypatch, dims, grid, frac):
+    # k_pyr (forward P2 and inverse P4, one LDS exchange each) against the
+    # numpy FFT convolution and against the LDS-staged y passes
+    Nx, Ny, Nz = dims
+    print(f"register y pass {dims} grid {grid} frac {frac}", flush=True)
+    R, b = problem(Nx, Ny, Nz, frac, seed=grid[1])
+    ref = O.aproduct_numpy_fft(R, Nx, Ny, Nz, 10.0, O.wavenumber(500.0), b, grid=grid)
+    monkeypatch.setenv("TDDA_YPASS", "1")
+    reg = T.AProductCore(ctx, R, Nx, Ny, Nz, 10.0, 500.0, grid=grid)
+    assert reg.ypass(T.C128) == "register" and reg.ypass(T.C64) == "register"
+    y64 = reg.Aproduct(b)
+    assert O.rel_l2(y64, ref) < TOL64
+    assert O.rel_l2(reg.Aproduct(b, precision=T.C64), ref) < TOL32
+    reg.close()
+    monkeypatch.setenv("TDDA_YPASS", "0")
+    lds = T.AProductCore(ctx, R, Nx, Ny, Nz, 10.0, 500.0, grid=grid)
+    assert lds.ypass(T.C128) == "lds"
+    z64 = lds.Aproduct(b)
+    lds.close()
+    assert O.rel_l2(y64, z64) < TOL64

@@ -180,6 +180,7 @@ void ctx_free(const tdda_ctx_s* c, void* p)
 struct TilePlan {
     int pz, pz1, tfy, tfx, ps_z, ps_y, ps_x;
     int xreg;  // P3 by the register-resident kernel k_pxr
+    int yreg;  // P2 / P4 by the register-resident kernel k_pyr
     size_t lds_z, lds_z1, lds_y, lds_x;
 };
 
@@ -187,6 +188,7 @@ struct TilePlan {
 struct TileCaps {
     int pz = 0, tfy = 0, tfx = 0;
     int xlds = 0;  // 1: P3 by the LDS-staged kernel even where k_pxr exists
+    int ylds = 0;  // 1: P2 / P4 by the LDS-staged kernel even where k_pyr exists
 };
 
 struct tdda_op_s {
@@ -307,6 +309,15 @@ size_t px_lds_slots(int xreg, int Mx, int tb, int tfx, int ps_x)
            (size_t)px_buffers<T>(Mx) * tfx * ps_x;
 }
 
+// LDS slots of a y-pass tile: twiddle table and data buffers
+template <typename T>
+size_t py_lds_slots(int yreg, int My, int tb, int tfy, int ps_y)
+{
+    if (yreg) return (size_t)pass_tw_slots(true, My, tb) + (size_t)tfy * ps_y;
+    return (size_t)pass_tw_slots(pass_length_compiled<T>(1, My) && pass_full_twiddles<T>(1, My), My, tb) +
+           (size_t)py_buffers<T>(My) * tfy * ps_y;
+}
+
 // Tile widths of the passes for element type T: a tile holds at most
 // `cap` elements (pencils x length, two LDS buffers of it), rows of the
 // strided passes are kept to whole 128-byte lines where the budget allows,
@@ -368,10 +379,18 @@ int plan_tiles(const tdda_op_s* op, TilePlan& t, const TileCaps& caps = TileCaps
         t.ps_x = op->Mx | 1;
     }
     if (!t.xreg && (t.tfx / 3) * op->Mx > 8 * kThreads) return fail(TDDA_ERR_UNSUPPORTED, "x pass tile too wide");
+    // register-resident y passes likewise (TDDA_YPASS=0 or the ylds cap keep
+    // the LDS-staged kernels): tiles of at most pyr_nf columns
+    t.yreg = pass_length_compiled<T>(3, op->My) && !caps.ylds && env_int("TDDA_YPASS", 1) != 0;
+    if (t.yreg) {
+        t.tfy = std::max(1, std::min(pyr_nf<T>(op->My), 3 * op->Mz));
+        if (caps.tfy > 0) t.tfy = std::min(t.tfy, caps.tfy);
+        t.ps_y = op->My | 1;
+    }
     auto tws = [&](int kind, int L, int tb) { return pass_tw_slots(pass_length_compiled<T>(kind, L) && pass_full_twiddles<T>(kind, L), L, tb); };
     t.lds_z = (size_t)(tws(0, op->Mz, op->pl[2].tb) + pz_buffers<T>(op->Mz) * 3 * t.pz * t.ps_z) * s;
     t.lds_z1 = (size_t)(tws(0, op->Mz, op->pl[2].tb) + pz_buffers<T>(op->Mz) * 3 * t.pz1 * t.ps_z) * s;
-    t.lds_y = (size_t)(tws(1, op->My, op->pl[1].tb) + py_buffers<T>(op->My) * t.tfy * t.ps_y) * s;
+    t.lds_y = py_lds_slots<T>(t.yreg, op->My, op->pl[1].tb, t.tfy, t.ps_y) * s;
     t.lds_x = px_lds_slots<T>(t.xreg, op->Mx, op->pl[0].tb, t.tfx, t.ps_x) * s;
     if (std::max({t.lds_z, t.lds_z1, t.lds_y, t.lds_x}) > op->ctx->lds_budget)
         return fail(TDDA_ERR_UNSUPPORTED, "pass tile exceeds the LDS budget");
@@ -447,6 +466,7 @@ void fill_jobs(const tdda_model_s* m, std::vector<JobDesc<T>>& out, const VecSet
         J.ps_y = tp.ps_y;
         J.ps_x = tp.ps_x;
         J.xreg = tp.xreg;
+        J.yreg = tp.yreg;
         J.cellmap = op->cellmap;
         J.green = green;
         J.twx = tw[0];
@@ -518,7 +538,8 @@ int build_groups(const tdda_ctx_s* c, JobTable<T>& jt, Put&& put)
         std::vector<int> key(nj);
         for (int j = 0; j < nj; j++) {
             const int L = len(jt.h[j]);
-            key[j] = (kind == 2 && jt.h[j].xreg) ? (L | kPxrKey) : pass_length_compiled<T>(kind, L) ? L : 0;
+            const bool reg = (kind == 2 && jt.h[j].xreg) || (kind == 1 && jt.h[j].yreg);
+            key[j] = reg ? (L | kPxrKey) : pass_length_compiled<T>(kind, L) ? L : 0;
         }
         std::vector<int> keys;
         for (int j = 0; j < nj; j++)
@@ -554,7 +575,7 @@ int build_groups(const tdda_ctx_s* c, JobTable<T>& jt, Put&& put)
     CHK(build(jt.gz1, 0, Lz, [](const JobTiles& t) { return t.z1; },
               [&](const JobDesc<T>& J) { return (size_t)(tws(0, J.Mz, J.plz.tb) + pz_buffers<T>(J.Mz) * 3 * J.pz1 * J.ps_z) * s; }));
     CHK(build(jt.gy, 1, [](const JobDesc<T>& J) { return J.My; }, [](const JobTiles& t) { return t.y; },
-              [&](const JobDesc<T>& J) { return (size_t)(tws(1, J.My, J.ply.tb) + py_buffers<T>(J.My) * J.tfy * J.ps_y) * s; }));
+              [&](const JobDesc<T>& J) { return py_lds_slots<T>(J.yreg, J.My, J.ply.tb, J.tfy, J.ps_y) * s; }));
     CHK(build(jt.gx, 2, [](const JobDesc<T>& J) { return J.Mx; }, [](const JobTiles& t) { return t.x; },
               [&](const JobDesc<T>& J) { return px_lds_slots<T>(J.xreg, J.Mx, J.plx.tb, J.tfx, J.ps_x) * s; }));
     return TDDA_OK;

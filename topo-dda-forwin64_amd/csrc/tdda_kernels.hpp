@@ -84,6 +84,7 @@ struct JobDesc {
     int nfx;     // tiles per ky row in P3: ceil(3 Mz / tfx)
     int ps_z, ps_y, ps_x;  // LDS pencil strides
     int xreg;              // P3 runs the register-resident kernel (k_pxr)
+    int yreg;              // P2 / P4 run the register-resident kernel (k_pyr)
     const int* cellmap;    // [Nx][Ny][Nz] -> site or -1; nullptr for the full box in bulk order
     const Cx<T>* green;
     const Cx<T>* twx;   // two-level twiddle tables (runtime-plan kernels)
@@ -780,6 +781,118 @@ __global__ void TDDA_PASS_BOUNDS k_py(const JobDesc<T>* __restrict__ jobs, const
         }
         if (tl >= ntot) break;
         cur = nxt;
+    }
+}
+
+// Register-resident y pass (P2 / P4) for lengths L = L1 * L2 with both
+// factors in the small-DFT set (pxr_l1, L1 >= L2); same four-step split as
+// the x pass (y = n1 + L1 n2, ky = k2 + L2 k1), with one LDS exchange:
+//   forward: thread (col, n1) loads w1 rows y = n1 + L1 n2 < Ny, DFT over n2,
+//            times w_L^(n1 k2), to LDS slot n1 L2 + k2; thread (col, k2)
+//            reads its L1 slots, DFT over n1, stores w2 rows ky = k2 + L2 k1;
+//   inverse: thread (col, k2) loads w2 rows k2 + L2 k1, inverse DFT over k1,
+//            times conj(w_L^(n1 k2)), to LDS; thread (col, n1) reads its L2
+//            slots, inverse DFT over k2, stores w1 rows y = n1 + L1 n2 < Ny.
+// A tile is (x, nf consecutive (kz, c) columns), nf * L1 <= kThreads.
+template <typename T>
+__host__ __device__ constexpr int pyr_nf(int L)
+{
+    return pxr_l1(L) <= 0 ? 0 : (kThreads / pxr_l1(L) >= 8 ? ((kThreads / pxr_l1(L)) / 8) * 8 : kThreads / pxr_l1(L));
+}
+
+// complex128 at 256 points needs the registers of three waves (with four
+// it spills); all others fit four.
+template <typename T>
+__host__ __device__ constexpr int pyr_waves(int L)
+{
+    return (sizeof(T) == 8 && L >= 256) ? 3 : 4;
+}
+
+template <typename T, bool INV, int LC>
+__global__ void __launch_bounds__(kThreads, pyr_waves<T>(LC)) k_pyr(const JobDesc<T>* __restrict__ jobs,
+                                                     const int* __restrict__ tstart, int njobs, int solver)
+{
+    constexpr int L1 = pxr_l1(LC), L2 = LC / L1, H2 = (L2 + 1) / 2;
+    static_assert(L1 > 0 && L1 >= L2, "length without a register split");
+    extern __shared__ __align__(16) char smem[];
+    Cx<T>* tw = reinterpret_cast<Cx<T>*>(smem);  // tw[q] = exp(-2 pi i q / L)
+    Cx<T>* A = tw + pass_tw_slots(true, LC, 0);
+    const int ntot = tstart[njobs];
+    for (int i = threadIdx.x; i < LC; i += kThreads) tw[i] = gld(jobs[0].twfy + i);
+    __syncthreads();
+    const int task = threadIdx.x;
+    for (int tl = blockIdx.x; tl < ntot; tl += gridDim.x) {
+        const STile t = s_tile(jobs, tstart, njobs, tl, false, solver != 0);
+        if (!t.live) continue;
+        const JobDesc<T>* J = jobs + t.job;
+        const int Ny = J->Ny, F = 3 * J->Mz, nf = t.nf, ps = J->ps_y;
+        Cx<T>* w1 = J->w1 + (size_t)t.outer * Ny * F + t.f0;
+        Cx<T>* w2 = J->w2 + (size_t)t.outer * LC * F + t.f0;
+        const int q = task / nf, col = task - q * nf;  // (n1 or k2, col)
+        if constexpr (!INV) {
+            if (q < L1) {
+                const int n1 = q;
+                Cx<T> v[L2];
+                sfor<0, L2>([&](auto c2) {
+                    constexpr int n2 = decltype(c2)::value;
+                    if constexpr (n2 >= H2) {
+                        v[n2] = Cx<T>{T(0), T(0)};
+                    } else {
+                        const int y = n1 + L1 * n2;
+                        v[n2] = y < Ny ? gld(w1 + (y * F + col)) : Cx<T>{T(0), T(0)};
+                    }
+                });
+                sdft<T, L2, false>(v);
+                Cx<T>* row = A + col * ps + n1 * L2;
+                row[0] = v[0];
+                sfor<1, L2>([&](auto c2) {
+                    constexpr int k2 = decltype(c2)::value;
+                    row[k2] = cmul(v[k2], tw[n1 * k2]);
+                });
+            }
+            __syncthreads();
+            if (q < L2) {
+                const int k2 = q;
+                Cx<T> v[L1];
+                const Cx<T>* cp = A + col * ps + k2;
+                sfor<0, L1>([&](auto c1) { v[decltype(c1)::value] = cp[decltype(c1)::value * L2]; });
+                sdft<T, L1, false>(v);
+                sfor<0, L1>([&](auto c1) {
+                    constexpr int k1 = decltype(c1)::value;
+                    gst(w2 + ((k2 + L2 * k1) * F + col), v[k1]);
+                });
+            }
+        } else {
+            if (q < L2) {
+                const int k2 = q;
+                Cx<T> v[L1];
+                sfor<0, L1>([&](auto c1) {
+                    constexpr int k1 = decltype(c1)::value;
+                    v[k1] = gld(w2 + ((k2 + L2 * k1) * F + col));
+                });
+                sdft<T, L1, true>(v);
+                Cx<T>* cp = A + col * ps + k2;
+                cp[0] = v[0];
+                sfor<1, L1>([&](auto c1) {
+                    constexpr int n1 = decltype(c1)::value;
+                    cp[n1 * L2] = cmulc(v[n1], tw[n1 * k2]);
+                });
+            }
+            __syncthreads();
+            if (q < L1) {
+                const int n1 = q;
+                Cx<T> v[L2];
+                const Cx<T>* row = A + col * ps + n1 * L2;
+                sfor<0, L2>([&](auto c2) { v[decltype(c2)::value] = row[decltype(c2)::value]; });
+                sdft<T, L2, true>(v);
+                sfor<0, H2>([&](auto c2) {
+                    constexpr int n2 = decltype(c2)::value;
+                    const int y = n1 + L1 * n2;
+                    if (y < Ny) gst(w1 + (y * F + col), v[n2]);
+                });
+            }
+        }
+        __syncthreads();
     }
 }
 

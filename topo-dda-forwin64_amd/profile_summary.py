@@ -24,10 +24,12 @@ import sys
 
 sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
 
-# pass kernels: name patterns (the x pass has a register-resident and an
-# LDS-staged kernel) and the pass names of bench.py
-PASS_NAMES = {("k_p1<",): "k_p1_zfwd", ("k_py<{t}, false",): "k_py_yfwd", ("k_pxr<", "k_px<"): "k_px_xgreen",
-              ("k_py<{t}, true",): "k_py_yinv", ("k_p5<",): "k_p5_zinv"}
+# pass kernels: name patterns (the x and y passes have register-resident and
+# LDS-staged kernels; the first pattern with dispatches wins) and the pass
+# names of bench.py
+PASS_NAMES = {("k_p1<",): "k_p1_zfwd", ("k_pyr<{t}, false", "k_py<{t}, false"): "k_py_yfwd",
+              ("k_pxr<", "k_px<"): "k_px_xgreen", ("k_pyr<{t}, true", "k_py<{t}, true"): "k_py_yinv",
+              ("k_p5<",): "k_p5_zinv"}
 
 
 def stats(path, top):

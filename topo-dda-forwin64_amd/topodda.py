@@ -332,10 +332,16 @@ class AProductCore:
         return tuple(out)
 
     def xpass(self, precision=C128):
-        """x-pass kernel (tdda_op_xpass): "register" or "lds"."""
+        """x-pass kernel (tdda_op_xpass bit 0): "register" or "lds"."""
         rc = lib().tdda_op_xpass(self.h, precision)
         _check(min(rc, 0))
-        return "register" if rc == 1 else "lds"
+        return "register" if rc & 1 else "lds"
+
+    def ypass(self, precision=C128):
+        """y-pass kernels (tdda_op_xpass bit 1): "register" or "lds"."""
+        rc = lib().tdda_op_xpass(self.h, precision)
+        _check(min(rc, 0))
+        return "register" if rc & 2 else "lds"
 
     def Aproduct(self, b, al=None, precision=C128):
         """y = A_offdiag b (+ al o b); b is (3N,) or (nrhs, 3N) complex128."""
