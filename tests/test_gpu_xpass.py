@@ -54,14 +54,20 @@ def test_register_x_pass(ctx, monkeypatch, dims, grid, frac):
     R, b = problem(Nx, Ny, Nz, frac, seed=grid[0])
     ref = O.aproduct_numpy_fft(R, Nx, Ny, Nz, 10.0, O.wavenumber(500.0), b, grid=grid)
     monkeypatch.setenv("TDDA_XPASS", "1")
-    reg = T.AProductCore(ctx, R, Nx, Ny, Nz, 10.0, 500.0, grid=grid)
-    assert reg.grid == grid
-    assert reg.xpass(T.C128) == "register" and reg.xpass(T.C64) == "register"
-    y64 = reg.Aproduct(b)
-    y32 = reg.Aproduct(b, precision=T.C64)
-    assert O.rel_l2(y64, ref) < TOL64
-    assert O.rel_l2(y32, ref) < TOL32
-    reg.close()
+    ys = []
+    for nt in ("512", "256"):  # workgroups of 512 threads (wide tiles) and of 256
+        monkeypatch.setenv("TDDA_XNT", nt)
+        reg = T.AProductCore(ctx, R, Nx, Ny, Nz, 10.0, 500.0, grid=grid)
+        assert reg.grid == grid
+        assert reg.xpass(T.C128) == "register" and reg.xpass(T.C64) == "register"
+        y64 = reg.Aproduct(b)
+        y32 = reg.Aproduct(b, precision=T.C64)
+        assert O.rel_l2(y64, ref) < TOL64, nt
+        assert O.rel_l2(y32, ref) < TOL32, nt
+        reg.close()
+        ys.append(y64)
+    # a tile's transforms do not depend on the tile width: the same bits
+    assert np.array_equal(ys[0], ys[1])
     monkeypatch.setenv("TDDA_XPASS", "0")
     lds = T.AProductCore(ctx, R, Nx, Ny, Nz, 10.0, 500.0, grid=grid)
     assert lds.xpass(T.C128) == "lds" and lds.xpass(T.C64) == "lds"

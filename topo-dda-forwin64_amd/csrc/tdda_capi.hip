@@ -180,6 +180,7 @@ void ctx_free(const tdda_ctx_s* c, void* p)
 struct TilePlan {
     int pz, pz1, tfy, tfx, ps_z, ps_y, ps_x;
     int xreg;  // P3 by the register-resident kernel k_pxr
+    int xwide; // k_pxr with 512 threads per workgroup (tiles up to twice as wide)
     int yreg;  // P2 / P4 by the register-resident kernel k_pyr
     size_t lds_z, lds_z1, lds_y, lds_x;
 };
@@ -188,6 +189,7 @@ struct TilePlan {
 struct TileCaps {
     int pz = 0, tfy = 0, tfx = 0;
     int xlds = 0;  // 1: P3 by the LDS-staged kernel even where k_pxr exists
+    int xnt = 0;   // threads per workgroup of k_pxr (256 or 512; 0: the planner's choice)
     int ylds = 0;  // 1: P2 / P4 by the LDS-staged kernel even where k_pyr exists
 };
 
@@ -373,8 +375,16 @@ int plan_tiles(const tdda_op_s* op, TilePlan& t, const TileCaps& caps = TileCaps
     // autotuner's xlds cap keep the LDS-staged kernel): tiles of at most
     // pxr_nf columns, odd pencil stride, full twiddle table
     t.xreg = pass_length_compiled<T>(3, op->Mx) && !caps.xlds && env_int("TDDA_XPASS", 1) != 0;
+    t.xwide = 0;
     if (t.xreg) {
-        t.tfx = std::max(3, std::min(pxr_nf<T>(op->Mx), 3 * op->Mz));
+        // 256 threads unless the autotuner measured the 512-thread kernel
+        // (tiles up to twice as wide) faster for this operator, or
+        // TDDA_XNT=512 is set; untuned operators keep one kernel per length,
+        // so a batch of them needs one x-pass launch per length
+        const int env_nt = env_int("TDDA_XNT", 0);
+        const int nt = caps.xnt ? caps.xnt : env_nt == 512 ? 512 : kThreads;
+        t.xwide = nt == 512;
+        t.tfx = std::max(3, std::min(pxr_nf<T>(op->Mx, nt), 3 * op->Mz));
         if (caps.tfx > 0) t.tfx = std::max(3, std::min(t.tfx, (caps.tfx / 3) * 3));
         t.ps_x = op->Mx | 1;
     }
@@ -392,6 +402,12 @@ int plan_tiles(const tdda_op_s* op, TilePlan& t, const TileCaps& caps = TileCaps
     t.lds_z1 = (size_t)(tws(0, op->Mz, op->pl[2].tb) + pz_buffers<T>(op->Mz) * 3 * t.pz1 * t.ps_z) * s;
     t.lds_y = py_lds_slots<T>(t.yreg, op->My, op->pl[1].tb, t.tfy, t.ps_y) * s;
     t.lds_x = px_lds_slots<T>(t.xreg, op->Mx, op->pl[0].tb, t.tfx, t.ps_x) * s;
+    if (t.xwide && t.lds_x > op->ctx->lds_budget) {  // a wide tile too large: 256 threads
+        t.xwide = 0;
+        t.tfx = std::max(3, std::min(pxr_nf<T>(op->Mx), 3 * op->Mz));
+        if (caps.tfx > 0) t.tfx = std::max(3, std::min(t.tfx, (caps.tfx / 3) * 3));
+        t.lds_x = px_lds_slots<T>(t.xreg, op->Mx, op->pl[0].tb, t.tfx, t.ps_x) * s;
+    }
     if (std::max({t.lds_z, t.lds_z1, t.lds_y, t.lds_x}) > op->ctx->lds_budget)
         return fail(TDDA_ERR_UNSUPPORTED, "pass tile exceeds the LDS budget");
     return TDDA_OK;
@@ -466,6 +482,7 @@ void fill_jobs(const tdda_model_s* m, std::vector<JobDesc<T>>& out, const VecSet
         J.ps_y = tp.ps_y;
         J.ps_x = tp.ps_x;
         J.xreg = tp.xreg;
+        J.xwide = tp.xwide;
         J.yreg = tp.yreg;
         J.cellmap = op->cellmap;
         J.green = green;
@@ -539,7 +556,8 @@ int build_groups(const tdda_ctx_s* c, JobTable<T>& jt, Put&& put)
         for (int j = 0; j < nj; j++) {
             const int L = len(jt.h[j]);
             const bool reg = (kind == 2 && jt.h[j].xreg) || (kind == 1 && jt.h[j].yreg);
-            key[j] = reg ? (L | kPxrKey) : pass_length_compiled<T>(kind, L) ? L : 0;
+            key[j] = reg ? (L | kPxrKey | ((kind == 2 && jt.h[j].xwide) ? kPxrWide : 0))
+                         : pass_length_compiled<T>(kind, L) ? L : 0;
         }
         std::vector<int> keys;
         for (int j = 0; j < nj; j++)

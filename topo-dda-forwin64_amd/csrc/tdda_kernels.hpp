@@ -85,6 +85,7 @@ struct JobDesc {
     int ps_z, ps_y, ps_x;  // LDS pencil strides
     int xreg;              // P3 runs the register-resident kernel (k_pxr)
     int yreg;              // P2 / P4 run the register-resident kernel (k_pyr)
+    int xwide;             // k_pxr with 512 threads per workgroup
     const int* cellmap;    // [Nx][Ny][Nz] -> site or -1; nullptr for the full box in bulk order
     const Cx<T>* green;
     const Cx<T>* twx;   // two-level twiddle tables (runtime-plan kernels)
@@ -1005,7 +1006,9 @@ __global__ void __launch_bounds__(kThreads, px_waves<T>(LC)) k_px(const JobDesc<
 
 // Register-resident x pass (P3) for lengths L = L1 * L2 with both factors
 // in the small-DFT set (pxr_l1, L1 >= L2).  A tile is (ky, nf consecutive
-// (kz, c) columns), nf a multiple of 3 with nf * L1 <= kThreads.  With
+// (kz, c) columns), nf a multiple of 3 with nf * L1 <= NT (threads of the
+// workgroup: 256, or 512 for tiles twice as wide -- longer HBM rows and
+// Green row segments at the same waves per SIMD).  With
 // x = n1 + L1 n2 and kx = k2 + L2 k1 (four-step split):
 //   A : thread (col, n1) loads x[n1 + L1 n2] from w2 (n2 < ceil(L2/2): the
 //       rest is zero padding), DFT over n2 in registers, times w_L^(n1 k2),
@@ -1023,9 +1026,9 @@ __global__ void __launch_bounds__(kThreads, px_waves<T>(LC)) k_px(const JobDesc<
 // every radix stage).  No prefetch: the resident workgroups of a CU overlap
 // each other's loads.
 template <typename T>
-__host__ __device__ constexpr int pxr_nf(int L)
+__host__ __device__ constexpr int pxr_nf(int L, int nt = kThreads)
 {
-    return pxr_l1(L) > 0 ? ((kThreads / pxr_l1(L)) / 3) * 3 : 0;
+    return pxr_l1(L) > 0 ? ((nt / pxr_l1(L)) / 3) * 3 : 0;
 }
 
 // Waves per SIMD k_pxr is compiled for, the most at which the compiler
@@ -1037,8 +1040,8 @@ __host__ __device__ constexpr int pxr_waves(int L)
     return (sizeof(T) == 8 && L >= 256) ? 3 : 4;
 }
 
-template <typename T, int LC>
-__global__ void __launch_bounds__(kThreads, pxr_waves<T>(LC)) k_pxr(const JobDesc<T>* __restrict__ jobs,
+template <typename T, int LC, int NT>
+__global__ void __launch_bounds__(NT, pxr_waves<T>(LC)) k_pxr(const JobDesc<T>* __restrict__ jobs,
                                                      const int* __restrict__ tstart, int njobs, int solver)
 {
     constexpr int L1 = pxr_l1(LC), L2 = LC / L1, H2 = (L2 + 1) / 2;
@@ -1048,7 +1051,7 @@ __global__ void __launch_bounds__(kThreads, pxr_waves<T>(LC)) k_pxr(const JobDes
     Cx<T>* A = tw + pass_tw_slots(true, LC, 0);
     const int ntot = tstart[njobs];
     // the full table of the L-th roots depends on L only: one per launch
-    for (int i = threadIdx.x; i < LC; i += kThreads) tw[i] = gld(jobs[0].twfx + i);
+    for (int i = threadIdx.x; i < LC; i += NT) tw[i] = gld(jobs[0].twfx + i);
     __syncthreads();
     const int task = threadIdx.x;
     for (int tl = blockIdx.x; tl < ntot; tl += gridDim.x) {
@@ -1102,7 +1105,7 @@ __global__ void __launch_bounds__(kThreads, pxr_waves<T>(LC)) k_pxr(const JobDes
             const T sy = (ky > My / 2) ? T(-1) : T(1);
             const Cx<T>* grow = J->green + (size_t)kyf * J->Hz * 6;
             const size_t gsx = (size_t)J->Hy * J->Hz * 6;
-            for (int e = task; e < nw; e += kThreads) {
+            for (int e = task; e < nw; e += NT) {
                 const int kxf = (int)((unsigned)e / (unsigned)nkz), kzl = e - kxf * nkz;
                 const int kz = t.f0 / 3 + kzl;
                 const int kzf = (kz > Mz / 2) ? Mz - kz : kz;

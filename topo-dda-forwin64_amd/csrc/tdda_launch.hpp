@@ -15,6 +15,7 @@ struct PassLaunch {
     size_t lds;
     hipStream_t stream;
     int ncu;
+    int threads = kThreads;  // workgroup size of the pass kernel
 };
 
 template <typename T>
@@ -25,9 +26,11 @@ template <typename T>
 void launch_py(int L, bool inv, int solver, const PassLaunch& pl, const JobDesc<T>* jobs, const int* tstart, int nj);
 template <typename T>
 void launch_px(int L, int solver, const PassLaunch& pl, const JobDesc<T>* jobs, const int* tstart, int nj);
-// Launch-group key bit of the register-resident x pass: launch_px runs
-// k_pxr<L> for L | kPxrKey.
+// Launch-group key bits of the register-resident x and y passes: launch_px
+// runs k_pxr<L> for L | kPxrKey (with 512 threads per workgroup when
+// kPxrWide is also set), launch_py k_pyr<L>.
 constexpr int kPxrKey = 1 << 20;
+constexpr int kPxrWide = 1 << 21;
 // true when a specialised kernel exists for this length and pass kind
 // (0: z pass, 1: y pass, 2: x pass, 3: register-resident x pass)
 template <typename T>
