@@ -137,3 +137,44 @@ def test_register_y_pass(ctx, monkeypatch, dims, grid, frac):
     z64 = lds.Aproduct(b)
     lds.close()
     assert O.rel_l2(y64, z64) < TOL64
+
+
+LONG = [  # complex64 register kernels with factors above 16: (dims, grid, axis)
+    ((63, 3, 2), (125, 8, 8), "x"),
+    ((81, 4, 2), (162, 8, 8), "x"),
+    ((135, 3, 2), (270, 8, 8), "x"),
+    ((256, 3, 2), (512, 8, 8), "x"),
+    ((313, 2, 2), (625, 8, 8), "x"),
+    ((365, 2, 2), (729, 8, 8), "x"),
+    ((480, 2, 2), (960, 8, 8), "x"),
+    ((512, 2, 2), (1024, 8, 8), "x"),
+    ((3, 135, 2), (8, 270, 8), "y"),
+    ((2, 300, 2), (8, 600, 8), "y"),
+    ((2, 512, 2), (8, 1024, 8), "y"),
+]
+
+
+@pytest.mark.parametrize("dims,grid,axis", LONG)
+def test_register_passes_long_complex64(ctx, monkeypatch, dims, grid, axis):
+    # lengths 125-1024: the complex64 x and y passes run k_pxr / k_pyr with
+    # 512-thread workgroups and factors up to 32; complex128 keeps the
+    # LDS-staged kernels there
+    Nx, Ny, Nz = dims
+    print(f"long register {axis} pass {dims} grid {grid}", flush=True)
+    R, b = problem(Nx, Ny, Nz, 1.0, seed=grid[0] + grid[1])
+    ref = O.aproduct_numpy_fft(R, Nx, Ny, Nz, 10.0, O.wavenumber(500.0), b, grid=grid)
+    monkeypatch.setenv("TDDA_XPASS", "1")
+    monkeypatch.setenv("TDDA_YPASS", "1")
+    reg = T.AProductCore(ctx, R, Nx, Ny, Nz, 10.0, 500.0, grid=grid)
+    kind = reg.xpass if axis == "x" else reg.ypass
+    assert kind(T.C64) == "register" and kind(T.C128) == "lds"
+    y32 = reg.Aproduct(b, precision=T.C64)
+    assert O.rel_l2(y32, ref) < TOL32
+    assert O.rel_l2(reg.Aproduct(b), ref) < TOL64
+    reg.close()
+    monkeypatch.setenv("TDDA_XPASS", "0")
+    monkeypatch.setenv("TDDA_YPASS", "0")
+    lds = T.AProductCore(ctx, R, Nx, Ny, Nz, 10.0, 500.0, grid=grid)
+    z32 = lds.Aproduct(b, precision=T.C64)
+    lds.close()
+    assert O.rel_l2(y32, z32) < 2 * TOL32

@@ -182,6 +182,7 @@ struct TilePlan {
     int xreg;  // P3 by the register-resident kernel k_pxr
     int xwide; // k_pxr with 512 threads per workgroup (tiles up to twice as wide)
     int yreg;  // P2 / P4 by the register-resident kernel k_pyr
+    int ywide; // k_pyr with 512 threads per workgroup (factors above 16)
     size_t lds_z, lds_z1, lds_y, lds_x;
 };
 
@@ -382,7 +383,9 @@ int plan_tiles(const tdda_op_s* op, TilePlan& t, const TileCaps& caps = TileCaps
         // TDDA_XNT=512 is set; untuned operators keep one kernel per length,
         // so a batch of them needs one x-pass launch per length
         const int env_nt = env_int("TDDA_XNT", 0);
-        const int nt = caps.xnt ? caps.xnt : env_nt == 512 ? 512 : kThreads;
+        // (lengths with a factor above 16 always take 512: 256 threads
+        // would give tiles of 6 columns)
+        const int nt = caps.xnt ? caps.xnt : (env_nt == 512 || pxr_l1(op->Mx) > 16) ? 512 : kThreads;
         t.xwide = nt == 512;
         t.tfx = std::max(3, std::min(pxr_nf<T>(op->Mx, nt), 3 * op->Mz));
         if (caps.tfx > 0) t.tfx = std::max(3, std::min(t.tfx, (caps.tfx / 3) * 3));
@@ -392,8 +395,10 @@ int plan_tiles(const tdda_op_s* op, TilePlan& t, const TileCaps& caps = TileCaps
     // register-resident y passes likewise (TDDA_YPASS=0 or the ylds cap keep
     // the LDS-staged kernels): tiles of at most pyr_nf columns
     t.yreg = pass_length_compiled<T>(3, op->My) && !caps.ylds && env_int("TDDA_YPASS", 1) != 0;
+    t.ywide = 0;
     if (t.yreg) {
-        t.tfy = std::max(1, std::min(pyr_nf<T>(op->My), 3 * op->Mz));
+        t.ywide = pxr_l1(op->My) > 16;  // 512 threads: rows of 16 columns instead of 8
+        t.tfy = std::max(1, std::min(pyr_nf<T>(op->My, t.ywide ? 512 : kThreads), 3 * op->Mz));
         if (caps.tfy > 0) t.tfy = std::min(t.tfy, caps.tfy);
         t.ps_y = op->My | 1;
     }
@@ -484,6 +489,7 @@ void fill_jobs(const tdda_model_s* m, std::vector<JobDesc<T>>& out, const VecSet
         J.xreg = tp.xreg;
         J.xwide = tp.xwide;
         J.yreg = tp.yreg;
+        J.ywide = tp.ywide;
         J.cellmap = op->cellmap;
         J.green = green;
         J.twx = tw[0];
@@ -556,7 +562,8 @@ int build_groups(const tdda_ctx_s* c, JobTable<T>& jt, Put&& put)
         for (int j = 0; j < nj; j++) {
             const int L = len(jt.h[j]);
             const bool reg = (kind == 2 && jt.h[j].xreg) || (kind == 1 && jt.h[j].yreg);
-            key[j] = reg ? (L | kPxrKey | ((kind == 2 && jt.h[j].xwide) ? kPxrWide : 0))
+            const bool wide = (kind == 2 && jt.h[j].xwide) || (kind == 1 && jt.h[j].ywide);
+            key[j] = reg ? (L | kPxrKey | (wide ? kPxrWide : 0))
                          : pass_length_compiled<T>(kind, L) ? L : 0;
         }
         std::vector<int> keys;

@@ -39,6 +39,12 @@ constexpr int kThreads = kThreadsFft;
 #define TDDA_PASS_WAVES 4
 #endif
 #define TDDA_PASS_BOUNDS __launch_bounds__(kThreads, TDDA_PASS_WAVES)
+// Register-resident kernels with a factor above 16 (complex64, lengths up to
+// 1024): waves per SIMD they are compiled for.
+#ifndef TDDA_LONG_WAVES
+#define TDDA_LONG_WAVES 2
+#endif
+
 #ifndef TDDA_EPT_C64
 #define TDDA_EPT_C64 12
 #endif
@@ -86,6 +92,7 @@ struct JobDesc {
     int xreg;              // P3 runs the register-resident kernel (k_pxr)
     int yreg;              // P2 / P4 run the register-resident kernel (k_pyr)
     int xwide;             // k_pxr with 512 threads per workgroup
+    int ywide;             // k_pyr with 512 threads per workgroup
     const int* cellmap;    // [Nx][Ny][Nz] -> site or -1; nullptr for the full box in bulk order
     const Cx<T>* green;
     const Cx<T>* twx;   // two-level twiddle tables (runtime-plan kernels)
@@ -794,11 +801,13 @@ __global__ void TDDA_PASS_BOUNDS k_py(const JobDesc<T>* __restrict__ jobs, const
 //   inverse: thread (col, k2) loads w2 rows k2 + L2 k1, inverse DFT over k1,
 //            times conj(w_L^(n1 k2)), to LDS; thread (col, n1) reads its L2
 //            slots, inverse DFT over k2, stores w1 rows y = n1 + L1 n2 < Ny.
-// A tile is (x, nf consecutive (kz, c) columns), nf * L1 <= kThreads.
+// A tile is (x, nf consecutive (kz, c) columns), nf * L1 <= NT (threads of
+// the workgroup: 256, or 512 for factors above 16, whose 256-thread tiles
+// would read rows of 8 columns).
 template <typename T>
-__host__ __device__ constexpr int pyr_nf(int L)
+__host__ __device__ constexpr int pyr_nf(int L, int nt = kThreads)
 {
-    return pxr_l1(L) <= 0 ? 0 : (kThreads / pxr_l1(L) >= 8 ? ((kThreads / pxr_l1(L)) / 8) * 8 : kThreads / pxr_l1(L));
+    return pxr_l1(L) <= 0 ? 0 : (nt / pxr_l1(L) >= 8 ? ((nt / pxr_l1(L)) / 8) * 8 : nt / pxr_l1(L));
 }
 
 // complex128 at 256 points needs the registers of three waves (with four
@@ -806,11 +815,11 @@ __host__ __device__ constexpr int pyr_nf(int L)
 template <typename T>
 __host__ __device__ constexpr int pyr_waves(int L)
 {
-    return (sizeof(T) == 8 && L >= 256) ? 3 : 4;
+    return (sizeof(T) == 8 && L >= 256) ? 3 : pxr_l1(L) > 16 ? TDDA_LONG_WAVES : 4;
 }
 
-template <typename T, bool INV, int LC>
-__global__ void __launch_bounds__(kThreads, pyr_waves<T>(LC)) k_pyr(const JobDesc<T>* __restrict__ jobs,
+template <typename T, bool INV, int LC, int NT = kThreads>
+__global__ void __launch_bounds__(NT, pyr_waves<T>(LC)) k_pyr(const JobDesc<T>* __restrict__ jobs,
                                                      const int* __restrict__ tstart, int njobs, int solver)
 {
     constexpr int L1 = pxr_l1(LC), L2 = LC / L1, H2 = (L2 + 1) / 2;
@@ -819,7 +828,7 @@ __global__ void __launch_bounds__(kThreads, pyr_waves<T>(LC)) k_pyr(const JobDes
     Cx<T>* tw = reinterpret_cast<Cx<T>*>(smem);  // tw[q] = exp(-2 pi i q / L)
     Cx<T>* A = tw + pass_tw_slots(true, LC, 0);
     const int ntot = tstart[njobs];
-    for (int i = threadIdx.x; i < LC; i += kThreads) tw[i] = gld(jobs[0].twfy + i);
+    for (int i = threadIdx.x; i < LC; i += NT) tw[i] = gld(jobs[0].twfy + i);
     __syncthreads();
     const int task = threadIdx.x;
     for (int tl = blockIdx.x; tl < ntot; tl += gridDim.x) {
@@ -1037,7 +1046,7 @@ __host__ __device__ constexpr int pxr_nf(int L, int nt = kThreads)
 template <typename T>
 __host__ __device__ constexpr int pxr_waves(int L)
 {
-    return (sizeof(T) == 8 && L >= 256) ? 3 : 4;
+    return (sizeof(T) == 8 && L >= 256) ? 3 : pxr_l1(L) > 16 ? TDDA_LONG_WAVES : 4;
 }
 
 template <typename T, int LC, int NT>
