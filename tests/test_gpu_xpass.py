@@ -148,6 +148,9 @@ LONG = [  # complex64 register kernels with factors above 16: (dims, grid, axis)
     ((365, 2, 2), (729, 8, 8), "x"),
     ((480, 2, 2), (960, 8, 8), "x"),
     ((512, 2, 2), (1024, 8, 8), "x"),
+    ((486, 2, 2), (972, 8, 8), "x"),
+    ((500, 2, 2), (1000, 8, 8), "x"),
+    ((2, 500, 2), (8, 1000, 8), "y"),
     ((3, 135, 2), (8, 270, 8), "y"),
     ((2, 300, 2), (8, 600, 8), "y"),
     ((2, 512, 2), (8, 1024, 8), "y"),
@@ -157,7 +160,7 @@ LONG = [  # complex64 register kernels with factors above 16: (dims, grid, axis)
 @pytest.mark.parametrize("dims,grid,axis", LONG)
 def test_register_passes_long_complex64(ctx, monkeypatch, dims, grid, axis):
     # lengths 125-1024: the complex64 x and y passes run k_pxr / k_pyr with
-    # 512-thread workgroups and factors up to 32; complex128 keeps the
+    # 512-thread workgroups and factors up to 40; complex128 keeps the
     # LDS-staged kernels there
     Nx, Ny, Nz = dims
     print(f"long register {axis} pass {dims} grid {grid}", flush=True)

@@ -158,7 +158,7 @@ __device__ __forceinline__ void dft(Cx<T>* v)
 
 // ---------------------------------------------------------------------------
 // Small DFTs of any length in {2, 3, 4, 5, 6, 8, 9, 10, 12, 15, 16, 18, 20,
-// 24, 25, 27, 30, 32}, held entirely in registers, for the register-resident
+// 24, 25, 27, 30, 32, 36, 40}, held entirely in registers, for the register-resident
 // x and y passes.  Lengths other
 // than 2, 3, 4, 5, 8 are split as R = A * B (decimation in time:
 // n = B a + b, k = k1 + A k2):
@@ -209,7 +209,8 @@ constexpr CosSin ct_root(int m, int R)
 constexpr bool sdft_ok(int R)
 {
     return R == 2 || R == 3 || R == 4 || R == 5 || R == 6 || R == 8 || R == 9 || R == 10 || R == 12 || R == 15 ||
-           R == 16 || R == 18 || R == 20 || R == 24 || R == 25 || R == 27 || R == 30 || R == 32;
+           R == 16 || R == 18 || R == 20 || R == 24 || R == 25 || R == 27 || R == 30 || R == 32 || R == 36 ||
+           R == 40;
 }
 // first factor A of the split R = A * B
 constexpr int sdft_split(int R) { return (R % 4 == 0) ? 4 : (R % 2 == 0) ? 2 : (R % 3 == 0) ? 3 : 5; }
