@@ -17,6 +17,7 @@ Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
 from __future__ import annotations
 
 import argparse
+import collections
 import json
 import math
 import os
@@ -353,6 +354,7 @@ def bench_serve(T, seconds, rate=200.0, seed=5):
     return {"workload": f"serve: Poisson {rate:g}/s for {seconds:g} s, dipoles log-U[8k, 2M], Nz in {{8,16,32}}, "
                         "1-4 RHS, half adjoint, mixed precision to 1e-8",
             "requests": len(reqs), "completed": len(ok), "failed": len(done) - len(ok),
+            "errors": dict(collections.Counter(err for _, tm, err in done if not tm).most_common(5)),
             "rhs_solved": nrhs, "solves_per_s": nrhs / (t_end - t0),
             "requests_per_s": len(ok) / (t_end - t0), "wall_s": t_end - t0,
             "latency_ms": {"p50": float(np.percentile(lat, 50) * 1e3), "p99": float(np.percentile(lat, 99) * 1e3)},

@@ -160,7 +160,18 @@ namespace {
 int ctx_malloc(const tdda_ctx_s* c, void** p, size_t bytes)
 {
     if (c->async_alloc) {
-        if (hipMallocAsync(p, bytes, c->stream) != hipSuccess) return TDDA_ERR_OOM;
+        if (hipMallocAsync(p, bytes, c->stream) != hipSuccess) {
+            // the pool keeps freed blocks; when they are too fragmented for
+            // this request, return the unused ones to the device and retry
+            (void)hipGetLastError();
+            hipMemPool_t pool;
+            if (hipStreamSynchronize(c->stream) != hipSuccess ||
+                hipDeviceGetDefaultMemPool(&pool, c->device) != hipSuccess ||
+                hipMemPoolTrimTo(pool, 0) != hipSuccess || hipMallocAsync(p, bytes, c->stream) != hipSuccess) {
+                (void)hipGetLastError();
+                return TDDA_ERR_OOM;
+            }
+        }
         // usable from any stream (and by blocking copies) once the stream has reached it
         if (hipStreamSynchronize(c->stream) != hipSuccess) return TDDA_ERR_HIP;
         return TDDA_OK;
