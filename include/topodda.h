@@ -120,7 +120,8 @@ typedef struct {
 int tdda_op_create(tdda_ctx ctx, const tdda_op_desc* desc, tdda_op* out);
 /* Periodic AProductCore (AProductCore.cpp:251-536): the tensor of each grid
  * offset is summed over the images m = -max_m..max_m, n = -max_n..max_n
- * shifted by (m Lm, n Ln, 0) (lengths in the units of d), each damped by
+ * shifted by (m Lm, n Ln, 0) -- physical lengths in the unit of d and lam
+ * (e.g. nm), added to the offsets (i d, j d, k d) -- each damped by
  * exp(-(0.01 K r)^4) (A_dic_generator, AProductCore.cpp:573-586).  At most
  * 1e6 images. */
 int tdda_op_create_periodic(tdda_ctx ctx, const tdda_op_desc* desc, int max_m, int max_n, double Lm, double Ln,
@@ -131,7 +132,7 @@ int tdda_op_create_periodic(tdda_ctx ctx, const tdda_op_desc* desc, int max_m, i
 typedef struct {
     int method;         /* TDDA_METHOD_* */
     int max_m, max_n;   /* images of a periodic structure; both < 0: not periodic */
-    double Lm, Ln;      /* periods (units of d) */
+    double Lm, Ln;      /* periods: physical lengths, same unit as d (not multiples of d) */
 } tdda_op_ext;
 /* AProductCore with the tensor method and, optionally, periodic images
  * (the reference's constructors, AProductCore.cpp:12-249 and 251-536, with
@@ -155,6 +156,11 @@ int tdda_op_tiles(tdda_op op, int precision, int* out4);
 int tdda_op_xpass(tdda_op op, int precision);
 /* AProductCore::~AProductCore (AProductCore.cpp:538-558). */
 int tdda_op_destroy(tdda_op op);
+/* The checks of tdda_op_create that need no device (sizes, grid, the
+ * 32-bit limit Nx * My * Mz * 3 < 2^31 of the pass kernels); grid3 <- the
+ * FFT grid the operator would use (may be NULL).  No counterpart in the
+ * reference (AProductCore.cpp:41-44 fixes the grid at 2N - 1).            */
+int tdda_op_plan(const tdda_op_desc* desc, int* grid3);
 /* grid3 <- (Mx, My, Mz); K <- wavenumber.  Either may be NULL. */
 int tdda_op_info(tdda_op op, int* grid3, double* K);
 
@@ -179,6 +185,9 @@ int tdda_green_tensor_fcd(double K, double d, double x, double y, double z, doub
 /* Material rule of AProductCore.cpp:19-24: eps /= nback^2, +0.01 if |eps| <= 1.01
  * (nmat complex values, in place).                                         */
 int tdda_material_rule(int nmat, double* material, double nback);
+/* The periodic constructor's variant (AProductCore.cpp:261-267): +0.01 only
+ * if |eps| < 1.01, so a material at exactly |eps| = 1.01 is kept.          */
+int tdda_material_rule_periodic(int nmat, double* material, double nback);
 /* Get_Alpha (tools.cpp:617-630): LDR polarizability. */
 int tdda_get_alpha(double lam, double K, double d, double diel_re, double diel_im, const double* nE3,
                    const double* nK3, double* out2);

@@ -97,14 +97,19 @@ void orc_get_alpha(double K, double d, double diel_re, double diel_im,
 /* ------------------------------------------------------------------------- */
 double orc_wavenumber(double lam, double nback) { return 2.0 * M_PI / (lam / nback); }
 
-void orc_material_rule(int nmat, double *mat2, double nback)
+/* |eps| is sqrt(norm(eps)) as in the reference; strict != 0 is the periodic */
+/* constructor's "< 1.01" (AProductCore.cpp:261-267).                          */
+void orc_material_rule_ex(int nmat, double *mat2, double nback, int strict)
 {
     for (int i = 0; i < nmat; i++) {
         cplx m = (mat2[2 * i] + I * mat2[2 * i + 1]) / (nback * nback);
-        if (cabs(m) <= 1.01) m = m + 0.01;
+        double a = sqrt(creal(m) * creal(m) + cimag(m) * cimag(m));
+        if (strict ? a < 1.01 : a <= 1.01) m = m + 0.01;
         mat2[2 * i] = creal(m); mat2[2 * i + 1] = cimag(m);
     }
 }
+
+void orc_material_rule(int nmat, double *mat2, double nback) { orc_material_rule_ex(nmat, mat2, nback, 0); }
 
 /* ------------------------------------------------------------------------- */
 /* Incident field.  DDAModel.cpp:47-51 (also reset_E, DDAModel.cpp:457-468):  */

@@ -78,6 +78,7 @@ def lib():
         L.orc_wavenumber.argtypes = [ctypes.c_double, ctypes.c_double]
         L.orc_wavenumber.restype = ctypes.c_double
         L.orc_material_rule.argtypes = [ctypes.c_int, _dp, ctypes.c_double]
+        L.orc_material_rule_ex.argtypes = [ctypes.c_int, _dp, ctypes.c_double, ctypes.c_int]
         L.orc_einc.argtypes = [ctypes.c_int, _ip, ctypes.c_double, ctypes.c_double, ctypes.c_double, _dp, _dp, _dp]
         L.orc_diel_al.argtypes = [ctypes.c_int, _dp, ctypes.c_int, _dp, ctypes.c_double, ctypes.c_double,
                                   ctypes.c_double, _dp, _dp, _dp, _dp]
@@ -129,10 +130,12 @@ def wavenumber(lam: float, nback: float = 1.0) -> float:
     return lib().orc_wavenumber(lam, nback)
 
 
-def material_rule(mat, nback: float = 1.0) -> np.ndarray:
-    """AProductCore.cpp:19-24 (divide by nback^2, +0.01 when |eps| <= 1.01)."""
+def material_rule(mat, nback: float = 1.0, periodic: bool = False) -> np.ndarray:
+    """AProductCore.cpp:19-24 (divide by nback^2, +0.01 when |eps| <= 1.01);
+    periodic: the periodic constructor's strict test |eps| < 1.01
+    (AProductCore.cpp:261-267)."""
     m = np.ascontiguousarray(np.asarray(mat, dtype=np.complex128).copy())
-    lib().orc_material_rule(len(m), _ptr(m.view(np.float64)), nback)
+    lib().orc_material_rule_ex(len(m), _ptr(m.view(np.float64)), nback, 1 if periodic else 0)
     return m
 
 

@@ -81,3 +81,39 @@ def test_host_setup_math_matches_oracle():
     assert np.allclose(d1, d2, rtol=1e-15, atol=0) and np.allclose(a1, a2, rtol=1e-14, atol=0)
     P = np.random.default_rng(1).standard_normal(len(R)) + 1j
     assert np.allclose(T.local_field(a1, P, d1), O.local_field(a2, P, d2), rtol=1e-14, atol=0)
+
+
+def test_material_rule_threshold_of_each_constructor():
+    # AProductCore.cpp:19-24 adds 0.01 when |eps| <= 1.01; the periodic
+    # constructor (AProductCore.cpp:261-267) only when |eps| < 1.01, so a
+    # material at exactly 1.01 differs between the two
+    mat = np.array([1.01, 1.0, 1.0100000000000002, 0.5 + 0.8j, 0.5 + 0.9j, 2.25 + 0.01j])
+    plain = T.material_rule(mat)
+    per = T.material_rule(mat, periodic=True)
+    assert plain[0] == 1.02 and per[0] == 1.01
+    assert plain[1] == per[1] == 1.01
+    assert plain[2] == per[2] == mat[2]              # just above 1.01: kept
+    assert plain[3] == per[3] == mat[3] + 0.01       # |eps| = 0.943
+    assert plain[4] == per[4] == mat[4]              # |eps| = 1.0296
+    assert plain[5] == per[5] == mat[5]
+    for periodic in (False, True):
+        for nback in (1.0, 1.3):
+            assert np.array_equal(T.material_rule(mat, nback, periodic), O.material_rule(mat, nback, periodic))
+
+
+def test_op_plan_grid_and_32bit_guard():
+    # grids the planner picks: smallest 2,3,5-smooth >= 2N - 1
+    assert T.op_plan(16, 16, 16) == (32, 32, 32)
+    assert T.op_plan(80, 80, 16) == (160, 160, 32)
+    assert T.op_plan(128, 128, 32) == (256, 256, 64)
+    assert T.op_plan(512, 512, 64) == (1024, 1024, 128)
+    # the pass kernels use 32-bit offsets within one right-hand side's
+    # Nx x My x Mz x 3 intermediate: larger structures are refused
+    T.op_plan(40, 1000, 16, grid=(80, 2048, 4096))  # 40 * 2048 * 4096 * 3 = 1.01e9 < 2^31
+    for args, grid in (((600, 600, 600), None), ((90, 1000, 16), (180, 2048, 4096))):  # 2.6e9, 2.26e9
+        try:
+            T.op_plan(*args, grid=grid)
+        except T.TddaError as e:
+            assert "2^31" in str(e)
+        else:
+            raise AssertionError("Nx * My * Mz * 3 >= 2^31 must be refused")

@@ -74,3 +74,20 @@ def test_tuned_model_alone_equals_batched(ctx):
         m.close()
     for c in cores:
         c.close()
+
+
+def test_tuner_keeps_its_y_pass_decision(ctx, monkeypatch):
+    # TDDA_TUNE_YLDS=1 makes the LDS-staged y kernels win the y step; the
+    # x step that follows must not drop that decision
+    Nx, Ny, Nz, d, lam = 24, 20, 8, 10.0, 500.0
+    core = T.AProductCore(ctx, None, Nx, Ny, Nz, d, lam)
+    assert core.ypass(T.C128) == "register" and core.ypass(T.C64) == "register"
+    monkeypatch.setenv("TDDA_TUNE_YLDS", "1")
+    core.autotune(reps=5)
+    monkeypatch.delenv("TDDA_TUNE_YLDS")
+    assert core.ypass(T.C128) == "lds" and core.ypass(T.C64) == "lds"
+    R = O.build_a_bulk(Nx, Ny, Nz)
+    b = np.random.default_rng(5).standard_normal(len(R)) + 0.5j
+    ref = O.aproduct_numpy_fft(R, Nx, Ny, Nz, d, O.wavenumber(lam), b, grid=core.grid)
+    assert O.rel_l2(core.Aproduct(b), ref) < 1e-12
+    core.close()

@@ -1096,17 +1096,27 @@ int tdda_green_tensor(double K, double x, double y, double z, double* out12)
     return TDDA_OK;
 }
 
-int tdda_material_rule(int nmat, double* material, double nback)
+// |eps| as the reference computes it, sqrt(norm(eps)), so that the 1.01
+// threshold falls on the same side for every input.
+static int material_rule(int nmat, double* material, double nback, bool strict)
 {
     if (nmat < 0 || (nmat > 0 && !material) || !(nback > 0)) return fail(TDDA_ERR_ARG, "bad material list");
     for (int i = 0; i < nmat; i++) {
         std::complex<double> m(material[2 * i], material[2 * i + 1]);
         m = m / (nback * nback);
-        if (std::abs(m) <= 1.01) m = m + 0.01;
+        const double a = std::sqrt(std::norm(m));
+        if (strict ? a < 1.01 : a <= 1.01) m = m + 0.01;
         material[2 * i] = m.real();
         material[2 * i + 1] = m.imag();
     }
     return TDDA_OK;
+}
+
+int tdda_material_rule(int nmat, double* material, double nback) { return material_rule(nmat, material, nback, false); }
+
+int tdda_material_rule_periodic(int nmat, double* material, double nback)
+{
+    return material_rule(nmat, material, nback, true);
 }
 
 static std::complex<double> ldr_alpha(double K, double d, std::complex<double> diel, const double* nE,
@@ -1244,17 +1254,18 @@ int tdda_op_create_periodic(tdda_ctx ctx, const tdda_op_desc* desc, int max_m, i
 
 namespace {
 
-int op_create(tdda_ctx ctx, const tdda_op_desc* desc, const PeriodicImages& per, tdda_op* out)
+// Checks of an operator description that need no device; M <- the FFT grid.
+int check_desc(const tdda_op_desc& D, int* M)
 {
-    if (!ctx || !desc || !out) return fail(TDDA_ERR_ARG, "null argument");
-    const tdda_op_desc& D = *desc;
     if (D.N <= 0 || D.Nx <= 0 || D.Ny <= 0 || D.Nz <= 0) return fail(TDDA_ERR_ARG, "lattice sizes must be positive");
     if (!(D.d > 0) || !(D.lam > 0) || !(D.nback > 0)) return fail(TDDA_ERR_ARG, "d, lam and nback must be positive");
     const long long box = (long long)D.Nx * D.Ny * D.Nz;
     if (box > (1LL << 30) || D.N > box) return fail(TDDA_ERR_ARG, "lattice too large or N exceeds the box");
-    int prec = D.precisions ? D.precisions : (TDDA_C128 | TDDA_C64);
+    const int prec = D.precisions ? D.precisions : (TDDA_C128 | TDDA_C64);
     if (prec & ~(TDDA_C128 | TDDA_C64)) return fail(TDDA_ERR_ARG, "unknown precision bits");
-    int M[3] = {D.Mx, D.My, D.Mz};
+    M[0] = D.Mx;
+    M[1] = D.My;
+    M[2] = D.Mz;
     const int Nn[3] = {D.Nx, D.Ny, D.Nz};
     for (int a = 0; a < 3; a++) {
         if (M[a] == 0) M[a] = next_smooth(2 * Nn[a] - 1);
@@ -1262,6 +1273,22 @@ int op_create(tdda_ctx ctx, const tdda_op_desc* desc, const PeriodicImages& per,
         if (!smooth235(M[a])) return fail(TDDA_ERR_UNSUPPORTED, "FFT grid sizes must have prime factors 2, 3, 5 only");
         if (M[a] > 4096) return fail(TDDA_ERR_UNSUPPORTED, "FFT length above 4096");
     }
+    // the pass kernels address one right-hand side's intermediates (the
+    // largest, after the y pass, holds Nx x My x Mz x 3 values; the site
+    // vectors 3N are smaller) with 32-bit offsets
+    if ((long long)D.Nx * M[1] * M[2] * 3 >= (1LL << 31))
+        return fail(TDDA_ERR_UNSUPPORTED, "structure too large: Nx * My * Mz * 3 must stay below 2^31");
+    return TDDA_OK;
+}
+
+int op_create(tdda_ctx ctx, const tdda_op_desc* desc, const PeriodicImages& per, tdda_op* out)
+{
+    if (!ctx || !desc || !out) return fail(TDDA_ERR_ARG, "null argument");
+    const tdda_op_desc& D = *desc;
+    int M[3];
+    CHK(check_desc(D, M));
+    const long long box = (long long)D.Nx * D.Ny * D.Nz;
+    const int prec = D.precisions ? D.precisions : (TDDA_C128 | TDDA_C64);
     HIPCHK(hipSetDevice(ctx->device));
     // bulk order (tools.cpp:497-510) on the full box needs no cell map;
     // R == NULL stands for that order
@@ -1430,6 +1457,16 @@ int tdda_op_destroy(tdda_op op)
         tdda_model_destroy(m);
     }
     op_release(op);
+    return TDDA_OK;
+}
+
+int tdda_op_plan(const tdda_op_desc* desc, int* grid3)
+{
+    if (!desc) return fail(TDDA_ERR_ARG, "null argument");
+    int M[3];
+    CHK(check_desc(*desc, M));
+    if (grid3)
+        for (int a = 0; a < 3; a++) grid3[a] = M[a];
     return TDDA_OK;
 }
 

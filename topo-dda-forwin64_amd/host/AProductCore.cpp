@@ -52,8 +52,11 @@ void AProductCore::build(const int* periodic_mn, const double* periodic_L, int p
     if (AMatrixMethod != "LDR" && AMatrixMethod != "FCD")
         throw std::invalid_argument("\"" + AMatrixMethod + "\" is not a valid polarizabiliy calculation method.");
     K = 2 * M_PI / (lam / nback);  // AProductCore.cpp:17
-    // AProductCore.cpp:19-24
-    check(tdda_material_rule((int)material.size(), reinterpret_cast<double*>(material.data()), nback));
+    // AProductCore.cpp:19-24 (periodic constructor: AProductCore.cpp:261-267,
+    // which tests |eps| < 1.01 instead of <= 1.01)
+    double* mat = reinterpret_cast<double*>(material.data());
+    check(periodic_mn ? tdda_material_rule_periodic((int)material.size(), mat, nback)
+                      : tdda_material_rule((int)material.size(), mat, nback));
     tdda_op_desc D{};
     D.N = CStr->get_N();
     D.Nx = CStr->get_Nx();

@@ -26,10 +26,10 @@ CONVERGED, BREAKDOWN, MAXIT = 1, 2, 3
 
 EXPORTS = [
     "tdda_last_error", "tdda_version", "tdda_ctx_create", "tdda_ctx_destroy", "tdda_ctx_sync",
-    "tdda_op_create", "tdda_op_destroy", "tdda_op_info", "tdda_aproduct", "tdda_green_tensor",
+    "tdda_op_create", "tdda_op_destroy", "tdda_op_info", "tdda_op_plan", "tdda_aproduct", "tdda_green_tensor",
     "tdda_model_create", "tdda_model_destroy", "tdda_model_set", "tdda_model_get", "tdda_model_apply",
     "tdda_solve_opts_default", "tdda_model_solve", "tdda_model_get_entries", "tdda_model_set_entries",
-    "tdda_model_time_passes", "tdda_model_time_pass", "tdda_material_rule", "tdda_get_alpha", "tdda_incident_field",
+    "tdda_model_time_passes", "tdda_model_time_pass", "tdda_material_rule", "tdda_material_rule_periodic", "tdda_get_alpha", "tdda_incident_field",
     "tdda_polarizability", "tdda_local_field", "tdda_batch_create", "tdda_batch_destroy", "tdda_batch_size",
     "tdda_batch_solve", "tdda_batch_apply", "tdda_batch_time_passes", "tdda_server_opts_default",
     "tdda_server_create", "tdda_server_destroy", "tdda_server_submit", "tdda_server_poll", "tdda_server_wait",
@@ -107,6 +107,7 @@ def lib():
                                               ctypes.c_double, ctypes.c_double, ctypes.POINTER(vp)]
         L.tdda_op_destroy.argtypes = [vp]
         L.tdda_op_info.argtypes = [vp, ctypes.POINTER(ctypes.c_int), dp]
+        L.tdda_op_plan.argtypes = [ctypes.POINTER(OpDesc), ctypes.POINTER(ctypes.c_int)]
         L.tdda_aproduct.argtypes = [vp, dp, dp, ctypes.c_int, dp, ctypes.c_int, ctypes.c_int]
         L.tdda_green_tensor.argtypes = [ctypes.c_double] * 4 + [dp]
         L.tdda_model_create.argtypes = [vp, ctypes.c_int, ctypes.POINTER(vp)]
@@ -120,6 +121,7 @@ def lib():
         L.tdda_model_time_passes.argtypes = [vp, ctypes.c_int, ctypes.c_int, dp]
         L.tdda_model_time_pass.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, dp]
         L.tdda_material_rule.argtypes = [ctypes.c_int, dp, ctypes.c_double]
+        L.tdda_material_rule_periodic.argtypes = [ctypes.c_int, dp, ctypes.c_double]
         L.tdda_get_alpha.argtypes = [ctypes.c_double] * 5 + [dp, dp, dp]
         L.tdda_incident_field.argtypes = [ctypes.c_int, ip] + [ctypes.c_double] * 3 + [dp, dp, dp]
         L.tdda_polarizability.argtypes = [ctypes.c_int, dp, ctypes.c_int, dp] + [ctypes.c_double] * 3 + [dp] * 4
@@ -216,10 +218,12 @@ def wavenumber(lam, nback=1.0):
     return 2 * math.pi / (lam / nback)
 
 
-def material_rule(material, nback=1.0):
-    """AProductCore.cpp:19-24."""
+def material_rule(material, nback=1.0, periodic=False):
+    """AProductCore.cpp:19-24 (+0.01 when |eps| <= 1.01); periodic: the
+    periodic constructor's |eps| < 1.01 (AProductCore.cpp:261-267)."""
     m = _c(material).copy()
-    _check(lib().tdda_material_rule(len(m), _dptr(m.view(np.float64)), nback))
+    fn = lib().tdda_material_rule_periodic if periodic else lib().tdda_material_rule
+    _check(fn(len(m), _dptr(m.view(np.float64)), nback))
     return m
 
 
@@ -286,6 +290,17 @@ def green_tensor_fcd(K, d, x, y, z):
     out = np.zeros(12)
     _check(lib().tdda_green_tensor_fcd(K, d, x, y, z, _dptr(out)))
     return out.view(np.complex128)
+
+
+def op_plan(Nx, Ny, Nz, d=10.0, lam=500.0, nback=1.0, N=None, grid=None):
+    """FFT grid an operator of this box would use, after the checks of
+    tdda_op_create that need no device (tdda_op_plan); raises TddaError
+    where tdda_op_create would refuse the description."""
+    g = grid or (0, 0, 0)
+    desc = OpDesc(Nx * Ny * Nz if N is None else N, Nx, Ny, Nz, None, d, lam, nback, g[0], g[1], g[2], C128 | C64)
+    g3 = (ctypes.c_int * 3)()
+    _check(lib().tdda_op_plan(ctypes.byref(desc), g3))
+    return tuple(g3)
 
 
 class AProductCore:
