@@ -173,6 +173,26 @@ int tdda_op_info(tdda_op op, int* grid3, double* K);
 int tdda_aproduct(tdda_op op, const double* b, double* y, int nrhs, const double* al, int al_per_rhs,
                   int precision);
 
+/* Allocation-free variant of tdda_aproduct.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  b, y and al are DEVICE pointers, 16-byte aligned,
+ * interleaved complex128 (nrhs * 3N values each; al holds nrhs * 3N values
+ * if al_per_rhs, else 3N; al NULL: no diagonal term).  The same passes and
+ * tiles as tdda_aproduct; the job tables, the pass intermediates and the
+ * complex64 staging live in one static device array of
+ * TDDA_NOALLOC_SCRATCH_BYTES per GPU, held until the process ends.  A call
+ * that needs more (about (Nx Ny Mz + Nx My Mz) * 48 B per right-hand side
+ * in complex128, half in complex64, plus 256 KiB of tables), or nrhs > 64,
+ * fails with TDDA_ERR_UNSUPPORTED / TDDA_ERR_ARG.  Runs on the operator's
+ * context stream and returns when y is written.  Calls no hipMalloc or
+ * hipFree.  Every call on one GPU uses that GPU's array: calls that overlap
+ * on one GPU (several threads or contexts) race and must be serialised by
+ * the caller.                                                              */
+#define TDDA_NOALLOC_SCRATCH_BYTES ((size_t)64 << 20)
+int tdda_aproduct_noalloc(tdda_op op, const double* b, double* y, int nrhs, const double* al, int al_per_rhs,
+                          int precision);
+
 /* AProductCore::LDR_inter (AProductCore.cpp:588-623) on the host: the six
  * unique components xx xy xz yy yz zz of the interaction tensor at offset
  * (x, y, z) for wavenumber K, as 12 doubles.                                */

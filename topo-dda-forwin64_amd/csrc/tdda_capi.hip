@@ -468,14 +468,13 @@ int alloc_vecs(tdda_model_s* m, VecSet<T>& v)
 // side).  r_vec / r0_vec let the outer residual table of the mixed solver
 // write its residual and r0 copy into scratch vectors.
 template <typename T>
-void fill_jobs(const tdda_model_s* m, std::vector<JobDesc<T>>& out, const VecSet<T>& v, const Cx<T>* green,
+void fill_jobs(const tdda_op_s* op, int nrhs, std::vector<JobDesc<T>>& out, const VecSet<T>& v, const Cx<T>* green,
                Cx<T>* const* tw, const TilePlan& tp, Cx<T>* r_vec, Cx<T>* r0_vec)
 {
-    const tdda_op_s* op = m->op;
-    const size_t n3 = m->n3;
+    const size_t n3 = 3 * (size_t)op->N;
     const size_t w1n = (size_t)op->Nx * op->Ny * op->Mz * 3;
     const size_t w2n = (size_t)op->Nx * op->My * op->Mz * 3;
-    for (int j = 0; j < m->nrhs; j++) {
+    for (int j = 0; j < nrhs; j++) {
         JobDesc<T> J;
         memset(&J, 0, sizeof(J));
         J.Nx = op->Nx;
@@ -1003,6 +1002,10 @@ static int time_pass_t(hipStream_t s, JobTable<T>& jt, int pass, int reps, doubl
     return TDDA_OK;
 }
 
+namespace {
+int noalloc_init(int device);  // tdda_noalloc.inc
+}
+
 // ---------------------------------------------------------------------------
 // exported functions
 // ---------------------------------------------------------------------------
@@ -1019,6 +1022,9 @@ int tdda_ctx_create(int device, tdda_ctx* out)
     HIPCHK(hipGetDeviceCount(&n));
     if (device < 0 || device >= n) return fail(TDDA_ERR_ARG, "device index out of range");
     HIPCHK(hipSetDevice(device));
+    // device address of the allocation-free entry point's static scratch
+    // array, looked up once per device with the device current
+    CHK(noalloc_init(device));
     hipDeviceProp_t prop;
     HIPCHK(hipGetDeviceProperties(&prop, device));
     tdda_ctx c = new tdda_ctx_s();
@@ -1500,16 +1506,16 @@ int tdda_model_create(tdda_op op, int nrhs, tdda_model* out)
     if (op->green64) {
         if ((rc = alloc_vecs<double>(m, m->v64)) != TDDA_OK) return bail(rc);
         Cx<double>* tw[6] = {op->tws64[0], op->tws64[1], op->tws64[2], op->tw64[0], op->tw64[1], op->tw64[2]};
-        fill_jobs<double>(m, m->j64.h, m->v64, op->green64, tw, op->tp64, m->v64.r, m->v64.r0);
+        fill_jobs<double>(op, nrhs, m->j64.h, m->v64, op->green64, tw, op->tp64, m->v64.r, m->v64.r0);
         // outer residual of the mixed solver: r -> p, its r0 copy -> t (scratch)
-        fill_jobs<double>(m, m->jo.h, m->v64, op->green64, tw, op->tp64, m->v64.p, m->v64.t);
+        fill_jobs<double>(op, nrhs, m->jo.h, m->v64, op->green64, tw, op->tp64, m->v64.p, m->v64.t);
         if ((rc = finalize_table<double>(m->allocs, op->ctx, m->j64)) != TDDA_OK) return bail(rc);
         if ((rc = finalize_table<double>(m->allocs, op->ctx, m->jo)) != TDDA_OK) return bail(rc);
     }
     if (op->green32) {
         if ((rc = alloc_vecs<float>(m, m->v32)) != TDDA_OK) return bail(rc);
         Cx<float>* tw[6] = {op->tws32[0], op->tws32[1], op->tws32[2], op->tw32[0], op->tw32[1], op->tw32[2]};
-        fill_jobs<float>(m, m->j32.h, m->v32, op->green32, tw, op->tp32, m->v32.r, m->v32.r0);
+        fill_jobs<float>(op, nrhs, m->j32.h, m->v32, op->green32, tw, op->tp32, m->v32.r, m->v32.r0);
         if ((rc = finalize_table<float>(m->allocs, op->ctx, m->j32)) != TDDA_OK) return bail(rc);
     }
     if (hipStreamSynchronize(op->ctx->stream) != hipSuccess) return bail(fail(TDDA_ERR_HIP, "model set-up"));
@@ -1931,3 +1937,4 @@ int tdda_batch_apply(tdda_batch b, int precision)
 #include "tdda_design.inc"
 #include "tdda_objective.inc"
 #include "tdda_tune.inc"
+#include "tdda_noalloc.inc"
