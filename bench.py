@@ -275,33 +275,9 @@ def bench_serve(T, seconds, rate=200.0, seed=5):
     residual of 1e-8.  Latency = result ready - scheduled arrival (both on
     the monotonic clock)."""
     import threading
-    rng = np.random.default_rng(seed)
-    pool = rng.random(3 * 2_100_000)
+    import workloads as W
+    reqs, pool = W.serve_stream(seconds, rate, seed)
     mat = T.material_rule(np.array([1.0, 2.25 + 0.01j]))
-    reqs = []
-    t = 0.0
-    while True:
-        t += rng.exponential(1.0 / rate)
-        if t > seconds:
-            break
-        n = math.exp(rng.uniform(math.log(8192), math.log(2_097_152)))
-        Nz = int(rng.choice([8, 16, 32]))
-        Nx = max(1, int(round(math.sqrt(n / Nz))))
-        N = Nx * Nx * Nz
-        nrhs = int(rng.integers(1, 5))
-        adjoint = bool(rng.random() < 0.5)
-        incs = [(np.array([0.0, 0.0, 1.0]), np.array([1.0, 0.0, 0.0]) if rng.random() < 0.5 else
-                 np.array([0.0, 1.0, 0.0])) for _ in range(nrhs)]
-        src = None
-        if adjoint:
-            src = []
-            for _ in range(nrhs):
-                sites = rng.choice(N, 100, replace=False)
-                idx = np.sort((3 * sites[:, None] + np.arange(3)[None, :]).reshape(-1)).astype(np.int32)
-                src.append((idx, rng.standard_normal(300) + 1j * rng.standard_normal(300)))
-        off = int(rng.integers(0, len(pool) - 3 * N + 1))
-        reqs.append(dict(at=t, Nx=Nx, Nz=Nz, N=N, lam=float(rng.uniform(450.0, 700.0)), incs=incs, src=src,
-                         off=off))
     srv = T.Server(0)
     done = []
     tickets = []

@@ -298,6 +298,9 @@ int tdda_batch_apply(tdda_batch b, int precision);
 /* Measurement hook (no reference counterpart): tdda_model_time_passes over
  * every right-hand side of the batch (each pass timed as all its launches). */
 int tdda_batch_time_passes(tdda_batch b, int precision, int reps, double* ms5);
+/* Measurement hook (no reference counterpart): tdda_model_time_pass over
+ * every right-hand side of the batch (all launches of one pass kind). */
+int tdda_batch_time_pass(tdda_batch b, int precision, int pass, int reps, double* ms);
 
 /* Request loop (no counterpart in the reference, which runs one task's
  * solves in sequence: T/main.cpp:584-598, EvoDDAModel.cpp:433-441).  A
@@ -331,6 +334,16 @@ typedef struct {
     const int* src_idx;       /* nrhs * nsrc distinct entries of the 3N vector      */
     const double* src_vals;   /* nrhs * nsrc complex values                        */
 } tdda_request;
+
+/* The DDAModel constructor on the device (DDAModel.cpp:12-69, with
+ * UpdateAlpha's eps interpolation, DDAModel.cpp:54-63, and Get_Alpha,
+ * tools.cpp:617-630): al and E of every right-hand side of model m from a
+ * request description, as the request loop builds them -- the 3N design
+ * values and the incidences cross PCIe instead of nrhs * 3N al and E.  The
+ * request's N, Nx, Ny, Nz and nrhs must match m (its op.d, lam and grid are
+ * ignored: m's operator supplies K and d); E is the plane wave, or the
+ * sparse source when nsrc > 0.  Also sets P = 0.  Synchronous.             */
+int tdda_model_setup(tdda_model m, const tdda_request* r);
 
 typedef struct {
     double t_submit, t_admit, t_done; /* seconds, steady clock of the server */

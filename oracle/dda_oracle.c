@@ -527,6 +527,11 @@ int orc_bicgstab(void *h, const double *al2, const double *E2, double *P2,
     return status;
 }
 
+void orc_set_num_threads(int n)
+{
+    if (n > 0) omp_set_num_threads(n);
+}
+
 int orc_num_threads(void)
 {
 #ifdef _OPENMP

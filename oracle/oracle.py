@@ -93,6 +93,7 @@ def lib():
                                    _ip, _dp]
         L.orc_bicgstab.restype = ctypes.c_int
         L.orc_num_threads.restype = ctypes.c_int
+        L.orc_set_num_threads.argtypes = [ctypes.c_int]
         _LIB = L
     return _LIB
 

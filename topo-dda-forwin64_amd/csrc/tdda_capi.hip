@@ -1902,6 +1902,23 @@ int tdda_batch_time_passes(tdda_batch b, int precision, int reps, double* ms5)
     return fail(TDDA_ERR_ARG, "precision must be TDDA_C128 or TDDA_C64");
 }
 
+int tdda_batch_time_pass(tdda_batch b, int precision, int pass, int reps, double* ms)
+{
+    if (!b || !ms || reps <= 0 || pass < 0 || pass > 4) return fail(TDDA_ERR_ARG, "bad argument");
+    HIPCHK(hipSetDevice(b->ctx->device));
+    hipStream_t s = b->ctx->stream;
+    if (precision == TDDA_C128) return time_pass_t<double>(s, b->j64, pass, reps, ms);
+    if (precision == TDDA_C64) {
+        if (!b->has32) return fail(TDDA_ERR_STATE, "batch has no complex64 storage");
+        for (int j = 0; j < b->nj; j++) {
+            const size_t n3 = 3 * (size_t)b->j64.h[j].N;
+            k_demote<<<(unsigned)((n3 + kThreads - 1) / kThreads), kThreads, 0, s>>>(b->j64.h[j].x, b->j32.h[j].x, n3);
+        }
+        return time_pass_t<float>(s, b->j32, pass, reps, ms);
+    }
+    return fail(TDDA_ERR_ARG, "precision must be TDDA_C128 or TDDA_C64");
+}
+
 int tdda_batch_apply(tdda_batch b, int precision)
 {
     if (!b) return fail(TDDA_ERR_ARG, "null batch");

@@ -31,12 +31,12 @@ EXPORTS = [
     "tdda_solve_opts_default", "tdda_model_solve", "tdda_model_get_entries", "tdda_model_set_entries",
     "tdda_model_time_passes", "tdda_model_time_pass", "tdda_material_rule", "tdda_material_rule_periodic", "tdda_get_alpha", "tdda_incident_field",
     "tdda_polarizability", "tdda_local_field", "tdda_batch_create", "tdda_batch_destroy", "tdda_batch_size",
-    "tdda_batch_solve", "tdda_batch_apply", "tdda_batch_time_passes", "tdda_server_opts_default",
+    "tdda_batch_solve", "tdda_batch_apply", "tdda_batch_time_passes", "tdda_batch_time_pass", "tdda_server_opts_default",
     "tdda_server_create", "tdda_server_destroy", "tdda_server_submit", "tdda_server_poll", "tdda_server_wait",
     "tdda_server_result", "tdda_server_get_stats", "tdda_design_create", "tdda_design_step", "tdda_design_get",
     "tdda_design_destroy", "tdda_beta_update", "tdda_op_create_periodic", "tdda_objective_point_e",
     "tdda_objective_integrated_e", "tdda_op_autotune", "tdda_op_tiles", "tdda_op_xpass", "tdda_op_create_ex",
-    "tdda_green_tensor_fcd", "tdda_field_at_points", "tdda_write_structure", "tdda_write_model_results",
+    "tdda_green_tensor_fcd", "tdda_field_at_points", "tdda_write_structure", "tdda_write_model_results", "tdda_model_setup", "tdda_aproduct_noalloc",
 ]
 
 
@@ -135,6 +135,7 @@ def lib():
         L.tdda_batch_solve.argtypes = [vp, ctypes.POINTER(SolveOpts), ctypes.POINTER(SolveInfo)]
         L.tdda_batch_apply.argtypes = [vp, ctypes.c_int]
         L.tdda_batch_time_passes.argtypes = [vp, ctypes.c_int, ctypes.c_int, dp]
+        L.tdda_batch_time_pass.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, dp]
         L.tdda_server_opts_default.argtypes = [ctypes.POINTER(ServerOpts)]
         L.tdda_server_opts_default.restype = None
         L.tdda_server_create.argtypes = [ctypes.c_int, ctypes.POINTER(ServerOpts), ctypes.POINTER(vp)]
@@ -145,6 +146,7 @@ def lib():
         L.tdda_server_result.argtypes = [vp, ctypes.c_longlong, dp, ctypes.POINTER(SolveInfo),
                                          ctypes.POINTER(RequestTimes)]
         L.tdda_server_get_stats.argtypes = [vp, ctypes.POINTER(ServerStats)]
+        L.tdda_model_setup.argtypes = [vp, ctypes.POINTER(Request)]
         L.tdda_design_create.argtypes = [vp, dp, ctypes.c_double, dp, ctypes.c_int, dp, ctypes.POINTER(vp)]
         L.tdda_design_step.argtypes = [vp, dp, ctypes.c_double, ctypes.c_double]
         L.tdda_design_get.argtypes = [vp, dp, dp, dp]
@@ -421,6 +423,18 @@ class DeviceModel:
         _check(lib().tdda_model_get(self.h, which, rhs, _dptr(out.view(np.float64))))
         return out.reshape(self.nrhs, self.n3) if rhs < 0 else out
 
+    def setup(self, t, material, incidences, E0=None, R=None, sources=None):
+        """al and E of every right-hand side built on the device from the
+        design values t (3N), the material list (after material_rule) and
+        the incidences [(nK, nE), ...] (tdda_model_setup: the DDAModel
+        constructor, DDAModel.cpp:12-69); sources: sparse right-hand sides
+        as in Server.submit.  P = 0 afterwards."""
+        c = self.core
+        q, keep = _request(c.Nx, c.Ny, c.Nz, c.d, c.lam, t, material, incidences, E0,
+                           c.R if R is None else R, sources, c.nback)
+        _check(lib().tdda_model_setup(self.h, ctypes.byref(q)))
+        del keep
+
     def get_entries(self, which, rhs, idx):
         idx = np.ascontiguousarray(idx, dtype=np.int32)
         out = np.zeros(len(idx), np.complex128)
@@ -492,6 +506,12 @@ class Batch:
         out = np.zeros(5)
         _check(lib().tdda_batch_time_passes(self.h, precision, reps, _dptr(out)))
         return out
+
+    def time_pass(self, pass_index, precision=C128, reps=20):
+        """Mean ms of all launches of one pass kind, repeated reps times back to back."""
+        out = ctypes.c_double()
+        _check(lib().tdda_batch_time_pass(self.h, precision, pass_index, reps, ctypes.byref(out)))
+        return out.value
 
     def close(self):
         if self.h:
@@ -619,6 +639,43 @@ class Design:
             pass
 
 
+def _request(Nx, Ny, Nz, d, lam, t, material, incidences, E0=None, R=None, sources=None, nback=1.0, grid=None):
+    """tdda_request of one structure; returns (request, arrays it points
+    into, which must stay alive while it is used)."""
+    t = _f(t)
+    N = len(t) // 3
+    g = grid or (0, 0, 0)
+    Rp = None
+    keep = [t]
+    if R is not None:
+        R = np.ascontiguousarray(R, dtype=np.int32)
+        Rp = R.ctypes.data_as(ctypes.POINTER(ctypes.c_int))
+        keep.append(R)
+    m = _c(material)
+    nrhs = len(incidences)
+    nK = _f(np.array([i[0] for i in incidences]).reshape(-1))
+    nE = _f(np.array([i[1] for i in incidences]).reshape(-1))
+    e0 = None if E0 is None else _f(E0)
+    keep += [m, nK, nE, e0]
+    q = Request()
+    q.op = OpDesc(N, Nx, Ny, Nz, Rp, d, lam, nback, g[0], g[1], g[2], 0)
+    q.diel_old = _dptr(t)
+    q.nmat = len(m)
+    q.material = _dptr(m.view(np.float64))
+    q.nrhs = nrhs
+    q.nK = _dptr(nK)
+    q.nE = _dptr(nE)
+    q.E0 = _dptr(e0) if e0 is not None else None
+    if sources:
+        idx = np.ascontiguousarray(np.concatenate([np.asarray(i, np.int32) for i, _ in sources]), dtype=np.int32)
+        val = _c(np.concatenate([np.asarray(v) for _, v in sources]))
+        q.nsrc = len(sources[0][0])
+        q.src_idx = idx.ctypes.data_as(ctypes.POINTER(ctypes.c_int))
+        q.src_vals = _dptr(val.view(np.float64))
+        keep += [idx, val]
+    return q, keep
+
+
 class Server:
     """Request loop with continuous batching on one device (own worker
     thread and stream).  submit() copies the request and returns a ticket;
@@ -640,35 +697,8 @@ class Server:
         """t: 3N design values; material: complex list after material_rule;
         incidences: [(nK, nE), ...]; sources: None or [(idx, vals), ...] per
         incidence (sparse right-hand sides, same count each)."""
-        t = _f(t)
-        N = len(t) // 3
-        g = grid or (0, 0, 0)
-        Rp = None
-        if R is not None:
-            R = np.ascontiguousarray(R, dtype=np.int32)
-            Rp = R.ctypes.data_as(ctypes.POINTER(ctypes.c_int))
-        m = _c(material)
-        nrhs = len(incidences)
-        nK = _f(np.array([i[0] for i in incidences]).reshape(-1))
-        nE = _f(np.array([i[1] for i in incidences]).reshape(-1))
-        e0 = None if E0 is None else _f(E0)
-        q = Request()
-        q.op = OpDesc(N, Nx, Ny, Nz, Rp, d, lam, nback, g[0], g[1], g[2], 0)
-        q.diel_old = _dptr(t)
-        q.nmat = len(m)
-        q.material = _dptr(m.view(np.float64))
-        q.nrhs = nrhs
-        q.nK = _dptr(nK)
-        q.nE = _dptr(nE)
-        q.E0 = _dptr(e0) if e0 is not None else None
-        keep = []
-        if sources:
-            idx = np.ascontiguousarray(np.concatenate([np.asarray(i, np.int32) for i, _ in sources]), dtype=np.int32)
-            val = _c(np.concatenate([np.asarray(v) for _, v in sources]))
-            q.nsrc = len(sources[0][0])
-            q.src_idx = idx.ctypes.data_as(ctypes.POINTER(ctypes.c_int))
-            q.src_vals = _dptr(val.view(np.float64))
-            keep = [idx, val]
+        q, keep = _request(Nx, Ny, Nz, d, lam, t, material, incidences, E0, R, sources, nback, grid)
+        N, nrhs = q.op.N, q.nrhs
         tk = ctypes.c_longlong()
         _check(lib().tdda_server_submit(self.h, ctypes.byref(q), ctypes.byref(tk)))
         del keep

@@ -12,6 +12,7 @@ Named workloads (sizes as in the benchmark definitions):
   topo80     80x80x16, d = 10 nm, lambda = 542 nm, cone-filtered column design
   mixed128   128x128x32, d = 8 nm, lambda = 600 nm, Bernoulli(0.5) binary design
   batch64    64 structures, Nx,Ny in 16..64, Nz in 8..16, 2 pols x 4 angles
+  serve      Poisson request stream, 8k-2M dipoles, 1-4 right-hand sides
 """
 from __future__ import annotations
 
@@ -111,3 +112,41 @@ def incidences():
         res.append((nK, T.ne_perp_in_xz(th)))
         res.append((nK, np.array([0.0, 1.0, 0.0])))
     return res
+
+
+def serve_stream(seconds: float, rate: float = 200.0, seed: int = 5):
+    """Request stream: Poisson arrivals at `rate`/s for `seconds`; dipole
+    count log-uniform in [8k, 2M], Nz in {8, 16, 32}, Nx = Ny =
+    round(sqrt(n / Nz)); design values U[0, 1) (eps between 1.01 and
+    2.25 + 0.01i); lambda U[450, 700] nm, d = 10 nm; 1-4 right-hand sides;
+    half forward (z-incident plane wave, x or y polarised), half adjoint
+    (complex sources at 100 seeded target dipoles, all three components).
+    Returns (requests, pool): each request's design values are
+    pool[off : off + 3N]."""
+    rng = np.random.default_rng(seed)
+    pool = rng.random(3 * 2_100_000)
+    reqs = []
+    t = 0.0
+    while True:
+        t += rng.exponential(1.0 / rate)
+        if t > seconds:
+            break
+        n = math.exp(rng.uniform(math.log(8192), math.log(2_097_152)))
+        Nz = int(rng.choice([8, 16, 32]))
+        Nx = max(1, int(round(math.sqrt(n / Nz))))
+        N = Nx * Nx * Nz
+        nrhs = int(rng.integers(1, 5))
+        adjoint = bool(rng.random() < 0.5)
+        incs = [(np.array([0.0, 0.0, 1.0]), np.array([1.0, 0.0, 0.0]) if rng.random() < 0.5 else
+                 np.array([0.0, 1.0, 0.0])) for _ in range(nrhs)]
+        src = None
+        if adjoint:
+            src = []
+            for _ in range(nrhs):
+                sites = rng.choice(N, 100, replace=False)
+                idx = np.sort((3 * sites[:, None] + np.arange(3)[None, :]).reshape(-1)).astype(np.int32)
+                src.append((idx, rng.standard_normal(300) + 1j * rng.standard_normal(300)))
+        off = int(rng.integers(0, len(pool) - 3 * N + 1))
+        reqs.append(dict(at=t, Nx=Nx, Nz=Nz, N=N, lam=float(rng.uniform(450.0, 700.0)), incs=incs, src=src,
+                         off=off))
+    return reqs, pool
