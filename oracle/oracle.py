@@ -119,6 +119,8 @@ def ref_lib():
         L.ref_model_results_full.argtypes = [ctypes.c_char_p, _dp, ctypes.c_int, _dp, ctypes.c_int, _ip,
                                              ctypes.c_int, _dp, _dp]
         L.ref_model_results_full.restype = ctypes.c_int
+        L.ref_spacepara_case.argtypes = [ctypes.c_char_p, ctypes.c_char_p]
+        L.ref_spacepara_case.restype = ctypes.c_int
         _REF = L
     return _REF
 

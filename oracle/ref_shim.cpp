@@ -13,6 +13,7 @@
 // DDAModel.cpp:589-616, restated with std::ofstream since DDAModel.cpp
 // includes CUDA headers).
 #include <complex>
+#include <cstdio>
 #include <fstream>
 #include <vector>
 #include "Tools.h"
@@ -150,4 +151,101 @@ int ref_model_results_full(const char* path, const double* P, int nP, const doub
     return fout.fail() ? -1 : 0;
 }
 
+
+// A design case through the reference's own Space / Structure /
+// FilterOption / SpacePara (input-geometry constructor, SpacePara.cpp:
+// 1087-1384; ChangeFilter) / CoreStructure (CoreStructure.cpp:6-104):
+// parameterization, cone-filter weights and UpdateStr steps.  Reads and
+// writes the text formats of topo-dda-forwin64_amd/host/dda_design_check.cpp,
+// which drives the build's host classes through the same case.
+static void ref_write_weights(FILE* f, SpacePara& sp)
+{
+    for (auto& w : *sp.get_FreeWeight()) {
+        fprintf(f, "%zu", w.size());
+        for (auto& q : w) fprintf(f, " %.17g %d", q.weight, q.position);
+        fprintf(f, "\n");
+    }
+}
+
+static void ref_write_vec(FILE* f, const VectorXd& v)
+{
+    for (int i = 0; i < v.size(); i++) fprintf(f, "%.17g\n", v(i));
+}
+
+int ref_spacepara_case(const char* in_path, const char* out_path)
+{
+    try {
+        ifstream in(in_path);
+        int Nx, Ny, Nz, bindz, ns;
+        in >> Nx >> Ny >> Nz >> bindz >> ns;
+        VectorXi total_space = build_a_bulk(Nx, Ny, Nz);
+        vector<Structure> ln;
+        vector<VectorXi> geoms(ns);
+        vector<int> paras(ns);
+        for (int s = 0; s < ns; s++) {
+            int n;
+            in >> paras[s] >> n;
+            geoms[s] = VectorXi::Zero(3 * n);
+            for (int k = 0; k < 3 * n; k++) in >> geoms[s](k);
+        }
+        Space S(&total_space, Nx, Ny, Nz, 0, &ln);
+        for (int s = 0; s < ns; s++) {
+            Structure st(S.get_total_space(), &geoms[s], paras[s] != 0);
+            S = S + st;
+        }
+        int nin;
+        in >> nin;
+        VectorXi inGeo = VectorXi::Zero(3 * nin);
+        VectorXd inDiel = VectorXd::Zero(3 * nin);
+        for (int k = 0; k < 3 * nin; k++) in >> inGeo(k);
+        for (int k = 0; k < nin; k++) {
+            double v;
+            in >> v;
+            inDiel(3 * k) = inDiel(3 * k + 1) = inDiel(3 * k + 2) = v;
+        }
+        int filter, nfl;
+        double bmin, bmax, ita;
+        string btype;
+        in >> filter >> bmin >> bmax >> ita >> btype >> nfl;
+        vector<filterinfo> fl(nfl);
+        for (auto& f : fl) in >> f.iteration >> f.rfilter;
+        string sym;
+        double ax, ay;
+        in >> sym >> ax >> ay;
+        int periodic, Lx, Ly;
+        in >> periodic >> Lx >> Ly;
+        FilterOption fo(bmin, bmax, ita, btype, fl);
+        SpacePara sp(&S, Vector3i(1, 1, bindz), &inGeo, &inDiel, filter != 0, &fo, sym, vector<double>{ax, ay},
+                     periodic != 0, Lx, Ly);
+        CoreStructure cs(&sp, 1.0);
+        FILE* f = fopen(out_path, "w");
+        if (!f) return -3;
+        const int N = cs.get_N();
+        const int NF = (int)sp.get_Free()->size();
+        fprintf(f, "%d %d %d\n", N, (int)sp.get_Para()->size(), NF);
+        for (int i = 0; i < N; i++) fprintf(f, "%d\n", (*sp.get_geometryPara())(i));
+        if (filter) ref_write_weights(f, sp);
+        int nsteps, max_it;
+        in >> nsteps >> max_it;
+        for (int k = 0; k < nsteps; k++) {
+            int it, change;
+            in >> it >> change;
+            VectorXd step = VectorXd::Zero(NF);
+            for (int i = 0; i < NF; i++) in >> step(i);
+            if (change && filter && fo.filterchange(it)) sp.ChangeFilter();
+            cs.UpdateStr(step, it, max_it);
+        }
+        ref_write_vec(f, *sp.get_Para());
+        if (filter) {
+            ref_write_vec(f, *sp.get_Para_origin());
+            ref_write_vec(f, *sp.get_Para_filtered());
+        }
+        ref_write_vec(f, *cs.get_diel_old());
+        if (filter) ref_write_weights(f, sp);
+        fclose(f);
+        return 0;
+    } catch (...) {
+        return -2;
+    }
+}
 }

@@ -27,6 +27,7 @@
 #include <stdint.h>
 
 #include "tdda_fft.hpp"
+#include "tdda_tilemap.hpp"
 
 namespace tdda {
 
@@ -540,37 +541,11 @@ __device__ __forceinline__ STile s_tile(const JobDesc<T>* __restrict__ jobs, con
     const int nfb = xpass ? J->nfx : J->nfy;
     const int tf = xpass ? J->tfx : J->tfy;
     const int r = tl - tstart[t.job];
-    if (!xpass) {
-        const int o = r / nfb;
-        t.f0 = (r - o * nfb) * tf;
-        t.outer = o;
-    } else {
-        // x pass: the tiles (ky, f) and (My - ky, f) read the same Green
-        // octant rows.  Paired tiles come in blocks of 16: eight (pair, f)
-        // slots, then the same slots' mirrors, so the two tiles of a pair
-        // are 8 apart in the launch's tile list -- on the same XCD (workgroups
-        // go to the 8 XCDs round robin, and persistent grids are multiples
-        // of 8) at about the same time, sharing its L2.  The unpaired rows
-        // ky = 0 and ky = My / 2 (My even) follow.
-        const int My = J->My, npair = (My - 1) / 2, paired = 2 * npair * nfb;
-        int fb, ky;
-        if (r < paired) {
-            const int q = r >> 4, w = r & 15;
-            const int rem = paired - 16 * q;
-            const int n = rem >= 16 ? 8 : rem >> 1;  // slots in this block
-            const int h = w >= n ? 1 : 0, slot = 8 * q + (w - h * n);
-            const int pair = slot / nfb;
-            fb = slot - pair * nfb;
-            ky = h ? My - 1 - pair : pair + 1;
-        } else {
-            const int s = r - paired;
-            const int row = s >= nfb ? 1 : 0;
-            fb = s - row * nfb;
-            ky = row ? My / 2 : 0;
-        }
-        t.f0 = fb * tf;
-        t.outer = ky;
-    }
+    int o, fb;
+    if (xpass) xtile_map(r, J->My, nfb, o, fb);
+    else ytile_map(r, J->Nx, nfb, o, fb);
+    t.f0 = fb * tf;
+    t.outer = o;
     t.nf = min(tf, J->Mz * 3 - t.f0);
     t.live = !(check_state && job_inactive(J->st));
     return t;

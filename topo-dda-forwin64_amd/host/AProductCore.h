@@ -71,3 +71,9 @@ private:
 };
 
 }  // namespace topodda
+
+// The reference's code names these classes without a namespace; define
+// TOPODDA_NO_GLOBAL_NAMES to keep them in namespace topodda only.
+#ifndef TOPODDA_NO_GLOBAL_NAMES
+using topodda::AProductCore;
+#endif

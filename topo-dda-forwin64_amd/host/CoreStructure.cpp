@@ -2,6 +2,8 @@
 
 #include "AProductCore.h"
 
+#include <tuple>
+
 namespace topodda {
 
 CoreStructure::CoreStructure(int Nx_, int Ny_, int Nz_, double d_, const VectorXi& R_, const VectorXd& diel_old_)
@@ -9,6 +11,65 @@ CoreStructure::CoreStructure(int Nx_, int Ny_, int Nz_, double d_, const VectorX
 {
     if (R_.size() % 3 != 0) throw std::invalid_argument("R must hold 3N coordinates");
     if (diel_old_.size() != R_.size()) throw std::invalid_argument("diel_old must hold 3N values");
+}
+
+CoreStructure::CoreStructure(SpacePara* spacepara_, double d_) : spacepara(spacepara_), d(d_)
+{
+    if (!spacepara) throw std::invalid_argument("CoreStructure: null SpacePara");
+    std::tie(Nx, Ny, Nz, N) = spacepara->get_space()->get_Ns();
+    R = spacepara->get_geometry();
+    diel_old = VectorXd::Zero(3 * (std::size_t)N);
+    diel_from_para();
+    diel_old_max = VectorXd::Zero(3 * (std::size_t)N);
+}
+
+void CoreStructure::diel_from_para()
+{
+    const VectorXi* gp = spacepara->get_geometryPara();
+    const VectorXd* Para = spacepara->get_Para();
+    for (int i = 0; i < N; i++) {
+        const double v = (*Para)((*gp)(i));
+        for (int c = 0; c < 3; c++) diel_old(3 * (std::size_t)i + c) = v;
+    }
+}
+
+void CoreStructure::UpdateStr(VectorXd step, int current_it, int Max_it)
+{
+    if (!spacepara) throw std::logic_error("CoreStructure::UpdateStr(step, ...) needs the SpacePara constructor");
+    VectorXd& Para = *spacepara->get_Para();
+    const VectorXi& Free = *spacepara->get_Free();
+    const std::size_t n = Free.size();
+    if (step.size() != n) throw std::invalid_argument("CoreStructure::UpdateStr: step.size != FreePara.size");
+    auto clamp01 = [](double& v) {
+        if (v >= 1) v = 1;
+        if (v <= 0) v = 0;
+    };
+    if (spacepara->get_Filter()) {
+        VectorXd& origin = *spacepara->get_Para_origin();
+        VectorXd& filtered = *spacepara->get_Para_filtered();
+        FilterOption* fo = spacepara->get_Filterstats();
+        fo->update_beta(current_it, Max_it);
+        for (std::size_t i = 0; i < n; i++) {
+            origin(Free(i)) += step(i);
+            clamp01(origin(Free(i)));
+        }
+        const std::vector<std::vector<WeightPara>>& W = *spacepara->get_FreeWeight();
+        for (std::size_t i = 0; i < n; i++) {
+            double num = 0.0, den = 0.0;
+            for (const WeightPara& w : W[i]) {
+                num += w.weight * origin(w.position);
+                den += w.weight;
+            }
+            filtered(Free(i)) = num / den;
+            Para(Free(i)) = fo->SmoothDensity(filtered(Free(i)));
+        }
+    } else {
+        for (std::size_t i = 0; i < n; i++) {
+            Para(Free(i)) += step(i);
+            clamp01(Para(Free(i)));
+        }
+    }
+    diel_from_para();
 }
 
 VectorXi CoreStructure::build_a_bulk(int Nx, int Ny, int Nz)

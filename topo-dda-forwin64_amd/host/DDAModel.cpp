@@ -74,6 +74,12 @@ void DDAModel::UpdateAlphaSingle(int idx)
 
 void DDAModel::change_E(VectorXcd E_) { E = E_; }
 
+void DDAModel::E_from_device()
+{
+    check(tdda_model_get(dev, TDDA_VEC_E, 0, reinterpret_cast<double*>(E.data())));
+    E_dev = E;
+}
+
 void DDAModel::reset_E() { compute_E(); }
 
 void DDAModel::InitializeP(VectorXcd& Initializer) { P = Initializer; }

@@ -74,6 +74,13 @@ public:
     // solves under FP64 refinement with a true-residual stop test.
     void set_solver_mode(int mode) { solver_mode = mode; }
     const tdda_solve_info& last_info() const { return info; }
+    // The device state (not in the reference): objectives evaluate P there
+    // and write adjoint sources into its E.  to_device uploads the host
+    // mirrors that changed (al, E) and P; E_from_device refreshes the host
+    // mirror of E.
+    tdda_model device_model() { return dev; }
+    void to_device() { sync_to_device(); }
+    void E_from_device();
 
 private:
     void compute_E();
@@ -96,3 +103,9 @@ private:
 };
 
 }  // namespace topodda
+
+// The reference's code names these classes without a namespace; define
+// TOPODDA_NO_GLOBAL_NAMES to keep them in namespace topodda only.
+#ifndef TOPODDA_NO_GLOBAL_NAMES
+using topodda::DDAModel;
+#endif

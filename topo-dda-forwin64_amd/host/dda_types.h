@@ -111,3 +111,14 @@ struct Matrix3cd {
 };
 
 }  // namespace topodda
+
+// The reference's code names these classes without a namespace; define
+// TOPODDA_NO_GLOBAL_NAMES to keep them in namespace topodda only.
+#ifndef TOPODDA_NO_GLOBAL_NAMES
+using topodda::cd;
+using topodda::VectorXcd;
+using topodda::VectorXd;
+using topodda::VectorXi;
+using topodda::Vector3d;
+using topodda::Matrix3cd;
+#endif

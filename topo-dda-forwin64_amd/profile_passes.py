@@ -5,11 +5,12 @@ of times, plus one device-to-device copy of known size, so that a
 calibration point for the counters (the copy moves exactly --copy-mib MiB
 in and out).
 
-usage: python3 topo-dda-forwin64_amd/profile_passes.py [--workload topo80|mixed128]
+usage: python3 topo-dda-forwin64_amd/profile_passes.py [--workload topo80|mixed128|batch64]
        [--precision c128|c64] [--reps 20] [--copy-mib 256] [--no-tune]
 
-The operator runs the autotuner first (as in bench.py), so its tuner's trial
-launches precede the timed passes in a trace.
+A single operator (topo80, mixed128) runs the autotuner first (as in
+bench.py), so its tuner's trial launches precede the timed passes in a
+trace; batch64 keeps the planner's tiles, as bench.py's batch does.
 """
 import argparse
 import ctypes
@@ -45,24 +46,42 @@ def main():
     ap.add_argument("--copy-mib", type=int, default=256)
     ap.add_argument("--no-tune", action="store_true", help="planner tiles instead of the autotuner's")
     a = ap.parse_args()
-    if a.workload == "topo80":
-        prob, _ = W.topo80()
-    elif a.workload == "mixed128":
-        prob = W.mixed128()
-    else:
-        raise SystemExit("unknown workload")
-    al, E, _ = prob.rhs(np.array([0.0, 0.0, 1.0]), np.array([1.0, 0.0, 0.0]))
     ctx = T.Context(0)
-    core = T.AProductCore(ctx, prob.R, prob.Nx, prob.Ny, prob.Nz, prob.d, prob.lam)
-    if not a.no_tune:
-        core.autotune()  # the tiles and x-pass kernel bench.py measures
-    m = T.DeviceModel(core, 1)
-    m.set(T.VEC_AL, al)
-    m.set(T.VEC_P, E)
     prec = T.C128 if a.precision == "c128" else T.C64
-    ms = m.time_passes(prec, reps=a.reps)
+    if a.workload == "batch64":
+        # the headline batch as bench.py builds it: planner tiles, 8
+        # incidences per structure, one launch per FFT length and pass
+        probs, incs = W.batch64(seed=4), W.incidences()
+        cores, models = [], []
+        for p in probs:
+            core = T.AProductCore(ctx, p.R, p.Nx, p.Ny, p.Nz, p.d, p.lam)
+            m = T.DeviceModel(core, len(incs))
+            rhs = [p.rhs(nK, nE)[:2] for nK, nE in incs]
+            m.set(T.VEC_AL, np.concatenate([al for al, _ in rhs]))
+            m.set(T.VEC_P, np.concatenate([e for _, e in rhs]))
+            cores.append(core)
+            models.append(m)
+        b = T.Batch(models)
+        ms = b.time_passes(prec, reps=a.reps)
+        grid = sorted({c.grid for c in cores})
+    else:
+        if a.workload == "topo80":
+            prob, _ = W.topo80()
+        elif a.workload == "mixed128":
+            prob = W.mixed128()
+        else:
+            raise SystemExit("unknown workload")
+        al, E, _ = prob.rhs(np.array([0.0, 0.0, 1.0]), np.array([1.0, 0.0, 0.0]))
+        core = T.AProductCore(ctx, prob.R, prob.Nx, prob.Ny, prob.Nz, prob.d, prob.lam)
+        if not a.no_tune:
+            core.autotune()  # the tiles and x-pass kernel bench.py measures
+        m = T.DeviceModel(core, 1)
+        m.set(T.VEC_AL, al)
+        m.set(T.VEC_P, E)
+        ms = m.time_passes(prec, reps=a.reps)
+        grid = core.grid
     copy_calibration(a.copy_mib)
-    print(json.dumps({"workload": a.workload, "precision": a.precision, "grid": core.grid,
+    print(json.dumps({"workload": a.workload, "precision": a.precision, "grid": grid,
                       "pass_ms": [float(x) for x in ms], "copy_mib": a.copy_mib}))
 
 

@@ -4,7 +4,8 @@
   python3 topo-dda-forwin64_amd/profile_summary.py stats  <kernel_stats.csv>  [--top 20]
   python3 topo-dda-forwin64_amd/profile_summary.py last <kernel_trace.csv> --kernel 'k_pxr<double, 160>' [--last 200]
   python3 topo-dda-forwin64_amd/profile_summary.py traffic <fetch counter_collection.csv>
-          <write counter_collection.csv> --precision c128|c64 --workload topo80 [--json out.json]
+          <write counter_collection.csv> --precision c128|c64 --workload topo80|mixed128|batch64
+          [--json profiles/traffic.json]
 
 `stats` prints the per-kernel duration table (markdown).  `last` prints the
 mean, min and max duration of the last N dispatches of one kernel (the
@@ -13,7 +14,8 @@ average FETCH_SIZE / WRITE_SIZE per dispatch of each pass kernel, scaled by
 the calibration that the plain device copy in the same run gives (the copy
 of --copy-mib MiB run by profile_passes.py reads and writes exactly that
 many bytes), next to the algorithmic bytes of the pass; with --json it
-writes {pass name: measured HBM bytes per dispatch}.
+merges {"<workload>_<precision>:<pass name>": measured HBM bytes per pass}
+into that file (bench.py reads profiles/traffic.json).
 """
 import argparse
 import collections
@@ -44,18 +46,36 @@ def stats(path, top):
 
 
 def last(path, kernel, n):
-    d = [(int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) / 1e3 for r in csv.DictReader(open(path))
-         if r["Kernel_Name"].replace("void ", "").startswith("tdda::" + kernel)]
-    tail = d[-n:]
-    print(f"`{kernel}`: {len(d)} dispatches; last {len(tail)}: mean {sum(tail) / len(tail):.2f} us, "
-          f"min {min(tail):.2f} us, max {max(tail):.2f} us")
+    """Mean, min and max of the last n dispatches of every kernel whose name
+    starts with `kernel` (a batch runs one kernel per FFT length); with
+    several, also the sum of their means (the duration of one pass over the
+    batch, launch gaps excluded)."""
+    per = collections.defaultdict(list)
+    for r in csv.DictReader(open(path)):
+        kn = r["Kernel_Name"].replace("void ", "")
+        if kn.startswith("tdda::" + kernel):
+            per[kn.split("(")[0]].append((int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) / 1e3)
+    total = 0.0
+    for kn, d in sorted(per.items()):
+        tail = d[-n:]
+        total += sum(tail) / len(tail)
+        print(f"`{kn}`: {len(d)} dispatches; last {len(tail)}: mean {sum(tail) / len(tail):.2f} us, "
+              f"min {min(tail):.2f} us, max {max(tail):.2f} us")
+    if len(per) > 1:
+        print(f"sum of the means of {len(per)} kernels: {total:.2f} us")
+
+
+LAST_ROW = {}
 
 
 def per_kernel(path, counter):
+    """Per-dispatch counter values of each kernel, in dispatch order; the row
+    of each kernel's last dispatch goes to LAST_ROW."""
     agg = collections.defaultdict(list)
-    for r in csv.DictReader(open(path)):
+    for k, r in enumerate(csv.DictReader(open(path))):
         if r["Counter_Name"] == counter:
             agg[r["Kernel_Name"]].append(float(r["Counter_Value"]) * 1024.0)
+            LAST_ROW[(counter, r["Kernel_Name"])] = k
     return agg
 
 
@@ -84,34 +104,55 @@ def traffic(fetch_csv, write_csv, precision, workload, copy_mib, out_json, last_
         n += 1
     wfac = copy_bytes / sum(cw[:n])
     ffac = copy_bytes / sum(cf[:n])
-    prob = W.topo80()[0] if workload == "topo80" else W.mixed128()
+    sys.path.insert(0, os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+    from bench import batch_bytes, pass_bytes
 
     class Core:
         pass
-    core = Core()
-    core.Nx, core.Ny, core.Nz, core.N = prob.Nx, prob.Ny, prob.Nz, prob.N
-    core.grid = tuple(next_smooth(2 * n - 1) for n in (prob.Nx, prob.Ny, prob.Nz))
-    sys.path.insert(0, os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
-    from bench import pass_bytes
-    alg = pass_bytes(core, s)
+
+    def core_of(p):
+        c = Core()
+        c.Nx, c.Ny, c.Nz, c.N = p.Nx, p.Ny, p.Nz, p.N
+        c.grid = tuple(next_smooth(2 * n - 1) for n in (p.Nx, p.Ny, p.Nz))
+        return c
+    if workload == "batch64":
+        alg = batch_bytes([(core_of(p), len(W.incidences())) for p in W.batch64(seed=4)], s)
+    else:
+        alg = pass_bytes(core_of(W.topo80()[0] if workload == "topo80" else W.mixed128()), s)
     print(f"calibration: device copy of {copy_mib} MiB in {n} dispatches; FETCH_SIZE x {ffac:.3f}, "
           f"WRITE_SIZE x {wfac:.3f}")
-    print("| pass | kernel | dispatches | fetch MB | write MB | measured MB | algorithmic MB | ratio |")
+    print("| pass | kernels | dispatches | fetch MB | write MB | measured MB | algorithmic MB | ratio |")
     print("|---|---|---|---|---|---|---|---|")
     res = {}
     for k, (pats, pname) in enumerate(PASS_NAMES.items()):
-        names = [kn for pat in pats for kn in fetch if f"tdda::{pat.format(t=t)}" in kn and f"<{t}" in kn]
+        # the first pattern with dispatches; a batch has one kernel per FFT
+        # length, whose per-dispatch bytes add up to one pass over the batch
+        names = []
+        for pat in pats:
+            names = [kn for kn in fetch if f"tdda::{pat.format(t=t)}" in kn and f"<{t}" in kn]
+            if names:
+                break
         if not names:
             continue
-        kn = names[0]
-        f = sum(fetch[kn]) / len(fetch[kn]) * ffac
-        w = sum(write.get(kn, [0.0])) / max(1, len(write.get(kn, []))) * wfac
-        res[pname] = int(f + w)
-        print(f"| {pname} | `{kn.split('(')[0].replace('void ', '')}` | {len(fetch[kn])} | {f / 1e6:.2f} | "
+        if workload != "batch64":
+            # one kernel per pass: the last one dispatched (the autotuner's
+            # trials of other kernels come first)
+            names = [max(names, key=lambda kn: LAST_ROW[("FETCH_SIZE", kn)])]
+        f = sum(sum(fetch[kn]) / len(fetch[kn]) for kn in names) * ffac
+        w = sum(sum(write.get(kn, [0.0])) / max(1, len(write.get(kn, []))) for kn in names) * wfac
+        res[f"{workload}_{precision}:{pname}"] = int(f + w)
+        label = names[0].split('(')[0].replace('void ', '') if len(names) == 1 else f"{len(names)} kernels"
+        print(f"| {pname} | `{label}` | {sum(len(fetch[kn]) for kn in names)} | {f / 1e6:.2f} | "
               f"{w / 1e6:.2f} | {(f + w) / 1e6:.2f} | {alg[k] / 1e6:.2f} | {(f + w) / alg[k]:.2f} |")
     if out_json:
+        # merged into the file's other workloads
+        prev = {}
+        if os.path.exists(out_json):
+            with open(out_json) as fh:
+                prev = json.load(fh)
+        prev.update(res)
         with open(out_json, "w") as fh:
-            json.dump(res, fh, indent=1)
+            json.dump(prev, fh, indent=1, sort_keys=True)
 
 
 def next_smooth(n):
