@@ -139,11 +139,13 @@ typedef struct {
  * their AMatrixMethod argument). */
 int tdda_op_create_ex(tdda_ctx ctx, const tdda_op_desc* desc, const tdda_op_ext* ext, tdda_op* out);
 /* Tile autotuner (no counterpart in the reference, which leaves tiling to
- * cuFFT): times the z passes with z tiles of 1, 1/2 and 1/4 of the planned
- * width, the y passes with each of their kernels (register-resident where
- * one exists for the length, LDS-staged) and the x pass with each of its
- * kernels at x tiles narrowed 3 columns at a time, each on a scratch model (reps launches per timing; <= 0: 20), and
- * keeps the fastest per precision.  Affects models created afterwards. */
+ * cuFFT): times the inverse z pass (P5) with z tiles of 1, 1/2 and 1/4 of the
+ * planned width and the forward z pass (P1) with 1 down to 1/8 of it, the y
+ * passes with each of their kernels (register-resident where one exists for
+ * the length, LDS-staged) and the x pass with each of its kernels at x tiles
+ * narrowed 3 columns at a time, each on a scratch model (reps launches per
+ * timing; <= 0: 20), and keeps the fastest per precision.  Affects models
+ * created afterwards. */
 int tdda_op_autotune(tdda_op op, int reps);
 /* Current tiles of one precision: out4 = z pencils per tile (P5), z pencils
  * per tile (P1), y columns per tile, x columns per tile. */
@@ -396,13 +398,40 @@ int tdda_server_get_stats(tdda_server s, tdda_server_stats* out);
  * nE[3] per right-hand side of m.  The design holds a reference to m. */
 int tdda_design_create(tdda_model m, const double* para_origin, double rfilter, const double* material, int nmat,
                        const double* inc, tdda_design* out);
-/* Para_origin += step (Nx * Ny host values; NULL: no step), clamped to
- * [0, 1]; Para_filtered = cone filter; Para = SmoothDensity(beta, eta); then
- * al of every right-hand side of the model (complex128 and, when present,
- * complex64 copies) from diel_old = Para of each site's column. */
+/* Any parameterization of SpacePara (SpacePara.h): the parameter of every
+ * site (geometryPara), the parameter values (Para: free and fixed), the
+ * free parameters (Free) and, with a filter, each free parameter's cone
+ * weights in SpacePara's order (FreeWeight, SpacePara.cpp:1232-1380: any
+ * radius, the periodic wrap, 4-fold shared parameters, fixed parameters
+ * among the weighted ones).  tdda_design_create is this with one free
+ * parameter per column of an extruded full box.                          */
+typedef struct {
+    int npara;                 /* Para values (free and fixed)                     */
+    int nfree;                 /* free parameters                                  */
+    const int* free_index;     /* nfree: index of each free parameter in Para      */
+    const int* site_param;     /* N: parameter of each site of the model's operator */
+    const double* para;        /* npara: initial Para (also Para_origin, Para_filtered) */
+    int filter;                /* 1: cone filter + SmoothDensity; 0: Para += step  */
+    const int* weight_start;   /* nfree + 1 offsets into weight_pos / weight_val   */
+    const int* weight_pos;     /* Para index of each weight                        */
+    const double* weight_val;  /* the weights                                      */
+} tdda_design_desc;
+/* Design state of model m for any parameterization (see tdda_design_desc);
+ * material / nmat / inc as tdda_design_create. */
+int tdda_design_create_ex(tdda_model m, const tdda_design_desc* desc, const double* material, int nmat,
+                          const double* inc, tdda_design* out);
+/* One CoreStructure::UpdateStr step (CoreStructure.cpp:31-104) for the
+ * nfree free parameters (step: nfree host values; NULL: no step): with a
+ * filter, Para_origin += step clamped to [0, 1], Para_filtered = weighted
+ * mean over the free parameter's weights, Para = SmoothDensity(beta, eta);
+ * without, Para += step clamped (beta and eta unused).  Then al of every
+ * right-hand side of the model (complex128 and, when present, complex64
+ * copies) from diel_old = Para of each site's parameter.  beta follows the
+ * caller's schedule (tdda_beta_update). */
 int tdda_design_step(tdda_design dz, const double* step, double beta, double eta);
-/* Copies Para_origin, Para_filtered and Para (each Nx * Ny; any may be
- * NULL). */
+/* Copies Para_origin, Para_filtered and Para (npara values each -- Nx * Ny
+ * for tdda_design_create; any may be NULL).  Without a filter only Para
+ * changes. */
 int tdda_design_get(tdda_design dz, double* para_origin, double* para_filtered, double* para);
 void tdda_design_destroy(tdda_design dz);
 /* FilterOption::update_beta (FilterOption.cpp:24-46): beta at iteration it
@@ -419,8 +448,8 @@ double tdda_beta_update(int kind, double it, double max_it, double beta_min, dou
  * model of the adjoint solve).
  * ---------------------------------------------------------------------- */
 /* ObjPointEDDAModel (ObjDDAModel.cpp:20-97): F = |E_ext - sum_i
- * A(r0 - d R_i) P_i| with E_ext = E0 nE exp(i K nK . r0); r0 in the units of
- * d (not scaled by d).  Source: -(1/F) sum_a A_ac conj(E_a). */
+ * A(r0 - d R_i) P_i| with E_ext = E0 nE exp(i K nK . r0); r0 is a physical
+ * position in the same length unit as d (not scaled by d).  Source: -(1/F) sum_a A_ac conj(E_a). */
 int tdda_objective_point_e(tdda_model m, int rhs, const double* r0, const double* nK, const double* nE, double E0,
                            double* value, tdda_model dst, int dst_rhs);
 /* ObjIntegratedEDDAModel (ObjDDAModel.cpp:100-177): F = log sum over the

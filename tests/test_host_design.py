@@ -7,57 +7,11 @@ weights in the same order (periodic wrap, 4-fold symmetry, fixed background
 parameters, a filter radius change) and the same design values bit for bit
 after several update steps."""
 import os
-import subprocess
 
-import numpy as np
 import pytest
 
+from design_cases import EXE, bulk, run_case
 from oracle import oracle as O
-
-ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-EXE = os.path.join(ROOT, "topo-dda-forwin64_amd", "dda_design_check")
-
-
-def bulk(xs, ys, zs):
-    return [(x, y, z) for x in xs for y in ys for z in zs]
-
-
-def case_text(Nx, Ny, Nz, bindz, structs, filt, sym, periodic, steps, seed):
-    """structs: [(para, sites)]; filt: (on, bmin, bmax, ita, type, [(it, r)]);
-    sym: (name, ax, ay); periodic: (on, Lx, Ly); steps: [(it, change)]."""
-    rng = np.random.default_rng(seed)
-    sites = [s for _, ss in structs for s in ss]
-    out = [f"{Nx} {Ny} {Nz} {bindz}", str(len(structs))]
-    for para, ss in structs:
-        out.append(f"{para} {len(ss)} " + " ".join(f"{x} {y} {z}" for x, y, z in ss))
-    vals = rng.random(len(sites))
-    out.append(str(len(sites)))
-    out.append(" ".join(f"{x} {y} {z}" for x, y, z in sites))
-    out.append(" ".join(repr(float(v)) for v in vals))
-    on, bmin, bmax, ita, kind, fl = filt
-    out.append(f"{int(on)} {bmin!r} {bmax!r} {ita!r} {kind} {len(fl)} " + " ".join(f"{i} {r!r}" for i, r in fl))
-    out.append(f"{sym[0]} {sym[1]!r} {sym[2]!r}")
-    out.append(f"{int(periodic[0])} {periodic[1]} {periodic[2]}")
-    return out, rng
-
-
-def run_case(tmp_path, name, Nx, Ny, Nz, bindz, structs, filt, sym, periodic, steps, nfree, seed=0):
-    lines, rng = case_text(Nx, Ny, Nz, bindz, structs, filt, sym, periodic, steps, seed)
-    lines.append(f"{len(steps)} {max(it for it, _ in steps) + 1}")
-    for it, change in steps:
-        lines.append(f"{it} {change} " + " ".join(repr(float(v)) for v in 0.2 * rng.standard_normal(nfree)))
-    case = tmp_path / f"{name}.txt"
-    case.write_text("\n".join(lines) + "\n")
-    ours, ref = tmp_path / f"{name}_ours.txt", tmp_path / f"{name}_ref.txt"
-    p = subprocess.run([EXE, str(case), str(ours)], capture_output=True, text=True, timeout=120)
-    assert p.returncode == 0, p.stderr
-    rc = O.ref_lib().ref_spacepara_case(str(case).encode(), str(ref).encode())
-    assert rc == 0
-    a, b = ours.read_text().split(), ref.read_text().split()
-    head = [int(x) for x in a[:3]]
-    assert head[2] == nfree, head
-    assert a == b, f"{name}: first difference at token {next(i for i, (x, y) in enumerate(zip(a, b)) if x != y)}"
-    return head
 
 
 needs_ref = pytest.mark.skipif(O.ref_lib() is None or not os.path.exists(EXE),

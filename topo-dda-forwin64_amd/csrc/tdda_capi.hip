@@ -143,6 +143,17 @@ int env_int(const char* name, int dflt)
 // ---------------------------------------------------------------------------
 // Handles are reference counted so that destroying them in any order is
 // safe: a model keeps its operator alive, an operator keeps its context.
+// Streams a pass fans its launch groups out over (besides the context's
+// own), so that the launches of different FFT lengths run concurrently and
+// fill each other's tails; fork and join are events on the main stream.
+constexpr int kAuxStreams = 3;
+struct AuxStreams {
+    int n = 0;
+    hipStream_t s[kAuxStreams] = {};
+    hipEvent_t fork = nullptr;
+    hipEvent_t join[kAuxStreams] = {};
+};
+
 struct tdda_ctx_s {
     int device;
     hipStream_t stream;
@@ -153,6 +164,7 @@ struct tdda_ctx_s {
     // stream-ordered allocation (request loop): hipFreeAsync does not
     // synchronise the device the way hipFree does
     bool async_alloc = false;
+    AuxStreams aux;
 };
 
 namespace {
@@ -199,7 +211,7 @@ struct TilePlan {
 
 // Upper limits on the planner's tile widths (0: none), set by the autotuner.
 struct TileCaps {
-    int pz = 0, tfy = 0, tfx = 0;
+    int pz = 0, pz1 = 0, tfy = 0, tfx = 0;  // pz: P5 (and P1 unless pz1), pz1: P1
     int xlds = 0;  // 1: P3 by the LDS-staged kernel even where k_pxr exists
     int xnt = 0;   // threads per workgroup of k_pxr (256 or 512; 0: the planner's choice)
     int ylds = 0;  // 1: P2 / P4 by the LDS-staged kernel even where k_pyr exists
@@ -257,6 +269,7 @@ struct JobTable {
     std::vector<PassGroup<T>> gz, gz1, gy, gx;  // P5, P1, P2/P4, P3
     SolverState* st = nullptr;
     double* part = nullptr;
+    const AuxStreams* aux = nullptr;  // the context's, for passes of several groups
 };
 
 struct tdda_model_s {
@@ -381,6 +394,7 @@ int plan_tiles(const tdda_op_s* op, TilePlan& t, const TileCaps& caps = TileCaps
         t.pz = std::min(t.pz, caps.pz);
         t.pz1 = std::min(t.pz1, caps.pz);
     }
+    if (caps.pz1 > 0) t.pz1 = std::min(t.pz1, caps.pz1);
     if (caps.tfy > 0) t.tfy = std::min(t.tfy, caps.tfy);
     if (caps.tfx > 0) t.tfx = std::max(3, std::min(t.tfx, (caps.tfx / 3) * 3));
     // register-resident x pass where its kernel exists (TDDA_XPASS=0 or the
@@ -500,6 +514,8 @@ void fill_jobs(const tdda_op_s* op, int nrhs, std::vector<JobDesc<T>>& out, cons
         J.xwide = tp.xwide;
         J.yreg = tp.yreg;
         J.ywide = tp.ywide;
+        J.gidx = j;
+        J.gsize = nrhs;
         J.cellmap = op->cellmap;
         J.green = green;
         J.twx = tw[0];
@@ -565,6 +581,7 @@ int build_groups(const tdda_ctx_s* c, JobTable<T>& jt, Put&& put)
     jt.tv = const_cast<int*>(put(hv.data(), (size_t)nj + 1));
     if (!jt.d || !jt.tv) return fail(TDDA_ERR_OOM, "job table upload");
     jt.nv = hv[nj];
+    jt.aux = &c->aux;
     // per pass kind: FFT length, pass kind of pass_length_compiled, tiles, LDS bytes
     auto build = [&](std::vector<PassGroup<T>>& groups, int kind, auto len, auto ntiles, auto lds) -> int {
         groups.clear();
@@ -651,30 +668,64 @@ int finalize_table(std::vector<void*>& arena, const tdda_ctx_s* c, JobTable<T>& 
     return build_groups(c, jt, put);
 }
 
+// Launches the groups of one pass: one group on the stream itself; several
+// spread over the stream and the context's auxiliary streams (largest
+// groups first, round robin), forked from and joined back into the stream,
+// so the pass is complete on the stream when the next work there starts.
+template <typename T, typename L>
+void fan_out(const JobTable<T>& jt, const std::vector<PassGroup<T>>& gs, hipStream_t s, L&& launch)
+{
+    const AuxStreams* a = jt.aux;
+    if (gs.size() <= 1 || !a || a->n == 0) {
+        for (const PassGroup<T>& g : gs) launch(g, s);
+        return;
+    }
+    std::vector<int> order(gs.size());
+    for (size_t i = 0; i < gs.size(); i++) order[i] = (int)i;
+    std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return gs[x].tiles > gs[y].tiles; });
+    const int nstreams = std::min<int>(a->n + 1, (int)gs.size());
+    (void)hipEventRecord(a->fork, s);
+    for (int k = 1; k < nstreams; k++) (void)hipStreamWaitEvent(a->s[k - 1], a->fork, 0);
+    for (size_t i = 0; i < order.size(); i++) {
+        const int k = (int)(i % nstreams);
+        launch(gs[order[i]], k == 0 ? s : a->s[k - 1]);
+    }
+    for (int k = 1; k < nstreams; k++) {
+        (void)hipEventRecord(a->join[k - 1], a->s[k - 1]);
+        (void)hipStreamWaitEvent(s, a->join[k - 1], 0);
+    }
+}
+
 template <typename T>
 void run_p1(const JobTable<T>& jt, int mode, hipStream_t s)
 {
-    for (const PassGroup<T>& g : jt.gz1) launch_p1<T>(g.L, mode, PassLaunch{dim3(g.tiles), g.lds, s, g.ncu}, g.d, g.ts, g.nj);
+    fan_out(jt, jt.gz1, s, [&](const PassGroup<T>& g, hipStream_t st) {
+        launch_p1<T>(g.L, mode, PassLaunch{dim3(g.tiles), g.lds, st, g.ncu}, g.d, g.ts, g.nj);
+    });
 }
 
 template <typename T>
 void run_p5(const JobTable<T>& jt, int mode, int diag, hipStream_t s)
 {
-    for (const PassGroup<T>& g : jt.gz)
-        launch_p5<T>(g.L, mode, diag, PassLaunch{dim3(g.tiles), g.lds, s, g.ncu}, g.d, g.ts, g.nj);
+    fan_out(jt, jt.gz, s, [&](const PassGroup<T>& g, hipStream_t st) {
+        launch_p5<T>(g.L, mode, diag, PassLaunch{dim3(g.tiles), g.lds, st, g.ncu}, g.d, g.ts, g.nj);
+    });
 }
 
 template <typename T>
 void run_py(const JobTable<T>& jt, bool inv, int solver, hipStream_t s)
 {
-    for (const PassGroup<T>& g : jt.gy)
-        launch_py<T>(g.L, inv, solver, PassLaunch{dim3(g.tiles), g.lds, s, g.ncu}, g.d, g.ts, g.nj);
+    fan_out(jt, jt.gy, s, [&](const PassGroup<T>& g, hipStream_t st) {
+        launch_py<T>(g.L, inv, solver, PassLaunch{dim3(g.tiles), g.lds, st, g.ncu}, g.d, g.ts, g.nj);
+    });
 }
 
 template <typename T>
 void run_px(const JobTable<T>& jt, int solver, hipStream_t s)
 {
-    for (const PassGroup<T>& g : jt.gx) launch_px<T>(g.L, solver, PassLaunch{dim3(g.tiles), g.lds, s, g.ncu}, g.d, g.ts, g.nj);
+    fan_out(jt, jt.gx, s, [&](const PassGroup<T>& g, hipStream_t st) {
+        launch_px<T>(g.L, solver, PassLaunch{dim3(g.tiles), g.lds, st, g.ncu}, g.d, g.ts, g.nj);
+    });
 }
 
 // one operator application: P1 .. P5
@@ -1040,6 +1091,22 @@ int tdda_ctx_create(int device, tdda_ctx* out)
         delete c;
         return fail(TDDA_ERR_HIP, std::string("hipStreamCreate: ") + hipGetErrorString(e));
     }
+    // auxiliary streams for the launch groups of a pass (default priority);
+    // TDDA_AUX_STREAMS=0 keeps every launch on the context's stream
+    const int naux = std::max(0, std::min(kAuxStreams, env_int("TDDA_AUX_STREAMS", kAuxStreams)));
+    if (naux > 0 && hipEventCreateWithFlags(&c->aux.fork, hipEventDisableTiming) == hipSuccess) {
+        for (int k = 0; k < naux; k++) {
+            if (hipStreamCreateWithFlags(&c->aux.s[k], hipStreamNonBlocking) != hipSuccess ||
+                hipEventCreateWithFlags(&c->aux.join[k], hipEventDisableTiming) != hipSuccess) {
+                (void)hipGetLastError();
+                if (c->aux.s[k]) (void)hipStreamDestroy(c->aux.s[k]);
+                c->aux.s[k] = nullptr;
+                break;
+            }
+            c->aux.n = k + 1;
+        }
+    }
+    (void)hipGetLastError();
     *out = c;
     return TDDA_OK;
 }
@@ -1049,6 +1116,12 @@ static void ctx_release(tdda_ctx ctx)
     if (--ctx->refs > 0) return;
     (void)hipSetDevice(ctx->device);
     (void)hipStreamSynchronize(ctx->stream);
+    for (int k = 0; k < ctx->aux.n; k++) {
+        (void)hipStreamSynchronize(ctx->aux.s[k]);
+        (void)hipStreamDestroy(ctx->aux.s[k]);
+        (void)hipEventDestroy(ctx->aux.join[k]);
+    }
+    if (ctx->aux.fork) (void)hipEventDestroy(ctx->aux.fork);
     (void)hipStreamDestroy(ctx->stream);
     delete ctx;
 }

@@ -94,6 +94,8 @@ struct JobDesc {
     int yreg;              // P2 / P4 run the register-resident kernel (k_pyr)
     int xwide;             // k_pxr with 512 threads per workgroup
     int ywide;             // k_pyr with 512 threads per workgroup
+    int gidx, gsize;       // place in its group of right-hand sides of one operator (consecutive jobs
+                           // in the table, equal tiles), whose x-pass tiles are interleaved; 0, 1 alone
     const int* cellmap;    // [Nx][Ny][Nz] -> site or -1; nullptr for the full box in bulk order
     const Cx<T>* green;
     const Cx<T>* twx;   // two-level twiddle tables (runtime-plan kernels)
@@ -540,10 +542,18 @@ __device__ __forceinline__ STile s_tile(const JobDesc<T>* __restrict__ jobs, con
     const JobDesc<T>* J = jobs + t.job;
     const int nfb = xpass ? J->nfx : J->nfy;
     const int tf = xpass ? J->tfx : J->tfy;
-    const int r = tl - tstart[t.job];
     int o, fb;
-    if (xpass) xtile_map(r, J->My, nfb, o, fb);
-    else ytile_map(r, J->Nx, nfb, o, fb);
+    if (xpass) {
+        // the x-pass tiles of a group of right-hand sides of one operator
+        // (consecutive jobs with equal tile counts) are interleaved
+        const int first = t.job - J->gidx;
+        int k;
+        xtile_map(tl - tstart[first], J->My, nfb, J->gsize, o, fb, k);
+        t.job = first + k;
+        J = jobs + t.job;
+    } else {
+        ytile_map(tl - tstart[t.job], J->Nx, nfb, o, fb);
+    }
     t.f0 = fb * tf;
     t.outer = o;
     t.nf = min(tf, J->Mz * 3 - t.f0);

@@ -306,61 +306,59 @@ static __global__ void __launch_bounds__(kThreads) k_request_fields(
     }
 }
 
-// Design update of an extruded structure: one value per (x, y) column,
-// column index x Ny + y.  Para_origin += step, clamped to [0, 1]
-// (CoreStructure.cpp:48-56).
-static __global__ void __launch_bounds__(kThreads) k_design_step(double* __restrict__ origin,
+// Design update (CoreStructure::UpdateStr, CoreStructure.cpp:31-104) over any
+// parameterization: free parameter i is Para index fi[i].  With a filter,
+// Para_origin += step, clamped to [0, 1] (CoreStructure.cpp:48-56); without,
+// Para itself.
+static __global__ void __launch_bounds__(kThreads) k_design_step(double* __restrict__ v,
+                                                              const int* __restrict__ fi,
                                                               const double* __restrict__ step, int n)
 {
     const int i = blockIdx.x * kThreads + threadIdx.x;
     if (i < n) {
-        double v = origin[i] + step[i];
-        v = v >= 1.0 ? 1.0 : v;
-        v = v <= 0.0 ? 0.0 : v;
-        origin[i] = v;
+        double x = v[fi[i]] + step[i];
+        x = x >= 1.0 ? 1.0 : x;
+        x = x <= 0.0 ? 0.0 : x;
+        v[fi[i]] = x;
     }
 }
 
-// Cone filter of the column values: the columns within dist^2 <= r^2 - 0.01
-// of a column, weighted r - dist (circlerange / calweight, SpacePara.cpp:37-48;
-// FreeWeight, SpacePara.cpp:1241-1270), normalised by the weights' sum and
-// summed in column order (CoreStructure.cpp:60-80), then projected by
-// SmoothDensity (FilterOption.cpp:49-60).  Contraction is off so that the
-// sums round like the reference's.
-static __global__ void __launch_bounds__(kThreads) k_design_filter(const double* __restrict__ origin, int Nx, int Ny,
-                                                                double r, int reach, double beta, double eta,
-                                                                double* __restrict__ filtered,
+// Cone filter of free parameter i: its weights (w, Para index) in
+// SpacePara's FreeWeight order (SpacePara.cpp:1232-1380), sum w origin /
+// sum w in that order (CoreStructure.cpp:60-80), then SmoothDensity
+// (FilterOption.cpp:49-60).  Contraction is off so that the sums round like
+// the reference's.
+static __global__ void __launch_bounds__(kThreads) k_design_filter(const double* __restrict__ origin,
+                                                                const int* __restrict__ fi,
+                                                                const int* __restrict__ ws,
+                                                                const int* __restrict__ wp,
+                                                                const double* __restrict__ wv, int n, double beta,
+                                                                double eta, double* __restrict__ filtered,
                                                                 double* __restrict__ para)
 {
 #pragma clang fp contract(off)
     const int i = blockIdx.x * kThreads + threadIdx.x;
-    if (i >= Nx * Ny) return;
-    const int xo = i / Ny, yo = i - xo * Ny;
+    if (i >= n) return;
     double num = 0.0, den = 0.0;
-    for (int x = max(0, xo - reach); x <= min(Nx - 1, xo + reach); x++) {
-        for (int y = max(0, yo - reach); y <= min(Ny - 1, yo + reach); y++) {
-            const double dx = double(x - xo), dy = double(y - yo);
-            const double d2 = dx * dx + dy * dy;
-            if (d2 <= r * r - 0.01) {
-                const double w = r - sqrt(d2);
-                num += w * origin[x * Ny + y];
-                den += w;
-            }
-        }
+    for (int k = ws[i]; k < ws[i + 1]; k++) {
+        const double w = wv[k];
+        num += w * origin[wp[k]];
+        den += w;
     }
     const double f = num / den;
-    filtered[i] = f;
+    filtered[fi[i]] = f;
     double p;
     if (f <= eta) p = eta * (exp(-beta * (1 - f / eta)) - (1 - f / eta) * exp(-beta));
     else p = (1 - eta) * (1 - exp(-beta * (f - eta) / (1 - eta)) + (f - eta) / (1 - eta) * exp(-beta)) + eta;
-    para[i] = p;
+    para[fi[i]] = p;
 }
 
-// al of every right-hand side of a full-box model from its column values
-// (DDAModel::UpdateAlpha, DDAModel.cpp:470-484; diel_old of site i is the
-// value of its column, CoreStructure.cpp:98-104).  inc: 6 doubles per
+// al of every right-hand side from the parameter of each site
+// (DDAModel::UpdateAlpha, DDAModel.cpp:470-484; diel_old of site i is
+// Para(geometryPara(i)), CoreStructure.cpp:98-104).  inc: 6 doubles per
 // right-hand side (nK[3], nE[3]).
-static __global__ void __launch_bounds__(kThreads) k_design_alpha(const double* __restrict__ para, int N, int Nz,
+static __global__ void __launch_bounds__(kThreads) k_design_alpha(const double* __restrict__ para,
+                                                               const int* __restrict__ site_param, int N,
                                                                const Cx<double>* __restrict__ mat, int nmat, double K,
                                                                double d, const double* __restrict__ inc, int nrhs,
                                                                Cx<double>* __restrict__ al)
@@ -371,7 +369,7 @@ static __global__ void __launch_bounds__(kThreads) k_design_alpha(const double* 
         const size_t j = g / n3;
         const size_t e = g - j * n3;
         const int i = (int)(e / 3);
-        al[g] = ldr_inv_alpha(interp_eps(para[i / Nz], mat, nmat), inc + 6 * j, K, d);
+        al[g] = ldr_inv_alpha(interp_eps(para[site_param[i]], mat, nmat), inc + 6 * j, K, d);
     }
 }
 

@@ -30,24 +30,32 @@ __host__ __device__ inline void ytile_map(int r, int nrows, int nfb, int& o, int
     fb = j;
 }
 
-// x pass: tile r of My x nfb -> (row ky, block fb).  Rows 1..npair pair with
-// My - 1 .. My - npair (npair = (My - 1) / 2); the unpaired rows 0 and, for
-// even My, My / 2 come last.
-__host__ __device__ inline void xtile_map(int r, int My, int nfb, int& ky, int& fb)
+// x pass: tile r of a group of R right-hand sides of one operator, each
+// My x nfb tiles -> (row ky, block fb, right-hand side k).  Rows 1..npair
+// pair with My - 1 .. My - npair (npair = (My - 1) / 2); all blocks of a
+// pair in all R right-hand sides go to one XCD, the R right-hand sides of a
+// (row, block) back to back, so that they read the same Green octant rows
+// from that XCD's L2.  The unpaired rows 0 and, for even My, My / 2 come
+// last.
+__host__ __device__ inline void xtile_map(int r, int My, int nfb, int R, int& ky, int& fb, int& k)
 {
-    const int npair = (My - 1) / 2, per = 2 * nfb, paired = npair * per;
+    const int npair = (My - 1) / 2, per = 2 * nfb * R, paired = npair * per;
     if (r < paired) {
         const int p0 = 8 * (r / (8 * per));
         const int m = npair - p0 < 8 ? npair - p0 : 8;
         const int rr = r - p0 * per;
         const int j = rr / m;
         const int pair = p0 + (rr - j * m);
-        fb = j >> 1;
-        ky = (j & 1) ? My - 1 - pair : pair + 1;
+        k = j % R;
+        const int jj = j / R;
+        fb = jj >> 1;
+        ky = (jj & 1) ? My - 1 - pair : pair + 1;
     } else {
         const int s = r - paired;
-        const int row = s >= nfb ? 1 : 0;
-        fb = s - row * nfb;
+        k = s % R;
+        const int ss = s / R;
+        const int row = ss >= nfb ? 1 : 0;
+        fb = ss - row * nfb;
         ky = row ? My / 2 : 0;
     }
 }

@@ -3,13 +3,17 @@
 
   python3 topo-dda-forwin64_amd/profile_summary.py stats  <kernel_stats.csv>  [--top 20]
   python3 topo-dda-forwin64_amd/profile_summary.py last <kernel_trace.csv> --kernel 'k_pxr<double, 160>' [--last 200]
+  python3 topo-dda-forwin64_amd/profile_summary.py span <kernel_trace.csv> --precision c64 --workload batch64 [--last 10]
   python3 topo-dda-forwin64_amd/profile_summary.py traffic <fetch counter_collection.csv>
           <write counter_collection.csv> --precision c128|c64 --workload topo80|mixed128|batch64
           [--json profiles/traffic.json]
 
 `stats` prints the per-kernel duration table (markdown).  `last` prints the
 mean, min and max duration of the last N dispatches of one kernel (the
-back-to-back timing launches that end a bench run).  `traffic` prints the
+back-to-back timing launches that end a bench run).  `span` prints, per
+pass, the wall span of each of the last N repetitions of the pass (first
+launch start to last launch end; what HIP events around the pass measure)
+and the summed kernel time.  `traffic` prints the
 average FETCH_SIZE / WRITE_SIZE per dispatch of each pass kernel, scaled by
 the calibration that the plain device copy in the same run gives (the copy
 of --copy-mib MiB run by profile_passes.py reads and writes exactly that
@@ -63,6 +67,44 @@ def last(path, kernel, n):
               f"min {min(tail):.2f} us, max {max(tail):.2f} us")
     if len(per) > 1:
         print(f"sum of the means of {len(per)} kernels: {total:.2f} us")
+
+
+def span(path, precision, workload, reps):
+    """Duration of one pass as a HIP-event pair around it sees it: the pass's
+    dispatches (a batch runs one launch per FFT length, spread over several
+    streams) are split into the last `reps` repetitions in start order, and
+    each repetition spans from its first start to its last end.  Also the sum
+    of the repetition's kernel durations (busy time, overlap counted twice)."""
+    t = "double" if precision == "c128" else "float"
+    rows = []
+    for r in csv.DictReader(open(path)):
+        kn = r["Kernel_Name"].replace("void ", "").split("(")[0]
+        rows.append((int(r["Start_Timestamp"]), int(r["End_Timestamp"]), kn))
+    rows.sort()
+    print("| pass | launches per pass | repetitions | span us (mean) | span us (min) | kernel busy us (mean) |")
+    print("|---|---|---|---|---|---|")
+    for pats, pname in PASS_NAMES.items():
+        names = set()
+        for pat in pats:
+            names = {kn for _, _, kn in rows if kn.startswith(f"tdda::{pat.format(t=t)}") and f"<{t}" in kn}
+            if names:
+                break
+        if not names:
+            continue
+        d = [r for r in rows if r[2] in names]
+        if workload != "batch64":
+            d = [r for r in d if r[2] == d[-1][2]]
+            g = 1
+        else:
+            g = len(names)
+        d = d[-reps * g:]
+        spans, busy = [], []
+        for k in range(0, len(d) - g + 1, g):
+            blk = d[k:k + g]
+            spans.append((max(e for _, e, _ in blk) - min(s for s, _, _ in blk)) / 1e3)
+            busy.append(sum(e - s for s, e, _ in blk) / 1e3)
+        print(f"| {pname} | {g} | {len(spans)} | {sum(spans) / len(spans):.2f} | {min(spans):.2f} | "
+              f"{sum(busy) / len(busy):.2f} |")
 
 
 LAST_ROW = {}
@@ -169,7 +211,7 @@ def next_smooth(n):
 
 def main():
     ap = argparse.ArgumentParser()
-    ap.add_argument("mode", choices=["stats", "traffic", "last"])
+    ap.add_argument("mode", choices=["stats", "traffic", "last", "span"])
     ap.add_argument("files", nargs="+")
     ap.add_argument("--top", type=int, default=20)
     ap.add_argument("--precision", default="c128")
@@ -184,6 +226,8 @@ def main():
         stats(a.files[0], a.top)
     elif a.mode == "last":
         last(a.files[0], a.kernel, a.last)
+    elif a.mode == "span":
+        span(a.files[0], a.precision, a.workload, a.last)
     else:
         traffic(a.files[0], a.files[1], a.precision, a.workload, a.copy_mib, a.json, a.last)
 

@@ -33,7 +33,7 @@ EXPORTS = [
     "tdda_polarizability", "tdda_local_field", "tdda_batch_create", "tdda_batch_destroy", "tdda_batch_size",
     "tdda_batch_solve", "tdda_batch_apply", "tdda_batch_time_passes", "tdda_batch_time_pass", "tdda_server_opts_default",
     "tdda_server_create", "tdda_server_destroy", "tdda_server_submit", "tdda_server_poll", "tdda_server_wait",
-    "tdda_server_result", "tdda_server_get_stats", "tdda_design_create", "tdda_design_step", "tdda_design_get",
+    "tdda_server_result", "tdda_server_get_stats", "tdda_design_create", "tdda_design_create_ex", "tdda_design_step", "tdda_design_get",
     "tdda_design_destroy", "tdda_beta_update", "tdda_op_create_periodic", "tdda_objective_point_e",
     "tdda_objective_integrated_e", "tdda_op_autotune", "tdda_op_tiles", "tdda_op_xpass", "tdda_op_create_ex",
     "tdda_green_tensor_fcd", "tdda_field_at_points", "tdda_write_structure", "tdda_write_model_results", "tdda_model_setup", "tdda_aproduct_noalloc",
@@ -148,6 +148,7 @@ def lib():
         L.tdda_server_get_stats.argtypes = [vp, ctypes.POINTER(ServerStats)]
         L.tdda_model_setup.argtypes = [vp, ctypes.POINTER(Request)]
         L.tdda_design_create.argtypes = [vp, dp, ctypes.c_double, dp, ctypes.c_int, dp, ctypes.POINTER(vp)]
+        L.tdda_design_create_ex.argtypes = [vp, ctypes.POINTER(DesignDesc), dp, ctypes.c_int, dp, ctypes.POINTER(vp)]
         L.tdda_design_step.argtypes = [vp, dp, ctypes.c_double, ctypes.c_double]
         L.tdda_design_get.argtypes = [vp, dp, dp, dp]
         L.tdda_design_destroy.argtypes = [vp]
@@ -594,36 +595,80 @@ def beta_update(kind, it, max_it, beta_min, beta_max):
     return lib().tdda_beta_update(kind, float(it), float(max_it), float(beta_min), float(beta_max))
 
 
+class DesignDesc(ctypes.Structure):
+    _ip = ctypes.POINTER(ctypes.c_int)
+    _dp = ctypes.POINTER(ctypes.c_double)
+    _fields_ = [("npara", ctypes.c_int), ("nfree", ctypes.c_int), ("free_index", _ip), ("site_param", _ip),
+                ("para", _dp), ("filter", ctypes.c_int), ("weight_start", _ip), ("weight_pos", _ip),
+                ("weight_val", _dp)]
+
+
 class Design:
-    """Device-resident design of an extruded full box: one parameter per
-    (x, y) column (index x * Ny + y) kept in HBM.  step() is
+    """Device-resident design kept in HBM.  step() is
     CoreStructure::UpdateStr (CoreStructure.cpp:31-104: add, clamp, cone
     filter, SmoothDensity) followed by DDAModel::UpdateAlpha
     (DDAModel.cpp:470-484) for every right-hand side of the model.
-    incidences: one (nK, nE) pair per right-hand side of the model."""
+    incidences: one (nK, nE) pair per right-hand side of the model.
 
-    def __init__(self, model: "DeviceModel", para_origin, rfilter, material, incidences):
+    Design(model, para_origin, rfilter, material, incidences): an extruded
+    full box, one parameter per (x, y) column (index x * Ny + y).
+    Design.general(...): any SpacePara parameterization."""
+
+    def __init__(self, model: "DeviceModel", para_origin, rfilter, material, incidences, _desc=None):
         self.model = model
-        po = np.ascontiguousarray(para_origin, dtype=np.float64)
         mat = np.ascontiguousarray(material, dtype=np.complex128)
         inc = np.ascontiguousarray([np.concatenate([np.asarray(k, float), np.asarray(e, float)])
                                     for k, e in incidences], dtype=np.float64).reshape(-1)
         if len(inc) != 6 * model.nrhs:
             raise TddaError("one (nK, nE) incidence per right-hand side of the model")
-        self.ncol = len(po)
         self.h = ctypes.c_void_p()
-        _check(lib().tdda_design_create(model.h, _dptr(po), float(rfilter), _dptr(mat.view(np.float64)), len(mat),
-                                        _dptr(inc), ctypes.byref(self.h)))
+        if _desc is None:
+            po = np.ascontiguousarray(para_origin, dtype=np.float64)
+            self.npara = self.nfree = len(po)
+            _check(lib().tdda_design_create(model.h, _dptr(po), float(rfilter), _dptr(mat.view(np.float64)),
+                                            len(mat), _dptr(inc), ctypes.byref(self.h)))
+        else:
+            d, keep = _desc
+            self.npara, self.nfree = d.npara, d.nfree
+            _check(lib().tdda_design_create_ex(model.h, ctypes.byref(d), _dptr(mat.view(np.float64)), len(mat),
+                                               _dptr(inc), ctypes.byref(self.h)))
+            del keep
+
+    @classmethod
+    def general(cls, model, site_param, para, free_index, material, incidences, weights=None):
+        """site_param: the parameter of each site of the model's operator
+        (SpacePara geometryPara); para: all parameter values; free_index: the
+        free parameters' indices in para; weights: None (no filter) or, per
+        free parameter, [(para index, weight), ...] in SpacePara's FreeWeight
+        order (tdda_design_create_ex)."""
+        ip = ctypes.POINTER(ctypes.c_int)
+        sp = np.ascontiguousarray(site_param, dtype=np.int32)
+        pa = np.ascontiguousarray(para, dtype=np.float64)
+        fi = np.ascontiguousarray(free_index, dtype=np.int32)
+        d = DesignDesc()
+        d.npara, d.nfree = len(pa), len(fi)
+        d.free_index = fi.ctypes.data_as(ip)
+        d.site_param = sp.ctypes.data_as(ip)
+        d.para = _dptr(pa)
+        keep = [sp, pa, fi]
+        if weights is not None:
+            ws = np.ascontiguousarray(np.concatenate([[0], np.cumsum([len(w) for w in weights])]), dtype=np.int32)
+            wp = np.ascontiguousarray([p for w in weights for p, _ in w], dtype=np.int32)
+            wv = np.ascontiguousarray([v for w in weights for _, v in w], dtype=np.float64)
+            d.filter = 1
+            d.weight_start, d.weight_pos, d.weight_val = ws.ctypes.data_as(ip), wp.ctypes.data_as(ip), _dptr(wv)
+            keep += [ws, wp, wv]
+        return cls(model, None, None, material, incidences, _desc=(d, keep))
 
     def step(self, step=None, beta=0.0, eta=0.5):
         st = None if step is None else np.ascontiguousarray(step, dtype=np.float64)
-        if st is not None and len(st) != self.ncol:
-            raise TddaError("step needs one value per column")
+        if st is not None and len(st) != self.nfree:
+            raise TddaError("step needs one value per free parameter")
         _check(lib().tdda_design_step(self.h, None if st is None else _dptr(st), float(beta), float(eta)))
 
     def get(self):
         """(Para_origin, Para_filtered, Para)."""
-        a, b, c = (np.zeros(self.ncol) for _ in range(3))
+        a, b, c = (np.zeros(self.npara) for _ in range(3))
         _check(lib().tdda_design_get(self.h, _dptr(a), _dptr(b), _dptr(c)))
         return a, b, c
 
