@@ -64,19 +64,29 @@ def log(msg):
 # ---------------------------------------------------------------------------
 def pass_bytes(core, s):
     """Algorithmic HBM bytes of the five passes of one application
-    Y = A P + al o P (one right-hand side), element size s bytes."""
+    Y = A P + al o P (one right-hand side), element size s bytes.  An
+    operator with fused z + y passes has no P1 / P4 launches: its P2 slot
+    holds P1 + P2 (read P, write w2) and its P5 slot P4 + P5 (read w2, al,
+    P; write Y) -- w1 never reaches HBM."""
     Nx, Ny, Nz = core.Nx, core.Ny, core.Nz
     Mx, My, Mz = core.grid
     n3 = 3 * core.N
     w1 = Nx * Ny * Mz * 3
     w2 = Nx * My * Mz * 3
     green = (Mx // 2 + 1) * (My // 2 + 1) * (Mz // 2 + 1) * 6
+    if core.zy_fused(core_prec(s)):
+        return np.array([0, n3 + w2, 2 * w2 + green, 0, w2 + 3 * n3], dtype=np.float64) * s
     return np.array([n3 + w1,              # P1: read P, write w1
                      w1 + w2,              # P2: read w1, write w2
                      2 * w2 + green,       # P3: read/write w2, read Green octant
                      w2 + w1,              # P4: read w2, write w1
                      w1 + 3 * n3],         # P5: read w1, al, P; write Y
                     dtype=np.float64) * s
+
+
+def core_prec(s):
+    import topodda as T
+    return T.C128 if s == 16 else T.C64
 
 
 def batch_bytes(cores_rhs, s):
@@ -97,7 +107,7 @@ def batch_bytes(cores_rhs, s):
 def pass_report(ms, b):
     ms = np.asarray(ms, dtype=np.float64)
     return {"ms": [round(float(x), 5) for x in ms],
-            "gbs": [round(float(bb / (t * 1e-3) / 1e9), 1) for bb, t in zip(b, ms)],
+            "gbs": [round(float(bb / (t * 1e-3) / 1e9), 1) if bb > 0 else None for bb, t in zip(b, ms)],
             "op_gbs": round(float(b.sum() / (ms.sum() * 1e-3) / 1e9), 1),
             "op_frac": round(float(b.sum() / (ms.sum() * 1e-3) / 1e9 / HBM_PEAK_GBS), 4)}
 

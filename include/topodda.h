@@ -152,9 +152,11 @@ int tdda_op_autotune(tdda_op op, int reps);
 int tdda_op_tiles(tdda_op op, int precision, int* out4);
 /* Pass kernels of one precision (no counterpart in the reference): bit 0
  * set when the x pass runs the register-resident kernel (else the
- * LDS-staged one), bit 1 likewise for the y passes; < 0 on error.  The
- * environment variables TDDA_XPASS=0 / TDDA_YPASS=0 make the planner keep
- * the LDS-staged kernels for operators created afterwards. */
+ * LDS-staged one), bit 1 likewise for the y passes, bit 2 when the z and y
+ * passes run fused (one workgroup per x slab, the z-transformed slab kept
+ * in LDS); < 0 on error.  The environment variables TDDA_XPASS=0 /
+ * TDDA_YPASS=0 / TDDA_FUSEZY=0 make the planner keep the LDS-staged or
+ * separate kernels for operators created afterwards. */
 int tdda_op_xpass(tdda_op op, int precision);
 /* AProductCore::~AProductCore (AProductCore.cpp:538-558). */
 int tdda_op_destroy(tdda_op op);

@@ -78,13 +78,18 @@ def test_tuned_model_alone_equals_batched(ctx):
 
 def test_tuner_keeps_its_y_pass_decision(ctx, monkeypatch):
     # TDDA_TUNE_YLDS=1 makes the LDS-staged y kernels win the y step; the
-    # x step that follows must not drop that decision
+    # x step that follows must not drop that decision.  The y step runs
+    # for separate z and y passes (TDDA_TUNE_NOZY=1 makes them win over the
+    # fused ones, which keep the register four-step)
     Nx, Ny, Nz, d, lam = 24, 20, 8, 10.0, 500.0
     core = T.AProductCore(ctx, None, Nx, Ny, Nz, d, lam)
     assert core.ypass(T.C128) == "register" and core.ypass(T.C64) == "register"
     monkeypatch.setenv("TDDA_TUNE_YLDS", "1")
+    monkeypatch.setenv("TDDA_TUNE_NOZY", "1")
     core.autotune(reps=5)
     monkeypatch.delenv("TDDA_TUNE_YLDS")
+    monkeypatch.delenv("TDDA_TUNE_NOZY")
+    assert not core.zy_fused(T.C128) and not core.zy_fused(T.C64)
     assert core.ypass(T.C128) == "lds" and core.ypass(T.C64) == "lds"
     R = O.build_a_bulk(Nx, Ny, Nz)
     b = np.random.default_rng(5).standard_normal(len(R)) + 0.5j

@@ -24,6 +24,7 @@
 #include "tdda_kernels.hpp"
 #include "tdda_setup.hpp"
 #include "tdda_launch.hpp"
+#include "tdda_zy_plan.hpp"
 
 using namespace tdda;
 
@@ -206,7 +207,8 @@ struct TilePlan {
     int xwide; // k_pxr with 512 threads per workgroup (tiles up to twice as wide)
     int yreg;  // P2 / P4 by the register-resident kernel k_pyr
     int ywide; // k_pyr with 512 threads per workgroup (factors above 16)
-    size_t lds_z, lds_z1, lds_y, lds_x;
+    int fzy;   // P1 + P2 and P4 + P5 by the fused kernels k_pzy / k_pyz (tdda_zy.hpp)
+    size_t lds_z, lds_z1, lds_y, lds_x, lds_zy;
 };
 
 // Upper limits on the planner's tile widths (0: none), set by the autotuner.
@@ -215,6 +217,7 @@ struct TileCaps {
     int xlds = 0;  // 1: P3 by the LDS-staged kernel even where k_pxr exists
     int xnt = 0;   // threads per workgroup of k_pxr (256 or 512; 0: the planner's choice)
     int ylds = 0;  // 1: P2 / P4 by the LDS-staged kernel even where k_pyr exists
+    int nozy = 0;  // 1: separate z and y passes even where the fused kernels fit
 };
 
 struct tdda_op_s {
@@ -266,7 +269,8 @@ struct JobTable {
     JobDesc<T>* d = nullptr;
     int* tv = nullptr;  // vector-tile starts, njobs+1
     int nv = 0;
-    std::vector<PassGroup<T>> gz, gz1, gy, gx;  // P5, P1, P2/P4, P3
+    std::vector<PassGroup<T>> gz, gz1, gy, gx;  // P5, P1, P2/P4, P3 (jobs with separate z and y passes)
+    std::vector<PassGroup<T>> gzy;              // fused P1 + P2 / P4 + P5 (jobs with fzy)
     SolverState* st = nullptr;
     double* part = nullptr;
     const AuxStreams* aux = nullptr;  // the context's, for passes of several groups
@@ -438,7 +442,24 @@ int plan_tiles(const tdda_op_s* op, TilePlan& t, const TileCaps& caps = TileCaps
         if (caps.tfx > 0) t.tfx = std::max(3, std::min(t.tfx, (caps.tfx / 3) * 3));
         t.lds_x = px_lds_slots<T>(t.xreg, op->Mx, op->pl[0].tb, t.tfx, t.ps_x) * s;
     }
-    if (std::max({t.lds_z, t.lds_z1, t.lds_y, t.lds_x}) > op->ctx->lds_budget)
+    // fused z + y passes where their kernels exist for (My, Mz) and a
+    // whole x slab fits in LDS with at least two workgroups per CU
+    // (TDDA_FUSEZY=0 or the autotuner's nozy cap keep the separate passes)
+    t.fzy = 0;
+    t.lds_zy = 0;
+    if (t.yreg && !t.ywide && zy_ylen_ok(op->My) && zy_zlen_ok<T>(op->Mz) && !caps.nozy &&
+        env_int("TDDA_FUSEZY", 1) != 0) {
+        const size_t lz = zy_lds_slots(op->My, op->Ny, op->Nz, op->Mz, t.tfy, t.ps_y) * s;
+        if (lds_resident_workgroups(lz + 512) >= 2 && lz <= op->ctx->lds_budget) {
+            t.fzy = 1;
+            t.lds_zy = lz;
+            // one z tile per slab: the fused inverse keeps one partial-sum
+            // slot per slab (k_finish counts Nx Ny / pz of them)
+            t.pz = t.pz1 = op->Ny;
+            t.lds_z = t.lds_z1 = 0;
+        }
+    }
+    if (std::max({t.lds_z, t.lds_z1, t.lds_y, t.lds_x, t.lds_zy}) > op->ctx->lds_budget)
         return fail(TDDA_ERR_UNSUPPORTED, "pass tile exceeds the LDS budget");
     return TDDA_OK;
 }
@@ -514,6 +535,7 @@ void fill_jobs(const tdda_op_s* op, int nrhs, std::vector<JobDesc<T>>& out, cons
         J.xwide = tp.xwide;
         J.yreg = tp.yreg;
         J.ywide = tp.ywide;
+        J.fzy = tp.fzy;
         J.gidx = j;
         J.gsize = nrhs;
         J.cellmap = op->cellmap;
@@ -583,19 +605,24 @@ int build_groups(const tdda_ctx_s* c, JobTable<T>& jt, Put&& put)
     jt.nv = hv[nj];
     jt.aux = &c->aux;
     // per pass kind: FFT length, pass kind of pass_length_compiled, tiles, LDS bytes
-    auto build = [&](std::vector<PassGroup<T>>& groups, int kind, auto len, auto ntiles, auto lds) -> int {
+    // kind: 0 z, 1 y, 2 x, 4 fused z + y; `fused`: the jobs the groups take
+    // (0: separate z and y passes, 1: fused, 2: all)
+    auto build = [&](std::vector<PassGroup<T>>& groups, int kind, int fused, auto len, auto ntiles,
+                     auto lds) -> int {
         groups.clear();
         std::vector<int> key(nj);
         for (int j = 0; j < nj; j++) {
             const int L = len(jt.h[j]);
             const bool reg = (kind == 2 && jt.h[j].xreg) || (kind == 1 && jt.h[j].yreg);
             const bool wide = (kind == 2 && jt.h[j].xwide) || (kind == 1 && jt.h[j].ywide);
-            key[j] = reg ? (L | kPxrKey | (wide ? kPxrWide : 0))
-                         : pass_length_compiled<T>(kind, L) ? L : 0;
+            key[j] = kind == 4 ? L
+                     : reg     ? (L | kPxrKey | (wide ? kPxrWide : 0))
+                     : pass_length_compiled<T>(kind, L) ? L : 0;
+            if (fused != 2 && (jt.h[j].fzy != 0) != (fused == 1)) key[j] = -1;  // not in these groups
         }
         std::vector<int> keys;
         for (int j = 0; j < nj; j++)
-            if (std::find(keys.begin(), keys.end(), key[j]) == keys.end()) keys.push_back(key[j]);
+            if (key[j] >= 0 && std::find(keys.begin(), keys.end(), key[j]) == keys.end()) keys.push_back(key[j]);
         for (int k : keys) {
             PassGroup<T> g;
             g.L = k;
@@ -622,14 +649,17 @@ int build_groups(const tdda_ctx_s* c, JobTable<T>& jt, Put&& put)
     };
     auto Lz = [](const JobDesc<T>& J) { return J.Mz; };
     auto tws = [](int kind, int L, int tb) { return pass_tw_slots(pass_length_compiled<T>(kind, L) && pass_full_twiddles<T>(kind, L), L, tb); };
-    CHK(build(jt.gz, 0, Lz, [](const JobTiles& t) { return t.z; },
+    CHK(build(jt.gz, 0, 0, Lz, [](const JobTiles& t) { return t.z; },
               [&](const JobDesc<T>& J) { return (size_t)(tws(0, J.Mz, J.plz.tb) + pz_buffers<T>(J.Mz) * 3 * J.pz * J.ps_z) * s; }));
-    CHK(build(jt.gz1, 0, Lz, [](const JobTiles& t) { return t.z1; },
+    CHK(build(jt.gz1, 0, 0, Lz, [](const JobTiles& t) { return t.z1; },
               [&](const JobDesc<T>& J) { return (size_t)(tws(0, J.Mz, J.plz.tb) + pz_buffers<T>(J.Mz) * 3 * J.pz1 * J.ps_z) * s; }));
-    CHK(build(jt.gy, 1, [](const JobDesc<T>& J) { return J.My; }, [](const JobTiles& t) { return t.y; },
+    CHK(build(jt.gy, 1, 0, [](const JobDesc<T>& J) { return J.My; }, [](const JobTiles& t) { return t.y; },
               [&](const JobDesc<T>& J) { return py_lds_slots<T>(J.yreg, J.My, J.ply.tb, J.tfy, J.ps_y) * s; }));
-    CHK(build(jt.gx, 2, [](const JobDesc<T>& J) { return J.Mx; }, [](const JobTiles& t) { return t.x; },
+    CHK(build(jt.gx, 2, 2, [](const JobDesc<T>& J) { return J.Mx; }, [](const JobTiles& t) { return t.x; },
               [&](const JobDesc<T>& J) { return px_lds_slots<T>(J.xreg, J.Mx, J.plx.tb, J.tfx, J.ps_x) * s; }));
+    // fused z + y: one tile per x slab
+    CHK(build(jt.gzy, 4, 1, [](const JobDesc<T>& J) { return J.My; }, [](const JobTiles& t) { return t.z; },
+              [&](const JobDesc<T>& J) { return zy_lds_slots(J.My, J.Ny, J.Nz, J.Mz, J.tfy, J.ps_y) * s; }));
     return TDDA_OK;
 }
 
@@ -672,28 +702,46 @@ int finalize_table(std::vector<void*>& arena, const tdda_ctx_s* c, JobTable<T>& 
 // spread over the stream and the context's auxiliary streams (largest
 // groups first, round robin), forked from and joined back into the stream,
 // so the pass is complete on the stream when the next work there starts.
-template <typename T, typename L>
-void fan_out(const JobTable<T>& jt, const std::vector<PassGroup<T>>& gs, hipStream_t s, L&& launch)
+//
+// A pass may take groups of two lists (gb may be empty): the separate y or
+// z pass of some jobs and the fused z + y pass of the others, which run
+// side by side.
+template <typename T, typename LA, typename LB>
+void fan_out2(const JobTable<T>& jt, const std::vector<PassGroup<T>>& ga, LA&& launch_a,
+              const std::vector<PassGroup<T>>& gb, LB&& launch_b, hipStream_t s)
 {
     const AuxStreams* a = jt.aux;
-    if (gs.size() <= 1 || !a || a->n == 0) {
-        for (const PassGroup<T>& g : gs) launch(g, s);
+    const size_t n = ga.size() + gb.size();
+    auto run = [&](size_t i, hipStream_t st) {
+        if (i < ga.size()) launch_a(ga[i], st);
+        else launch_b(gb[i - ga.size()], st);
+    };
+    if (n <= 1 || !a || a->n == 0) {
+        for (size_t i = 0; i < n; i++) run(i, s);
         return;
     }
-    std::vector<int> order(gs.size());
-    for (size_t i = 0; i < gs.size(); i++) order[i] = (int)i;
-    std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return gs[x].tiles > gs[y].tiles; });
-    const int nstreams = std::min<int>(a->n + 1, (int)gs.size());
+    auto tiles = [&](size_t i) { return i < ga.size() ? ga[i].tiles : gb[i - ga.size()].tiles; };
+    std::vector<size_t> order(n);
+    for (size_t i = 0; i < n; i++) order[i] = i;
+    std::stable_sort(order.begin(), order.end(), [&](size_t x, size_t y) { return tiles(x) > tiles(y); });
+    const int nstreams = std::min<int>(a->n + 1, (int)n);
     (void)hipEventRecord(a->fork, s);
     for (int k = 1; k < nstreams; k++) (void)hipStreamWaitEvent(a->s[k - 1], a->fork, 0);
     for (size_t i = 0; i < order.size(); i++) {
         const int k = (int)(i % nstreams);
-        launch(gs[order[i]], k == 0 ? s : a->s[k - 1]);
+        run(order[i], k == 0 ? s : a->s[k - 1]);
     }
     for (int k = 1; k < nstreams; k++) {
         (void)hipEventRecord(a->join[k - 1], a->s[k - 1]);
         (void)hipStreamWaitEvent(s, a->join[k - 1], 0);
     }
+}
+
+template <typename T, typename L>
+void fan_out(const JobTable<T>& jt, const std::vector<PassGroup<T>>& gs, hipStream_t s, L&& launch)
+{
+    static const std::vector<PassGroup<T>> none;
+    fan_out2(jt, gs, launch, none, launch, s);
 }
 
 template <typename T>
@@ -704,20 +752,39 @@ void run_p1(const JobTable<T>& jt, int mode, hipStream_t s)
     });
 }
 
+// P5 of the jobs with separate passes, and P4 + P5 of the fused jobs
 template <typename T>
 void run_p5(const JobTable<T>& jt, int mode, int diag, hipStream_t s)
 {
-    fan_out(jt, jt.gz, s, [&](const PassGroup<T>& g, hipStream_t st) {
-        launch_p5<T>(g.L, mode, diag, PassLaunch{dim3(g.tiles), g.lds, st, g.ncu}, g.d, g.ts, g.nj);
-    });
+    fan_out2(
+        jt, jt.gz,
+        [&](const PassGroup<T>& g, hipStream_t st) {
+            launch_p5<T>(g.L, mode, diag, PassLaunch{dim3(g.tiles), g.lds, st, g.ncu}, g.d, g.ts, g.nj);
+        },
+        jt.gzy,
+        [&](const PassGroup<T>& g, hipStream_t st) {
+            launch_pyz<T>(g.L, mode, mode == OUT_PLAIN ? diag : 1, PassLaunch{dim3(g.tiles), g.lds, st, g.ncu}, g.d,
+                          g.ts, g.nj);
+        },
+        s);
 }
 
+// P2 / P4 of the jobs with separate passes; forward (P2) also runs P1 + P2
+// of the fused jobs (`inmode`: their input mode, as P1's)
 template <typename T>
-void run_py(const JobTable<T>& jt, bool inv, int solver, hipStream_t s)
+void run_py(const JobTable<T>& jt, bool inv, int solver, hipStream_t s, int inmode = IN_PLAIN)
 {
-    fan_out(jt, jt.gy, s, [&](const PassGroup<T>& g, hipStream_t st) {
-        launch_py<T>(g.L, inv, solver, PassLaunch{dim3(g.tiles), g.lds, st, g.ncu}, g.d, g.ts, g.nj);
-    });
+    static const std::vector<PassGroup<T>> none;
+    fan_out2(
+        jt, jt.gy,
+        [&](const PassGroup<T>& g, hipStream_t st) {
+            launch_py<T>(g.L, inv, solver, PassLaunch{dim3(g.tiles), g.lds, st, g.ncu}, g.d, g.ts, g.nj);
+        },
+        inv ? none : jt.gzy,
+        [&](const PassGroup<T>& g, hipStream_t st) {
+            launch_pzy<T>(g.L, inmode, PassLaunch{dim3(g.tiles), g.lds, st, g.ncu}, g.d, g.ts, g.nj);
+        },
+        s);
 }
 
 template <typename T>
@@ -733,7 +800,7 @@ template <typename T>
 void launch_apply(const JobTable<T>& jt, int nj, int inmode, int outmode, int diag, int solver, hipStream_t s)
 {
     run_p1<T>(jt, inmode, s);
-    run_py<T>(jt, false, solver, s);
+    run_py<T>(jt, false, solver, s, inmode);
     run_px<T>(jt, solver, s);
     run_py<T>(jt, true, solver, s);
     run_p5<T>(jt, outmode, diag, s);

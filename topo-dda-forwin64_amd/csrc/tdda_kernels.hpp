@@ -94,6 +94,7 @@ struct JobDesc {
     int yreg;              // P2 / P4 run the register-resident kernel (k_pyr)
     int xwide;             // k_pxr with 512 threads per workgroup
     int ywide;             // k_pyr with 512 threads per workgroup
+    int fzy;               // P1 + P2 and P4 + P5 by the fused kernels k_pzy / k_pyz (one tile per x slab)
     int gidx, gsize;       // place in its group of right-hand sides of one operator (consecutive jobs
                            // in the table, equal tiles), whose x-pass tiles are interleaved; 0, 1 alone
     const int* cellmap;    // [Nx][Ny][Nz] -> site or -1; nullptr for the full box in bulk order

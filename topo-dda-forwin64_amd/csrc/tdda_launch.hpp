@@ -26,6 +26,14 @@ template <typename T>
 void launch_py(int L, bool inv, int solver, const PassLaunch& pl, const JobDesc<T>* jobs, const int* tstart, int nj);
 template <typename T>
 void launch_px(int L, int solver, const PassLaunch& pl, const JobDesc<T>* jobs, const int* tstart, int nj);
+// Fused z + y passes (tdda_zy.hpp) of an operator whose y length has a
+// register split up to 128 points (zy_ylen_ok) and whose z length is one of
+// the register DFT lengths (zy_zlen_ok): k_pzy replaces P1 and P2, k_pyz P4
+// and P5.
+template <typename T>
+void launch_pzy(int L, int mode, const PassLaunch& pl, const JobDesc<T>* jobs, const int* tstart, int nj);
+template <typename T>
+void launch_pyz(int L, int mode, int diag, const PassLaunch& pl, const JobDesc<T>* jobs, const int* tstart, int nj);
 // Launch-group key bits of the register-resident x and y passes: launch_px
 // runs k_pxr<L> for L | kPxrKey (with 512 threads per workgroup when
 // kPxrWide is also set), launch_py k_pyr<L>.

@@ -361,6 +361,13 @@ class AProductCore:
         _check(min(rc, 0))
         return "register" if rc & 2 else "lds"
 
+    def zy_fused(self, precision=C128):
+        """True when the z and y passes run fused (tdda_op_xpass bit 2): one
+        workgroup per x slab keeps the z-transformed slab in LDS."""
+        rc = lib().tdda_op_xpass(self.h, precision)
+        _check(min(rc, 0))
+        return bool(rc & 4)
+
     def Aproduct(self, b, al=None, precision=C128):
         """y = A_offdiag b (+ al o b); b is (3N,) or (nrhs, 3N) complex128."""
         b = np.ascontiguousarray(b, dtype=np.complex128)
