@@ -669,7 +669,7 @@ template <typename T>
 size_t job_part_size(const JobDesc<T>& J)
 {
     const JobTiles t = job_tiles(J);
-    return 4 * (size_t)std::max(t.z, t.v);
+    return kPart * (size_t)std::max(t.z, t.v);
 }
 
 // Completes a job table from its host descriptors: per-job solver state,
@@ -812,14 +812,23 @@ void launch_apply(const JobTable<T>& jt, int nj, int inmode, int outmode, int di
     }
 }
 
-// one BiCGSTAB iteration (or the unrolled first step when state.first)
+// one BiCGSTAB iteration (or the unrolled first step when state.first):
+// two operator applications and their finishing launches; the iteration's
+// updates of x and r are applied by the next iteration's P1 (or k_settle)
 template <typename T>
 void launch_iteration(const JobTable<T>& jt, int nj, hipStream_t s)
 {
     launch_apply<T>(jt, nj, IN_P, OUT_AP, 1, 1, s);
     launch_apply<T>(jt, nj, IN_T, OUT_AT, 1, 1, s);
-    k_update<T><<<dim3(jt.nv), dim3(kThreads), 0, s>>>(jt.d, jt.tv, nj);
-    k_finish<T, FIN_UPDATE><<<dim3(nj), dim3(kThreads), 0, s>>>(jt.d, nj);
+}
+
+// x and r of the jobs that have stopped with their last update pending
+// (after the iterations of a solve, before x or r is read)
+template <typename T>
+void launch_settle(const JobTable<T>& jt, int nj, hipStream_t s)
+{
+    k_settle<T><<<dim3(jt.nv), dim3(kThreads), 0, s>>>(jt.d, jt.tv, nj);
+    k_finish<T, FIN_SETTLE><<<dim3(nj), dim3(kThreads), 0, s>>>(jt.d, nj);
 }
 
 // The job tables and host buffers one solve runs on: a model's own, or the
@@ -949,6 +958,8 @@ int solve_fp64(SolveSet& S, const tdda_solve_opts* o, tdda_solve_info* info)
     }
     CHK(check_launch());
     CHK(run_iterations<double>(S, *S.j64, hst, o->max_it, o->check_every));
+    launch_settle<double>(*S.j64, nj, S.s);
+    CHK(check_launch());
     for (int j = 0; j < nj; j++) {
         info[j].status = hst[j].status;
         info[j].iterations = hst[j].iter;
@@ -1016,6 +1027,7 @@ int solve_mixed(SolveSet& S, const tdda_solve_opts* o, tdda_solve_info* info)
         launch_init_zero<float>(*S.j32, nj, s);
         CHK(check_launch());
         CHK(run_iterations<float>(S, *S.j32, hin, o->inner_max_it, o->check_every));
+        launch_settle<float>(*S.j32, nj, s);
         for (int j = 0; j < nj; j++) {
             if (!active[j]) continue;
             inner_it[j] += hin[j].iter;

@@ -73,11 +73,22 @@ enum SolveStatus : int {
 
 // Per right-hand-side BiCGSTAB scalars (always FP64, also for the complex64
 // inner solver) -- the scalars of DDAModel::bicgstab, DDAModel.cpp:145-160.
+//
+// One iteration runs in two operator applications with their finishing
+// launches: P1 forms p (and first applies the previous iteration's pending
+// updates of x and r), P5 forms Ap with r0.Ap -> alpha; P1 forms t, P5 forms
+// At with the dot products from which the finishing launch takes zeta, the
+// new rho = r0.r and |r| of r = t - zeta At without r itself.  `pend` marks
+// x += alpha p + zeta t, r = t - zeta At as pending: the next P1 applies it,
+// or k_settle once the job has stopped.
 struct SolverState {
     double alpha[2], zeta[2], rho[2], rho_prev[2];
     double enorm, rnorm, err, tol;
-    int iter, maxit, status, first;
+    int iter, maxit, status, first, pend;
 };
+
+// Partial sums per tile of a reducing pass (the At pass needs eight).
+constexpr int kPart = 8;
 
 template <typename T>
 struct JobDesc {
@@ -251,7 +262,7 @@ __device__ __forceinline__ void block_sum(double* v, double* red)
     }
 }
 
-// Stores this tile's four partial sums (valid in thread 0 after block_sum).
+// Stores this tile's kPart partial sums (valid in thread 0 after block_sum).
 // The totals are formed by k_finish in a separate launch: a last-workgroup
 // ticket inside the pass would need a device-scope release fence per tile,
 // which on a multi-XCD GPU writes back the XCD's L2 each time.
@@ -259,13 +270,31 @@ __device__ __forceinline__ void store_partials(double* part, int tile, const dou
 {
     if (threadIdx.x == 0) {
 #pragma unroll
-        for (int k = 0; k < 4; k++) *(TDDA_GLOBAL double*)(part + tile * 4 + k) = v[k];
+        for (int k = 0; k < kPart; k++) *(TDDA_GLOBAL double*)(part + tile * kPart + k) = v[k];
     }
 }
 
 __device__ __forceinline__ bool job_inactive(const SolverState* st)
 {
     return st != nullptr && gstate(st)->status != ST_RUNNING;
+}
+
+// Dot products of the At pass: conj(At) t, conj(At) At (zeta,
+// DDAModel.cpp:225), and |t|^2, conj(r0) t, conj(r0) At, from which the
+// finishing launch forms r0.r and |r|^2 of r = t - zeta At (DDAModel.cpp:
+// 230-232) without a pass over r.
+template <typename T>
+__device__ __forceinline__ void at_sums(double* acc, Cx<T> at, Cx<T> tv, Cx<T> r0)
+{
+    const double2 a = to_d(at), b = to_d(tv), c = to_d(r0);
+    acc[0] += a.x * b.x + a.y * b.y;  // conj(At) t
+    acc[1] += a.x * b.y - a.y * b.x;
+    acc[2] += a.x * a.x + a.y * a.y;  // conj(At) At
+    acc[3] += b.x * b.x + b.y * b.y;  // |t|^2
+    acc[4] += c.x * b.x + c.y * b.y;  // conj(r0) t
+    acc[5] += c.x * b.y - c.y * b.x;
+    acc[6] += c.x * a.x + c.y * a.y;  // conj(r0) At
+    acc[7] += c.x * a.y - c.y * a.x;
 }
 
 // ---------------------------------------------------------------------------
@@ -570,7 +599,8 @@ __global__ void TDDA_PASS_BOUNDS k_p1(const JobDesc<T>* __restrict__ jobs, const
     extern __shared__ __align__(16) char smem[];
     constexpr bool FT = LC > 0 && pass_full_twiddles<T>(0, LC);
     constexpr bool IP = LC > 0 && pz_inplace<T>(LC);  // one LDS buffer, in-place transforms
-    constexpr int NV = (MODE == IN_P) ? 3 : ((MODE == IN_T) ? 2 : 1);
+    // IN_P: t, At, x, p, Ap (the pending update, then p); the first step: r
+    constexpr int NV = (MODE == IN_P) ? 5 : ((MODE == IN_T) ? 2 : 1);
     constexpr int EP = ept_len<T>(LC) / 3;
     const int ntot = tstart[njobs];
     Pre<T, EP> pr[NV];
@@ -585,7 +615,7 @@ __global__ void TDDA_PASS_BOUNDS k_p1(const JobDesc<T>* __restrict__ jobs, const
             Cx<T>* twl = reinterpret_cast<Cx<T>*>(smem);
             Cx<T>* A = twl + pass_tw_slots(FT, Mz, J->plz.tb);
             load_pass_twiddles<T, FT>(twl, J->twz, J->twfz, J->plz);
-            Cx<T> sa{T(0), T(0)}, sb{T(0), T(0)};
+            Cx<T> sa{T(0), T(0)}, sb{T(0), T(0)}, sc{T(0), T(0)};
             bool first = false;
             if (MODE == IN_P) {
                 const TDDA_GLOBAL SolverState* st = gstate(J->st);
@@ -597,6 +627,7 @@ __global__ void TDDA_PASS_BOUNDS k_p1(const JobDesc<T>* __restrict__ jobs, const
                     cdiv(st->rho[0], st->rho[1], st->rho_prev[0], st->rho_prev[1], q2r, q2i);
                     sa = from_d<T>(q1r * q2r - q1i * q2i, q1r * q2i + q1i * q2r);
                     sb = from_d<T>(st->zeta[0], st->zeta[1]);
+                    sc = from_d<T>(st->alpha[0], st->alpha[1]);
                 }
             } else if (MODE == IN_T) {
                 sa = from_d<T>(gstate(J->st)->alpha[0], gstate(J->st)->alpha[1]);
@@ -621,8 +652,18 @@ __global__ void TDDA_PASS_BOUNDS k_p1(const JobDesc<T>* __restrict__ jobs, const
                         idx = base * 3 + e;
                     }
                     if (MODE == IN_P) {
-                        // p = r + beta (p - u), u = zeta Ap   (DDAModel.cpp:219)
-                        if (!first) v = cadd(pr[0].get(k), cmul(sa, csub(pr[1].get(k), cmul(sb, pr[2].get(k)))));
+                        if (!first) {
+                            // the previous iteration's P += alpha p + z (z = zeta r - alpha u =
+                            // zeta t) and r = t - zeta At (DDAModel.cpp:226-230), then
+                            // p = r + beta (p - u), u = zeta Ap   (DDAModel.cpp:219)
+                            const Cx<T> tv = pr[0].get(k), pv = pr[3].get(k);
+                            const Cx<T> rn = csub(tv, cmul(sb, pr[1].get(k)));
+                            v = cadd(rn, cmul(sa, csub(pv, cmul(sb, pr[4].get(k)))));
+                            if (idx >= 0) {
+                                gst(J->r + idx, rn);
+                                gst(J->x + idx, cadd(pr[2].get(k), cadd(cmul(sc, pv), cmul(sb, tv))));
+                            }
+                        }
                         if (idx >= 0) gst(J->p + idx, v);
                     } else {
                         // t = r - alpha Ap   (DDAModel.cpp:222)
@@ -645,6 +686,7 @@ __global__ void TDDA_PASS_BOUNDS k_p1(const JobDesc<T>* __restrict__ jobs, const
             nxt = z_tile(jobs, tstart, njobs, tl, 1, MODE != IN_PLAIN);
             if (nxt.live) {
                 const JobDesc<T>* J = jobs + nxt.job;
+                const bool pfirst = MODE == IN_P && gstate(J->st)->first != 0;
                 const int Nz = J->Nz;
                 const int nin = nxt.np * Nz * 3;
                 const int base = nxt.q0 * Nz;
@@ -671,9 +713,15 @@ __global__ void TDDA_PASS_BOUNDS k_p1(const JobDesc<T>* __restrict__ jobs, const
                     if (MODE == IN_PLAIN) pr[0].set(k, gld(J->in + idx));
                     else if (MODE == IN_X) pr[0].set(k, gld(J->x + idx));
                     else if (MODE == IN_P) {
-                        pr[0].set(k, gld(J->r + idx));
-                        pr[1].set(k, gld(J->p + idx));
-                        pr[2].set(k, gld(J->ap + idx));
+                        if (pfirst) {
+                            pr[0].set(k, gld(J->r + idx));
+                        } else {
+                            pr[0].set(k, gld(J->t + idx));
+                            pr[1].set(k, gld(J->at + idx));
+                            pr[2].set(k, gld(J->x + idx));
+                            pr[3].set(k, gld(J->p + idx));
+                            pr[4].set(k, gld(J->ap + idx));
+                        }
                     } else {
                         pr[0].set(k, gld(J->r + idx));
                         pr[1].set(k, gld(J->ap + idx));
@@ -1161,7 +1209,7 @@ __global__ void TDDA_PASS_BOUNDS k_p5(const JobDesc<T>* __restrict__ jobs, const
     extern __shared__ __align__(16) char smem[];
     constexpr bool FT = LC > 0 && pass_full_twiddles<T>(0, LC);
     constexpr bool IP = LC > 0 && pz_inplace<T>(LC);  // one LDS buffer, in-place transforms
-    __shared__ double red[16 * 4];
+    __shared__ double red[16 * kPart];
     constexpr int EP = ept_len<T>(LC);
     const int ntot = tstart[njobs];
     Pre<T, EP> pr;
@@ -1210,7 +1258,7 @@ __global__ void TDDA_PASS_BOUNDS k_p5(const JobDesc<T>* __restrict__ jobs, const
             const Cx<T>* res;
             if constexpr (IP) res = ct_fft_ip<T, LC, true, np_max<T>(LC), kPruneOut, FT>(A, cur.np * 3, twl);
             else res = pass_fft<T, LC, true, kPruneOut, FT>(A, B, cur.np * 3, ps, J->plz, twl);
-            double acc[4] = {0.0, 0.0, 0.0, 0.0};
+            double acc[kPart] = {};
             const int nout = cur.np * Nz * 3;
             const int base = cur.q0 * Nz;
             RowWalk w(Nz * 3, 1.0f / (float)(Nz * 3));
@@ -1251,14 +1299,11 @@ __global__ void TDDA_PASS_BOUNDS k_p5(const JobDesc<T>* __restrict__ jobs, const
                     const Cx<T> tv = gld(J->t + i);
                     const Cx<T> at = cadd(conv, cmul(gld(J->al + i), tv));
                     gst(J->at + i, at);
-                    const double2 a = to_d(at), b = to_d(tv);
-                    acc[0] += a.x * b.x + a.y * b.y;  // conj(At) t
-                    acc[1] += a.x * b.y - a.y * b.x;
-                    acc[2] += a.x * a.x + a.y * a.y;  // conj(At) At
+                    at_sums(acc, at, tv, gld(J->r0 + i));
                 }
             }
             if (MODE != OUT_PLAIN) {
-                block_sum<4>(acc, red);
+                block_sum<kPart>(acc, red);
                 store_partials(J->part, cur.jt, acc);
             }
             __syncthreads();
@@ -1279,13 +1324,13 @@ template <typename T>
 __global__ void __launch_bounds__(kThreads) k_init_zero(const JobDesc<T>* __restrict__ jobs,
                                                         const int* __restrict__ tstart, int njobs)
 {
-    __shared__ double red[16 * 4];
+    __shared__ double red[16 * kPart];
     const int job = find_job(tstart, njobs, blockIdx.x);
     const JobDesc<T>& J = jobs[job];
     if (job_inactive(J.st)) return;
     const int tile = blockIdx.x - tstart[job];
     const int n = J.N * 3;
-    double acc[4] = {0.0, 0.0, 0.0, 0.0};
+    double acc[kPart] = {};
     const int i0 = tile * kThreads * kVecPer;
     for (int k = 0; k < kVecPer; k++) {
         const int i = i0 + k * kThreads + threadIdx.x;
@@ -1298,68 +1343,61 @@ __global__ void __launch_bounds__(kThreads) k_init_zero(const JobDesc<T>* __rest
             acc[0] += d.x * d.x + d.y * d.y;
         }
     }
-    block_sum<4>(acc, red);
+    block_sum<kPart>(acc, red);
     store_partials(J.part, tile, acc);
 }
 
-// u = zeta Ap; z = zeta r - alpha u; P += alpha p + z; rl = r; r = t - zeta At
-// (DDAModel.cpp:226-230) with the stop test of DDAModel.cpp:232 and the
-// breakdown test of DDAModel.cpp:193 for the next iteration.
+// The pending update of a job that has stopped (converged, broken down or
+// out of iterations): P += alpha p + z with z = zeta t, r = t - zeta At
+// (DDAModel.cpp:226-230).  A running job's pending update is applied by its
+// next P1; the finishing launch FIN_SETTLE then clears the flag.
 template <typename T>
-__global__ void __launch_bounds__(kThreads) k_update(const JobDesc<T>* __restrict__ jobs,
+__global__ void __launch_bounds__(kThreads) k_settle(const JobDesc<T>* __restrict__ jobs,
                                                      const int* __restrict__ tstart, int njobs)
 {
-    __shared__ double red[16 * 4];
     const int job = find_job(tstart, njobs, blockIdx.x);
     const JobDesc<T>& J = jobs[job];
-    if (job_inactive(J.st)) return;
+    const TDDA_GLOBAL SolverState* stc = gstate(J.st);
+    if (stc->status == ST_RUNNING || !stc->pend) return;
     const int tile = blockIdx.x - tstart[job];
     const int n = J.N * 3;
-    const TDDA_GLOBAL SolverState* stc = gstate(J.st);
     const Cx<T> al = from_d<T>(stc->alpha[0], stc->alpha[1]);
     const Cx<T> ze = from_d<T>(stc->zeta[0], stc->zeta[1]);
-    double acc[4] = {0.0, 0.0, 0.0, 0.0};
     const int i0 = tile * kThreads * kVecPer;
     for (int k = 0; k < kVecPer; k++) {
         const int i = i0 + k * kThreads + threadIdx.x;
         if (i < n) {
-            const Cx<T> r = gld(J.r + i);
-            const Cx<T> u = cmul(ze, gld(J.ap + i));
-            const Cx<T> z = csub(cmul(ze, r), cmul(al, u));
-            gst(J.x + i, cadd(cadd(gld(J.x + i), cmul(al, gld(J.p + i))), z));
-            const Cx<T> rn = csub(gld(J.t + i), cmul(ze, gld(J.at + i)));
-            gst(J.r + i, rn);
-            const double2 a = to_d(gld(J.r0 + i)), b = to_d(rn);
-            acc[0] += b.x * b.x + b.y * b.y;
-            acc[1] += a.x * b.x + a.y * b.y;
-            acc[2] += a.x * b.y - a.y * b.x;
+            const Cx<T> tv = gld(J.t + i);
+            gst(J.r + i, csub(tv, cmul(ze, gld(J.at + i))));
+            gst(J.x + i, cadd(gld(J.x + i), cadd(cmul(al, gld(J.p + i)), cmul(ze, tv))));
         }
     }
-    block_sum<4>(acc, red);
-    store_partials(J.part, tile, acc);
 }
 
 // Totals of the partial sums of one reducing pass and the scalar step that
 // follows it.  One workgroup per job; the per-tile partials are summed in a
 // fixed order (thread-strided, then block_sum), so results do not depend on
 // which other jobs share the launches.
-enum { FIN_RES = 0, FIN_AP = 1, FIN_AT = 2, FIN_INIT = 3, FIN_UPDATE = 4 };
+enum { FIN_RES = 0, FIN_AP = 1, FIN_AT = 2, FIN_INIT = 3, FIN_SETTLE = 4 };
 
 template <typename T, int KIND>
 __global__ void __launch_bounds__(kThreads) k_finish(const JobDesc<T>* __restrict__ jobs, int njobs)
 {
-    __shared__ double red[16 * 4];
+    __shared__ double red[16 * kPart];
     const JobDesc<T>& J = jobs[blockIdx.x];
+    if (KIND == FIN_SETTLE) {
+        if (threadIdx.x == 0 && J.st->status != ST_RUNNING) J.st->pend = 0;
+        return;
+    }
     if (job_inactive(J.st)) return;
-    const int ntiles = (KIND == FIN_INIT || KIND == FIN_UPDATE)
-                           ? (int)((3 * (size_t)J.N + kThreads * kVecPer - 1) / (kThreads * kVecPer))
-                           : (J.Nx * J.Ny + J.pz - 1) / J.pz;
-    double acc[4] = {0.0, 0.0, 0.0, 0.0};
+    const int ntiles = KIND == FIN_INIT ? (int)((3 * (size_t)J.N + kThreads * kVecPer - 1) / (kThreads * kVecPer))
+                                        : (J.Nx * J.Ny + J.pz - 1) / J.pz;
+    double acc[kPart] = {};
     for (int i = threadIdx.x; i < ntiles; i += kThreads) {
 #pragma unroll
-        for (int k = 0; k < 4; k++) acc[k] += *(const TDDA_GLOBAL double*)(J.part + i * 4 + k);
+        for (int k = 0; k < kPart; k++) acc[k] += *(const TDDA_GLOBAL double*)(J.part + i * kPart + k);
     }
-    block_sum<4>(acc, red);
+    block_sum<kPart>(acc, red);
     if (threadIdx.x != 0) return;
     SolverState* st = J.st;
     const double* tot = acc;
@@ -1371,20 +1409,27 @@ __global__ void __launch_bounds__(kThreads) k_finish(const JobDesc<T>* __restric
     } else if (KIND == FIN_AP) {
         // alpha = r0.r / r0.Ap   (DDAModel.cpp:168, 221)
         cdiv(st->rho[0], st->rho[1], tot[0], tot[1], st->alpha[0], st->alpha[1]);
-    } else if (KIND == FIN_AT) {
-        // zeta = At.t / At.At   (DDAModel.cpp:172, 225)
-        cdiv(tot[0], tot[1], tot[2], 0.0, st->zeta[0], st->zeta[1]);
     } else if (KIND == FIN_INIT) {
         st->rho[0] = tot[0];
         st->rho[1] = 0.0;
         st->enorm = sqrt(tot[0]);
         st->rnorm = sqrt(tot[0]);
-    } else {
+    } else {  // FIN_AT
+        // zeta = At.t / At.At   (DDAModel.cpp:172, 225)
+        double zr, zi;
+        cdiv(tot[0], tot[1], tot[2], 0.0, zr, zi);
+        st->zeta[0] = zr;
+        st->zeta[1] = zi;
+        // r = t - zeta At (DDAModel.cpp:230), pending: rl = r, then
+        // r0.r = r0.t - zeta r0.At and
+        // |r|^2 = |t|^2 - 2 Re(conj(zeta) At.t) + |zeta|^2 At.At
+        st->pend = 1;
         st->rho_prev[0] = st->rho[0];
         st->rho_prev[1] = st->rho[1];
-        st->rho[0] = tot[1];
-        st->rho[1] = tot[2];
-        st->rnorm = sqrt(tot[0]);
+        st->rho[0] = tot[4] - (zr * tot[6] - zi * tot[7]);
+        st->rho[1] = tot[5] - (zr * tot[7] + zi * tot[6]);
+        const double rr = tot[3] - 2.0 * (zr * tot[0] + zi * tot[1]) + (zr * zr + zi * zi) * tot[2];
+        st->rnorm = sqrt(fmax(rr, 0.0));
         st->err = st->rnorm / st->enorm;
         if (st->first) {
             st->first = 0;

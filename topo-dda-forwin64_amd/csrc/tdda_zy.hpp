@@ -123,7 +123,7 @@ __global__ void __launch_bounds__(kThreads, (zy_waves<T, false>())) k_pzy(const 
         const int Ny = J->Ny, Nz = J->Nz, Mz = J->Mz, F = 3 * Mz, rs = zy_row_stride(Mz);
         Cx<T>* S = ZB + Ny * rs;
         // 1. the slab's inputs -> S[y][z][c], with the fused vector updates
-        Cx<T> sa{T(0), T(0)}, sb{T(0), T(0)};
+        Cx<T> sa{T(0), T(0)}, sb{T(0), T(0)}, sc{T(0), T(0)};
         bool first = false;
         if (mode == IN_P) {
             const TDDA_GLOBAL SolverState* st = gstate(J->st);
@@ -135,27 +135,35 @@ __global__ void __launch_bounds__(kThreads, (zy_waves<T, false>())) k_pzy(const 
                 cdiv(st->rho[0], st->rho[1], st->rho_prev[0], st->rho_prev[1], q2r, q2i);
                 sa = from_d<T>(q1r * q2r - q1i * q2i, q1r * q2i + q1i * q2r);
                 sb = from_d<T>(st->zeta[0], st->zeta[1]);
+                sc = from_d<T>(st->alpha[0], st->alpha[1]);
             }
         } else if (mode == IN_T) {
             sa = from_d<T>(gstate(J->st)->alpha[0], gstate(J->st)->alpha[1]);
         }
         const int nin = Ny * Nz * 3;
         const int base = x * Ny * Nz;  // cell index of (x, 0, 0)
-        const Cx<T>* va = mode == IN_PLAIN ? J->in : mode == IN_X ? J->x : J->r;
-        const Cx<T>* vb = mode == IN_P ? J->p : J->ap;
-        const bool two = mode == IN_T || (mode == IN_P && !first);
+        // loads: a = in / x / r / t; b = Ap (IN_T) or At (IN_P); IN_P after the
+        // first step also c = x, d = p, f = Ap (the pending update, then p)
+        const bool upd = mode == IN_P && !first;
+        const Cx<T>* va = mode == IN_PLAIN ? J->in : mode == IN_X ? J->x : upd ? J->t : J->r;
+        const Cx<T>* vb = mode == IN_T ? J->ap : J->at;
+        const bool two = mode == IN_T || upd;
         for (int e0 = threadIdx.x; e0 < nin; e0 += zy_group<T>() * kThreads) {
             int idx[zy_group<T>()];
-            Cx<T> a[zy_group<T>()], b[zy_group<T>()], c[zy_group<T>()];
+            Cx<T> a[zy_group<T>()], b[zy_group<T>()], c[zy_group<T>()], d[zy_group<T>()], f[zy_group<T>()];
 #pragma unroll
             for (int u = 0; u < zy_group<T>(); u++) {
                 const int e = e0 + u * kThreads;
                 idx[u] = e < nin ? zy_index(J, base, e) : -1;
-                a[u] = b[u] = c[u] = Cx<T>{T(0), T(0)};
+                a[u] = b[u] = c[u] = d[u] = f[u] = Cx<T>{T(0), T(0)};
                 if (idx[u] >= 0) {
                     a[u] = gld(va + idx[u]);
                     if (two) b[u] = gld(vb + idx[u]);
-                    if (mode == IN_P && !first) c[u] = gld(J->ap + idx[u]);
+                    if (upd) {
+                        c[u] = gld(J->x + idx[u]);
+                        d[u] = gld(J->p + idx[u]);
+                        f[u] = gld(J->ap + idx[u]);
+                    }
                 }
             }
 #pragma unroll
@@ -165,8 +173,15 @@ __global__ void __launch_bounds__(kThreads, (zy_waves<T, false>())) k_pzy(const 
                 Cx<T> v = a[u];
                 if (idx[u] >= 0) {
                     if (mode == IN_P) {
-                        // p = r + beta (p - u), u = zeta Ap   (DDAModel.cpp:219)
-                        if (!first) v = cadd(v, cmul(sa, csub(b[u], cmul(sb, c[u]))));
+                        if (upd) {
+                            // the previous iteration's P += alpha p + z (z = zeta r - alpha u =
+                            // zeta t) and r = t - zeta At (DDAModel.cpp:226-230), then
+                            // p = r + beta (p - u), u = zeta Ap   (DDAModel.cpp:219)
+                            const Cx<T> rn = csub(a[u], cmul(sb, b[u]));
+                            gst(J->r + idx[u], rn);
+                            gst(J->x + idx[u], cadd(c[u], cadd(cmul(sc, d[u]), cmul(sb, a[u]))));
+                            v = cadd(rn, cmul(sa, csub(d[u], cmul(sb, f[u]))));
+                        }
                         gst(J->p + idx[u], v);
                     } else if (mode == IN_T) {
                         // t = r - alpha Ap   (DDAModel.cpp:222)
@@ -239,7 +254,7 @@ __global__ void __launch_bounds__(kThreads, (zy_waves<T, true>())) k_pyz(const J
     constexpr int L1 = pxr_l1(LY), L2 = LY / L1, H2 = (L2 + 1) / 2;
     static_assert(L1 > 0 && L1 >= L2, "length without a register split");
     extern __shared__ __align__(16) char smem[];
-    __shared__ double red[16 * 4];
+    __shared__ double red[16 * kPart];
     Cx<T>* tw = reinterpret_cast<Cx<T>*>(smem);
     Cx<T>* ZB = tw + pass_tw_slots(true, LY, 0);
     const int ntot = tstart[njobs];
@@ -312,14 +327,15 @@ __global__ void __launch_bounds__(kThreads, (zy_waves<T, true>())) k_pyz(const J
         }
         __syncthreads();
         // 3. crop to the sites, diagonal and solver epilogue (k_p5's)
-        double acc[4] = {0.0, 0.0, 0.0, 0.0};
+        double acc[kPart] = {};
         const int nout = Ny * Nz * 3;
         const int base = x * Ny * Nz;
-        // per mode: a = al, b = the vector al multiplies, c = E (OUT_RES) or r0 (OUT_AP)
+        // per mode: a = al, b = the vector al multiplies, c = E (OUT_RES) or
+        // r0 (OUT_AP, OUT_AT)
         const bool use_al = mode != OUT_PLAIN || diag;
         const Cx<T>* vb = mode == OUT_PLAIN ? J->in : mode == OUT_RES ? J->x : mode == OUT_AP ? J->p : J->t;
         const Cx<T>* vc = mode == OUT_RES ? J->e : J->r0;
-        const bool use_c = mode == OUT_RES || mode == OUT_AP;
+        const bool use_c = mode != OUT_PLAIN;
         for (int e0 = threadIdx.x; e0 < nout; e0 += zy_group<T>() * kThreads) {
             int idx[zy_group<T>()];
             Cx<T> a[zy_group<T>()], b[zy_group<T>()], c[zy_group<T>()];
@@ -360,15 +376,12 @@ __global__ void __launch_bounds__(kThreads, (zy_waves<T, true>())) k_pyz(const J
                 } else {  // OUT_AT
                     const Cx<T> at = cadd(conv, cmul(a[u], b[u]));
                     gst(J->at + i, at);
-                    const double2 ad = to_d(at), td = to_d(b[u]);
-                    acc[0] += ad.x * td.x + ad.y * td.y;  // conj(At) t
-                    acc[1] += ad.x * td.y - ad.y * td.x;
-                    acc[2] += ad.x * ad.x + ad.y * ad.y;  // conj(At) At
+                    at_sums(acc, at, b[u], c[u]);
                 }
             }
         }
         if (mode != OUT_PLAIN) {
-            block_sum<4>(acc, red);
+            block_sum<kPart>(acc, red);
             store_partials(J->part, x, acc);
         }
         __syncthreads();

@@ -6,7 +6,7 @@ calibration point for the counters (the copy moves exactly --copy-mib MiB
 in and out).
 
 usage: python3 topo-dda-forwin64_amd/profile_passes.py [--workload topo80|mixed128|batch64]
-       [--precision c128|c64] [--reps 20] [--copy-mib 256] [--no-tune]
+       [--precision c128|c64] [--reps 20] [--copy-mib 256] [--no-tune] [--plan-out plan.json]
 
 A single operator (topo80, mixed128) runs the autotuner first (as in
 bench.py), so its tuner's trial launches precede the timed passes in a
@@ -45,6 +45,9 @@ def main():
     ap.add_argument("--reps", type=int, default=20)
     ap.add_argument("--copy-mib", type=int, default=256)
     ap.add_argument("--no-tune", action="store_true", help="planner tiles instead of the autotuner's")
+    ap.add_argument("--plan-out", default=None,
+                    help="JSON file: per precision, whether each operator runs fused z + y passes "
+                         "(profile_summary.py traffic reads it from the parent of the counter directories)")
     a = ap.parse_args()
     ctx = T.Context(0)
     prec = T.C128 if a.precision == "c128" else T.C64
@@ -80,6 +83,10 @@ def main():
         m.set(T.VEC_P, E)
         ms = m.time_passes(prec, reps=a.reps)
         grid = core.grid
+    if a.plan_out:
+        cl = cores if a.workload == "batch64" else [core]
+        with open(a.plan_out, "w") as fh:
+            json.dump({p: [c.zy_fused(q) for c in cl] for p, q in (("c128", T.C128), ("c64", T.C64))}, fh)
     copy_calibration(a.copy_mib)
     print(json.dumps({"workload": a.workload, "precision": a.precision, "grid": grid,
                       "pass_ms": [float(x) for x in ms], "copy_mib": a.copy_mib}))

@@ -36,6 +36,25 @@ sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
 PASS_NAMES = {("k_p1<",): "k_p1_zfwd", ("k_pyr<{t}, false", "k_py<{t}, false"): "k_py_yfwd",
               ("k_pxr<", "k_px<"): "k_px_xgreen", ("k_pyr<{t}, true", "k_py<{t}, true"): "k_py_yinv",
               ("k_p5<",): "k_p5_zinv"}
+# operators with fused z + y passes run k_pzy in the P2 slot (P1 + P2) and
+# k_pyz in the P5 slot (P4 + P5), beside the separate passes of the others
+FUSED = {"k_py_yfwd": "k_pzy<{t}", "k_p5_zinv": "k_pyz<{t}"}
+
+
+def pass_kernels(names, pats, pname, t, contains=False):
+    """Kernel names of one pass slot: those of the first pattern that has
+    dispatches, plus the slot's fused kernels."""
+    def match(kn, pat):
+        full = f"tdda::{pat.format(t=t)}"
+        return (full in kn if contains else kn.startswith(full)) and f"<{t}" in kn
+    sel = set()
+    for pat in pats:
+        sel = {kn for kn in names if match(kn, pat)}
+        if sel:
+            break
+    if pname in FUSED:
+        sel |= {kn for kn in names if match(kn, FUSED[pname])}
+    return sel
 
 
 def stats(path, top):
@@ -84,11 +103,7 @@ def span(path, precision, workload, reps):
     print("| pass | launches per pass | repetitions | span us (mean) | span us (min) | kernel busy us (mean) |")
     print("|---|---|---|---|---|---|")
     for pats, pname in PASS_NAMES.items():
-        names = set()
-        for pat in pats:
-            names = {kn for _, _, kn in rows if kn.startswith(f"tdda::{pat.format(t=t)}") and f"<{t}" in kn}
-            if names:
-                break
+        names = pass_kernels({kn for _, _, kn in rows}, pats, pname, t)
         if not names:
             continue
         d = [r for r in rows if r[2] in names]
@@ -149,16 +164,26 @@ def traffic(fetch_csv, write_csv, precision, workload, copy_mib, out_json, last_
     sys.path.insert(0, os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
     from bench import batch_bytes, pass_bytes
 
-    class Core:
-        pass
+    # which operators ran fused z + y passes: profile_passes.py's plan file
+    # beside the counter files (one flag per structure, in workload order)
+    plan = None
+    pf = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(fetch_csv))), "plan.json")
+    if os.path.exists(pf):
+        with open(pf) as fh:
+            plan = json.load(fh).get(precision)
 
-    def core_of(p):
+    class Core:
+        def zy_fused(self, prec):
+            return self.fused
+
+    def core_of(p, k=0):
         c = Core()
         c.Nx, c.Ny, c.Nz, c.N = p.Nx, p.Ny, p.Nz, p.N
         c.grid = tuple(next_smooth(2 * n - 1) for n in (p.Nx, p.Ny, p.Nz))
+        c.fused = bool(plan[k]) if plan else False
         return c
     if workload == "batch64":
-        alg = batch_bytes([(core_of(p), len(W.incidences())) for p in W.batch64(seed=4)], s)
+        alg = batch_bytes([(core_of(p, k), len(W.incidences())) for k, p in enumerate(W.batch64(seed=4))], s)
     else:
         alg = pass_bytes(core_of(W.topo80()[0] if workload == "topo80" else W.mixed128()), s)
     print(f"calibration: device copy of {copy_mib} MiB in {n} dispatches; FETCH_SIZE x {ffac:.3f}, "
@@ -169,11 +194,7 @@ def traffic(fetch_csv, write_csv, precision, workload, copy_mib, out_json, last_
     for k, (pats, pname) in enumerate(PASS_NAMES.items()):
         # the first pattern with dispatches; a batch has one kernel per FFT
         # length, whose per-dispatch bytes add up to one pass over the batch
-        names = []
-        for pat in pats:
-            names = [kn for kn in fetch if f"tdda::{pat.format(t=t)}" in kn and f"<{t}" in kn]
-            if names:
-                break
+        names = sorted(pass_kernels(set(fetch), pats, pname, t, contains=True))
         if not names:
             continue
         if workload != "batch64":
