@@ -155,8 +155,9 @@ def bench_batch64(T, W, ctx, steps, warmup, batch_tiles):
     cores = [T.AProductCore(ctx, p.R, p.Nx, p.Ny, p.Nz, p.d, p.lam, p.nback) for p in probs]
     tuned = TUNE and batch_tiles == "tuned"
     if tuned:
+        # each operator tuned on a scratch model of its 8 right-hand sides
         for c in cores:
-            c.autotune()
+            c.autotune(nrhs=len(incs))
     models, rhs_all = [], []
     for p, core in zip(probs, cores):
         rhs = [p.rhs(nK, nE)[:2] for nK, nE in incs]

@@ -139,14 +139,19 @@ typedef struct {
  * their AMatrixMethod argument). */
 int tdda_op_create_ex(tdda_ctx ctx, const tdda_op_desc* desc, const tdda_op_ext* ext, tdda_op* out);
 /* Tile autotuner (no counterpart in the reference, which leaves tiling to
- * cuFFT): times the inverse z pass (P5) with z tiles of 1, 1/2 and 1/4 of the
- * planned width and the forward z pass (P1) with 1 down to 1/8 of it, the y
- * passes with each of their kernels (register-resident where one exists for
- * the length, LDS-staged) and the x pass with each of its kernels at x tiles
- * narrowed 3 columns at a time, each on a scratch model (reps launches per
- * timing; <= 0: 20), and keeps the fastest per precision.  Affects models
- * created afterwards. */
+ * cuFFT): times the fused z + y passes against the separate ones (where the
+ * planner fuses), the inverse z pass (P5) with z tiles of 1, 1/2 and 1/4 of
+ * the planned width and the forward z pass (P1) with 1 down to 1/8 of it,
+ * the y passes with each of their kernels (register-resident where one
+ * exists for the length, LDS-staged) and the x pass with each of its kernels
+ * at x tiles narrowed 3 columns at a time, each on a scratch model of one
+ * right-hand side (reps launches per timing; <= 0: 20), and keeps the
+ * fastest per precision.  Affects models created afterwards. */
 int tdda_op_autotune(tdda_op op, int reps);
+/* tdda_op_autotune on a scratch model of nrhs right-hand sides (1..4096):
+ * the tiles for the number of right-hand sides the operator will carry (a
+ * structure with several incidences; no counterpart in the reference). */
+int tdda_op_autotune_ex(tdda_op op, int reps, int nrhs);
 /* Current tiles of one precision: out4 = z pencils per tile (P5), z pencils
  * per tile (P1), y columns per tile, x columns per tile. */
 int tdda_op_tiles(tdda_op op, int precision, int* out4);

@@ -35,7 +35,7 @@ EXPORTS = [
     "tdda_server_create", "tdda_server_destroy", "tdda_server_submit", "tdda_server_poll", "tdda_server_wait",
     "tdda_server_result", "tdda_server_get_stats", "tdda_design_create", "tdda_design_create_ex", "tdda_design_step", "tdda_design_get",
     "tdda_design_destroy", "tdda_beta_update", "tdda_op_create_periodic", "tdda_objective_point_e",
-    "tdda_objective_integrated_e", "tdda_op_autotune", "tdda_op_tiles", "tdda_op_xpass", "tdda_op_create_ex",
+    "tdda_objective_integrated_e", "tdda_op_autotune", "tdda_op_autotune_ex", "tdda_op_tiles", "tdda_op_xpass", "tdda_op_create_ex",
     "tdda_green_tensor_fcd", "tdda_field_at_points", "tdda_write_structure", "tdda_write_model_results", "tdda_model_setup", "tdda_aproduct_noalloc",
 ]
 
@@ -163,6 +163,7 @@ def lib():
         L.tdda_write_model_results.argtypes = [ctypes.c_char_p, ctypes.c_int, dp, ctypes.c_int, dp, ctypes.c_int, ip,
                                                ctypes.c_int, dp, dp]
         L.tdda_op_autotune.argtypes = [vp, ctypes.c_int]
+        L.tdda_op_autotune_ex.argtypes = [vp, ctypes.c_int, ctypes.c_int]
         L.tdda_op_tiles.argtypes = [vp, ctypes.c_int, ip]
         L.tdda_op_xpass.argtypes = [vp, ctypes.c_int]
         L.tdda_objective_point_e.argtypes = [vp, ctypes.c_int, dp, dp, dp, ctypes.c_double, dp, vp, ctypes.c_int]
@@ -339,9 +340,10 @@ class AProductCore:
         self.grid = tuple(g3)
         self.K = K.value
 
-    def autotune(self, reps=20):
-        """Tile autotuner (tdda_op_autotune): affects models created afterwards."""
-        _check(lib().tdda_op_autotune(self.h, int(reps)))
+    def autotune(self, reps=20, nrhs=1):
+        """Tile autotuner (tdda_op_autotune_ex) on a scratch model of nrhs
+        right-hand sides: affects models created afterwards."""
+        _check(lib().tdda_op_autotune_ex(self.h, int(reps), int(nrhs)))
 
     def tiles(self, precision=C128):
         """(z pencils P5, z pencils P1, y columns, x columns) per tile."""
