@@ -17,16 +17,23 @@ def test_autotuned_operator_matches_oracle(ctx, dims):
     d, lam = 10.0, 500.0
     print(f"autotune {dims}", flush=True)
     core = T.AProductCore(ctx, None, Nx, Ny, Nz, d, lam)
-    before = {p: core.tiles(p) for p in (T.C128, T.C64)}
+    before = {p: (core.tiles(p), core.zy_fused(p)) for p in (T.C128, T.C64)}
     core.autotune(reps=10)
     Mz = core.grid[2]
     for p in (T.C128, T.C64):
         pz, pz1, tfy, tfx = core.tiles(p)
-        bz, bz1, by, bx = before[p]
-        # z tiles only narrow; the x and y tiles are those of the kernel the
-        # tuner chose (register-resident with 256 or 512 threads, or
-        # LDS-staged), each within one ky / x row of 3 Mz columns
-        assert 1 <= pz <= bz and 1 <= pz1 <= bz1
+        (bz, bz1, by, bx), bf = before[p]
+        # fused z + y passes: one z tile per x slab (Ny pencils); separate
+        # passes: z tiles only narrow from the planner's; the x and y tiles
+        # are those of the kernel the tuner chose (register-resident with 256
+        # or 512 threads, or LDS-staged), each within one ky / x row of 3 Mz
+        # columns
+        if core.zy_fused(p):
+            assert pz == pz1 == Ny
+        elif not bf:
+            assert 1 <= pz <= bz and 1 <= pz1 <= bz1
+        else:
+            assert pz >= 1 and pz1 >= 1
         assert 1 <= tfy <= 3 * Mz and 3 <= tfx <= 3 * Mz and tfx % 3 == 0
         assert core.xpass(p) in ("register", "lds") and core.ypass(p) in ("register", "lds")
     R = O.build_a_bulk(Nx, Ny, Nz)
@@ -95,4 +102,23 @@ def test_tuner_keeps_its_y_pass_decision(ctx, monkeypatch):
     b = np.random.default_rng(5).standard_normal(len(R)) + 0.5j
     ref = O.aproduct_numpy_fft(R, Nx, Ny, Nz, d, O.wavenumber(lam), b, grid=core.grid)
     assert O.rel_l2(core.Aproduct(b), ref) < 1e-12
+    core.close()
+
+
+def test_autotune_for_several_right_hand_sides(ctx):
+    # tdda_op_autotune_ex: the tuner's scratch model carries nrhs right-hand
+    # sides; the tuned operator still matches the oracle for every one
+    Nx, Ny, Nz, d, lam = 24, 20, 10, 10.0, 500.0
+    core = T.AProductCore(ctx, None, Nx, Ny, Nz, d, lam)
+    core.autotune(reps=5, nrhs=4)
+    R = O.build_a_bulk(Nx, Ny, Nz)
+    rng = np.random.default_rng(8)
+    b = rng.standard_normal((4, len(R))) + 1j * rng.standard_normal((4, len(R)))
+    y = core.Aproduct(b)
+    for j in range(4):
+        ref = O.aproduct_numpy_fft(R, Nx, Ny, Nz, d, O.wavenumber(lam), b[j], grid=core.grid)
+        assert O.rel_l2(y[j], ref) < 1e-12
+    for bad in (0, 5000):
+        with pytest.raises(T.TddaError):
+            core.autotune(reps=5, nrhs=bad)
     core.close()
