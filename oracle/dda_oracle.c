@@ -231,7 +231,16 @@ static void fft1(const fftplan1 *p, cplx *x, cplx *y, int sign)
     if (L == 1) return;
     cplx *src = x, *dst = y;
     int Ns = 1;
-    cplx v[64], w[64];
+    /* radix buffers: on the stack up to radix 64, else on the heap (the
+     * reference's 2N-1 grids have prime factors up to 127 for N <= 64) */
+    int rmax = 1;
+    for (int s = 0; s < p->nf; s++) rmax = p->fac[s] > rmax ? p->fac[s] : rmax;
+    cplx vs[64], ws[64];
+    cplx *v = vs, *w = ws;
+    if (rmax > 64) {
+        v = (cplx *)malloc(sizeof(cplx) * (size_t)rmax);
+        w = (cplx *)malloc(sizeof(cplx) * (size_t)rmax);
+    }
     for (int s = 0; s < p->nf; s++) {
         int R = p->fac[s];
         int LR = L / R;
@@ -257,6 +266,10 @@ static void fft1(const fftplan1 *p, cplx *x, cplx *y, int sign)
         }
         cplx *tmp = src; src = dst; dst = tmp;
         Ns *= R;
+    }
+    if (v != vs) {
+        free(v);
+        free(w);
     }
     if (src != x) memcpy(x, src, sizeof(cplx) * (size_t)L);
 }

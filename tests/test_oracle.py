@@ -161,7 +161,8 @@ def test_planted_green_mistakes_are_caught():
 
 
 # --- FFT and operator --------------------------------------------------------
-@pytest.mark.parametrize("L", [1, 2, 3, 4, 5, 8, 12, 30, 31, 64, 160, 243])
+# 101, 127 and 2 * 67: the reference's 2N - 1 grids have prime factors above 64
+@pytest.mark.parametrize("L", [1, 2, 3, 4, 5, 8, 12, 30, 31, 64, 160, 243, 101, 127, 134])
 def test_fft1d_matches_numpy(L):
     x = np.random.default_rng(L).standard_normal(L) + 1j * np.random.default_rng(L + 1).standard_normal(L)
     assert np.allclose(O.fft1d(x, -1), np.fft.fft(x), rtol=0, atol=1e-13 * max(1, np.abs(x).sum()))
