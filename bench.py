@@ -192,6 +192,26 @@ def bench_batch64(T, W, ctx, steps, warmup, batch_tiles):
     # sample of right-hand sides for the field error: one per 8th structure
     sample = [(s, s % len(incs)) for s in range(0, len(probs), 8)]
     P_mixed = {(s, j): models[s].get(T.VEC_P, j) for s, j in sample}
+    # the same batch with right Jacobi preconditioning (opt-in solver
+    # option, not the reference's iteration): same system, same 1e-8 stop
+    batch.solve(mode=T.MODE_MIXED, tol=TOL, x0_zero=1, true_residual=1, precond=T.PRECOND_JACOBI)
+    ctx.sync()
+    tsj, okj = [], True
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        t1 = time.perf_counter()
+        jinfo = batch.solve(mode=T.MODE_MIXED, tol=TOL, x0_zero=1, true_residual=1, precond=T.PRECOND_JACOBI)
+        tsj.append(time.perf_counter() - t1)
+        okj = okj and converged(jinfo)
+    ctx.sync()
+    dtj = time.perf_counter() - t0
+    out["batched_mixed_jacobi"] = {"solves_per_s": nrhs * steps / dtj, "ms_per_step": dtj / steps * 1e3,
+                                   "latency_ms": lat_stats(tsj),
+                                   "iterations_mean": float(np.mean([i["iterations"] for i in jinfo])),
+                                   "outer_mean": float(np.mean([i["outer"] for i in jinfo])),
+                                   "true_residual_max": max(i["true_residual"] for i in jinfo),
+                                   "all_converged": okj}
+    P_jac = {(s, j): models[s].get(T.VEC_P, j) for s, j in sample}
     # batched FP64
     batch.solve(mode=T.MODE_FP64, tol=TOL, x0_zero=1, true_residual=1)
     t0 = time.perf_counter()
@@ -221,7 +241,10 @@ def bench_batch64(T, W, ctx, steps, warmup, batch_tiles):
         m.close()
     # plain FP64: one right-hand side at a time, each operator with its
     # autotuned tiles (inputs resident; uploads between solves untimed)
+    # (the Jacobi-preconditioned FP64 solve of each right-hand side timed
+    # beside it)
     lat, pinfo, P_plain = [], [], {}
+    latj, pinfoj = [], []
     for s, (core, rhs) in enumerate(zip(cores, rhs_all)):
         if TUNE and not tuned:
             core.autotune()
@@ -230,22 +253,32 @@ def bench_batch64(T, W, ctx, steps, warmup, batch_tiles):
             one.set(T.VEC_AL, a)
             one.set(T.VEC_E, e)
             t0 = time.perf_counter()
+            pinfoj.append(one.solve(mode=T.MODE_FP64, tol=TOL, x0_zero=1, true_residual=1,
+                                    precond=T.PRECOND_JACOBI)[0])
+            latj.append(time.perf_counter() - t0)
+            t0 = time.perf_counter()
             pinfo.append(one.solve(mode=T.MODE_FP64, tol=TOL, x0_zero=1, true_residual=1)[0])
             lat.append(time.perf_counter() - t0)
             if (s, j) in P_mixed:
                 P_plain[(s, j)] = one.get(T.VEC_P)[0]
         one.close()
-    out["plain_fp64"] = {"solves_per_s": len(lat) / sum(lat), "latency_ms": lat_stats(lat),
-                         "iterations_mean": float(np.mean([i["iterations"] for i in pinfo])),
-                         "true_residual_max": max(i["true_residual"] for i in pinfo),
-                         "all_converged": converged(pinfo)}
+    for key, ls, inf in (("plain_fp64", lat, pinfo), ("plain_fp64_jacobi", latj, pinfoj)):
+        out[key] = {"solves_per_s": len(ls) / sum(ls), "latency_ms": lat_stats(ls),
+                    "iterations_mean": float(np.mean([i["iterations"] for i in inf])),
+                    "true_residual_max": max(i["true_residual"] for i in inf),
+                    "all_converged": converged(inf)}
     out["batched_mixed_vs_plain_fp64"] = round(out["batched_mixed"]["solves_per_s"]
                                                / out["plain_fp64"]["solves_per_s"], 3)
     out["batched_fp64_vs_plain_fp64"] = round(out["batched_fp64"]["solves_per_s"]
                                               / out["plain_fp64"]["solves_per_s"], 3)
+    out["batched_mixed_jacobi_vs_plain_fp64"] = round(out["batched_mixed_jacobi"]["solves_per_s"]
+                                                      / out["plain_fp64"]["solves_per_s"], 3)
+    out["batched_mixed_jacobi_vs_plain_fp64_jacobi"] = round(out["batched_mixed_jacobi"]["solves_per_s"]
+                                                             / out["plain_fp64_jacobi"]["solves_per_s"], 3)
     cpu_cases = [dict(name=f"s{s}/rhs{j}", R=probs[s].R, dims=(probs[s].Nx, probs[s].Ny, probs[s].Nz),
                       d=probs[s].d, K=probs[s].K, grid=cores[s].grid, al=rhs_all[s][j][0], E=rhs_all[s][j][1],
-                      gpu={"batched_mixed": P_mixed[(s, j)], "plain_fp64": P_plain[(s, j)]}) for s, j in sample]
+                      gpu={"batched_mixed": P_mixed[(s, j)], "plain_fp64": P_plain[(s, j)],
+                           "batched_mixed_jacobi": P_jac[(s, j)]}) for s, j in sample]
     for c in cores:
         c.close()
     return out, roof, cpu_cases
@@ -268,18 +301,21 @@ def bench_mixed128(T, W, ctx, reps):
            "xpass": {"c128": core.xpass(T.C128), "c64": core.xpass(T.C64)},
            "ypass": {"c128": core.ypass(T.C128), "c64": core.ypass(T.C64)}}
     P = {}
-    for mode, name in ((T.MODE_FP64, "fp64"), (T.MODE_MIXED, "mixed")):
-        m.solve(mode=mode, tol=TOL, x0_zero=1, true_residual=1)
+    for mode, name, pc in ((T.MODE_FP64, "fp64", T.PRECOND_NONE), (T.MODE_MIXED, "mixed", T.PRECOND_NONE),
+                           (T.MODE_FP64, "fp64_jacobi", T.PRECOND_JACOBI),
+                           (T.MODE_MIXED, "mixed_jacobi", T.PRECOND_JACOBI)):
+        m.solve(mode=mode, tol=TOL, x0_zero=1, true_residual=1, precond=pc)
         lat, infos = [], []
         for _ in range(reps):
             t0 = time.perf_counter()
-            infos.append(m.solve(mode=mode, tol=TOL, x0_zero=1, true_residual=1)[0])
+            infos.append(m.solve(mode=mode, tol=TOL, x0_zero=1, true_residual=1, precond=pc)[0])
             lat.append(time.perf_counter() - t0)
         P[name] = m.get(T.VEC_P)[0]
         out[name] = {"solves_per_s": reps / sum(lat), "latency_ms": lat_stats(lat),
                      "iterations": infos[-1]["iterations"], "outer": infos[-1]["outer"],
                      "true_residual_max": max(i["true_residual"] for i in infos), "all_converged": converged(infos)}
     out["mixed_vs_plain_fp64"] = round(out["mixed"]["solves_per_s"] / out["fp64"]["solves_per_s"], 3)
+    out["mixed_jacobi_vs_plain_fp64"] = round(out["mixed_jacobi"]["solves_per_s"] / out["fp64"]["solves_per_s"], 3)
     m.set(T.VEC_P, E)
     seq = {p: m.time_passes(p, reps=20) for p in (T.C128, T.C64)}
     b = {T.C128: pass_bytes(core, 16), T.C64: pass_bytes(core, 8)}
@@ -294,21 +330,21 @@ def bench_mixed128(T, W, ctx, reps):
     m.close()
     core.close()
     case = dict(name="mixed128", R=prob.R, dims=(prob.Nx, prob.Ny, prob.Nz), d=prob.d, K=prob.K, grid=core.grid,
-                al=al, E=E, gpu={"fp64": P["fp64"], "mixed": P["mixed"]})
+                al=al, E=E, gpu={k: P[k] for k in ("fp64", "mixed", "mixed_jacobi")})
     return out, roofs, case
 
 
-def run_step(T, F, A, targets, diel, mode, lat):
+def run_step(T, F, A, targets, diel, mode, lat, pc=0):
     """One topology step: forward solve, adjoint source, adjoint solve."""
     t0 = time.perf_counter()
-    fi = F.solve(mode=mode, tol=TOL, x0_zero=1, true_residual=1)[0]
+    fi = F.solve(mode=mode, tol=TOL, x0_zero=1, true_residual=1, precond=pc)[0]
     t1 = time.perf_counter()
     idx = (3 * targets[:, None] + np.arange(3)[None, :]).reshape(-1)
     P = F.get_entries(T.VEC_P, 0, idx)
     al_t = F.al_host[idx]
     g = np.conj(al_t * P * 3.0 / (2.0 + diel[idx]))
     A.set_entries(T.VEC_E, 0, idx, g)
-    ai = A.solve(mode=mode, tol=TOL, x0_zero=1, true_residual=1)[0]
+    ai = A.solve(mode=mode, tol=TOL, x0_zero=1, true_residual=1, precond=pc)[0]
     t2 = time.perf_counter()
     lat.append(t1 - t0)
     lat.append(t2 - t1)
@@ -335,15 +371,16 @@ def bench_topo80(T, W, ctx, steps, warmup):
                        "(100 targets) per step", "grid": list(core.grid), "dipoles": prob.N,
            "xpass_c128": core.xpass(T.C128), "ypass_c128": core.ypass(T.C128)}
     P = {}
-    for mode, name in ((T.MODE_FP64, "fp64"), (T.MODE_MIXED, "mixed")):
+    for mode, name, pc in ((T.MODE_FP64, "fp64", T.PRECOND_NONE), (T.MODE_MIXED, "mixed", T.PRECOND_NONE),
+                           (T.MODE_MIXED, "mixed_jacobi", T.PRECOND_JACOBI)):
         lat = []
         for _ in range(warmup):
-            run_step(T, F, A, targets, diel, mode, lat)
+            run_step(T, F, A, targets, diel, mode, lat, pc)
         lat, infos = [], []
         ctx.sync()
         t0 = time.perf_counter()
         for _ in range(steps):
-            infos.append(run_step(T, F, A, targets, diel, mode, lat))
+            infos.append(run_step(T, F, A, targets, diel, mode, lat, pc))
         ctx.sync()
         dt = time.perf_counter() - t0
         out[name] = {"solves_per_s": 2 * steps / dt, "ms_per_step": dt / steps * 1e3, "latency_ms": lat_stats(lat),
@@ -352,6 +389,7 @@ def bench_topo80(T, W, ctx, steps, warmup):
                      "all_converged": all(converged(pair) for pair in infos)}
         P[name] = (F.get(T.VEC_P)[0], A.get(T.VEC_E)[0], A.get(T.VEC_P)[0])
     out["mixed_vs_plain_fp64"] = round(out["mixed"]["solves_per_s"] / out["fp64"]["solves_per_s"], 3)
+    out["mixed_jacobi_vs_plain_fp64"] = round(out["mixed_jacobi"]["solves_per_s"] / out["fp64"]["solves_per_s"], 3)
     F.set(T.VEC_P, E)
     out["aproduct_passes"] = {"c128": pass_report(F.time_passes(T.C128, reps=50), pass_bytes(core, 16)),
                               "c64": pass_report(F.time_passes(T.C64, reps=50), pass_bytes(core, 8))}
@@ -720,6 +758,11 @@ def main():
         "batched_fp64": head["batched_fp64"],
         "batched_mixed_vs_plain_fp64": head["batched_mixed_vs_plain_fp64"],
         "batched_mixed": bm,
+        "jacobi": {"note": "opt-in right Jacobi preconditioning (BiCGSTAB on A diag(al)^-1, not the reference's "
+                           "iteration): same systems, same 1e-8 true-residual stop",
+                   "batched_mixed": head["batched_mixed_jacobi"], "plain_fp64": head["plain_fp64_jacobi"],
+                   "batched_mixed_vs_plain_fp64": head["batched_mixed_jacobi_vs_plain_fp64"],
+                   "batched_mixed_vs_plain_fp64_jacobi": head["batched_mixed_jacobi_vs_plain_fp64_jacobi"]},
         "aproduct_passes": head["aproduct_passes"],
         "roofline": roof,
         "roofline_by_workload": roofs,

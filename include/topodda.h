@@ -268,7 +268,15 @@ typedef struct {
     int max_outer;     /* mixed: refinement cycles                        */
     int check_every;   /* iterations between host status polls (0: auto)  */
     int true_residual; /* 1: also compute ||E - A P|| / ||E|| in FP64     */
+    int precond;       /* TDDA_PRECOND_NONE (the reference's BiCGSTAB) or
+                          TDDA_PRECOND_JACOBI: BiCGSTAB on A D^-1 with
+                          D = diag(al) (right preconditioning: the same
+                          residual E - A P and stop tests, P = D^-1 y at the
+                          end); no counterpart in the reference          */
 } tdda_solve_opts;
+
+#define TDDA_PRECOND_NONE 0
+#define TDDA_PRECOND_JACOBI 1
 
 typedef struct {
     int status;      /* TDDA_SOLVE_*                                        */

@@ -110,11 +110,13 @@ def test_fused_solves_match_oracle(ctx, dims, frac):
             assert res <= 1e-8, (name, mode, res)
             P[(name, mode)] = (x, info["iterations"])
             m.close()
-    # FP64: the same iteration as the separate passes up to FFT rounding
-    # (rounding moves the stop by a few iterations: 59 vs 61 seen on
-    # 24 x 20 x 10); both solutions within the 1e-8 residual's reach
+    # FP64: the same iteration as the separate passes up to FFT rounding.
+    # Near 1e-8 this problem's residual plateaus, so rounding moves the stop
+    # by several iterations (59 vs 61 and 61 vs 68 seen on 24 x 20 x 10 with
+    # two orders of FMA contraction); both solutions within the 1e-8
+    # residual's reach
     (xf, itf), (xs, its) = P[("fused", T.MODE_FP64)], P[("separate", T.MODE_FP64)]
-    assert abs(itf - its) <= max(3, its // 10)
+    assert abs(itf - its) <= max(3, its // 5)
     assert O.rel_l2(xf, xs) < 5e-6
     sep.close()
     core.close()

@@ -831,6 +831,14 @@ void launch_settle(const JobTable<T>& jt, int nj, hipStream_t s)
     k_finish<T, FIN_SETTLE><<<dim3(nj), dim3(kThreads), 0, s>>>(jt.d, nj);
 }
 
+// right Jacobi preconditioning: P = D^-1 y of the settled jobs that have
+// stopped (DIV), or y = D P before iterating from a given P (!DIV)
+template <typename T, bool DIV>
+void launch_prec_x(const JobTable<T>& jt, int nj, hipStream_t s)
+{
+    k_prec_x<T, DIV><<<dim3(jt.nv), dim3(kThreads), 0, s>>>(jt.d, jt.tv, nj);
+}
+
 // The job tables and host buffers one solve runs on: a model's own, or the
 // concatenation of several models' (a batch).
 struct SolveSet {
@@ -924,7 +932,7 @@ int run_iterations(SolveSet& S, JobTable<T>& jt, SolverState* hst, int max_total
 
 template <typename T>
 int init_states(SolveSet& S, JobTable<T>& jt, SolverState* hst, const std::vector<int>& active, double tol,
-                int maxit, const std::vector<double>* tols = nullptr)
+                int maxit, const std::vector<double>* tols = nullptr, int prec = 0)
 {
     for (int j = 0; j < S.nj; j++) {
         SolverState& St = hst[j];
@@ -932,6 +940,7 @@ int init_states(SolveSet& S, JobTable<T>& jt, SolverState* hst, const std::vecto
         St.tol = tols ? (*tols)[j] : tol;
         St.maxit = maxit;
         St.first = 1;
+        St.prec = prec;
         St.status = active[j] ? ST_RUNNING : ST_CONVERGED;
     }
     HIPCHK(hipMemcpyAsync(jt.st, hst, sizeof(SolverState) * S.nj, hipMemcpyHostToDevice, S.s));
@@ -950,15 +959,18 @@ int solve_fp64(SolveSet& S, const tdda_solve_opts* o, tdda_solve_info* info)
     const int nj = S.nj;
     SolverState* hst = S.h_st;
     std::vector<int> active(nj, 1);
-    CHK(init_states<double>(S, *S.j64, hst, active, o->tol, o->max_it));
+    const int prec = o->precond == TDDA_PRECOND_JACOBI;
+    CHK(init_states<double>(S, *S.j64, hst, active, o->tol, o->max_it, nullptr, prec));
     if (o->x0_zero) {
         launch_init_zero<double>(*S.j64, nj, S.s);
     } else {
         launch_apply<double>(*S.j64, nj, IN_X, OUT_RES, 1, 1, S.s);
+        if (prec) launch_prec_x<double, false>(*S.j64, nj, S.s);
     }
     CHK(check_launch());
     CHK(run_iterations<double>(S, *S.j64, hst, o->max_it, o->check_every));
     launch_settle<double>(*S.j64, nj, S.s);
+    if (prec) launch_prec_x<double, true>(*S.j64, nj, S.s);
     CHK(check_launch());
     for (int j = 0; j < nj; j++) {
         info[j].status = hst[j].status;
@@ -1023,7 +1035,8 @@ int solve_mixed(SolveSet& S, const tdda_solve_opts* o, tdda_solve_info* info)
                 k_demote<<<blocks(j), kThreads, 0, s>>>(S.jo->h[j].r, const_cast<Cx<float>*>(S.j32->h[j].e), n3of(j));
         std::vector<double> itol(nj);
         for (int j = 0; j < nj; j++) itol[j] = inner_tolerance(o, rel[j]);
-        CHK(init_states<float>(S, *S.j32, hin, active, o->inner_tol, o->inner_max_it, &itol));
+        CHK(init_states<float>(S, *S.j32, hin, active, o->inner_tol, o->inner_max_it, &itol,
+                               o->precond == TDDA_PRECOND_JACOBI));
         launch_init_zero<float>(*S.j32, nj, s);
         CHK(check_launch());
         CHK(run_iterations<float>(S, *S.j32, hin, o->inner_max_it, o->check_every));
@@ -1032,7 +1045,8 @@ int solve_mixed(SolveSet& S, const tdda_solve_opts* o, tdda_solve_info* info)
             if (!active[j]) continue;
             inner_it[j] += hin[j].iter;
             outer[j] += 1;
-            k_accum<<<blocks(j), kThreads, 0, s>>>(S.j64->h[j].x, S.j32->h[j].x, n3of(j));
+            k_accum<<<blocks(j), kThreads, 0, s>>>(S.j64->h[j].x, S.j32->h[j].x, n3of(j),
+                                                  o->precond == TDDA_PRECOND_JACOBI ? S.j64->h[j].al : nullptr);
         }
         CHK(check_launch());
         CHK(true_residual(S, rel, active));
@@ -1050,6 +1064,8 @@ int solve_mixed(SolveSet& S, const tdda_solve_opts* o, tdda_solve_info* info)
 int solve_set(SolveSet& S, const tdda_solve_opts* opts, tdda_solve_info* info)
 {
     if (opts->max_it < 0 || !(opts->tol >= 0)) return fail(TDDA_ERR_ARG, "bad iteration limit or tolerance");
+    if (opts->precond != TDDA_PRECOND_NONE && opts->precond != TDDA_PRECOND_JACOBI)
+        return fail(TDDA_ERR_ARG, "unknown preconditioner");
     if (opts->mode == TDDA_MODE_FP64) {
         CHK(solve_fp64(S, opts, info));
         if (opts->true_residual) {
@@ -1931,6 +1947,7 @@ void tdda_solve_opts_default(tdda_solve_opts* o)
     o->max_outer = 8;
     o->check_every = 0;
     o->true_residual = 0;
+    o->precond = TDDA_PRECOND_NONE;
 }
 
 int tdda_model_solve(tdda_model m, const tdda_solve_opts* opts, tdda_solve_info* info)

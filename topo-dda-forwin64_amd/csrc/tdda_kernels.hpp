@@ -85,6 +85,7 @@ struct SolverState {
     double alpha[2], zeta[2], rho[2], rho_prev[2];
     double enorm, rnorm, err, tol;
     int iter, maxit, status, first, pend;
+    int prec;  // right Jacobi preconditioning: the solve runs on A D^-1, D = diag(al)
 };
 
 // Partial sums per tile of a reducing pass (the At pass needs eight).
@@ -277,6 +278,14 @@ __device__ __forceinline__ void store_partials(double* part, int tile, const dou
 __device__ __forceinline__ bool job_inactive(const SolverState* st)
 {
     return st != nullptr && gstate(st)->status != ST_RUNNING;
+}
+
+// a / b
+template <typename T>
+__device__ __forceinline__ Cx<T> cdivv(Cx<T> a, Cx<T> b)
+{
+    const T inv = T(1) / (b.x * b.x + b.y * b.y);
+    return Cx<T>{(a.x * b.x + a.y * b.y) * inv, (a.y * b.x - a.x * b.y) * inv};
 }
 
 // Dot products of the At pass: conj(At) t, conj(At) At (zeta,
@@ -599,8 +608,10 @@ __global__ void TDDA_PASS_BOUNDS k_p1(const JobDesc<T>* __restrict__ jobs, const
     extern __shared__ __align__(16) char smem[];
     constexpr bool FT = LC > 0 && pass_full_twiddles<T>(0, LC);
     constexpr bool IP = LC > 0 && pz_inplace<T>(LC);  // one LDS buffer, in-place transforms
-    // IN_P: t, At, x, p, Ap (the pending update, then p); the first step: r
-    constexpr int NV = (MODE == IN_P) ? 5 : ((MODE == IN_T) ? 2 : 1);
+    // IN_P: t, At, x, p, Ap (the pending update, then p); the first step: r.
+    // IN_P / IN_T with right Jacobi preconditioning also al (the operator
+    // input is p / al or t / al), in the last slot.
+    constexpr int NV = (MODE == IN_P) ? 6 : ((MODE == IN_T) ? 3 : 1);
     constexpr int EP = ept_len<T>(LC) / 3;
     const int ntot = tstart[njobs];
     Pre<T, EP> pr[NV];
@@ -617,6 +628,7 @@ __global__ void TDDA_PASS_BOUNDS k_p1(const JobDesc<T>* __restrict__ jobs, const
             load_pass_twiddles<T, FT>(twl, J->twz, J->twfz, J->plz);
             Cx<T> sa{T(0), T(0)}, sb{T(0), T(0)}, sc{T(0), T(0)};
             bool first = false;
+            const bool prec = (MODE == IN_P || MODE == IN_T) && gstate(J->st)->prec != 0;
             if (MODE == IN_P) {
                 const TDDA_GLOBAL SolverState* st = gstate(J->st);
                 first = st->first != 0;
@@ -670,6 +682,7 @@ __global__ void TDDA_PASS_BOUNDS k_p1(const JobDesc<T>* __restrict__ jobs, const
                         v = csub(pr[0].get(k), cmul(sa, pr[1].get(k)));
                         if (idx >= 0) gst(J->t + idx, v);
                     }
+                    if (prec && idx >= 0) v = cdivv(v, pr[NV - 1].get(k));
                 }
                 A[(pl * 3 + c) * ps + lpad<T>(z)] = v;
             }
@@ -687,6 +700,7 @@ __global__ void TDDA_PASS_BOUNDS k_p1(const JobDesc<T>* __restrict__ jobs, const
             if (nxt.live) {
                 const JobDesc<T>* J = jobs + nxt.job;
                 const bool pfirst = MODE == IN_P && gstate(J->st)->first != 0;
+                const bool pprec = (MODE == IN_P || MODE == IN_T) && gstate(J->st)->prec != 0;
                 const int Nz = J->Nz;
                 const int nin = nxt.np * Nz * 3;
                 const int base = nxt.q0 * Nz;
@@ -726,6 +740,7 @@ __global__ void TDDA_PASS_BOUNDS k_p1(const JobDesc<T>* __restrict__ jobs, const
                         pr[0].set(k, gld(J->r + idx));
                         pr[1].set(k, gld(J->ap + idx));
                     }
+                    if ((MODE == IN_P || MODE == IN_T) && pprec) pr[NV - 1].set(k, gld(J->al + idx));
                 }
             }
         }
@@ -1259,6 +1274,9 @@ __global__ void TDDA_PASS_BOUNDS k_p5(const JobDesc<T>* __restrict__ jobs, const
             if constexpr (IP) res = ct_fft_ip<T, LC, true, np_max<T>(LC), kPruneOut, FT>(A, cur.np * 3, twl);
             else res = pass_fft<T, LC, true, kPruneOut, FT>(A, B, cur.np * 3, ps, J->plz, twl);
             double acc[kPart] = {};
+            // right Jacobi preconditioning: the operator is A D^-1, whose
+            // diagonal term on v is v itself
+            const bool prec = (MODE == OUT_AP || MODE == OUT_AT) && gstate(J->st)->prec != 0;
             const int nout = cur.np * Nz * 3;
             const int base = cur.q0 * Nz;
             RowWalk w(Nz * 3, 1.0f / (float)(Nz * 3));
@@ -1290,14 +1308,15 @@ __global__ void TDDA_PASS_BOUNDS k_p5(const JobDesc<T>* __restrict__ jobs, const
                     acc[0] += rd.x * rd.x + rd.y * rd.y;
                     acc[1] += ed.x * ed.x + ed.y * ed.y;
                 } else if (MODE == OUT_AP) {
-                    const Cx<T> ap = cadd(conv, cmul(gld(J->al + i), gld(J->p + i)));
+                    const Cx<T> pv = gld(J->p + i);
+                    const Cx<T> ap = cadd(conv, prec ? pv : cmul(gld(J->al + i), pv));
                     gst(J->ap + i, ap);
                     const double2 a = to_d(gld(J->r0 + i)), b = to_d(ap);
                     acc[0] += a.x * b.x + a.y * b.y;  // conj(r0) Ap
                     acc[1] += a.x * b.y - a.y * b.x;
                 } else {  // OUT_AT
                     const Cx<T> tv = gld(J->t + i);
-                    const Cx<T> at = cadd(conv, cmul(gld(J->al + i), tv));
+                    const Cx<T> at = cadd(conv, prec ? tv : cmul(gld(J->al + i), tv));
                     gst(J->at + i, at);
                     at_sums(acc, at, tv, gld(J->r0 + i));
                 }
@@ -1370,6 +1389,29 @@ __global__ void __launch_bounds__(kThreads) k_settle(const JobDesc<T>* __restric
             const Cx<T> tv = gld(J.t + i);
             gst(J.r + i, csub(tv, cmul(ze, gld(J.at + i))));
             gst(J.x + i, cadd(gld(J.x + i), cadd(cmul(al, gld(J.p + i)), cmul(ze, tv))));
+        }
+    }
+}
+
+// Right Jacobi preconditioning: the iterate y of A D^-1 y = E becomes
+// P = D^-1 y (DIV) once the job has stopped and settled; a solve from a
+// given P starts from y = D P (!DIV).
+template <typename T, bool DIV>
+__global__ void __launch_bounds__(kThreads) k_prec_x(const JobDesc<T>* __restrict__ jobs,
+                                                     const int* __restrict__ tstart, int njobs)
+{
+    const int job = find_job(tstart, njobs, blockIdx.x);
+    const JobDesc<T>& J = jobs[job];
+    const TDDA_GLOBAL SolverState* stc = gstate(J.st);
+    if (!stc->prec || (DIV && stc->status == ST_RUNNING)) return;
+    const int tile = blockIdx.x - tstart[job];
+    const int n = J.N * 3;
+    const int i0 = tile * kThreads * kVecPer;
+    for (int k = 0; k < kVecPer; k++) {
+        const int i = i0 + k * kThreads + threadIdx.x;
+        if (i < n) {
+            const Cx<T> xv = gld(J.x + i), a = gld(J.al + i);
+            gst(J.x + i, DIV ? cdivv(xv, a) : cmul(xv, a));
         }
     }
 }

@@ -22,16 +22,19 @@ static __global__ void __launch_bounds__(kThreads) k_demote(const Cx<double>* __
     }
 }
 
-// x64 += c128(x32)
+// x64 += c128(x32), or x64 += c128(x32) / al64 when the inner solve ran
+// with right Jacobi preconditioning (al64 not null: x32 is y of A D^-1 y = r)
 static __global__ void __launch_bounds__(kThreads) k_accum(Cx<double>* __restrict__ x, const Cx<float>* __restrict__ d,
-                                                    size_t n)
+                                                    size_t n, const Cx<double>* __restrict__ al = nullptr)
 {
     const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) {
         const Cx<float> v = d[i];
+        Cx<double> dv{double(v.x), double(v.y)};
+        if (al) dv = cdivv(dv, al[i]);
         Cx<double> a = x[i];
-        a.x += double(v.x);
-        a.y += double(v.y);
+        a.x += dv.x;
+        a.y += dv.y;
         x[i] = a;
     }
 }

@@ -143,22 +143,27 @@ __global__ void __launch_bounds__(kThreads, (zy_waves<T, false>())) k_pzy(const 
         const int nin = Ny * Nz * 3;
         const int base = x * Ny * Nz;  // cell index of (x, 0, 0)
         // loads: a = in / x / r / t; b = Ap (IN_T) or At (IN_P); IN_P after the
-        // first step also c = x, d = p, f = Ap (the pending update, then p)
+        // first step also c = x, d = p, f = Ap (the pending update, then p);
+        // with right Jacobi preconditioning g = al (the operator input is
+        // p / al or t / al)
         const bool upd = mode == IN_P && !first;
+        const bool prec = (mode == IN_P || mode == IN_T) && gstate(J->st)->prec != 0;
         const Cx<T>* va = mode == IN_PLAIN ? J->in : mode == IN_X ? J->x : upd ? J->t : J->r;
         const Cx<T>* vb = mode == IN_T ? J->ap : J->at;
         const bool two = mode == IN_T || upd;
         for (int e0 = threadIdx.x; e0 < nin; e0 += zy_group<T>() * kThreads) {
             int idx[zy_group<T>()];
-            Cx<T> a[zy_group<T>()], b[zy_group<T>()], c[zy_group<T>()], d[zy_group<T>()], f[zy_group<T>()];
+            Cx<T> a[zy_group<T>()], b[zy_group<T>()], c[zy_group<T>()], d[zy_group<T>()], f[zy_group<T>()],
+                g[zy_group<T>()];
 #pragma unroll
             for (int u = 0; u < zy_group<T>(); u++) {
                 const int e = e0 + u * kThreads;
                 idx[u] = e < nin ? zy_index(J, base, e) : -1;
-                a[u] = b[u] = c[u] = d[u] = f[u] = Cx<T>{T(0), T(0)};
+                a[u] = b[u] = c[u] = d[u] = f[u] = g[u] = Cx<T>{T(0), T(0)};
                 if (idx[u] >= 0) {
                     a[u] = gld(va + idx[u]);
                     if (two) b[u] = gld(vb + idx[u]);
+                    if (prec) g[u] = gld(J->al + idx[u]);
                     if (upd) {
                         c[u] = gld(J->x + idx[u]);
                         d[u] = gld(J->p + idx[u]);
@@ -188,6 +193,7 @@ __global__ void __launch_bounds__(kThreads, (zy_waves<T, false>())) k_pzy(const 
                         v = csub(v, cmul(sa, b[u]));
                         gst(J->t + idx[u], v);
                     }
+                    if (prec) v = cdivv(v, g[u]);
                 }
                 S[e] = v;
             }
@@ -332,7 +338,10 @@ __global__ void __launch_bounds__(kThreads, (zy_waves<T, true>())) k_pyz(const J
         const int base = x * Ny * Nz;
         // per mode: a = al, b = the vector al multiplies, c = E (OUT_RES) or
         // r0 (OUT_AP, OUT_AT)
-        const bool use_al = mode != OUT_PLAIN || diag;
+        // right Jacobi preconditioning: the operator is A D^-1, whose
+        // diagonal term on v is v itself (no al)
+        const bool prec = (mode == OUT_AP || mode == OUT_AT) && gstate(J->st)->prec != 0;
+        const bool use_al = (mode != OUT_PLAIN || diag) && !prec;
         const Cx<T>* vb = mode == OUT_PLAIN ? J->in : mode == OUT_RES ? J->x : mode == OUT_AP ? J->p : J->t;
         const Cx<T>* vc = mode == OUT_RES ? J->e : J->r0;
         const bool use_c = mode != OUT_PLAIN;
@@ -345,10 +354,8 @@ __global__ void __launch_bounds__(kThreads, (zy_waves<T, true>())) k_pyz(const J
                 idx[u] = e < nout ? zy_index(J, base, e) : -1;
                 a[u] = b[u] = c[u] = Cx<T>{T(0), T(0)};
                 if (idx[u] >= 0) {
-                    if (use_al) {
-                        a[u] = gld(J->al + idx[u]);
-                        b[u] = gld(vb + idx[u]);
-                    }
+                    if (use_al) a[u] = gld(J->al + idx[u]);
+                    if (use_al || prec) b[u] = gld(vb + idx[u]);
                     if (use_c) c[u] = gld(vc + idx[u]);
                 }
             }
@@ -368,13 +375,13 @@ __global__ void __launch_bounds__(kThreads, (zy_waves<T, true>())) k_pyz(const J
                     acc[0] += rd.x * rd.x + rd.y * rd.y;
                     acc[1] += ed.x * ed.x + ed.y * ed.y;
                 } else if (mode == OUT_AP) {
-                    const Cx<T> ap = cadd(conv, cmul(a[u], b[u]));
+                    const Cx<T> ap = cadd(conv, prec ? b[u] : cmul(a[u], b[u]));
                     gst(J->ap + i, ap);
                     const double2 r0 = to_d(c[u]), bd = to_d(ap);
                     acc[0] += r0.x * bd.x + r0.y * bd.y;  // conj(r0) Ap
                     acc[1] += r0.x * bd.y - r0.y * bd.x;
                 } else {  // OUT_AT
-                    const Cx<T> at = cadd(conv, cmul(a[u], b[u]));
+                    const Cx<T> at = cadd(conv, prec ? b[u] : cmul(a[u], b[u]));
                     gst(J->at + i, at);
                     at_sums(acc, at, b[u], c[u]);
                 }

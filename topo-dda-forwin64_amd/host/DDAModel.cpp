@@ -107,6 +107,7 @@ void DDAModel::bicgstab(int MAX_ITERATION, double MAX_ERROR)
     o.max_it = MAX_ITERATION;
     o.tol = MAX_ERROR;
     o.mode = solver_mode;
+    o.precond = solver_precond;
     o.x0_zero = 0;
     check(tdda_model_solve(dev, &o, &info));
     check(tdda_model_get(dev, TDDA_VEC_P, 0, reinterpret_cast<double*>(P.data())));

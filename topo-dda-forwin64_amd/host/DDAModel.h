@@ -73,6 +73,11 @@ public:
     // reference algorithm in complex128; TDDA_MODE_MIXED runs complex64 inner
     // solves under FP64 refinement with a true-residual stop test.
     void set_solver_mode(int mode) { solver_mode = mode; }
+    // TDDA_PRECOND_JACOBI (not in the reference): BiCGSTAB on A diag(al)^-1,
+    // the same residual and solution in far fewer iterations when 1/alpha
+    // varies across the sites; TDDA_PRECOND_NONE (default) is the
+    // reference's unpreconditioned iteration.
+    void set_solver_precond(int precond) { solver_precond = precond; }
     const tdda_solve_info& last_info() const { return info; }
     // The device state (not in the reference): objectives evaluate P there
     // and write adjoint sources into its E.  to_device uploads the host
@@ -98,6 +103,7 @@ private:
     int ITERATION = 0;
     double Error = 0.0;
     int solver_mode = TDDA_MODE_FP64;
+    int solver_precond = TDDA_PRECOND_NONE;
     tdda_solve_info info{};
     tdda_model dev = nullptr;
 };

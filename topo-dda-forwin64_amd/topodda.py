@@ -23,6 +23,7 @@ C64 = 2
 VEC_AL, VEC_E, VEC_P, VEC_Y = 0, 1, 2, 3
 MODE_FP64, MODE_MIXED = 0, 1
 CONVERGED, BREAKDOWN, MAXIT = 1, 2, 3
+PRECOND_NONE, PRECOND_JACOBI = 0, 1
 
 EXPORTS = [
     "tdda_last_error", "tdda_version", "tdda_ctx_create", "tdda_ctx_destroy", "tdda_ctx_sync",
@@ -54,7 +55,8 @@ class OpDesc(ctypes.Structure):
 class SolveOpts(ctypes.Structure):
     _fields_ = [("max_it", ctypes.c_int), ("tol", ctypes.c_double), ("mode", ctypes.c_int),
                 ("x0_zero", ctypes.c_int), ("inner_tol", ctypes.c_double), ("inner_max_it", ctypes.c_int),
-                ("max_outer", ctypes.c_int), ("check_every", ctypes.c_int), ("true_residual", ctypes.c_int)]
+                ("max_outer", ctypes.c_int), ("check_every", ctypes.c_int), ("true_residual", ctypes.c_int),
+                ("precond", ctypes.c_int)]
 
 
 class SolveInfo(ctypes.Structure):
