@@ -494,17 +494,18 @@ def bench_periodic(T, W, ctx, reps):
             "tensor_evals_per_s": round(evals / float(np.median(ts)), 0)}
 
 
-def bench_serve(T, seconds, rate=200.0, seed=5):
+def bench_serve(T, seconds, rate=200.0, seed=5, precond=0):
     """Request stream (workloads.serve_stream): Poisson arrivals at
     `rate`/s for `seconds`, 8k-2M dipoles, 1-4 right-hand sides, half
-    adjoint; mixed precision to a true residual of 1e-8.  Latency =
-    result ready - scheduled arrival (monotonic clock); service = result
-    ready - admission."""
+    adjoint; mixed precision to a true residual of 1e-8 (with right Jacobi
+    preconditioning when `precond` is PRECOND_JACOBI).  Latency = result
+    ready - scheduled arrival (monotonic clock); service = result ready -
+    admission."""
     import threading
     import workloads as W
     reqs, pool = W.serve_stream(seconds, rate, seed)
     mat = T.material_rule(np.array([1.0, 2.25 + 0.01j]))
-    srv = T.Server(0)
+    srv = T.Server(0, precond=precond)
     done = []
     tickets = []
     lock = threading.Condition()
@@ -555,7 +556,8 @@ def bench_serve(T, seconds, rate=200.0, seed=5):
     nrhs = sum(len(infos) for _, _, infos in ok)
     tr = [i["true_residual"] for _, _, infos in ok for i in infos]
     return {"workload": f"serve: Poisson {rate:g}/s for {seconds:g} s, dipoles log-U[8k, 2M], Nz in {{8,16,32}}, "
-                        "1-4 RHS, half adjoint, mixed precision to 1e-8",
+                        "1-4 RHS, half adjoint, mixed precision to 1e-8"
+                        + (", right Jacobi preconditioning" if precond else ""),
             "requests": len(reqs), "completed": len(ok), "failed": len(done) - len(ok),
             "errors": dict(collections.Counter(err for _, tm, err in done if not tm).most_common(5)),
             "rhs_solved": nrhs, "solves_per_s": nrhs / (t_end - t0),
@@ -678,7 +680,7 @@ def main():
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--workloads", default="single16,topo80,mixed128,design,periodic,serve",
+    ap.add_argument("--workloads", default="single16,topo80,mixed128,design,periodic,serve,serve_jacobi",
                     help="workloads reported beside the headline (comma list, or none)")
     ap.add_argument("--extra-reps", type=int, default=3)
     ap.add_argument("--serve-seconds", type=float, default=1.0, help="arrival window of the request stream")
@@ -706,6 +708,8 @@ def main():
         log(f"workload {w}")
         if w == "serve":
             extras[w] = bench_serve(T, args.serve_seconds)
+        elif w == "serve_jacobi":
+            extras[w] = bench_serve(T, args.serve_seconds, precond=T.PRECOND_JACOBI)
         elif w == "mixed128":
             extras[w], r3, c3 = bench_mixed128(T, W, ctx, args.extra_reps)
             roofs["mixed128_c128"], roofs["mixed128_c64"] = r3["c128"], r3["c64"]

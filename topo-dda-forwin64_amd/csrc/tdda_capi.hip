@@ -257,6 +257,7 @@ struct PassGroup {
     int tiles = 0;               // tiles of the launch (upper bound of its persistent grid)
     int ncu = 0;
     size_t lds = 0;
+    std::vector<int> jobs;       // the group's jobs (indices into the table)
 };
 
 // Jobs of one set of launches.  Each pass runs one launch per group of jobs
@@ -274,6 +275,9 @@ struct JobTable {
     SolverState* st = nullptr;
     double* part = nullptr;
     const AuxStreams* aux = nullptr;  // the context's, for passes of several groups
+    // while the solver iterates: the jobs not yet seen stopped; a pass
+    // skips the groups with none (empty: every group runs)
+    std::vector<char> live;
 };
 
 struct tdda_model_s {
@@ -631,6 +635,7 @@ int build_groups(const tdda_ctx_s* c, JobTable<T>& jt, Put&& put)
             for (int j = 0; j < nj; j++) {
                 if (key[j] != k) continue;
                 sub.push_back(jt.h[j]);
+                g.jobs.push_back(j);
                 subt.push_back(ntiles(tiles[j]));
                 g.lds = std::max(g.lds, lds(jt.h[j]));
             }
@@ -705,25 +710,31 @@ int finalize_table(std::vector<void*>& arena, const tdda_ctx_s* c, JobTable<T>& 
 //
 // A pass may take groups of two lists (gb may be empty): the separate y or
 // z pass of some jobs and the fused z + y pass of the others, which run
-// side by side.
+// side by side.  Groups whose jobs have all stopped (jt.live) are not
+// launched: their kernels would return at once on every tile.
 template <typename T, typename LA, typename LB>
 void fan_out2(const JobTable<T>& jt, const std::vector<PassGroup<T>>& ga, LA&& launch_a,
               const std::vector<PassGroup<T>>& gb, LB&& launch_b, hipStream_t s)
 {
     const AuxStreams* a = jt.aux;
-    const size_t n = ga.size() + gb.size();
+    auto group = [&](size_t i) -> const PassGroup<T>& { return i < ga.size() ? ga[i] : gb[i - ga.size()]; };
     auto run = [&](size_t i, hipStream_t st) {
         if (i < ga.size()) launch_a(ga[i], st);
         else launch_b(gb[i - ga.size()], st);
     };
+    std::vector<size_t> order;
+    order.reserve(ga.size() + gb.size());
+    for (size_t i = 0; i < ga.size() + gb.size(); i++) {
+        bool any = jt.live.empty();
+        for (size_t k = 0; !any && k < group(i).jobs.size(); k++) any = jt.live[group(i).jobs[k]] != 0;
+        if (any) order.push_back(i);
+    }
+    const size_t n = order.size();
     if (n <= 1 || !a || a->n == 0) {
-        for (size_t i = 0; i < n; i++) run(i, s);
+        for (size_t i : order) run(i, s);
         return;
     }
-    auto tiles = [&](size_t i) { return i < ga.size() ? ga[i].tiles : gb[i - ga.size()].tiles; };
-    std::vector<size_t> order(n);
-    for (size_t i = 0; i < n; i++) order[i] = i;
-    std::stable_sort(order.begin(), order.end(), [&](size_t x, size_t y) { return tiles(x) > tiles(y); });
+    std::stable_sort(order.begin(), order.end(), [&](size_t x, size_t y) { return group(x).tiles > group(y).tiles; });
     const int nstreams = std::min<int>(a->n + 1, (int)n);
     (void)hipEventRecord(a->fork, s);
     for (int k = 1; k < nstreams; k++) (void)hipStreamWaitEvent(a->s[k - 1], a->fork, 0);
@@ -888,6 +899,13 @@ int run_iterations(SolveSet& S, JobTable<T>& jt, SolverState* hst, int max_total
     };
     std::vector<double> prev_err(nj, -1.0);
     std::vector<int> prev_it(nj, 0);
+    // jobs seen stopped leave the launch groups; the table runs every group
+    // again once the iterations end
+    jt.live.assign(nj, 1);
+    struct LiveReset {
+        std::vector<char>& v;
+        ~LiveReset() { v.clear(); }
+    } live_reset{jt.live};
     int cur = 0;
     CHK(issue(0, check_every > 0 ? check_every : 4));
     if (issued < limit) CHK(issue(1, check_every > 0 ? check_every : 4));
@@ -898,7 +916,10 @@ int run_iterations(SolveSet& S, JobTable<T>& jt, SolverState* hst, int max_total
         bool running = false;
         int need = 0;
         for (int j = 0; j < nj; j++) {
-            if (St[j].status != ST_RUNNING) continue;
+            if (St[j].status != ST_RUNNING) {
+                jt.live[j] = 0;
+                continue;
+            }
             running = true;
             int rem = 4;
             if (prev_err[j] > 0 && St[j].iter > prev_it[j] && St[j].err < prev_err[j] && St[j].err > 0) {
@@ -941,6 +962,7 @@ int init_states(SolveSet& S, JobTable<T>& jt, SolverState* hst, const std::vecto
         St.maxit = maxit;
         St.first = 1;
         St.prec = prec;
+        St.idle = !active[j];
         St.status = active[j] ? ST_RUNNING : ST_CONVERGED;
     }
     HIPCHK(hipMemcpyAsync(jt.st, hst, sizeof(SolverState) * S.nj, hipMemcpyHostToDevice, S.s));
@@ -1008,15 +1030,20 @@ double inner_tolerance(const tdda_solve_opts* o, double rel)
     return std::max(o->inner_tol, 0.5 * o->tol / rel);
 }
 
+// One mixed-precision glue launch over the whole set (k_mix).
+template <int OP>
+void launch_mix(SolveSet& S, int prec)
+{
+    k_mix<OP><<<dim3(S.j64->nv), dim3(kThreads), 0, S.s>>>(S.j64->d, S.jo->d, S.j32->d, S.j64->tv, S.nj, prec);
+}
+
 int solve_mixed(SolveSet& S, const tdda_solve_opts* o, tdda_solve_info* info)
 {
     hipStream_t s = S.s;
     const int nj = S.nj;
     if (!S.j32) return fail(TDDA_ERR_STATE, "mixed solve needs operators built with TDDA_C64");
-    auto n3of = [&](int j) { return 3 * (size_t)S.j64->h[j].N; };
-    auto blocks = [&](int j) { return (unsigned)((n3of(j) + kThreads - 1) / kThreads); };
-    if (o->x0_zero)
-        for (int j = 0; j < nj; j++) HIPCHK(hipMemsetAsync(S.j64->h[j].x, 0, sizeof(Cx<double>) * n3of(j), s));
+    const int prec = o->precond == TDDA_PRECOND_JACOBI;
+    if (o->x0_zero) launch_mix<MIX_ZERO_X>(S, prec);
     std::vector<int> active(nj, 1);
     std::vector<double> rel(nj, 0.0);
     std::vector<int> inner_it(nj, 0), outer(nj, 0);
@@ -1029,14 +1056,11 @@ int solve_mixed(SolveSet& S, const tdda_solve_opts* o, tdda_solve_info* info)
             any |= active[j] != 0;
         }
         if (!any) break;
-        // inner right-hand side = r (complex64)
-        for (int j = 0; j < nj; j++)
-            if (active[j])
-                k_demote<<<blocks(j), kThreads, 0, s>>>(S.jo->h[j].r, const_cast<Cx<float>*>(S.j32->h[j].e), n3of(j));
         std::vector<double> itol(nj);
         for (int j = 0; j < nj; j++) itol[j] = inner_tolerance(o, rel[j]);
-        CHK(init_states<float>(S, *S.j32, hin, active, o->inner_tol, o->inner_max_it, &itol,
-                               o->precond == TDDA_PRECOND_JACOBI));
+        CHK(init_states<float>(S, *S.j32, hin, active, o->inner_tol, o->inner_max_it, &itol, prec));
+        // inner right-hand side = r (complex64) of the cycle's jobs
+        launch_mix<MIX_DEMOTE>(S, prec);
         launch_init_zero<float>(*S.j32, nj, s);
         CHK(check_launch());
         CHK(run_iterations<float>(S, *S.j32, hin, o->inner_max_it, o->check_every));
@@ -1045,9 +1069,8 @@ int solve_mixed(SolveSet& S, const tdda_solve_opts* o, tdda_solve_info* info)
             if (!active[j]) continue;
             inner_it[j] += hin[j].iter;
             outer[j] += 1;
-            k_accum<<<blocks(j), kThreads, 0, s>>>(S.j64->h[j].x, S.j32->h[j].x, n3of(j),
-                                                  o->precond == TDDA_PRECOND_JACOBI ? S.j64->h[j].al : nullptr);
         }
+        launch_mix<MIX_ACCUM>(S, prec);
         CHK(check_launch());
         CHK(true_residual(S, rel, active));
     }

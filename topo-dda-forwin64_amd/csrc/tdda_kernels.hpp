@@ -86,6 +86,7 @@ struct SolverState {
     double enorm, rnorm, err, tol;
     int iter, maxit, status, first, pend;
     int prec;  // right Jacobi preconditioning: the solve runs on A D^-1, D = diag(al)
+    int idle;  // not part of this solve phase (a refinement cycle after it converged)
 };
 
 // Partial sums per tile of a reducing pass (the At pass needs eight).
