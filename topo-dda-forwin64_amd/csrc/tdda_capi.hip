@@ -25,6 +25,7 @@
 #include "tdda_setup.hpp"
 #include "tdda_launch.hpp"
 #include "tdda_zy_plan.hpp"
+#include "tdda_grid.hpp"
 
 using namespace tdda;
 
@@ -1465,7 +1466,7 @@ int check_desc(const tdda_op_desc& D, int* M)
     M[2] = D.Mz;
     const int Nn[3] = {D.Nx, D.Ny, D.Nz};
     for (int a = 0; a < 3; a++) {
-        if (M[a] == 0) M[a] = next_smooth(2 * Nn[a] - 1);
+        if (M[a] == 0) M[a] = default_grid_len(a, 2 * Nn[a] - 1, prec);
         if (M[a] < 2 * Nn[a] - 1) return fail(TDDA_ERR_ARG, "FFT grid smaller than 2N-1 along an axis");
         if (!smooth235(M[a])) return fail(TDDA_ERR_UNSUPPORTED, "FFT grid sizes must have prime factors 2, 3, 5 only");
         if (M[a] > 4096) return fail(TDDA_ERR_UNSUPPORTED, "FFT length above 4096");
