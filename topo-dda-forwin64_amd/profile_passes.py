@@ -86,7 +86,9 @@ def main():
     if a.plan_out:
         cl = cores if a.workload == "batch64" else [core]
         with open(a.plan_out, "w") as fh:
-            json.dump({p: [c.zy_fused(q) for c in cl] for p, q in (("c128", T.C128), ("c64", T.C64))}, fh)
+            plan = {p: [c.zy_fused(q) for c in cl] for p, q in (("c128", T.C128), ("c64", T.C64))}
+            plan["grids"] = [list(c.grid) for c in cl]
+            json.dump(plan, fh)
     copy_calibration(a.copy_mib)
     print(json.dumps({"workload": a.workload, "precision": a.precision, "grid": grid,
                       "pass_ms": [float(x) for x in ms], "copy_mib": a.copy_mib}))

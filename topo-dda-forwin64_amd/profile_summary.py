@@ -166,11 +166,12 @@ def traffic(fetch_csv, write_csv, precision, workload, copy_mib, out_json, last_
 
     # which operators ran fused z + y passes: profile_passes.py's plan file
     # beside the counter files (one flag per structure, in workload order)
-    plan = None
+    plan, grids = None, None
     pf = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(fetch_csv))), "plan.json")
     if os.path.exists(pf):
         with open(pf) as fh:
-            plan = json.load(fh).get(precision)
+            pj = json.load(fh)
+        plan, grids = pj.get(precision), pj.get("grids")
 
     class Core:
         def zy_fused(self, prec):
@@ -179,7 +180,9 @@ def traffic(fetch_csv, write_csv, precision, workload, copy_mib, out_json, last_
     def core_of(p, k=0):
         c = Core()
         c.Nx, c.Ny, c.Nz, c.N = p.Nx, p.Ny, p.Nz, p.N
-        c.grid = tuple(next_smooth(2 * n - 1) for n in (p.Nx, p.Ny, p.Nz))
+        # the operator's grid as the planner chose it (plan file), else the
+        # smallest smooth lengths
+        c.grid = tuple(grids[k]) if grids else tuple(next_smooth(2 * n - 1) for n in (p.Nx, p.Ny, p.Nz))
         c.fused = bool(plan[k]) if plan else False
         return c
     if workload == "batch64":
