@@ -367,7 +367,7 @@ def bench_topo80(T, W, ctx, steps, warmup):
         m.set(T.VEC_AL, al)
         m.al_host = al
     F.set(T.VEC_E, E)
-    out = {"workload": "topo80: 80x80x16 slab, d=10nm, lambda=542nm, grid 160x160x32, forward + adjoint "
+    out = {"workload": "topo80: 80x80x16 slab, d=10nm, lambda=542nm, forward + adjoint "
                        "(100 targets) per step", "grid": list(core.grid), "dipoles": prob.N,
            "xpass_c128": core.xpass(T.C128), "ypass_c128": core.ypass(T.C128)}
     P = {}
@@ -475,7 +475,7 @@ def bench_design(T, W, ctx, reps):
 
 def bench_periodic(T, W, ctx, reps):
     """Periodic operator set-up on the device (tdda_op_create_periodic): the
-    image-lattice tensor of an 80 x 80 x 16 unit cell (grid 160 x 160 x 32)
+    image-lattice tensor of an 80 x 80 x 16 unit cell (the planner's grid)
     with 21 x 21 images, then its spectrum -- the reference sums the images
     on one host thread (AProductCore.cpp:301-536)."""
     pm = pn = 10
@@ -489,7 +489,7 @@ def bench_periodic(T, W, ctx, reps):
         grid = core.grid
         core.close()
     evals = (2 * Nx - 1) * (2 * Ny - 1) * (2 * Nz - 1) * (2 * pm + 1) * (2 * pn + 1)
-    return {"workload": "periodic: 80x80x16 cell, grid 160x160x32, 21x21 images, Lm = Ln = 800 nm",
+    return {"workload": "periodic: 80x80x16 cell, 21x21 images, Lm = Ln = 800 nm",
             "grid": list(grid), "images": (2 * pm + 1) * (2 * pn + 1), "setup_ms": round(float(np.median(ts)) * 1e3, 1),
             "tensor_evals_per_s": round(evals / float(np.median(ts)), 0)}
 

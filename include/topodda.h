@@ -103,8 +103,9 @@ int tdda_ctx_sync(tdda_ctx ctx);
  * order (tools.cpp:497-510).  grid = (Mx, My, Mz) chooses the zero-padded FFT grid; 0 lets
  * the planner pick: My, Mz the smallest {2,3,5}-smooth sizes >= 2N_a - 1
  * (My: 75 and 125 skipped for 80 and 128), Mx the length within 30 % above
- * 2Nx - 1 whose x pass costs least on MI355X (a measured table; the x pass
- * moves the same bytes for any Mx).  Any grid >= 2N_a - 1 gives the same
+ * 2Nx - 1 whose x pass costs least on MI355X when that is more than 5 %
+ * below the smallest smooth length's (a measured table; the x pass moves
+ * the same bytes for any Mx).  Any grid >= 2N_a - 1 gives the same
  * linear convolution as the reference's 2N_a - 1.
  * K = 2 pi nback / lam (AProductCore.cpp:17).                               */
 typedef struct {

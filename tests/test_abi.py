@@ -104,15 +104,17 @@ def test_material_rule_threshold_of_each_constructor():
 def test_op_plan_grid_and_32bit_guard():
     # grids the planner picks: y and z the smallest 2,3,5-smooth >= 2N - 1
     # (y: 75 and 125 skipped), x among the lengths within 30 % above 2Nx - 1
-    # the one whose slower precision is closest to its best x-pass time
+    # the one whose slower precision is closest to its best x-pass time, if
+    # more than 5 % better than the smallest smooth length
     assert T.op_plan(16, 16, 16) == (32, 32, 32)
     assert T.op_plan(63, 63, 12) == (128, 128, 24)  # 125 = 5^3: 1.5x / 2.4x the x-pass time of 128
     assert T.op_plan(38, 38, 8)[1:] == (80, 15)
     assert T.op_plan(48, 20, 5) == (100, 40, 9)     # 96 = 12 x 8 costs more than 100 = 10 x 10
-    assert T.op_plan(100, 10, 4)[0] == 256          # 200 = 20 x 10 has no complex128 register kernel
+    assert T.op_plan(100, 10, 4)[0] == 200
+    assert T.op_plan(121, 10, 4)[0] == 256          # 243 = 27 x 9 and 250 = 25 x 10 cost more
     assert T.op_plan(200, 10, 4)[0] == 400
     assert T.op_plan(400, 10, 4)[0] == 800          # beyond the table: next smooth length
-    assert T.op_plan(80, 80, 16) == (160, 160, 32)
+    assert T.op_plan(80, 80, 16) == (160, 160, 32)  # 162 measures within 5 % of 160: the smaller stays
     assert T.op_plan(128, 128, 32) == (256, 256, 64)
     assert T.op_plan(512, 512, 64) == (1024, 1024, 128)
     # the pass kernels use 32-bit offsets within one right-hand side's

@@ -122,3 +122,19 @@ def test_autotune_for_several_right_hand_sides(ctx):
         with pytest.raises(T.TddaError):
             core.autotune(reps=5, nrhs=bad)
     core.close()
+
+
+def test_autotune_keeps_wide_complex128_register_x_pass(ctx):
+    # x length 162 = 18 x 9: a factor above 16, which the register kernels
+    # run with 512 threads in both precisions; the tuner must time that
+    # kernel for complex128 too (the LDS-staged one is twice as slow)
+    Nx, Ny, Nz, d, lam = 81, 32, 12, 10.0, 500.0
+    core = T.AProductCore(ctx, None, Nx, Ny, Nz, d, lam, grid=(162, 0, 0))
+    core.autotune(reps=5)
+    assert core.xpass(T.C128) == "register" and core.xpass(T.C64) == "register"
+    R = O.build_a_bulk(Nx, Ny, Nz)
+    b = np.random.default_rng(9).standard_normal(len(R)) + 0.25j
+    ref = O.aproduct_numpy_fft(R, Nx, Ny, Nz, d, O.wavenumber(lam), b, grid=core.grid)
+    assert O.rel_l2(core.Aproduct(b), ref) < 1e-12
+    assert O.rel_l2(core.Aproduct(b, precision=T.C64), ref) < 5e-6
+    core.close()

@@ -142,6 +142,20 @@ def test_register_y_pass(ctx, monkeypatch, dims, grid, frac):
 LONG = [  # complex64 register kernels with factors above 16: (dims, grid, axis)
     ((63, 3, 2), (125, 8, 8), "x"),
     ((81, 4, 2), (162, 8, 8), "x"),
+    ((100, 3, 2), (200, 8, 8), "x"),
+    ((108, 2, 2), (216, 8, 8), "x"),
+    ((122, 2, 2), (243, 8, 8), "x"),
+    ((162, 2, 2), (324, 8, 8), "x"),
+    ((180, 2, 2), (360, 8, 8), "x"),
+    ((200, 2, 2), (400, 8, 8), "x"),
+    ((203, 2, 2), (405, 8, 8), "x"),
+    ((243, 2, 2), (486, 8, 8), "x"),
+    ((240, 2, 2), (480, 8, 8), "x"),
+    ((250, 2, 2), (500, 8, 8), "x"),
+    ((270, 2, 2), (540, 8, 8), "x"),
+    ((2, 200, 2), (8, 400, 8), "y"),
+    ((2, 81, 2), (8, 162, 8), "y"),
+    ((2, 125, 2), (8, 250, 8), "y"),
     ((135, 3, 2), (270, 8, 8), "x"),
     ((256, 3, 2), (512, 8, 8), "x"),
     ((313, 2, 2), (625, 8, 8), "x"),
@@ -157,11 +171,17 @@ LONG = [  # complex64 register kernels with factors above 16: (dims, grid, axis)
 ]
 
 
+# complex128 register kernels with a factor above 16: the lengths up to 540
+# whose transforms fit the registers of two or three waves
+MID128 = {125, 162, 200, 216, 243, 250, 270, 288, 300, 320, 324, 360, 375, 384, 400, 405, 432, 450, 480, 486,
+          500, 512, 540}
+
+
 @pytest.mark.parametrize("dims,grid,axis", LONG)
 def test_register_passes_long_complex64(ctx, monkeypatch, dims, grid, axis):
     # lengths 125-1024: the complex64 x and y passes run k_pxr / k_pyr with
-    # 512-thread workgroups and factors up to 40; complex128 keeps the
-    # LDS-staged kernels there
+    # 512-thread workgroups and factors up to 40; complex128 runs them for
+    # the lengths of MID128 and keeps the LDS-staged kernels for the others
     Nx, Ny, Nz = dims
     print(f"long register {axis} pass {dims} grid {grid}", flush=True)
     R, b = problem(Nx, Ny, Nz, 1.0, seed=grid[0] + grid[1])
@@ -170,14 +190,18 @@ def test_register_passes_long_complex64(ctx, monkeypatch, dims, grid, axis):
     monkeypatch.setenv("TDDA_YPASS", "1")
     reg = T.AProductCore(ctx, R, Nx, Ny, Nz, 10.0, 500.0, grid=grid)
     kind = reg.xpass if axis == "x" else reg.ypass
-    assert kind(T.C64) == "register" and kind(T.C128) == "lds"
+    L = grid[0] if axis == "x" else grid[1]
+    assert kind(T.C64) == "register" and kind(T.C128) == ("register" if L in MID128 else "lds")
     y32 = reg.Aproduct(b, precision=T.C64)
     assert O.rel_l2(y32, ref) < TOL32
-    assert O.rel_l2(reg.Aproduct(b), ref) < TOL64
+    y64 = reg.Aproduct(b)
+    assert O.rel_l2(y64, ref) < TOL64
     reg.close()
     monkeypatch.setenv("TDDA_XPASS", "0")
     monkeypatch.setenv("TDDA_YPASS", "0")
     lds = T.AProductCore(ctx, R, Nx, Ny, Nz, 10.0, 500.0, grid=grid)
     z32 = lds.Aproduct(b, precision=T.C64)
+    z64 = lds.Aproduct(b)
     lds.close()
     assert O.rel_l2(y32, z32) < 2 * TOL32
+    assert O.rel_l2(y64, z64) < TOL64

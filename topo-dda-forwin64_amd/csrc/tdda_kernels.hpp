@@ -860,12 +860,20 @@ __host__ __device__ constexpr int pyr_nf(int L, int nt = kThreads)
     return pxr_l1(L) <= 0 ? 0 : (nt / pxr_l1(L) >= 8 ? ((nt / pxr_l1(L)) / 8) * 8 : nt / pxr_l1(L));
 }
 
-// complex128 at 256 points needs the registers of three waves (with four
-// it spills); all others fit four.
+// complex128 from 256 points: three waves, two where the inner factor of
+// its split reaches 20 (400 = 20 x 20 .. 540 = 27 x 20 spill at three)
+template <typename T>
+__host__ __device__ constexpr int wide_c128_waves(int L)
+{
+    return (pxr_l1(L) > 0 && L / pxr_l1(L) >= 20) ? 2 : 3;
+}
+
+// complex128 from 256 points needs the registers of three waves (with four
+// it spills), two where its inner factor reaches 20; all others fit four.
 template <typename T>
 __host__ __device__ constexpr int pyr_waves(int L)
 {
-    return (sizeof(T) == 8 && L >= 256) ? 3 : pxr_l1(L) > 16 ? TDDA_LONG_WAVES : 4;
+    return (sizeof(T) == 8 && L >= 256) ? wide_c128_waves<T>(L) : pxr_l1(L) > 16 ? TDDA_LONG_WAVES : 4;
 }
 
 template <typename T, bool INV, int LC, int NT = kThreads>
@@ -1092,11 +1100,13 @@ __host__ __device__ constexpr int pxr_nf(int L, int nt = kThreads)
 
 // Waves per SIMD k_pxr is compiled for, the most at which the compiler
 // keeps it out of scratch: four (complex64 50-64 VGPRs, complex128 73-122),
-// three for complex128 at 256 points (131 VGPRs; with four it spills).
+// three for complex128 from 256 points (131-165 VGPRs; with four they
+// spill), two where complex128's inner factor reaches 20 (400 = 20 x 20 ..
+// 540 = 27 x 20 spill at three).
 template <typename T>
 __host__ __device__ constexpr int pxr_waves(int L)
 {
-    return (sizeof(T) == 8 && L >= 256) ? 3 : pxr_l1(L) > 16 ? TDDA_LONG_WAVES : 4;
+    return (sizeof(T) == 8 && L >= 256) ? wide_c128_waves<T>(L) : pxr_l1(L) > 16 ? TDDA_LONG_WAVES : 4;
 }
 
 template <typename T, int LC, int NT>
