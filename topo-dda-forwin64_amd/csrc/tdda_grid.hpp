@@ -16,25 +16,31 @@ namespace tdda {
 // x-pass time (us) per length on MI355X: a 16 x 32 x 12 box with 8
 // right-hand sides (so the loads and stores are the same for every length),
 // complex64 and complex128, the kernel the planner picks for that length
-// (register-resident up to 540 in both precisions).  Measured with topo-dda-forwin64_amd/profile_lengths.py.
+// (register-resident up to 540 in both precisions, above 540 in complex64
+// only: complex128 runs the LDS transform there).  Measured with topo-dda-forwin64_amd/profile_lengths.py.
 struct XLenCost {
     int L;
     float c64, c128;
 };
 constexpr XLenCost kXLenCost[] = {
-    {32, 7.31f, 9.07f}, {36, 7.30f, 9.48f}, {40, 7.95f, 10.30f}, {45, 8.50f, 11.20f},
-    {48, 8.17f, 10.74f}, {50, 8.44f, 11.02f}, {54, 8.69f, 11.71f}, {60, 9.25f, 12.52f},
-    {64, 9.46f, 14.86f}, {72, 10.27f, 15.59f}, {75, 12.04f, 19.60f}, {80, 10.26f, 15.64f},
-    {81, 10.69f, 15.09f}, {90, 11.18f, 16.00f}, {96, 12.55f, 21.12f}, {100, 11.60f, 16.86f},
-    {108, 12.78f, 21.63f}, {120, 15.22f, 24.13f}, {125, 21.80f, 51.98f}, {128, 14.18f, 22.77f},
-    {135, 16.41f, 27.73f}, {144, 15.75f, 25.66f}, {150, 17.51f, 28.87f}, {160, 17.65f, 33.01f},
-    {162, 16.86f, 31.88f}, {180, 21.49f, 35.89f}, {192, 20.36f, 35.00f}, {200, 18.94f, 36.21f},
-    {216, 24.49f, 50.28f}, {225, 26.17f, 49.30f}, {240, 26.52f, 49.67f}, {243, 34.65f, 69.24f},
-    {250, 29.15f, 67.75f}, {256, 26.51f, 48.71f}, {270, 33.02f, 59.24f}, {288, 33.27f, 60.62f},
-    {300, 33.81f, 61.08f}, {320, 33.97f, 61.49f}, {324, 36.22f, 66.39f}, {360, 37.59f, 70.01f},
-    {375, 49.61f, 93.75f}, {384, 48.97f, 88.75f}, {400, 40.14f, 72.85f}, {405, 52.72f, 99.05f},
-    {432, 52.15f, 95.79f}, {450, 55.64f, 103.56f}, {480, 76.19f, 98.39f}, {486, 56.50f, 107.33f},
-    {500, 59.16f, 105.74f}, {512, 60.66f, 110.46f}, {540, 81.24f, 169.49f},
+    {32, 7.32f, 9.09f}, {36, 7.31f, 9.47f}, {40, 7.96f, 10.30f}, {45, 8.47f, 11.23f},
+    {48, 8.17f, 10.75f}, {50, 8.45f, 11.02f}, {54, 8.69f, 11.71f}, {60, 9.24f, 12.54f},
+    {64, 9.45f, 14.96f}, {72, 10.33f, 15.60f}, {75, 12.06f, 19.52f}, {80, 10.27f, 15.68f},
+    {81, 10.66f, 15.01f}, {90, 11.19f, 15.94f}, {96, 12.58f, 21.15f}, {100, 11.62f, 16.70f},
+    {108, 12.83f, 21.56f}, {120, 15.24f, 23.98f}, {125, 21.80f, 51.81f}, {128, 14.22f, 22.77f},
+    {135, 16.41f, 27.49f}, {144, 15.75f, 25.45f}, {150, 17.52f, 28.66f}, {160, 17.67f, 32.95f},
+    {162, 16.90f, 31.42f}, {180, 21.47f, 35.42f}, {192, 20.36f, 34.49f}, {200, 18.98f, 35.48f},
+    {216, 24.54f, 49.96f}, {225, 26.22f, 48.91f}, {240, 26.48f, 49.11f}, {243, 34.53f, 68.50f},
+    {250, 29.21f, 68.18f}, {256, 26.61f, 48.77f}, {270, 32.95f, 59.30f}, {288, 33.21f, 59.63f},
+    {300, 33.65f, 59.38f}, {320, 33.85f, 61.11f}, {324, 36.04f, 66.07f}, {360, 37.28f, 68.84f},
+    {375, 49.22f, 92.88f}, {384, 48.52f, 87.77f}, {400, 39.96f, 72.84f}, {405, 52.44f, 98.44f},
+    {432, 51.66f, 93.75f}, {450, 54.98f, 101.85f}, {480, 75.92f, 97.41f}, {486, 55.35f, 106.11f},
+    {500, 58.63f, 104.12f}, {512, 60.22f, 109.76f}, {540, 80.12f, 169.05f}, {576, 83.16f, 399.07f},
+    {600, 85.83f, 420.16f}, {625, 89.91f, 433.97f}, {640, 104.62f, 410.90f}, {648, 90.35f, 512.22f},
+    {675, 93.46f, 518.06f}, {720, 114.11f, 551.53f}, {729, 98.42f, 612.23f}, {750, 114.18f, 536.81f},
+    {768, 112.50f, 480.86f}, {800, 117.26f, 752.64f}, {810, 126.21f, 1101.62f}, {864, 125.62f, 991.51f},
+    {900, 134.46f, 1002.25f}, {960, 134.89f, 857.64f}, {972, 153.72f, 1158.07f}, {1000, 157.67f, 876.11f},
+    {1024, 134.79f, 891.96f},
 };
 
 inline int grid_next_smooth(int n)

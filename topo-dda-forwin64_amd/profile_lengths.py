@@ -1,5 +1,5 @@
 """Measures the x-pass cost table of the grid planner (kXLenCost in
-csrc/tdda_grid.hpp): for every 2,3,5-smooth x length from 32 to 540, the
+csrc/tdda_grid.hpp): for every 2,3,5-smooth x length from 32 to 1024, the
 x pass of a 16 x 32 x 12 box with 8 right-hand sides (so every length loads
 and stores the same rows), complex64 and complex128, best of two runs of 40
 launches.  Prints the table rows.  Needs a GPU:
@@ -25,7 +25,7 @@ def smooth(n):
 def main():
     ctx = T.Context(0)
     rows = []
-    for L in [L for L in range(32, 541) if smooth(L)]:
+    for L in [L for L in range(32, 1025) if smooth(L)]:
         core = T.AProductCore(ctx, None, 16, 32, 12, 10.0, 500.0, grid=(L, 0, 0))
         m = T.DeviceModel(core, 8)
         v = np.random.default_rng(1).standard_normal(3 * core.N * 8) + 0j
