@@ -1007,12 +1007,15 @@ int solve_fp64(SolveSet& S, const tdda_solve_opts* o, tdda_solve_info* info)
 
 // FP64 true residual r = E - A x into jo's r vector; rel <- ||r|| / ||E|| for
 // the active jobs.
-int true_residual(SolveSet& S, std::vector<double>& rel, const std::vector<int>& active)
+// zero_x: x = 0 first (r = E exactly, since A 0 = 0: no operator
+// application, one pass over E).
+int true_residual(SolveSet& S, std::vector<double>& rel, const std::vector<int>& active, bool zero_x = false)
 {
     const int nj = S.nj;
     SolverState* hst = S.h_st + nj;
     CHK(init_states<double>(S, *S.jo, hst, active, 0.0, 0));
-    launch_apply<double>(*S.jo, nj, IN_X, OUT_RES, 1, 1, S.s);
+    if (zero_x) launch_init_zero<double>(*S.jo, nj, S.s);
+    else launch_apply<double>(*S.jo, nj, IN_X, OUT_RES, 1, 1, S.s);
     CHK(check_launch());
     HIPCHK(hipMemcpyAsync(hst, S.jo->st, sizeof(SolverState) * nj, hipMemcpyDeviceToHost, S.s));
     HIPCHK(hipStreamSynchronize(S.s));
@@ -1044,11 +1047,10 @@ int solve_mixed(SolveSet& S, const tdda_solve_opts* o, tdda_solve_info* info)
     const int nj = S.nj;
     if (!S.j32) return fail(TDDA_ERR_STATE, "mixed solve needs operators built with TDDA_C64");
     const int prec = o->precond == TDDA_PRECOND_JACOBI;
-    if (o->x0_zero) launch_mix<MIX_ZERO_X>(S, prec);
     std::vector<int> active(nj, 1);
     std::vector<double> rel(nj, 0.0);
     std::vector<int> inner_it(nj, 0), outer(nj, 0);
-    CHK(true_residual(S, rel, active));
+    CHK(true_residual(S, rel, active, o->x0_zero != 0));
     SolverState* hin = S.h_st + 2 * nj;
     for (int c = 0; c < o->max_outer; c++) {
         bool any = false;

@@ -41,13 +41,12 @@ static __global__ void __launch_bounds__(kThreads) k_accum(Cx<double>* __restric
 
 // The same glue over a whole job set in one launch (the vector tiles of
 // the set's tables, which share one tiling: it depends on N only).
-//   MIX_ZERO_X  x64 = 0 (every job)
 //   MIX_DEMOTE  e32 = c64(r of the residual table)
 //   MIX_ACCUM   x64 += c128(x32), divided by al64 with right Jacobi
 //               preconditioning
-// DEMOTE and ACCUM skip the jobs whose complex64 state is idle (not part of
-// this refinement cycle).
-enum { MIX_ZERO_X = 0, MIX_DEMOTE = 1, MIX_ACCUM = 2 };
+// Both skip the jobs whose complex64 state is idle (not part of this
+// refinement cycle).
+enum { MIX_DEMOTE = 1, MIX_ACCUM = 2 };
 
 template <int OP>
 static __global__ void __launch_bounds__(kThreads) k_mix(const JobDesc<double>* __restrict__ j64,
@@ -56,16 +55,14 @@ static __global__ void __launch_bounds__(kThreads) k_mix(const JobDesc<double>* 
                                                          const int* __restrict__ tv, int njobs, int prec)
 {
     const int job = find_job(tv, njobs, blockIdx.x);
-    if (OP != MIX_ZERO_X && gstate(j32[job].st)->idle) return;
+    if (gstate(j32[job].st)->idle) return;
     const int tile = blockIdx.x - tv[job];
     const int n = j64[job].N * 3;
     const int i0 = tile * kThreads * kVecPer;
     for (int k = 0; k < kVecPer; k++) {
         const int i = i0 + k * kThreads + threadIdx.x;
         if (i >= n) break;
-        if (OP == MIX_ZERO_X) {
-            gst(j64[job].x + i, Cx<double>{0.0, 0.0});
-        } else if (OP == MIX_DEMOTE) {
+        if (OP == MIX_DEMOTE) {
             const Cx<double> v = gld(jo[job].r + i);
             gst(const_cast<Cx<float>*>(j32[job].e) + i, Cx<float>{float(v.x), float(v.y)});
         } else {
