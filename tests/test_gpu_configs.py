@@ -6,7 +6,8 @@ the CPU) and its BiCGSTAB (DDAModel.cpp:131-260):
     of the returned P recomputed by the oracle operator;
   * 64 structures (16..64 x 16..64 x 8..16) x 8 incidences in one batch:
     every right-hand side's true residual by the oracle operator, and three
-    structures solved alone equal the batch bit for bit;
+    structures solved alone equal the batch bit for bit; the same batch with
+    Jacobi preconditioning, residuals by the oracle operator;
   * a request stream drawn from the serving workload (8k-2M dipoles,
     1-4 right-hand sides, forward and adjoint): oracle residuals up to
     ~300k dipoles, an independent complex128 device residual for the
@@ -99,6 +100,48 @@ def test_batch64_full_batch(ctx):
         assert [i["iterations"] for i in io] == [i["iterations"] for i in infos[8 * s:8 * s + 8]]
         assert np.array_equal(one.get(T.VEC_P), models[s].get(T.VEC_P))
         one.close()
+    batch.close()
+    for m in models:
+        m.close()
+    for c in cores:
+        c.close()
+
+
+def test_batch64_full_batch_jacobi(ctx):
+    """The same 512 right-hand sides with right Jacobi preconditioning (an
+    opt-in solver option, A diag(al)^-1): every true residual <= 1e-8 by the
+    oracle operator, and P within what two 1e-8 residuals allow of the
+    unpreconditioned batch's."""
+    probs = W.batch64(seed=4)
+    incs = W.incidences()
+    print(f"batch64 jacobi: {len(probs)} structures x {len(incs)} incidences", flush=True)
+    cores, models, data = [], [], []
+    for p in probs:
+        core = T.AProductCore(ctx, p.R, p.Nx, p.Ny, p.Nz, p.d, p.lam, p.nback)
+        rhs = [p.rhs(nK, nE)[:2] for nK, nE in incs]
+        m = T.DeviceModel(core, len(incs))
+        m.set(T.VEC_AL, np.concatenate([a for a, _ in rhs]))
+        m.set(T.VEC_E, np.concatenate([e for _, e in rhs]))
+        cores.append(core)
+        models.append(m)
+        data.append(rhs)
+    batch = T.Batch(models)
+    plain = batch.solve(mode=T.MODE_MIXED, tol=TOL, x0_zero=1, true_residual=1)
+    P0 = [m.get(T.VEC_P) for m in models]
+    infos = batch.solve(mode=T.MODE_MIXED, tol=TOL, x0_zero=1, true_residual=1, precond=T.PRECOND_JACOBI)
+    assert all(i["status"] == T.CONVERGED and i["true_residual"] <= TOL for i in infos)
+    # the diagonal scaling removes most of the iterations
+    assert sum(i["iterations"] for i in infos) * 4 <= sum(i["iterations"] for i in plain)
+    k = 0
+    for p, core, m, rhs, p0 in zip(probs, cores, models, data, P0):
+        op = O.OrcOperator(p.R, p.Nx, p.Ny, p.Nz, p.d, p.K, grid=core.grid)
+        P = m.get(T.VEC_P)
+        for j, (al, E) in enumerate(rhs):
+            res = oracle_residual(op, al, E, P[j])
+            assert res <= RES_BAR, (p.name, j, res)
+            assert O.rel_l2(P[j], p0[j]) < 1e-6, (p.name, j)
+            k += 1
+        op.close()
     batch.close()
     for m in models:
         m.close()
