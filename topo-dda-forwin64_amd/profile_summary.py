@@ -122,6 +122,38 @@ def span(path, precision, workload, reps):
               f"{sum(busy) / len(busy):.2f} |")
 
 
+def sq(counter_csvs, trace_csv, top, clock_ghz=2.4, simds=1024):
+    """Per-kernel SQ counter summary of one or more --pmc runs and a kernel
+    trace of the same workload (durations): VALU busy = VALU instructions x
+    4 cycles (a wave64 instruction on a 16-lane SIMD) over the SIMD cycles
+    of the kernel's mean duration; wait = SQ_WAIT_ANY / SQ_WAVE_CYCLES; LDS
+    bank-conflict cycles per LDS-active cycle; instructions per wave."""
+    agg = collections.defaultdict(lambda: collections.defaultdict(list))
+    for f in counter_csvs:
+        for r in csv.DictReader(open(f)):
+            kn = r["Kernel_Name"].split("(")[0].replace("void ", "")
+            agg[kn][r["Counter_Name"]].append(float(r["Counter_Value"]))
+    dur = collections.defaultdict(list)
+    for r in csv.DictReader(open(trace_csv)):
+        kn = r["Kernel_Name"].split("(")[0].replace("void ", "")
+        dur[kn].append((int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) / 1e3)
+    print("| kernel | us | VALU busy % | wait % of wave cycles | LDS bank-conflict cycles per LDS-active cycle "
+          "| per wave: VALU, LDS, SALU, VMEM rd, VMEM wr |")
+    print("|---|---|---|---|---|---|")
+    names = [kn for kn in agg if kn in dur and any(k in kn for k in ("k_pxr", "k_pzy", "k_pyz", "k_pyr", "k_p1", "k_p5"))]
+    for kn in sorted(names, key=lambda k: -sum(dur[k]))[:top]:
+        c = {n: sum(v) / len(v) for n, v in agg[kn].items()}
+        us = sum(dur[kn]) / len(dur[kn])
+        cyc = us * 1e-6 * clock_ghz * 1e9 * simds
+        w = c.get("SQ_WAVES", 1) or 1
+        print(f"| {kn} | {us:.1f} | {100 * c.get('SQ_INSTS_VALU', 0) * 4 / cyc:.0f} | "
+              f"{100 * c.get('SQ_WAIT_ANY', 0) / max(c.get('SQ_WAVE_CYCLES', 1), 1):.0f} | "
+              f"{c.get('SQ_LDS_BANK_CONFLICT', 0) / max(c.get('SQ_ACTIVE_INST_LDS', 1), 1):.2f} | "
+              f"{c.get('SQ_INSTS_VALU', 0) / w:.0f}, {c.get('SQ_INSTS_LDS', 0) / w:.0f}, "
+              f"{c.get('SQ_INSTS_SALU', 0) / w:.0f}, {c.get('SQ_INSTS_VMEM_RD', 0) / w:.0f}, "
+              f"{c.get('SQ_INSTS_VMEM_WR', 0) / w:.0f} |")
+
+
 LAST_ROW = {}
 
 
@@ -235,7 +267,7 @@ def next_smooth(n):
 
 def main():
     ap = argparse.ArgumentParser()
-    ap.add_argument("mode", choices=["stats", "traffic", "last", "span"])
+    ap.add_argument("mode", choices=["stats", "traffic", "last", "span", "sq"])
     ap.add_argument("files", nargs="+")
     ap.add_argument("--top", type=int, default=20)
     ap.add_argument("--precision", default="c128")
@@ -252,6 +284,9 @@ def main():
         last(a.files[0], a.kernel, a.last)
     elif a.mode == "span":
         span(a.files[0], a.precision, a.workload, a.last)
+    elif a.mode == "sq":
+        # files: the counter CSVs, then the kernel-trace CSV
+        sq(a.files[:-1], a.files[-1], a.top)
     else:
         traffic(a.files[0], a.files[1], a.precision, a.workload, a.copy_mib, a.json, a.last)
 
