@@ -101,6 +101,21 @@ def test_material_rule_threshold_of_each_constructor():
             assert np.array_equal(T.material_rule(mat, nback, periodic), O.material_rule(mat, nback, periodic))
 
 
+def test_op_plan_grid_lengths_are_valid():
+    # every planner length is 2,3,5-smooth, at least 2N - 1 (the linear
+    # convolution) and at most 30 % above it (the x-pass cost table's window)
+    def smooth(n):
+        for p in (2, 3, 5):
+            while n % p == 0:
+                n //= p
+        return n == 1
+    for n in list(range(2, 140)) + list(range(140, 700, 7)):
+        g = T.op_plan(n, n if n < 300 else 8, 3)
+        for L, N in zip(g, (n, n if n < 300 else 8, 3)):
+            assert smooth(L) and L >= 2 * N - 1, (n, g)
+        assert g[0] <= 1.3 * (2 * n - 1) + 1, (n, g)
+
+
 def test_op_plan_grid_and_32bit_guard():
     # grids the planner picks: y and z the smallest 2,3,5-smooth >= 2N - 1
     # (y: 75 and 125 skipped), x among the lengths within 30 % above 2Nx - 1
