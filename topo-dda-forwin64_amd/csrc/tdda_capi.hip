@@ -678,6 +678,15 @@ size_t job_part_size(const JobDesc<T>& J)
     return kPart * (size_t)std::max(t.z, t.v);
 }
 
+// Upper bound of the bytes build_groups uploads for a table of nj jobs
+// (descriptors: the table's own and the group subsets, each job in at most
+// four groups; tile starts; 256-byte alignment of every array).
+template <typename T>
+size_t table_bound(size_t nj)
+{
+    return 5 * nj * sizeof(JobDesc<T>) + (11 * nj + 16) * sizeof(int) + 256 * (2 + 9 * nj) + 1024;
+}
+
 // Completes a job table from its host descriptors: per-job solver state,
 // partial-sum buffers and counters, then the launch groups, all in device
 // memory owned by `arena`.
@@ -694,14 +703,25 @@ int finalize_table(std::vector<void*>& arena, const tdda_ctx_s* c, JobTable<T>& 
         jt.h[j].st = jt.st + j;
         jt.h[j].part = jt.part + poff[j];
     }
+    // the descriptor and tile arrays of all launch groups in one device
+    // block, filled from one host image with one copy (a request of the
+    // serving loop builds three tables: one allocation each, not dozens)
+    const size_t cap = table_bound<T>((size_t)nj);
+    char* dblk = nullptr;
+    CHK(dev_alloc(c, arena, &dblk, cap));
+    std::vector<char> img(cap);
+    size_t used = 0;
     auto put = [&](const auto* host, size_t n) {
         using U = std::remove_const_t<std::remove_pointer_t<decltype(host)>>;
-        U* d = nullptr;
-        if (dev_alloc(c, arena, &d, n) != TDDA_OK) return (const U*)nullptr;
-        if (hipMemcpy(d, host, sizeof(U) * n, hipMemcpyHostToDevice) != hipSuccess) return (const U*)nullptr;
-        return (const U*)d;
+        const size_t off = (used + 255) & ~(size_t)255, bytes = sizeof(U) * n;
+        if (off + bytes > cap) return (const U*)nullptr;
+        memcpy(img.data() + off, host, bytes);
+        used = off + bytes;
+        return reinterpret_cast<const U*>(dblk + off);
     };
-    return build_groups(c, jt, put);
+    CHK(build_groups(c, jt, put));
+    HIPCHK(hipMemcpy(dblk, img.data(), used, hipMemcpyHostToDevice));
+    return TDDA_OK;
 }
 
 // Launches the groups of one pass: one group on the stream itself; several
