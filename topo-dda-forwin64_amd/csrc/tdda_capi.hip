@@ -259,6 +259,7 @@ struct PassGroup {
     int ncu = 0;
     size_t lds = 0;
     std::vector<int> jobs;       // the group's jobs (indices into the table)
+    const int4* tmap = nullptr;  // x pass: (job, ky, f0, nf) per tile, or none (the kernel decodes)
 };
 
 // Jobs of one set of launches.  Each pass runs one launch per group of jobs
@@ -593,8 +594,38 @@ JobTiles job_tiles(const JobDesc<T>& J)
 // kernel) with its own descriptor subset, tile prefix sums, LDS size and
 // persistent grid.  `put(host, count)` copies an array to device memory and
 // returns its device address (nullptr on failure).
+// Tile map of an x-pass launch group: per tile the job (within the
+// group), ky, first column and column count, by the arithmetic of s_tile
+// (interleaved right-hand sides of one operator, mirrored ky order).
+template <typename T>
+std::vector<int4> xpass_tile_map(const std::vector<JobDesc<T>>& sub, const std::vector<int>& ts)
+{
+    std::vector<int4> m(ts.back());
+    int j0 = 0;
+    for (int r = 0; r < ts.back(); r++) {
+        while (ts[j0 + 1] <= r) j0++;
+        const JobDesc<T>& J0 = sub[j0];
+        const int first = j0 - J0.gidx;
+        int ky, fb, k;
+        xtile_map(r - ts[first], J0.My, J0.nfx, J0.gsize, ky, fb, k);
+        const int job = first + k;
+        const int f0 = fb * J0.tfx;
+        m[r] = make_int4(job, ky, f0, std::min(J0.tfx, sub[job].Mz * 3 - f0));
+    }
+    return m;
+}
+
+// Bytes of the x-pass tile maps of a table (build_groups with xmap).
+template <typename T>
+size_t xmap_bound(const JobTable<T>& jt)
+{
+    size_t n = 0;
+    for (const JobDesc<T>& J : jt.h) n += (size_t)J.My * J.nfx;
+    return n * sizeof(int4) + 256 * (jt.h.size() + 1);
+}
+
 template <typename T, typename Put>
-int build_groups(const tdda_ctx_s* c, JobTable<T>& jt, Put&& put)
+int build_groups(const tdda_ctx_s* c, JobTable<T>& jt, Put&& put, bool xmap = false)
 {
     const int nj = (int)jt.h.size();
     const size_t s = sizeof(Cx<T>);
@@ -649,6 +680,11 @@ int build_groups(const tdda_ctx_s* c, JobTable<T>& jt, Put&& put)
             g.d = (sub.size() == (size_t)nj) ? jt.d : put(sub.data(), sub.size());
             g.ts = put(ts.data(), ts.size());
             if (!g.d || !g.ts) return fail(TDDA_ERR_OOM, "job table upload");
+            if (xmap && kind == 2 && (k & kPxrKey)) {
+                const std::vector<int4> tm = xpass_tile_map(sub, ts);
+                g.tmap = put(tm.data(), tm.size());
+                if (!g.tmap) return fail(TDDA_ERR_OOM, "job table upload");
+            }
             groups.push_back(g);
         }
         return TDDA_OK;
@@ -706,7 +742,7 @@ int finalize_table(std::vector<void*>& arena, const tdda_ctx_s* c, JobTable<T>& 
     // the descriptor and tile arrays of all launch groups in one device
     // block, filled from one host image with one copy (a request of the
     // serving loop builds three tables: one allocation each, not dozens)
-    const size_t cap = table_bound<T>((size_t)nj);
+    const size_t cap = table_bound<T>((size_t)nj) + xmap_bound(jt);
     char* dblk = nullptr;
     CHK(dev_alloc(c, arena, &dblk, cap));
     std::vector<char> img(cap);
@@ -719,7 +755,7 @@ int finalize_table(std::vector<void*>& arena, const tdda_ctx_s* c, JobTable<T>& 
         used = off + bytes;
         return reinterpret_cast<const U*>(dblk + off);
     };
-    CHK(build_groups(c, jt, put));
+    CHK(build_groups(c, jt, put, true));
     HIPCHK(hipMemcpy(dblk, img.data(), used, hipMemcpyHostToDevice));
     return TDDA_OK;
 }
@@ -823,7 +859,7 @@ template <typename T>
 void run_px(const JobTable<T>& jt, int solver, hipStream_t s)
 {
     fan_out(jt, jt.gx, s, [&](const PassGroup<T>& g, hipStream_t st) {
-        launch_px<T>(g.L, solver, PassLaunch{dim3(g.tiles), g.lds, st, g.ncu}, g.d, g.ts, g.nj);
+        launch_px<T>(g.L, solver, PassLaunch{dim3(g.tiles), g.lds, st, g.ncu}, g.d, g.ts, g.nj, g.tmap);
     });
 }
 

@@ -601,6 +601,23 @@ __device__ __forceinline__ STile s_tile(const JobDesc<T>* __restrict__ jobs, con
     return t;
 }
 
+// The x-pass tile tl from a tile map the host built with s_tile's
+// arithmetic: (job, ky, f0, nf) per tile, one scalar load instead of the
+// job search and the interleaving divisions.
+template <typename T>
+__device__ __forceinline__ STile m_tile(const JobDesc<T>* __restrict__ jobs, const int4* __restrict__ tmap, int tl,
+                                        bool check_state)
+{
+    const int4 m = tmap[tl];
+    STile t;
+    t.job = m.x;
+    t.outer = m.y;
+    t.f0 = m.z;
+    t.nf = m.w;
+    t.live = !(check_state && job_inactive(jobs[t.job].st));
+    return t;
+}
+
 // P1: z-forward.  Tile = pz1 consecutive (x, y) pencils.
 template <typename T, int MODE, int LC>
 __global__ void TDDA_PASS_BOUNDS k_p1(const JobDesc<T>* __restrict__ jobs, const int* __restrict__ tstart,
@@ -1111,7 +1128,8 @@ __host__ __device__ constexpr int pxr_waves(int L)
 
 template <typename T, int LC, int NT>
 __global__ void __launch_bounds__(NT, pxr_waves<T>(LC)) k_pxr(const JobDesc<T>* __restrict__ jobs,
-                                                     const int* __restrict__ tstart, int njobs, int solver)
+                                                     const int* __restrict__ tstart, int njobs, int solver,
+                                                     const int4* __restrict__ tmap)
 {
     constexpr int L1 = pxr_l1(LC), L2 = LC / L1, H2 = (L2 + 1) / 2;
     static_assert(L1 > 0 && L1 >= L2, "length without a register split");
@@ -1124,7 +1142,7 @@ __global__ void __launch_bounds__(NT, pxr_waves<T>(LC)) k_pxr(const JobDesc<T>* 
     __syncthreads();
     const int task = threadIdx.x;
     for (int tl = blockIdx.x; tl < ntot; tl += gridDim.x) {
-        const STile t = s_tile(jobs, tstart, njobs, tl, true, solver != 0);
+        const STile t = tmap ? m_tile(jobs, tmap, tl, solver != 0) : s_tile(jobs, tstart, njobs, tl, true, solver != 0);
         if (!t.live) continue;
         const JobDesc<T>* J = jobs + t.job;
         const int Nx = J->Nx, My = J->My, Mz = J->Mz, F = 3 * Mz, nf = t.nf, ky = t.outer, ps = J->ps_x;

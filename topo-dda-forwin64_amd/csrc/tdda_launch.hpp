@@ -25,7 +25,8 @@ void launch_p5(int L, int mode, int diag, const PassLaunch& pl, const JobDesc<T>
 template <typename T>
 void launch_py(int L, bool inv, int solver, const PassLaunch& pl, const JobDesc<T>* jobs, const int* tstart, int nj);
 template <typename T>
-void launch_px(int L, int solver, const PassLaunch& pl, const JobDesc<T>* jobs, const int* tstart, int nj);
+void launch_px(int L, int solver, const PassLaunch& pl, const JobDesc<T>* jobs, const int* tstart, int nj,
+               const int4* tmap = nullptr);
 // Fused z + y passes (tdda_zy.hpp) of an operator whose y length has a
 // register split up to 128 points (zy_ylen_ok) and whose z length is one of
 // the register DFT lengths (zy_zlen_ok): k_pzy replaces P1 and P2, k_pyz P4
