@@ -260,6 +260,7 @@ struct PassGroup {
     size_t lds = 0;
     std::vector<int> jobs;       // the group's jobs (indices into the table)
     const int4* tmap = nullptr;  // x pass: (job, ky, f0, nf) per tile, or none (the kernel decodes)
+    const int2* zmap = nullptr;  // fused z + y passes: (job, x slab) per tile, or none
 };
 
 // Jobs of one set of launches.  Each pass runs one launch per group of jobs
@@ -615,13 +616,18 @@ std::vector<int4> xpass_tile_map(const std::vector<JobDesc<T>>& sub, const std::
     return m;
 }
 
-// Bytes of the x-pass tile maps of a table (build_groups with xmap).
+// Bytes of the tile maps of a table (build_groups with xmap): x pass and
+// fused z + y passes.
 template <typename T>
 size_t xmap_bound(const JobTable<T>& jt)
 {
     size_t n = 0;
-    for (const JobDesc<T>& J : jt.h) n += (size_t)J.My * J.nfx;
-    return n * sizeof(int4) + 256 * (jt.h.size() + 1);
+    size_t z = 0;
+    for (const JobDesc<T>& J : jt.h) {
+        n += (size_t)J.My * J.nfx;
+        z += (size_t)J.Nx;
+    }
+    return n * sizeof(int4) + z * sizeof(int2) + 512 * (jt.h.size() + 1);
 }
 
 template <typename T, typename Put>
@@ -684,6 +690,13 @@ int build_groups(const tdda_ctx_s* c, JobTable<T>& jt, Put&& put, bool xmap = fa
                 const std::vector<int4> tm = xpass_tile_map(sub, ts);
                 g.tmap = put(tm.data(), tm.size());
                 if (!g.tmap) return fail(TDDA_ERR_OOM, "job table upload");
+            }
+            if (xmap && kind == 4) {
+                std::vector<int2> zm(ts.back());
+                for (size_t q = 0; q + 1 < ts.size(); q++)
+                    for (int r = ts[q]; r < ts[q + 1]; r++) zm[r] = make_int2((int)q, r - ts[q]);
+                g.zmap = put(zm.data(), zm.size());
+                if (!g.zmap) return fail(TDDA_ERR_OOM, "job table upload");
             }
             groups.push_back(g);
         }
@@ -832,7 +845,7 @@ void run_p5(const JobTable<T>& jt, int mode, int diag, hipStream_t s)
         jt.gzy,
         [&](const PassGroup<T>& g, hipStream_t st) {
             launch_pyz<T>(g.L, mode, mode == OUT_PLAIN ? diag : 1, PassLaunch{dim3(g.tiles), g.lds, st, g.ncu}, g.d,
-                          g.ts, g.nj);
+                          g.ts, g.nj, g.zmap);
         },
         s);
 }
@@ -850,7 +863,7 @@ void run_py(const JobTable<T>& jt, bool inv, int solver, hipStream_t s, int inmo
         },
         inv ? none : jt.gzy,
         [&](const PassGroup<T>& g, hipStream_t st) {
-            launch_pzy<T>(g.L, inmode, PassLaunch{dim3(g.tiles), g.lds, st, g.ncu}, g.d, g.ts, g.nj);
+            launch_pzy<T>(g.L, inmode, PassLaunch{dim3(g.tiles), g.lds, st, g.ncu}, g.d, g.ts, g.nj, g.zmap);
         },
         s);
 }

@@ -32,9 +32,11 @@ void launch_px(int L, int solver, const PassLaunch& pl, const JobDesc<T>* jobs, 
 // the register DFT lengths (zy_zlen_ok): k_pzy replaces P1 and P2, k_pyz P4
 // and P5.
 template <typename T>
-void launch_pzy(int L, int mode, const PassLaunch& pl, const JobDesc<T>* jobs, const int* tstart, int nj);
+void launch_pzy(int L, int mode, const PassLaunch& pl, const JobDesc<T>* jobs, const int* tstart, int nj,
+                const int2* tmap = nullptr);
 template <typename T>
-void launch_pyz(int L, int mode, int diag, const PassLaunch& pl, const JobDesc<T>* jobs, const int* tstart, int nj);
+void launch_pyz(int L, int mode, int diag, const PassLaunch& pl, const JobDesc<T>* jobs, const int* tstart, int nj,
+                const int2* tmap = nullptr);
 // Launch-group key bits of the register-resident x and y passes: launch_px
 // runs k_pxr<L> for L | kPxrKey (with 512 threads per workgroup when
 // kPxrWide is also set), launch_py k_pyr<L>.

@@ -103,10 +103,18 @@ __device__ __forceinline__ void zy_zpencil(int Mz, const Cx<T>* src, Cx<T>* dst,
     }
 }
 
+// (job, x slab) of fused tile tl by the tile starts.
+__device__ __forceinline__ int2 zy_tile(const int* __restrict__ tstart, int njobs, int tl)
+{
+    const int job = find_job(tstart, njobs, tl);
+    return make_int2(job, tl - tstart[job]);
+}
+
 // P1 + P2 fused.  mode: IN_PLAIN, IN_X, IN_P, IN_T (as k_p1).
 template <typename T, int LY>
 __global__ void __launch_bounds__(kThreads, (zy_waves<T, false>())) k_pzy(const JobDesc<T>* __restrict__ jobs,
-                                                                const int* __restrict__ tstart, int njobs, int mode)
+                                                                const int* __restrict__ tstart, int njobs, int mode,
+                                                                const int2* __restrict__ tmap)
 {
     constexpr int L1 = pxr_l1(LY), L2 = LY / L1, H2 = (L2 + 1) / 2;
     static_assert(L1 > 0 && L1 >= L2, "length without a register split");
@@ -116,10 +124,12 @@ __global__ void __launch_bounds__(kThreads, (zy_waves<T, false>())) k_pzy(const 
     const int ntot = tstart[njobs];
     for (int i = threadIdx.x; i < LY; i += kThreads) tw[i] = gld(jobs[0].twfy + i);
     for (int tl = blockIdx.x; tl < ntot; tl += gridDim.x) {
-        const int job = find_job(tstart, njobs, tl);
+        // (job, x slab) of the tile: the host's tile map, else the search
+        const int2 jx = tmap ? tmap[tl] : zy_tile(tstart, njobs, tl);
+        const int job = jx.x;
         const JobDesc<T>* J = jobs + job;
         if (mode != IN_PLAIN && job_inactive(J->st)) continue;
-        const int x = tl - tstart[job];
+        const int x = jx.y;
         const int Ny = J->Ny, Nz = J->Nz, Mz = J->Mz, F = 3 * Mz, rs = zy_row_stride(Mz);
         Cx<T>* S = ZB + Ny * rs;
         // 1. the slab's inputs -> S[y][z][c], with the fused vector updates
@@ -255,7 +265,7 @@ __global__ void __launch_bounds__(kThreads, (zy_waves<T, false>())) k_pzy(const 
 template <typename T, int LY>
 __global__ void __launch_bounds__(kThreads, (zy_waves<T, true>())) k_pyz(const JobDesc<T>* __restrict__ jobs,
                                                                 const int* __restrict__ tstart, int njobs, int mode,
-                                                                int diag)
+                                                                int diag, const int2* __restrict__ tmap)
 {
     constexpr int L1 = pxr_l1(LY), L2 = LY / L1, H2 = (L2 + 1) / 2;
     static_assert(L1 > 0 && L1 >= L2, "length without a register split");
@@ -267,10 +277,11 @@ __global__ void __launch_bounds__(kThreads, (zy_waves<T, true>())) k_pyz(const J
     for (int i = threadIdx.x; i < LY; i += kThreads) tw[i] = gld(jobs[0].twfy + i);
     __syncthreads();
     for (int tl = blockIdx.x; tl < ntot; tl += gridDim.x) {
-        const int job = find_job(tstart, njobs, tl);
+        const int2 jx = tmap ? tmap[tl] : zy_tile(tstart, njobs, tl);
+        const int job = jx.x;
         const JobDesc<T>* J = jobs + job;
         if (mode != OUT_PLAIN && job_inactive(J->st)) continue;
-        const int x = tl - tstart[job];
+        const int x = jx.y;
         const int Ny = J->Ny, Nz = J->Nz, Mz = J->Mz, F = 3 * Mz, rs = zy_row_stride(Mz);
         Cx<T>* S = ZB + Ny * rs;
         Cx<T>* A = S;
