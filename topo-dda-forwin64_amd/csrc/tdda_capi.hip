@@ -259,7 +259,7 @@ struct PassGroup {
     int ncu = 0;
     size_t lds = 0;
     std::vector<int> jobs;       // the group's jobs (indices into the table)
-    const int4* tmap = nullptr;  // x pass: (job, ky, f0, nf) per tile, or none (the kernel decodes)
+    const int4* tmap = nullptr;  // register x / y pass: (job, ky or x, f0, nf) per tile, or none (the kernel decodes)
     const int2* zmap = nullptr;  // fused z + y passes: (job, x slab) per tile, or none
 };
 
@@ -599,6 +599,22 @@ JobTiles job_tiles(const JobDesc<T>& J)
 // group), ky, first column and column count, by the arithmetic of s_tile
 // (interleaved right-hand sides of one operator, mirrored ky order).
 template <typename T>
+std::vector<int4> ypass_tile_map(const std::vector<JobDesc<T>>& sub, const std::vector<int>& ts)
+{
+    std::vector<int4> m(ts.back());
+    int j = 0;
+    for (int r = 0; r < ts.back(); r++) {
+        while (ts[j + 1] <= r) j++;
+        const JobDesc<T>& J = sub[j];
+        int x, fb;
+        ytile_map(r - ts[j], J.Nx, J.nfy, x, fb);
+        const int f0 = fb * J.tfy;
+        m[r] = make_int4(j, x, f0, std::min(J.tfy, J.Mz * 3 - f0));
+    }
+    return m;
+}
+
+template <typename T>
 std::vector<int4> xpass_tile_map(const std::vector<JobDesc<T>>& sub, const std::vector<int>& ts)
 {
     std::vector<int4> m(ts.back());
@@ -616,15 +632,15 @@ std::vector<int4> xpass_tile_map(const std::vector<JobDesc<T>>& sub, const std::
     return m;
 }
 
-// Bytes of the tile maps of a table (build_groups with xmap): x pass and
-// fused z + y passes.
+// Bytes of the tile maps of a table (build_groups with xmap): register x
+// and y passes and fused z + y passes.
 template <typename T>
 size_t xmap_bound(const JobTable<T>& jt)
 {
     size_t n = 0;
     size_t z = 0;
     for (const JobDesc<T>& J : jt.h) {
-        n += (size_t)J.My * J.nfx;
+        n += (size_t)J.My * J.nfx + (size_t)J.Nx * J.nfy;
         z += (size_t)J.Nx;
     }
     return n * sizeof(int4) + z * sizeof(int2) + 512 * (jt.h.size() + 1);
@@ -686,8 +702,8 @@ int build_groups(const tdda_ctx_s* c, JobTable<T>& jt, Put&& put, bool xmap = fa
             g.d = (sub.size() == (size_t)nj) ? jt.d : put(sub.data(), sub.size());
             g.ts = put(ts.data(), ts.size());
             if (!g.d || !g.ts) return fail(TDDA_ERR_OOM, "job table upload");
-            if (xmap && kind == 2 && (k & kPxrKey)) {
-                const std::vector<int4> tm = xpass_tile_map(sub, ts);
+            if (xmap && (kind == 1 || kind == 2) && (k & kPxrKey)) {
+                const std::vector<int4> tm = kind == 2 ? xpass_tile_map(sub, ts) : ypass_tile_map(sub, ts);
                 g.tmap = put(tm.data(), tm.size());
                 if (!g.tmap) return fail(TDDA_ERR_OOM, "job table upload");
             }
@@ -859,7 +875,7 @@ void run_py(const JobTable<T>& jt, bool inv, int solver, hipStream_t s, int inmo
     fan_out2(
         jt, jt.gy,
         [&](const PassGroup<T>& g, hipStream_t st) {
-            launch_py<T>(g.L, inv, solver, PassLaunch{dim3(g.tiles), g.lds, st, g.ncu}, g.d, g.ts, g.nj);
+            launch_py<T>(g.L, inv, solver, PassLaunch{dim3(g.tiles), g.lds, st, g.ncu}, g.d, g.ts, g.nj, g.tmap);
         },
         inv ? none : jt.gzy,
         [&](const PassGroup<T>& g, hipStream_t st) {

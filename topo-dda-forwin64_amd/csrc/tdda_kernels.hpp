@@ -895,7 +895,8 @@ __host__ __device__ constexpr int pyr_waves(int L)
 
 template <typename T, bool INV, int LC, int NT = kThreads>
 __global__ void __launch_bounds__(NT, pyr_waves<T>(LC)) k_pyr(const JobDesc<T>* __restrict__ jobs,
-                                                     const int* __restrict__ tstart, int njobs, int solver)
+                                                     const int* __restrict__ tstart, int njobs, int solver,
+                                                     const int4* __restrict__ tmap)
 {
     constexpr int L1 = pxr_l1(LC), L2 = LC / L1, H2 = (L2 + 1) / 2;
     static_assert(L1 > 0 && L1 >= L2, "length without a register split");
@@ -907,7 +908,7 @@ __global__ void __launch_bounds__(NT, pyr_waves<T>(LC)) k_pyr(const JobDesc<T>* 
     __syncthreads();
     const int task = threadIdx.x;
     for (int tl = blockIdx.x; tl < ntot; tl += gridDim.x) {
-        const STile t = s_tile(jobs, tstart, njobs, tl, false, solver != 0);
+        const STile t = tmap ? m_tile(jobs, tmap, tl, solver != 0) : s_tile(jobs, tstart, njobs, tl, false, solver != 0);
         if (!t.live) continue;
         const JobDesc<T>* J = jobs + t.job;
         const int Ny = J->Ny, F = 3 * J->Mz, nf = t.nf, ps = J->ps_y;

@@ -23,7 +23,8 @@ void launch_p1(int L, int mode, const PassLaunch& pl, const JobDesc<T>* jobs, co
 template <typename T>
 void launch_p5(int L, int mode, int diag, const PassLaunch& pl, const JobDesc<T>* jobs, const int* tstart, int nj);
 template <typename T>
-void launch_py(int L, bool inv, int solver, const PassLaunch& pl, const JobDesc<T>* jobs, const int* tstart, int nj);
+void launch_py(int L, bool inv, int solver, const PassLaunch& pl, const JobDesc<T>* jobs, const int* tstart, int nj,
+               const int4* tmap = nullptr);
 template <typename T>
 void launch_px(int L, int solver, const PassLaunch& pl, const JobDesc<T>* jobs, const int* tstart, int nj,
                const int4* tmap = nullptr);
