@@ -111,7 +111,9 @@ def pass_kind(name):
 
 
 def spans(path):
-    rows = list(csv.DictReader(open(path)))
+    """Prints the tables; returns {"loop_ms", "busy_ms", "idle_ms", "span_us": {kind: (p, t)}}."""
+    with open(path) as fh:
+        rows = list(csv.DictReader(fh))
     for r in rows:
         r["s"], r["e"] = int(r["Start_Timestamp"]), int(r["End_Timestamp"])
     rows.sort(key=lambda r: r["s"])
@@ -167,6 +169,8 @@ def spans(path):
     print("|---|---|---|")
     for n, (c, d) in sorted(agg.items(), key=lambda kv: -kv[1][1])[:12]:
         print(f"| `{n}` | {c} | {d / 1e6:.3f} |")
+    return {"loop_ms": (t1 - t0) / 1e6, "busy_ms": busy / 1e6, "idle_ms": (t1 - t0 - busy) / 1e6,
+            "span_us": {k: (statistics.mean(ev), statistics.mean(od)) for k, (ev, od) in span.items()}}
 
 
 def main():
