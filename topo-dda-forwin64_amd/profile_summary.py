@@ -4,6 +4,7 @@
   python3 topo-dda-forwin64_amd/profile_summary.py stats  <kernel_stats.csv>  [--top 20]
   python3 topo-dda-forwin64_amd/profile_summary.py last <kernel_trace.csv> --kernel 'k_pxr<double, 160>' [--last 200]
   python3 topo-dda-forwin64_amd/profile_summary.py span <kernel_trace.csv> --precision c64 --workload batch64 [--last 10]
+  python3 topo-dda-forwin64_amd/profile_summary.py window <kernel_trace.csv> --kernel 'k_pxr<float' --reps 20 --per 13 [--skip 0] [--distinct]
   python3 topo-dda-forwin64_amd/profile_summary.py traffic <fetch counter_collection.csv>
           <write counter_collection.csv> --precision c128|c64 --workload topo80|mixed128|batch64
           [--json profiles/traffic.json]
@@ -13,7 +14,12 @@ mean, min and max duration of the last N dispatches of one kernel (the
 back-to-back timing launches that end a bench run).  `span` prints, per
 pass, the wall span of each of the last N repetitions of the pass (first
 launch start to last launch end; what HIP events around the pass measure)
-and the summed kernel time.  `traffic` prints the
+and the summed kernel time.  `window` takes the dispatches whose name
+contains --kernel, drops the last --skip of them, splits the last
+--reps x --per of the rest into --reps runs of --per consecutive dispatches
+(one repetition of a pass over a batch: one launch per FFT length), and
+prints the wall span of each run (what HIP events around it measure).
+`traffic` prints the
 average FETCH_SIZE / WRITE_SIZE per dispatch of each pass kernel, scaled by
 the calibration that the plain device copy in the same run gives (the copy
 of --copy-mib MiB run by profile_passes.py reads and writes exactly that
@@ -253,6 +259,49 @@ def traffic(fetch_csv, write_csv, precision, workload, copy_mib, out_json, last_
             json.dump(prev, fh, indent=1, sort_keys=True)
 
 
+def window(trace_csv, kernel, reps, per, skip, distinct=False):
+    """Wall span of each of the last `reps` runs of `per` consecutive
+    dispatches of the kernels whose name contains `kernel`.  With
+    `distinct`, the runs end where the last `reps` x `per` dispatches in
+    which every run holds the same `per` different kernels end (one launch
+    per FFT length of a batch, as opposed to later single-operator
+    launches), and `skip` counts back from there."""
+    with open(trace_csv) as fh:
+        rows = [r for r in csv.DictReader(fh) if kernel in r["Kernel_Name"]]
+    rows.sort(key=lambda r: int(r["Start_Timestamp"]))
+    need = reps * per
+    end = len(rows)
+    if distinct:
+        def ok(e):
+            # every run: the same `per` different kernels
+            if e < need:
+                return False
+            first = {r["Kernel_Name"] for r in rows[e - per:e]}
+            return len(first) == per and all({r["Kernel_Name"] for r in rows[e - (k + 1) * per:e - k * per]} == first
+                                             for k in range(1, reps))
+        while end >= need and not ok(end):
+            end -= 1
+        if end < need:
+            sys.exit(f"no {reps} runs of {per} different {kernel!r} kernels")
+    rows = rows[:end - skip] if skip else rows[:end]
+    if len(rows) < need:
+        sys.exit(f"{len(rows)} dispatches match {kernel!r}, {need} needed")
+    rows = rows[-need:]
+    spans, busy = [], []
+    for k in range(reps):
+        run = rows[k * per:(k + 1) * per]
+        s0 = min(int(r["Start_Timestamp"]) for r in run)
+        e1 = max(int(r["End_Timestamp"]) for r in run)
+        spans.append((e1 - s0) / 1e3)
+        busy.append(sum(int(r["End_Timestamp"]) - int(r["Start_Timestamp"]) for r in run) / 1e3)
+    names = sorted({r["Kernel_Name"].split("(")[0].replace("void ", "") for r in rows})
+    whole = (max(int(r["End_Timestamp"]) for r in rows) - int(rows[0]["Start_Timestamp"])) / 1e3
+    print(f"`{kernel}`: {reps} runs of {per} dispatches ({len(names)} kernels), skipping {skip}: "
+          f"span mean {sum(spans) / reps:.2f} us, min {min(spans):.2f} us, max {max(spans):.2f} us; "
+          f"first start to last end / runs {whole / reps:.2f} us; kernel busy (summed) mean {sum(busy) / reps:.2f} us")
+    return spans
+
+
 def next_smooth(n):
     m = n
     while True:
@@ -267,7 +316,7 @@ def next_smooth(n):
 
 def main():
     ap = argparse.ArgumentParser()
-    ap.add_argument("mode", choices=["stats", "traffic", "last", "span", "sq"])
+    ap.add_argument("mode", choices=["stats", "traffic", "last", "span", "sq", "window"])
     ap.add_argument("files", nargs="+")
     ap.add_argument("--top", type=int, default=20)
     ap.add_argument("--precision", default="c128")
@@ -277,8 +326,15 @@ def main():
     ap.add_argument("--kernel", default="k_pxr<double, 160>")
     ap.add_argument("--last", type=int, default=200,
                     help="last: dispatches averaged; traffic: last dispatches per pass kernel (0: all)")
+    ap.add_argument("--reps", type=int, default=20, help="window: runs")
+    ap.add_argument("--per", type=int, default=1, help="window: dispatches per run")
+    ap.add_argument("--skip", type=int, default=0, help="window: matching dispatches dropped from the end")
+    ap.add_argument("--distinct", action="store_true",
+                    help="window: end at the last runs that each hold --per different kernels")
     a = ap.parse_args()
-    if a.mode == "stats":
+    if a.mode == "window":
+        window(a.files[0], a.kernel, a.reps, a.per, a.skip, a.distinct)
+    elif a.mode == "stats":
         stats(a.files[0], a.top)
     elif a.mode == "last":
         last(a.files[0], a.kernel, a.last)
