@@ -706,7 +706,8 @@ def main():
     global TUNE
     TUNE = not args.no_tune
     if args.gpus != 1:
-        print(f"bench.py runs on one GPU only (asked for {args.gpus}); the sharding design is in DESIGN.md.",
+        print(f"bench.py runs on one GPU only (asked for {args.gpus}): several GPUs would each take whole "
+              "structures of the batch, one process per GPU, and that path is not built.",
               file=sys.stderr)
         sys.exit(2)
 
