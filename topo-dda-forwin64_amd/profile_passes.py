@@ -7,6 +7,7 @@ in and out).
 
 usage: python3 topo-dda-forwin64_amd/profile_passes.py [--workload topo80|mixed128|batch64]
        [--precision c128|c64] [--reps 20] [--copy-mib 256] [--no-tune] [--plan-out plan.json]
+       [--incidences N]
 
 A single operator (topo80, mixed128) runs the autotuner first (as in
 bench.py), so its tuner's trial launches precede the timed passes in a
@@ -45,6 +46,8 @@ def main():
     ap.add_argument("--reps", type=int, default=20)
     ap.add_argument("--copy-mib", type=int, default=256)
     ap.add_argument("--no-tune", action="store_true", help="planner tiles instead of the autotuner's")
+    ap.add_argument("--incidences", type=int, default=0,
+                    help="batch64: right-hand sides per structure, the first N of the 8 incidences (0: all)")
     ap.add_argument("--plan-out", default=None,
                     help="JSON file: per precision, whether each operator runs fused z + y passes "
                          "(profile_summary.py traffic reads it from the parent of the counter directories)")
@@ -55,6 +58,8 @@ def main():
         # the headline batch as bench.py builds it: planner tiles, 8
         # incidences per structure, one launch per FFT length and pass
         probs, incs = W.batch64(seed=4), W.incidences()
+        if a.incidences > 0:
+            incs = incs[:a.incidences]
         cores, models = [], []
         for p in probs:
             core = T.AProductCore(ctx, p.R, p.Nx, p.Ny, p.Nz, p.d, p.lam)
@@ -88,6 +93,7 @@ def main():
         with open(a.plan_out, "w") as fh:
             plan = {p: [c.zy_fused(q) for c in cl] for p, q in (("c128", T.C128), ("c64", T.C64))}
             plan["grids"] = [list(c.grid) for c in cl]
+            plan["nrhs"] = len(incs) if a.workload == "batch64" else 1
             json.dump(plan, fh)
     copy_calibration(a.copy_mib)
     print(json.dumps({"workload": a.workload, "precision": a.precision, "grid": grid,

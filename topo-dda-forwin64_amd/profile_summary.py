@@ -204,12 +204,12 @@ def traffic(fetch_csv, write_csv, precision, workload, copy_mib, out_json, last_
 
     # which operators ran fused z + y passes: profile_passes.py's plan file
     # beside the counter files (one flag per structure, in workload order)
-    plan, grids = None, None
+    plan, grids, nrhs = None, None, len(W.incidences())
     pf = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(fetch_csv))), "plan.json")
     if os.path.exists(pf):
         with open(pf) as fh:
             pj = json.load(fh)
-        plan, grids = pj.get(precision), pj.get("grids")
+        plan, grids, nrhs = pj.get(precision), pj.get("grids"), pj.get("nrhs", nrhs)
 
     class Core:
         def zy_fused(self, prec):
@@ -224,7 +224,7 @@ def traffic(fetch_csv, write_csv, precision, workload, copy_mib, out_json, last_
         c.fused = bool(plan[k]) if plan else False
         return c
     if workload == "batch64":
-        alg = batch_bytes([(core_of(p, k), len(W.incidences())) for k, p in enumerate(W.batch64(seed=4))], s)
+        alg = batch_bytes([(core_of(p, k), nrhs) for k, p in enumerate(W.batch64(seed=4))], s)
     else:
         alg = pass_bytes(core_of(W.topo80()[0] if workload == "topo80" else W.mixed128()), s)
     print(f"calibration: device copy of {copy_mib} MiB in {n} dispatches; FETCH_SIZE x {ffac:.3f}, "
